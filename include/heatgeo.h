@@ -1,0 +1,211 @@
+/*
+ * heatgeo.h -- C ABI of libheatgeo, the MI355X (gfx950) heat-method geodesic library.
+ *
+ * The boundary a host binding (ctypes here, see ordito_amd/_lib.py) calls to back ordito's
+ * array-in/array-out heat path: ``ordito.heat.heat_geodesic`` (ordito heat.py:370-549),
+ * ``ordito.heat.heat_operators`` (heat.py:200-367), ``ordito.heat.HeatSolver``
+ * (heat.py:3491-4456) and the per-mesh cache ``ordito.mesh.Trimesh.heat_solver`` /
+ * ``release_factorizations`` (mesh.py:2187-2265, 3457-3495).
+ *
+ * Conventions
+ *  - Every pointer named d_* is device memory on the current HIP device; sizes are element counts.
+ *  - Vertices are float32 xyz triples (V,3), faces a flat int32 buffer (3F,), sources int32,
+ *    fields float64 laid out column-major over queries: column q of a (k, V) field starts at q*V.
+ *  - Every function returns an hg status code; hg_last_error() holds the calling thread's last
+ *    message. HG_ERR_VALUE maps to Python ValueError, HG_ERR_RUNTIME to RuntimeError.
+ *  - Work is issued on the mesh's stream (hg_mesh_set_stream); functions that return results
+ *    through host memory synchronize that stream.
+ *  - A mesh is not safe for concurrent use from two threads (the reference's rule,
+ *    heat.py:298-308: two solves against one operator run on one stream).
+ */
+#ifndef HEATGEO_H
+#define HEATGEO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HG_OK 0
+#define HG_ERR_VALUE 1   /* bad argument: out-of-range source, shape mismatch, unknown method */
+#define HG_ERR_RUNTIME 2 /* HIP failure, missing state, dead mesh */
+#define HG_ERR_NOT_PD 3  /* a non-positive pivot: the operator is not definite (cholesky.py:90-118) */
+#define HG_ERR_BUDGET 4  /* the factorization would exceed the memory budget */
+
+/* Discretizations (heat.py:2002-2012): the lumped mass and the mean edge length of t = h^2. */
+#define HG_METHOD_MESHLAB 0     /* barycentric mass, h over every face's three edges */
+#define HG_METHOD_POTPOURRI3D 2 /* barycentric mass, h over the unique edges */
+
+/* How a solve runs (HeatSolver policy, heat.py:4348-4423). */
+#define HG_MODE_ITERATE 0  /* settled Jacobi-CG heat solve; Jacobi-Chebyshev CG Poisson solve */
+#define HG_MODE_FACTORED 1 /* refined solves through the held sparse Cholesky factor */
+
+/* Precision of a factored solve. */
+#define HG_PRECISION_FP64 0  /* FP64 solve blocks and vectors, FP64 refinement */
+#define HG_PRECISION_MIXED 1 /* FP32 solve blocks, FP64 residuals and refinement */
+
+/* Which systems hg_factor builds. */
+#define HG_SYSTEM_HEAT 1
+#define HG_SYSTEM_POISSON 2
+
+typedef struct hg_mesh hg_mesh;
+
+/* Device views of the assembled operators: heat_operators' 8-tuple (heat.py:169-178). */
+typedef struct hg_operator_view {
+  int64_t n_vertices;
+  int64_t n_faces;
+  int64_t nnz;
+  const int32_t* offsets;   /* (V+1) CSR row offsets shared by the three operators */
+  const int32_t* columns;   /* (nnz) */
+  const double* laplacian;  /* (nnz) L, igl sign convention (negative diagonal) */
+  const double* heat;       /* (nnz) M - t L */
+  const double* poisson;    /* (nnz) -L */
+  const double* mass;       /* (V) barycentric lumped mass */
+  const float* cot;         /* (F,3) half-cotangents */
+  const float* normals;     /* (F,3) unit face normals */
+  const float* areas;       /* (F) face areas */
+  double t;                 /* the diffusion time the heat system was built at */
+  int32_t settle_rounds;    /* the settle window of the heat solve (heat.py:5234-5243) */
+  int32_t time_from_edges;  /* 1: t is the device-formed squared mean face-edge length */
+} hg_operator_view;
+
+/* Statistics of the last solve on a mesh. */
+typedef struct hg_solve_stats {
+  int32_t heat_rounds;       /* CG rounds that stepped, or refinement solves */
+  int32_t poisson_rounds;
+  int32_t heat_refinements;  /* refinement solves of the factored heat step */
+  int32_t poisson_refinements;
+  int32_t backward_error_failed; /* 1 when the settled iterate failed its backward-error check */
+  int32_t reserved;
+} hg_solve_stats;
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* hg_last_error(void);
+
+/* Library version string. */
+const char* hg_version(void);
+
+/*
+ * Bind a mesh: copies the caller's device buffers (V,3) float32 and (3F,) int32 into library-owned
+ * memory and builds the operator pattern. Replaces what ordito.mesh.Trimesh(vertices, faces) plus
+ * mesh_operator_pattern (laplacian.py:1071-1389) hold. Faces with an index outside [0, V) are a
+ * ValueError. stream: a hipStream_t or NULL for the null stream.
+ */
+int hg_mesh_create(const float* d_vertices, int64_t n_vertices, const int32_t* d_faces,
+                   int64_t n_faces, void* stream, hg_mesh** out_mesh);
+
+/* Free a mesh and everything it holds (operators, factors). */
+int hg_mesh_destroy(hg_mesh* mesh);
+
+/* Issue the mesh's later work on this stream. */
+int hg_mesh_set_stream(hg_mesh* mesh, void* stream);
+
+/*
+ * Assemble the heat operators at time t (heat_operators / _heat_operators, heat.py:200-367,
+ * 2015-2206). t = NaN selects the default h^2 with the method's h (heat.py:2002-2012); with
+ * solver_default_time set it is the solver's rule instead (HeatSolver._vertex_time,
+ * heat.py:3714-3736): raised to (diag/400)^2 when that is longer, with the settle window grown to
+ * match. Re-assembling drops any held factorization.
+ */
+int hg_operators(hg_mesh* mesh, double t, int32_t method, int32_t solver_default_time);
+
+/* Device views of the operators (assembling at the default t first if needed). */
+int hg_operator_views(hg_mesh* mesh, hg_operator_view* out_view);
+
+/*
+ * Factor the systems named by `systems` (HeatSolver.factor, heat.py:3863-3888; the Poisson
+ * factorization shares the heat analysis, cholesky.py:313-362). Sets *out_factored to 1 when
+ * every named system holds a factor. `precision` HG_PRECISION_MIXED also keeps FP32 solve blocks.
+ */
+int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_factored);
+
+/* Whether each system holds a factor: bit 0 heat, bit 1 Poisson. */
+int hg_factored(hg_mesh* mesh, int32_t* out_systems);
+
+/*
+ * Diffuse: solve (M - tL) u = rhs for n_columns columns of V entries (HeatSolver.diffuse,
+ * heat.py:3890-3914). HG_MODE_ITERATE runs the settled Jacobi-CG (solve_spd_settled,
+ * linalg.py:2490-2727) and its componentwise backward-error check; on failure it factors the
+ * heat system and solves again refined (the reference's fallback), reporting it in *out_flags
+ * (bit 0: the iterate failed its check, bit 1: a factorization was built). HG_MODE_FACTORED solves
+ * through the held factor, refined componentwise to 1e-12 (linalg.py:2659-2665), factoring first
+ * if none is held. d_solution is overwritten (the iterate starts from zero).
+ */
+int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
+               int32_t mode, int32_t precision, int32_t* out_flags);
+
+/*
+ * Solve the Poisson system -L phi = rhs (HeatSolver.solve_poisson, heat.py:4062-4151): iterated
+ * by conjugate gradient under the Jacobi-Chebyshev polynomial to relative residual 1e-8, or
+ * through the held factor with normwise refinement to 1e-8 (heat.py:4138-4140). One row per
+ * singular component is pinned and each component's constant set to zero mean.
+ */
+int hg_solve_poisson(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
+                     int32_t mode, int32_t precision);
+
+/*
+ * The heat method's right-hand side: -div of the unit gradient of each heat column
+ * (_distance_from_heat / unit_gradient_divergence, heat.py:3052-3131, kernels/heat.py:626-687),
+ * summed per vertex in ascending corner order.
+ */
+int hg_unit_gradient_divergence(hg_mesh* mesh, const double* d_heat, double* d_neg_divergence,
+                                int64_t n_columns);
+
+/*
+ * Seed the source indicators: column q gets 1 at each of its sources d_sources[off[q]..off[q+1])
+ * (kernels/heat.py:599-623). An index outside [0, V) is a ValueError (heat.py:5488-5503).
+ * d_source_offsets is a host array of n_columns+1 entries.
+ */
+int hg_seed_sources(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_offsets,
+                    int64_t n_columns, double* d_u0);
+
+/*
+ * Shift each column so its mean over its sources is zero and orient it positive
+ * (_shift_to_sources, heat.py:3134-3162; kernels/heat.py:3383-3424).
+ */
+int hg_shift_to_sources(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
+                        const int64_t* source_offsets, int64_t n_columns);
+
+/*
+ * The whole path for n_queries source sets (heat_geodesic, heat.py:370-609): seed, diffuse,
+ * divergence, Poisson, shift. Query q's sources are d_sources[off[q]..off[q+1]) with
+ * source_offsets a host array; its (V,) float64 distance goes to d_out + q*V. A query with no
+ * sources returns zeros. Statistics of the call go to *out_stats when not NULL.
+ */
+int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_offsets,
+                int64_t n_queries, double* d_out, int32_t mode, int32_t precision,
+                hg_solve_stats* out_stats);
+
+/* Device bytes the held factorizations take (HeatSolver.nbytes, heat.py:3838-3847). */
+int hg_nbytes(hg_mesh* mesh, int64_t* out_bytes);
+
+/* Drop every held factorization; the operators stay (HeatSolver.release, heat.py:3849-3861). */
+int hg_release(hg_mesh* mesh);
+
+/* Analysis statistics of the held plan: supernodes, levels, stored solve-block entries, flops,
+ * arena entries, largest front. */
+int hg_plan_stats(hg_mesh* mesh, int64_t* out_stats6);
+
+/*
+ * Host-only access to the Cholesky analysis (sparse_cholesky's ordering and symbolic phase,
+ * cholesky.py:538-1273), for inspection and tests; needs no GPU. The pattern is a symmetric CSR
+ * with sorted rows, xyz float32 (n,3) host coordinates for the geometric nested dissection.
+ */
+typedef struct hg_plan hg_plan;
+int hg_plan_create(const int32_t* offsets, const int32_t* columns, int64_t n, const float* xyz,
+                   int32_t leaf_rows, int32_t top_rows, hg_plan** out_plan);
+/* Sizes: [n_super, n_levels, total structure rows, total children, lower entries, arena entries,
+ *         solve-block entries, vector entries]. */
+int hg_plan_sizes(hg_plan* plan, int64_t* out8);
+/* Copy one array out: 0 perm, 1 col_begin, 2 ncol, 3 nrow, 4 parent, 5 level, 6 row_ptr, 7 rows,
+ * 8 rel, 9 child_ptr, 10 children, 11 level_ptr, 12 level_nodes (int32); 13 front_off, 14 w_off,
+ * 15 vec_off, 16 entry_dst (int64); 17 entry_src, 18 entry_row, 19 entry_col (int32). */
+int hg_plan_copy(hg_plan* plan, int32_t field, void* out);
+int hg_plan_destroy(hg_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEATGEO_H */

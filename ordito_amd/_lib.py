@@ -1,0 +1,172 @@
+"""
+ctypes binding of ``libheatgeo.so`` (the C ABI declared in ``include/heatgeo.h``).
+
+The library is loaded from this package's directory; there is no fallback: a missing library or a
+missing GPU raises. Status codes map to the exceptions ordito raises on the same conditions
+(``heat.py:5488-5503`` ValueError on a bad source, ``cholesky.py:90-118``
+NotPositiveDefiniteError(ValueError), RuntimeError otherwise).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "libheatgeo.so"
+
+HG_OK = 0
+HG_ERR_VALUE = 1
+HG_ERR_RUNTIME = 2
+HG_ERR_NOT_PD = 3
+HG_ERR_BUDGET = 4
+
+HG_METHOD_MESHLAB = 0
+HG_METHOD_POTPOURRI3D = 2
+HG_MODE_ITERATE = 0
+HG_MODE_FACTORED = 1
+HG_PRECISION_FP64 = 0
+HG_PRECISION_MIXED = 1
+HG_SYSTEM_HEAT = 1
+HG_SYSTEM_POISSON = 2
+
+# The C ABI's exported symbols, in header order; the load test checks every one resolves.
+EXPORTED = (
+    "hg_last_error",
+    "hg_version",
+    "hg_mesh_create",
+    "hg_mesh_destroy",
+    "hg_mesh_set_stream",
+    "hg_operators",
+    "hg_operator_views",
+    "hg_factor",
+    "hg_factored",
+    "hg_diffuse",
+    "hg_solve_poisson",
+    "hg_unit_gradient_divergence",
+    "hg_seed_sources",
+    "hg_shift_to_sources",
+    "hg_geodesic",
+    "hg_nbytes",
+    "hg_release",
+    "hg_plan_stats",
+    "hg_plan_create",
+    "hg_plan_sizes",
+    "hg_plan_copy",
+    "hg_plan_destroy",
+)
+
+
+class NotPositiveDefiniteError(ValueError):
+    """The operator met a pivot that is not positive (``cholesky.py:90-118``)."""
+
+
+class OperatorView(ctypes.Structure):
+    _fields_ = [
+        ("n_vertices", ctypes.c_int64),
+        ("n_faces", ctypes.c_int64),
+        ("nnz", ctypes.c_int64),
+        ("offsets", ctypes.c_void_p),
+        ("columns", ctypes.c_void_p),
+        ("laplacian", ctypes.c_void_p),
+        ("heat", ctypes.c_void_p),
+        ("poisson", ctypes.c_void_p),
+        ("mass", ctypes.c_void_p),
+        ("cot", ctypes.c_void_p),
+        ("normals", ctypes.c_void_p),
+        ("areas", ctypes.c_void_p),
+        ("t", ctypes.c_double),
+        ("settle_rounds", ctypes.c_int32),
+        ("time_from_edges", ctypes.c_int32),
+    ]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("heat_rounds", ctypes.c_int32),
+        ("poisson_rounds", ctypes.c_int32),
+        ("heat_refinements", ctypes.c_int32),
+        ("poisson_refinements", ctypes.c_int32),
+        ("backward_error_failed", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict[str, int]:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved"}
+
+
+_lib: ctypes.CDLL | None = None
+_lock = threading.Lock()
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+
+_SIGNATURES = {
+    "hg_last_error": ([], ctypes.c_char_p),
+    "hg_version": ([], ctypes.c_char_p),
+    "hg_mesh_create": ([_P, _I64, _P, _I64, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "hg_mesh_destroy": ([_P], ctypes.c_int),
+    "hg_mesh_set_stream": ([_P, _P], ctypes.c_int),
+    "hg_operators": ([_P, ctypes.c_double, _I32, _I32], ctypes.c_int),
+    "hg_operator_views": ([_P, ctypes.POINTER(OperatorView)], ctypes.c_int),
+    "hg_factor": ([_P, _I32, _I32, _PI32], ctypes.c_int),
+    "hg_factored": ([_P, _PI32], ctypes.c_int),
+    "hg_diffuse": ([_P, _P, _P, _I64, _I32, _I32, _PI32], ctypes.c_int),
+    "hg_solve_poisson": ([_P, _P, _P, _I64, _I32, _I32], ctypes.c_int),
+    "hg_unit_gradient_divergence": ([_P, _P, _P, _I64], ctypes.c_int),
+    "hg_seed_sources": ([_P, _P, _PI64, _I64, _P], ctypes.c_int),
+    "hg_shift_to_sources": ([_P, _P, _P, _PI64, _I64], ctypes.c_int),
+    "hg_geodesic": ([_P, _P, _PI64, _I64, _P, _I32, _I32, ctypes.POINTER(SolveStats)], ctypes.c_int),
+    "hg_nbytes": ([_P, _PI64], ctypes.c_int),
+    "hg_release": ([_P], ctypes.c_int),
+    "hg_plan_stats": ([_P, _PI64], ctypes.c_int),
+    "hg_plan_create": ([_P, _P, _I64, _P, _I32, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "hg_plan_sizes": ([_P, _PI64], ctypes.c_int),
+    "hg_plan_copy": ([_P, _I32, _P], ctypes.c_int),
+    "hg_plan_destroy": ([_P], ctypes.c_int),
+}
+
+
+def load() -> ctypes.CDLL:
+    """Load the library once; raise if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIBRARY_PATH.exists():
+                raise RuntimeError(
+                    f"{LIBRARY_PATH} is not built: run `python -c 'import __graft_entry__ as g; "
+                    "g.build()'` (hipcc, gfx950); there is no CPU fallback"
+                )
+            lib = ctypes.CDLL(str(LIBRARY_PATH))
+            for name, (argtypes, restype) in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = argtypes
+                fn.restype = restype
+            _lib = lib
+        return _lib
+
+
+def check(status: int) -> None:
+    """Raise the exception a status code names, with the library's message."""
+    if status == HG_OK:
+        return
+    message = load().hg_last_error().decode(errors="replace")
+    if status == HG_ERR_VALUE:
+        raise ValueError(message)
+    if status == HG_ERR_NOT_PD:
+        raise NotPositiveDefiniteError(message)
+    if status == HG_ERR_BUDGET:
+        raise MemoryError(message)
+    raise RuntimeError(message)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))
+
+
+def offsets_array(offsets) -> ctypes.Array:
+    values = [int(v) for v in offsets]
+    return (ctypes.c_int64 * len(values))(*values)

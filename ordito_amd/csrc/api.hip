@@ -1,0 +1,758 @@
+// The C ABI of libheatgeo (include/heatgeo.h): mesh state, operator assembly, the heat and Poisson
+// solves in both modes, the divergence, the shift and the fused geodesic path.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "cholesky.h"
+
+namespace hg {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& message) { g_last_error = message; }
+
+// Constants of the reference's solver policy.
+constexpr int32_t kHeatCheckRounds = 16;          // heat.py:147
+constexpr double kHeatChangeTolerance = 1e-6;     // heat.py:151
+constexpr int32_t kHeatSettleRounds = 192;        // heat.py:152
+constexpr double kDiffusionDecay = 400.0;         // heat.py:158
+constexpr double kSettleBackwardError = 1e-6;     // linalg.py:179
+constexpr double kSettleDirectTolerance = 1e-12;  // linalg.py:181
+constexpr double kPoissonTolerance = 1e-8;        // heat.py:108, 4138-4140
+constexpr double kMixedHeatTolerance = 1e-10;     // mixed path: FP64 refinement target
+constexpr int32_t kRefineRounds = 12;             // cholesky.py:74
+constexpr int32_t kMixedRefineRounds = 24;
+constexpr int32_t kChebyshevSteps = 12;           // linalg.py CHEBYSHEV_DEGREE: 11 products
+constexpr double kChebyshevInterval = 80.0;       // linalg.py CHEBYSHEV_INTERVAL
+constexpr double kChebyshevCap = 0.5;
+
+}  // namespace hg
+
+using namespace hg;
+
+struct hg_mesh {
+  hipStream_t stream = nullptr;
+  int64_t n = 0, f = 0;
+  std::vector<float> h_xyz;
+  std::vector<int32_t> h_faces;
+  HostPattern pattern;
+  DeviceArray<float> xyz;
+  DeviceArray<int32_t> faces, offsets, columns, corner_ptr, corner_slots, vertex_ptr,
+      vertex_corners;
+  // Operators.
+  bool assembled = false;
+  double t = 0.0;
+  int32_t settle_rounds = kHeatSettleRounds;
+  int32_t time_from_edges = 1;
+  DeviceArray<float> cot, normals, areas, heat32;
+  DeviceArray<double> mass, lap, heat, poisson, heat_inv, poisson_inv, scalars, edge_partials;
+  double poisson_bound = -1.0;
+  // Components: the Poisson system pins each component's last-eliminated row.
+  int32_t n_comp = 0;
+  std::vector<int32_t> labels;
+  DeviceArray<int32_t> comp_ptr, comp_members;
+  DeviceArray<int32_t> poisson_pinned_new, poisson_dropped_old, heat_pinned_new, heat_dropped_old;
+  std::vector<int64_t> poisson_pin_slots, heat_pin_slots;
+  bool has_heat_pins = false;
+  // Cholesky.
+  std::unique_ptr<DevicePlan> plan;
+  Factor heat_factor, poisson_factor;
+  // Workspaces.
+  CgWorkspace cg;
+  SolveWorkspace solve;
+  DeviceArray<double> fields, corner_terms, shift_partials, shift_values;
+  DeviceArray<int32_t> d_sources, invalid;
+  DeviceArray<int64_t> d_owner, d_offsets;
+  int64_t field_capacity = 0, corner_capacity = 0;
+  hg_solve_stats stats{};
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& body) {
+  try {
+    body();
+    return HG_OK;
+  } catch (const Error& e) {
+    set_error(e.message);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_error("host out of memory");
+    return HG_ERR_RUNTIME;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return HG_ERR_RUNTIME;
+  }
+}
+
+void require_mesh(hg_mesh* mesh) {
+  HG_REQUIRE(mesh != nullptr, HG_ERR_RUNTIME, "the mesh handle is null (released or never created)");
+}
+
+// ---- operators --------------------------------------------------------------------------------
+
+// The mean unique-edge length: each edge once, over the pattern's strict upper triangle.
+double host_mean_unique_edge(const hg_mesh& m) {
+  double total = 0.0;
+  int64_t count = 0;
+  for (int64_t i = 0; i < m.n; ++i) {
+    for (int32_t k = m.pattern.offsets[i]; k < m.pattern.offsets[i + 1]; ++k) {
+      int32_t j = m.pattern.columns[k];
+      if (j <= i) continue;
+      const float* a = &m.h_xyz[3 * i];
+      const float* b = &m.h_xyz[3 * j];
+      float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+      total += static_cast<double>(sqrtf(dx * dx + dy * dy + dz * dz));
+      ++count;
+    }
+  }
+  return count > 0 ? total / static_cast<double>(count) : 0.0;
+}
+
+double host_mean_face_edge(const hg_mesh& m) {
+  if (m.f == 0) return 0.0;
+  double total = 0.0;
+  for (int64_t f = 0; f < m.f; ++f) {
+    for (int e = 0; e < 3; ++e) {
+      const float* a = &m.h_xyz[3 * m.h_faces[3 * f + e]];
+      const float* b = &m.h_xyz[3 * m.h_faces[3 * f + (e + 1) % 3]];
+      float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+      total += static_cast<double>(sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+  }
+  return total / static_cast<double>(3 * m.f);
+}
+
+double host_diffusion_floor(const hg_mesh& m) {
+  if (m.n == 0) return 0.0;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t v = 0; v < m.n; ++v)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], m.h_xyz[3 * v + d]);
+      hi[d] = std::max(hi[d], m.h_xyz[3 * v + d]);
+    }
+  double s = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double e = static_cast<double>(hi[d]) - static_cast<double>(lo[d]);
+    s += e * e;
+  }
+  double diag = std::sqrt(s);
+  return (diag / kDiffusionDecay) * (diag / kDiffusionDecay);
+}
+
+void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default) {
+  const hipStream_t st = m.stream;
+  const int64_t n = m.n, nnz = static_cast<int64_t>(m.pattern.columns.size());
+  const int32_t face_edges = method == HG_METHOD_MESHLAB ? 1 : 0;
+  // The time rule of HeatSolver._vertex_time (heat.py:3714-3736) and _settle_window (5234-5243).
+  const double mean = face_edges ? host_mean_face_edge(m) : host_mean_unique_edge(m);
+  const double def = mean * mean;
+  double t_value = t_request;
+  int32_t from_edges = 0;
+  if (std::isnan(t_request)) {
+    const double floor = solver_default ? host_diffusion_floor(m) : 0.0;
+    if (floor > def) t_value = floor;
+    else from_edges = 1;
+  }
+  if (from_edges || !(t_value > def && def > 0.0)) m.settle_rounds = kHeatSettleRounds;
+  else m.settle_rounds = static_cast<int32_t>(std::ceil(kHeatSettleRounds * std::sqrt(t_value / def)));
+  m.time_from_edges = from_edges;
+
+  m.cot.allocate(3 * m.f);
+  m.normals.allocate(3 * m.f);
+  m.areas.allocate(m.f);
+  m.mass.allocate(n);
+  m.lap.allocate(nnz);
+  m.heat.allocate(nnz);
+  m.poisson.allocate(nnz);
+  m.heat32.allocate(nnz);
+  m.heat_inv.allocate(n);
+  m.poisson_inv.allocate(n);
+  m.scalars.allocate(4);
+  m.edge_partials.allocate(2 * edge_sum_blocks(n));
+  launch_face_terms(m.xyz.ptr, m.faces.ptr, m.f, m.cot.ptr, m.normals.ptr, m.areas.ptr, st);
+  launch_mass(m.vertex_ptr.ptr, m.vertex_corners.ptr, m.areas.ptr, n, m.mass.ptr, st);
+  launch_edge_sums(m.offsets.ptr, m.columns.ptr, m.corner_ptr.ptr, m.xyz.ptr, n, face_edges,
+                   m.edge_partials.ptr, from_edges, from_edges ? 0.0 : t_value, m.scalars.ptr, st);
+  launch_operator_rows(m.offsets.ptr, m.columns.ptr, m.corner_ptr.ptr, m.corner_slots.ptr,
+                       m.cot.ptr, m.mass.ptr, m.scalars.ptr, n, m.lap.ptr, m.heat.ptr,
+                       m.poisson.ptr, m.heat32.ptr, m.heat_inv.ptr, m.poisson_inv.ptr, st);
+  double host_scalars[4] = {0, 0, 0, 0};
+  HG_HIP(hipMemcpyAsync(host_scalars, m.scalars.ptr, sizeof(host_scalars), hipMemcpyDeviceToHost, st));
+  HG_HIP(hipStreamSynchronize(st));
+  m.t = host_scalars[3];
+  m.assembled = true;
+  m.poisson_bound = -1.0;
+  m.heat_factor.reset();
+  m.poisson_factor.reset();
+}
+
+void ensure_operators(hg_mesh& m) {
+  if (!m.assembled) assemble(m, std::numeric_limits<double>::quiet_NaN(), HG_METHOD_MESHLAB, true);
+}
+
+// ---- pins and the factorization -----------------------------------------------------------------
+
+void ensure_plan(hg_mesh& m) {
+  if (m.plan) return;
+  AnalysisOptions options;
+  Plan plan = analyse(m.pattern, m.h_xyz.data(), options);
+  auto device = std::make_unique<DevicePlan>();
+  device->upload(plan, m.stream);
+  // Pins: the Poisson system's one row per component, its last-eliminated vertex
+  // (kernels/cholesky.py:195-214); the heat system's empty rows (unreferenced vertices).
+  const int64_t n = m.n;
+  std::vector<int32_t> last(m.n_comp, -1);
+  for (int64_t v = 0; v < n; ++v) {
+    int32_t c = m.labels[v];
+    if (last[c] < 0 || plan.iperm[v] > plan.iperm[last[c]]) last[c] = static_cast<int32_t>(v);
+  }
+  std::vector<int32_t> pinned_new(n, 0), dropped_old(n, 0);
+  std::vector<int32_t> heat_pinned_new(n, 0), heat_dropped_old(n, 0);
+  auto front_slot = [&](int32_t new_index) {
+    auto it = std::upper_bound(plan.col_begin.begin(), plan.col_begin.end(), new_index);
+    int32_t s = static_cast<int32_t>(it - plan.col_begin.begin()) - 1;
+    int64_t m_s = plan.ncol[s] + plan.nrow[s];
+    int64_t local = new_index - plan.col_begin[s];
+    return plan.front_off[s] + local * m_s + local;
+  };
+  m.poisson_pin_slots.clear();
+  m.heat_pin_slots.clear();
+  for (int32_t c = 0; c < m.n_comp; ++c) {
+    int32_t v = last[c];
+    pinned_new[plan.iperm[v]] = 1;
+    dropped_old[v] = 1;
+    m.poisson_pin_slots.push_back(front_slot(plan.iperm[v]));
+  }
+  m.has_heat_pins = false;
+  for (int64_t v = 0; v < n; ++v) {
+    if (m.pattern.diagonal[v] < 0) {
+      heat_pinned_new[plan.iperm[v]] = 1;
+      heat_dropped_old[v] = 1;
+      m.heat_pin_slots.push_back(front_slot(plan.iperm[v]));
+      m.has_heat_pins = true;
+    }
+  }
+  m.poisson_pinned_new.upload(pinned_new, m.stream);
+  m.poisson_dropped_old.upload(dropped_old, m.stream);
+  m.heat_pinned_new.upload(heat_pinned_new, m.stream);
+  m.heat_dropped_old.upload(heat_dropped_old, m.stream);
+  HG_HIP(hipStreamSynchronize(m.stream));
+  m.plan = std::move(device);
+}
+
+void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
+  ensure_operators(m);
+  ensure_plan(m);
+  if (system == HG_SYSTEM_HEAT) {
+    if (!m.heat_factor.held())
+      factor_numeric(*m.plan, m.heat.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
+                     m.heat_pin_slots, m.heat_factor, keep_fp32, m.stream);
+    else if (keep_fp32 && !m.heat_factor.w32.ptr)
+      factor_numeric(*m.plan, m.heat.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
+                     m.heat_pin_slots, m.heat_factor, true, m.stream);
+  } else {
+    if (!m.poisson_factor.held() || (keep_fp32 && !m.poisson_factor.w32.ptr))
+      factor_numeric(*m.plan, m.poisson.ptr, m.poisson_pinned_new.ptr, m.poisson_pin_slots,
+                     m.poisson_factor, keep_fp32, m.stream);
+  }
+}
+
+// ---- solves -----------------------------------------------------------------------------------
+
+int32_t diffuse_iterated(hg_mesh& m, const double* rhs, double* x, int64_t k) {
+  int32_t total = 0;
+  for (int64_t q0 = 0; q0 < k; q0 += 64) {
+    int64_t kk = std::min<int64_t>(64, k - q0);
+    CgProblem p;
+    p.n = m.n;
+    p.n_columns = kk;
+    p.offsets = m.offsets.ptr;
+    p.columns = m.columns.ptr;
+    p.values32 = m.heat32.ptr;
+    p.values64 = m.heat.ptr;
+    p.inv_diag = m.heat_inv.ptr;
+    p.rhs = rhs + q0 * m.n;
+    p.x = x + q0 * m.n;
+    p.settle = 1;
+    p.check_rounds = kHeatCheckRounds;
+    p.change_tolerance = kHeatChangeTolerance;
+    p.settle_rounds = m.settle_rounds;
+    total = std::max(total, run_cg(p, m.cg, m.stream));
+  }
+  return total;
+}
+
+bool backward_error_fails(hg_mesh& m, const double* x, const double* b, int64_t k) {
+  if (m.invalid.size < 1) m.invalid.allocate(1);
+  HG_HIP(hipMemsetAsync(m.invalid.ptr, 0, sizeof(int32_t), m.stream));
+  launch_backward_error(m.offsets.ptr, m.columns.ptr, m.heat.ptr, x, b, m.n, k,
+                        kSettleBackwardError, m.invalid.ptr, m.stream);
+  int32_t flag = 0;
+  HG_HIP(hipMemcpyAsync(&flag, m.invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
+  HG_HIP(hipStreamSynchronize(m.stream));
+  return flag != 0;
+}
+
+int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t precision) {
+  const bool mixed = precision == HG_PRECISION_MIXED;
+  factor_system(m, HG_SYSTEM_HEAT, mixed);
+  return refined_solve(*m.plan, m.heat_factor, mixed, m.offsets.ptr, m.columns.ptr, m.heat.ptr,
+                       rhs, x, k, mixed ? kMixedHeatTolerance : kSettleDirectTolerance, true,
+                       mixed ? kMixedRefineRounds : kRefineRounds,
+                       m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
+                       m.has_heat_pins ? m.heat_dropped_old.ptr : nullptr, m.solve, m.stream);
+}
+
+// The settled iterate, verified, falling back to the factorization (solve_spd_settled,
+// linalg.py:2659-2727). Returns the flags of hg_diffuse.
+int32_t diffuse(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t mode,
+                int32_t precision) {
+  ensure_operators(m);
+  int32_t flags = 0;
+  if (mode == HG_MODE_FACTORED || m.heat_factor.held()) {
+    m.stats.heat_refinements = diffuse_factored(m, rhs, x, k, precision);
+    m.stats.heat_rounds = 0;
+    return flags;
+  }
+  m.stats.heat_rounds = diffuse_iterated(m, rhs, x, k);
+  m.stats.heat_refinements = 0;
+  if (backward_error_fails(m, x, rhs, k)) {
+    flags |= 1;
+    m.stats.backward_error_failed = 1;
+    bool was_held = m.heat_factor.held();
+    m.stats.heat_refinements = diffuse_factored(m, rhs, x, k, HG_PRECISION_FP64);
+    if (!was_held) flags |= 2;
+  }
+  return flags;
+}
+
+double poisson_bound(hg_mesh& m) {
+  if (m.poisson_bound < 0.0) {
+    int32_t blocks = static_cast<int32_t>((std::max<int64_t>(m.n, 1) + kBlock - 1) / kBlock);
+    DeviceArray<double> partials(blocks), out(1);
+    launch_dominance_bound(m.offsets.ptr, m.columns.ptr, m.poisson.ptr, m.n, partials.ptr, out.ptr,
+                           m.stream);
+    double bound = 0.0;
+    HG_HIP(hipMemcpyAsync(&bound, out.ptr, sizeof(double), hipMemcpyDeviceToHost, m.stream));
+    HG_HIP(hipStreamSynchronize(m.stream));
+    m.poisson_bound = bound;
+  }
+  return m.poisson_bound;
+}
+
+void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t mode,
+                   int32_t precision) {
+  ensure_operators(m);
+  const bool factored = mode == HG_MODE_FACTORED || m.poisson_factor.held() || m.heat_factor.held();
+  if (factored) {
+    const bool mixed = precision == HG_PRECISION_MIXED;
+    factor_system(m, HG_SYSTEM_POISSON, mixed);
+    m.stats.poisson_refinements = refined_solve(
+        *m.plan, m.poisson_factor, mixed, m.offsets.ptr, m.columns.ptr, m.poisson.ptr, rhs, x, k,
+        kPoissonTolerance, false, mixed ? kMixedRefineRounds : kRefineRounds,
+        m.poisson_pinned_new.ptr, m.poisson_dropped_old.ptr, m.solve, m.stream);
+    m.stats.poisson_rounds = 0;
+  } else {
+    // The Jacobi-Chebyshev interval (linalg.py:6074-6090): [max(min(80/n, 0.5), 1 - r), 1 + r],
+    // r the largest Gershgorin ratio of D^-1 A, at least 1e-3.
+    double r = std::max(poisson_bound(m), 1e-3);
+    double lo = std::max(std::min(kChebyshevInterval / static_cast<double>(m.n), kChebyshevCap), 1.0 - r);
+    double hi = 1.0 + r;
+    int32_t rounds = 0;
+    for (int64_t q0 = 0; q0 < k; q0 += 64) {
+      int64_t kk = std::min<int64_t>(64, k - q0);
+      CgProblem p;
+      p.n = m.n;
+      p.n_columns = kk;
+      p.offsets = m.offsets.ptr;
+      p.columns = m.columns.ptr;
+      p.values64 = m.poisson.ptr;
+      p.inv_diag = m.poisson_inv.ptr;
+      p.rhs = rhs + q0 * m.n;
+      p.x = x + q0 * m.n;
+      p.settle = 0;
+      p.tolerance = kPoissonTolerance;
+      p.chebyshev_steps = kChebyshevSteps;
+      p.lambda_min = lo;
+      p.lambda_max = hi;
+      rounds = std::max(rounds, run_cg(p, m.cg, m.stream));
+    }
+    m.stats.poisson_rounds = rounds;
+    m.stats.poisson_refinements = 0;
+  }
+  remove_null_space(m.comp_ptr.ptr, m.comp_members.ptr, m.n_comp, m.n, k, x, m.stream);
+}
+
+void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets, int64_t k) {
+  const int64_t total = offsets[k] - offsets[0];
+  std::vector<int64_t> owner(static_cast<size_t>(std::max<int64_t>(total, 1)));
+  for (int64_t q = 0; q < k; ++q)
+    for (int64_t e = offsets[q]; e < offsets[q + 1]; ++e) owner[e - offsets[0]] = q;
+  std::vector<int64_t> rel_offsets(k + 1);
+  for (int64_t q = 0; q <= k; ++q) rel_offsets[q] = offsets[q] - offsets[0];
+  m.d_owner.upload(owner, m.stream);
+  m.d_offsets.upload(rel_offsets, m.stream);
+  if (m.d_sources.size < std::max<int64_t>(total, 1)) m.d_sources.allocate(std::max<int64_t>(total, 1));
+  if (total > 0)
+    HG_HIP(hipMemcpyAsync(m.d_sources.ptr, d_sources + offsets[0], sizeof(int32_t) * total,
+                          hipMemcpyDefault, m.stream));
+}
+
+void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0) {
+  const int64_t total = offsets[k] - offsets[0];
+  HG_HIP(hipMemsetAsync(u0, 0, sizeof(double) * m.n * k, m.stream));
+  if (m.invalid.size < 1) m.invalid.allocate(1);
+  HG_HIP(hipMemsetAsync(m.invalid.ptr, 0, sizeof(int32_t), m.stream));
+  launch_seed(m.d_sources.ptr, m.d_owner.ptr, total, m.n, u0, m.invalid.ptr, m.stream);
+  int32_t flag = 0;
+  HG_HIP(hipMemcpyAsync(&flag, m.invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
+  HG_HIP(hipStreamSynchronize(m.stream));
+  HG_REQUIRE(flag == 0, HG_ERR_VALUE,
+             "every entry of sources must be a vertex index in [0, " + std::to_string(m.n) + ")");
+}
+
+void divergence(hg_mesh& m, const double* heat, double* out, int64_t k) {
+  if (m.corner_capacity < 3 * m.f * k) {
+    m.corner_terms.allocate(std::max<int64_t>(3 * m.f * k, 1));
+    m.corner_capacity = 3 * m.f * k;
+  }
+  launch_divergence(m.xyz.ptr, m.faces.ptr, m.normals.ptr, m.areas.ptr, m.cot.ptr,
+                    m.vertex_ptr.ptr, m.vertex_corners.ptr, heat, m.n, m.f, k, m.corner_terms.ptr,
+                    out, m.stream);
+}
+
+void shift(hg_mesh& m, double* phi, int64_t k) {
+  int32_t blocks = shift_partial_blocks(m.n);
+  if (m.shift_partials.size < blocks * k) m.shift_partials.allocate(blocks * k);
+  if (m.shift_values.size < 2 * k) m.shift_values.allocate(2 * k);
+  launch_shift(phi, m.d_sources.ptr, m.d_offsets.ptr, m.n, k, m.shift_partials.ptr,
+               m.shift_values.ptr, m.stream);
+}
+
+double* reserve_fields(hg_mesh& m, int64_t k) {
+  if (m.field_capacity < 3 * m.n * k) {
+    m.fields.allocate(std::max<int64_t>(3 * m.n * k, 1));
+    m.field_capacity = 3 * m.n * k;
+  }
+  return m.fields.ptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hg_last_error(void) { return g_last_error.c_str(); }
+
+const char* hg_version(void) { return "heatgeo 0.1.0 gfx950"; }
+
+int hg_mesh_create(const float* d_vertices, int64_t n_vertices, const int32_t* d_faces,
+                   int64_t n_faces, void* stream, hg_mesh** out_mesh) {
+  return guarded([&] {
+    HG_REQUIRE(out_mesh != nullptr, HG_ERR_VALUE, "out_mesh is null");
+    HG_REQUIRE(n_vertices >= 0 && n_faces >= 0, HG_ERR_VALUE, "negative size");
+    HG_REQUIRE(n_vertices < (1ll << 31) && 3 * n_faces < (1ll << 31), HG_ERR_VALUE,
+               "mesh too large for int32 indices");
+    auto m = std::make_unique<hg_mesh>();
+    m->stream = static_cast<hipStream_t>(stream);
+    m->n = n_vertices;
+    m->f = n_faces;
+    m->h_xyz.resize(3 * n_vertices);
+    m->h_faces.resize(3 * n_faces);
+    if (n_vertices > 0)
+      HG_HIP(hipMemcpy(m->h_xyz.data(), d_vertices, sizeof(float) * 3 * n_vertices, hipMemcpyDefault));
+    if (n_faces > 0)
+      HG_HIP(hipMemcpy(m->h_faces.data(), d_faces, sizeof(int32_t) * 3 * n_faces, hipMemcpyDefault));
+    m->pattern = build_pattern(n_vertices, m->h_faces.data(), n_faces);
+    m->labels = component_labels(m->pattern, &m->n_comp);
+    std::vector<int32_t> comp_ptr(m->n_comp + 1, 0), members(n_vertices);
+    for (int64_t v = 0; v < n_vertices; ++v) comp_ptr[m->labels[v] + 1]++;
+    for (int32_t c = 0; c < m->n_comp; ++c) comp_ptr[c + 1] += comp_ptr[c];
+    {
+      std::vector<int32_t> at(comp_ptr.begin(), comp_ptr.end() - 1);
+      for (int64_t v = 0; v < n_vertices; ++v) members[at[m->labels[v]]++] = static_cast<int32_t>(v);
+    }
+    hipStream_t st = m->stream;
+    m->xyz.upload(m->h_xyz, st);
+    m->faces.upload(m->h_faces, st);
+    m->offsets.upload(m->pattern.offsets, st);
+    m->columns.upload(m->pattern.columns, st);
+    m->corner_ptr.upload(m->pattern.corner_ptr, st);
+    m->corner_slots.upload(m->pattern.corner_slots, st);
+    m->vertex_ptr.upload(m->pattern.vertex_ptr, st);
+    m->vertex_corners.upload(m->pattern.vertex_corners, st);
+    m->comp_ptr.upload(comp_ptr, st);
+    m->comp_members.upload(members, st);
+    HG_HIP(hipStreamSynchronize(st));
+    *out_mesh = m.release();
+  });
+}
+
+int hg_mesh_destroy(hg_mesh* mesh) {
+  return guarded([&] {
+    if (mesh) {
+      (void)hipStreamSynchronize(mesh->stream);
+      delete mesh;
+    }
+  });
+}
+
+int hg_mesh_set_stream(hg_mesh* mesh, void* stream) {
+  return guarded([&] {
+    require_mesh(mesh);
+    mesh->stream = static_cast<hipStream_t>(stream);
+  });
+}
+
+int hg_operators(hg_mesh* mesh, double t, int32_t method, int32_t solver_default_time) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(method == HG_METHOD_MESHLAB || method == HG_METHOD_POTPOURRI3D, HG_ERR_VALUE,
+               "method must be 'meshlab' or 'potpourri3d'");
+    HG_REQUIRE(std::isnan(t) || t > 0.0, HG_ERR_VALUE, "t must be positive");
+    assemble(*mesh, t, method, solver_default_time != 0);
+  });
+}
+
+int hg_operator_views(hg_mesh* mesh, hg_operator_view* out) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(out != nullptr, HG_ERR_VALUE, "out_view is null");
+    ensure_operators(*mesh);
+    out->n_vertices = mesh->n;
+    out->n_faces = mesh->f;
+    out->nnz = static_cast<int64_t>(mesh->pattern.columns.size());
+    out->offsets = mesh->offsets.ptr;
+    out->columns = mesh->columns.ptr;
+    out->laplacian = mesh->lap.ptr;
+    out->heat = mesh->heat.ptr;
+    out->poisson = mesh->poisson.ptr;
+    out->mass = mesh->mass.ptr;
+    out->cot = mesh->cot.ptr;
+    out->normals = mesh->normals.ptr;
+    out->areas = mesh->areas.ptr;
+    out->t = mesh->t;
+    out->settle_rounds = mesh->settle_rounds;
+    out->time_from_edges = mesh->time_from_edges;
+  });
+}
+
+int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_factored) {
+  return guarded([&] {
+    require_mesh(mesh);
+    const bool mixed = precision == HG_PRECISION_MIXED;
+    if (systems & HG_SYSTEM_HEAT) factor_system(*mesh, HG_SYSTEM_HEAT, mixed);
+    if (systems & HG_SYSTEM_POISSON) factor_system(*mesh, HG_SYSTEM_POISSON, mixed);
+    if (out_factored) {
+      bool ok = (!(systems & HG_SYSTEM_HEAT) || mesh->heat_factor.held()) &&
+                (!(systems & HG_SYSTEM_POISSON) || mesh->poisson_factor.held());
+      *out_factored = ok ? 1 : 0;
+    }
+  });
+}
+
+int hg_factored(hg_mesh* mesh, int32_t* out_systems) {
+  return guarded([&] {
+    require_mesh(mesh);
+    *out_systems = (mesh->heat_factor.held() ? 1 : 0) | (mesh->poisson_factor.held() ? 2 : 0);
+  });
+}
+
+int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
+               int32_t mode, int32_t precision, int32_t* out_flags) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
+    mesh->stats = hg_solve_stats{};
+    int32_t flags = diffuse(*mesh, d_rhs, d_solution, n_columns, mode, precision);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+    if (out_flags) *out_flags = flags;
+  });
+}
+
+int hg_solve_poisson(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
+                     int32_t mode, int32_t precision) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
+    solve_poisson(*mesh, d_rhs, d_solution, n_columns, mode, precision);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+  });
+}
+
+int hg_unit_gradient_divergence(hg_mesh* mesh, const double* d_heat, double* d_neg_divergence,
+                                int64_t n_columns) {
+  return guarded([&] {
+    require_mesh(mesh);
+    ensure_operators(*mesh);
+    divergence(*mesh, d_heat, d_neg_divergence, n_columns);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+  });
+}
+
+int hg_seed_sources(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_offsets,
+                    int64_t n_columns, double* d_u0) {
+  return guarded([&] {
+    require_mesh(mesh);
+    upload_sources(*mesh, d_sources, source_offsets, n_columns);
+    seed(*mesh, source_offsets, n_columns, d_u0);
+  });
+}
+
+int hg_shift_to_sources(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
+                        const int64_t* source_offsets, int64_t n_columns) {
+  return guarded([&] {
+    require_mesh(mesh);
+    upload_sources(*mesh, d_sources, source_offsets, n_columns);
+    shift(*mesh, d_phi, n_columns);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+  });
+}
+
+int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_offsets,
+                int64_t n_queries, double* d_out, int32_t mode, int32_t precision,
+                hg_solve_stats* out_stats) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(n_queries >= 0, HG_ERR_VALUE, "negative query count");
+    HG_REQUIRE(source_offsets != nullptr, HG_ERR_VALUE, "source_offsets is null");
+    hg_mesh& m = *mesh;
+    m.stats = hg_solve_stats{};
+    const int64_t n = m.n, k = n_queries;
+    if (k == 0) return;
+    if (n == 0 || m.f == 0) {
+      if (n > 0) HG_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * n * k, m.stream));
+      HG_HIP(hipStreamSynchronize(m.stream));
+      return;
+    }
+    ensure_operators(m);
+    upload_sources(m, d_sources, source_offsets, k);
+    double* fields = reserve_fields(m, k);
+    double* u0 = fields;
+    double* heat = fields + n * k;
+    double* negdiv = fields + 2 * n * k;
+    seed(m, source_offsets, k, u0);
+    int32_t flags = diffuse(m, u0, heat, k, mode, precision);
+    m.stats.backward_error_failed = flags & 1;
+    divergence(m, heat, negdiv, k);
+    solve_poisson(m, negdiv, d_out, k, mode, precision);
+    shift(m, d_out, k);
+    HG_HIP(hipStreamSynchronize(m.stream));
+    if (out_stats) *out_stats = m.stats;
+  });
+}
+
+int hg_nbytes(hg_mesh* mesh, int64_t* out_bytes) {
+  return guarded([&] {
+    require_mesh(mesh);
+    int64_t bytes = mesh->heat_factor.bytes() + mesh->poisson_factor.bytes();
+    if (bytes > 0 && mesh->plan) bytes += mesh->plan->bytes();
+    *out_bytes = bytes;
+  });
+}
+
+int hg_release(hg_mesh* mesh) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+    mesh->heat_factor.reset();
+    mesh->poisson_factor.reset();
+    mesh->plan.reset();
+  });
+}
+
+int hg_plan_stats(hg_mesh* mesh, int64_t* out) {
+  return guarded([&] {
+    require_mesh(mesh);
+    ensure_operators(*mesh);
+    ensure_plan(*mesh);
+    const DevicePlan& p = *mesh->plan;
+    out[0] = p.n_super;
+    out[1] = p.n_levels;
+    out[2] = p.nnz_w;
+    out[3] = static_cast<int64_t>(p.flops);
+    out[4] = p.arena_size;
+    out[5] = p.max_front;
+  });
+}
+
+struct hg_plan {
+  Plan plan;
+};
+
+int hg_plan_create(const int32_t* offsets, const int32_t* columns, int64_t n, const float* xyz,
+                   int32_t leaf_rows, int32_t top_rows, hg_plan** out_plan) {
+  return guarded([&] {
+    HG_REQUIRE(out_plan != nullptr && n >= 0, HG_ERR_VALUE, "bad arguments");
+    HostPattern pattern;
+    pattern.n = n;
+    pattern.offsets.assign(offsets, offsets + n + 1);
+    pattern.columns.assign(columns, columns + offsets[n]);
+    AnalysisOptions options;
+    if (leaf_rows > 0) options.leaf_rows = leaf_rows;
+    if (top_rows > 0) options.top_rows = top_rows;
+    auto p = std::make_unique<hg_plan>();
+    p->plan = analyse(pattern, xyz, options);
+    *out_plan = p.release();
+  });
+}
+
+int hg_plan_sizes(hg_plan* plan, int64_t* out) {
+  return guarded([&] {
+    HG_REQUIRE(plan != nullptr, HG_ERR_VALUE, "null plan");
+    const Plan& p = plan->plan;
+    out[0] = p.n_super;
+    out[1] = p.n_levels;
+    out[2] = static_cast<int64_t>(p.rows.size());
+    out[3] = static_cast<int64_t>(p.children.size());
+    out[4] = static_cast<int64_t>(p.entry_src.size());
+    out[5] = p.arena_size;
+    out[6] = p.w_size;
+    out[7] = p.vec_size;
+  });
+}
+
+int hg_plan_copy(hg_plan* plan, int32_t field, void* out) {
+  return guarded([&] {
+    HG_REQUIRE(plan != nullptr && out != nullptr, HG_ERR_VALUE, "null argument");
+    const Plan& p = plan->plan;
+    auto put32 = [&](const std::vector<int32_t>& v) {
+      if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(int32_t));
+    };
+    auto put64 = [&](const std::vector<int64_t>& v) {
+      if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(int64_t));
+    };
+    switch (field) {
+      case 0: put32(p.perm); break;
+      case 1: put32(p.col_begin); break;
+      case 2: put32(p.ncol); break;
+      case 3: put32(p.nrow); break;
+      case 4: put32(p.parent); break;
+      case 5: put32(p.level); break;
+      case 6: put32(p.row_ptr); break;
+      case 7: put32(p.rows); break;
+      case 8: put32(p.rel); break;
+      case 9: put32(p.child_ptr); break;
+      case 10: put32(p.children); break;
+      case 11: put32(p.level_ptr); break;
+      case 12: put32(p.level_nodes); break;
+      case 13: put64(p.front_off); break;
+      case 14: put64(p.w_off); break;
+      case 15: put64(p.vec_off); break;
+      case 16: put64(p.entry_dst); break;
+      case 17: put32(p.entry_src); break;
+      case 18: put32(p.entry_row); break;
+      case 19: put32(p.entry_col); break;
+      default: throw Error{HG_ERR_VALUE, "unknown plan field"};
+    }
+  });
+}
+
+int hg_plan_destroy(hg_plan* plan) {
+  return guarded([&] { delete plan; });
+}
+
+}  // extern "C"

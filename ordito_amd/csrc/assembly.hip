@@ -1,0 +1,438 @@
+// Assembly of the heat method's operators and the per-face / per-vertex passes of a query:
+// face terms, lumped mass, the time step's edge sums, the fused row kernel (L, M - tL, -L), the
+// source seeding, the unit-gradient divergence and the source shift.
+//
+// Atomic-free: every face -> vertex sum is a gather over the vertex's corners in ascending corner
+// order, and every reduction a fixed tree, so a call is bit-reproducible. Multiply-add
+// contraction is off in this file: the float32 face terms and the float64 sums are formed one
+// rounding per operation, exactly the arithmetic the reference's kernels spell.
+#pragma clang fp contract(off)
+
+#include "kernels.h"
+
+namespace hg {
+namespace {
+
+struct F3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ F3 load3(const float* p, int64_t v) {
+  return F3{p[3 * v], p[3 * v + 1], p[3 * v + 2]};
+}
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 neg(F3 a) { return F3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 u, F3 v) {
+  return F3{u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 widen(F3 a) { return D3{a.x, a.y, a.z}; }
+__device__ __forceinline__ D3 sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 add(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 scale(double s, D3 a) { return D3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(D3 u, D3 v) {
+  return D3{u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+
+// cross(b - a, c - a) formed at the corner opposite the longest edge (the largest angle), ties to
+// corner a then b: ordito kernels/predicates.py:106-130.
+__device__ __forceinline__ F3 area_vector(F3 a, F3 b, F3 c) {
+  F3 ab = sub(b, a), bc = sub(c, b), ca = sub(a, c);
+  float l_ab = dot(ab, ab), l_bc = dot(bc, bc), l_ca = dot(ca, ca);
+  if (l_bc >= l_ca && l_bc >= l_ab) return cross(ab, neg(ca));
+  if (l_ca >= l_ab) return cross(bc, neg(ab));
+  return cross(ca, neg(bc));
+}
+
+}  // namespace
+
+// Per face: float32 half-cotangents (dot / (2 |cross|), kernels/laplacian.py:93-106), unit normal
+// and area (kernels/triangles.py:332-358). One thread a face, 48 bytes read, 28 written.
+__global__ void face_terms_kernel(const float* __restrict__ xyz, const int32_t* __restrict__ faces,
+                                  int64_t n_faces, float* __restrict__ cot,
+                                  float* __restrict__ normals, float* __restrict__ areas) {
+  int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  F3 v0 = load3(xyz, faces[3 * f]), v1 = load3(xyz, faces[3 * f + 1]),
+     v2 = load3(xyz, faces[3 * f + 2]);
+  F3 c = area_vector(v0, v1, v2);
+  float norm = sqrtf(dot(c, c));
+  F3 n = c;
+  if (norm > 0.0f) n = F3{c.x / norm, c.y / norm, c.z / norm};
+  normals[3 * f] = n.x;
+  normals[3 * f + 1] = n.y;
+  normals[3 * f + 2] = n.z;
+  areas[f] = 0.5f * norm;
+  float denominator = 2.0f * norm;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  if (denominator > 0.0f) {
+    c0 = dot(sub(v1, v0), sub(v2, v0)) / denominator;
+    c1 = dot(sub(v2, v1), sub(v0, v1)) / denominator;
+    c2 = dot(sub(v0, v2), sub(v1, v2)) / denominator;
+  }
+  cot[3 * f] = c0;
+  cot[3 * f + 1] = c1;
+  cot[3 * f + 2] = c2;
+}
+
+// m_i = (sum of the float64-widened areas of the faces at i, in ascending corner order) / 3:
+// kernels/heat.py:125-131 then 423-425.
+__global__ void mass_kernel(const int32_t* __restrict__ vertex_ptr,
+                            const int32_t* __restrict__ vertex_corners,
+                            const float* __restrict__ areas, int64_t n, double* __restrict__ mass) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sum = 0.0;
+  for (int32_t k = vertex_ptr[i]; k < vertex_ptr[i + 1]; ++k)
+    sum += static_cast<double>(areas[vertex_corners[k] / 3]);
+  mass[i] = sum / 3.0;
+}
+
+// First stage of (sum of w |v_j - v_i|, sum of w) over the strict upper triangle, w the faces on
+// the edge (MeshLab's face-edge mean, kernels/heat.py:143-235): one thread a row, a fixed block
+// tree, one partial pair a block.
+__global__ void edge_sums_kernel(const int32_t* __restrict__ offsets,
+                                 const int32_t* __restrict__ columns,
+                                 const int32_t* __restrict__ corner_ptr,
+                                 const float* __restrict__ xyz, int64_t n, int32_t face_edges,
+                                 double* __restrict__ partials) {
+  __shared__ double s_len[kBlock];
+  __shared__ double s_cnt[kBlock];
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double len = 0.0, cnt = 0.0;
+  if (i < n) {
+    F3 vi = load3(xyz, i);
+    for (int32_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+      int32_t j = columns[k];
+      if (j <= i) continue;
+      F3 d = sub(load3(xyz, j), vi);
+      double w = face_edges ? static_cast<double>(corner_ptr[k + 1] - corner_ptr[k]) : 1.0;
+      len += w * static_cast<double>(sqrtf(dot(d, d)));
+      cnt += w;
+    }
+  }
+  s_len[threadIdx.x] = len;
+  s_cnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      s_len[threadIdx.x] += s_len[threadIdx.x + stride];
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + stride];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = s_len[0];
+    partials[2 * blockIdx.x + 1] = s_cnt[0];
+  }
+}
+
+// Fold the block partials in block order, and form the heat system's factor: -t, or -h^2 from the
+// sums (kernels/linalg.py:1593-1609). scalars = [sum, count, factor, t].
+__global__ void edge_sums_final_kernel(const double* __restrict__ partials, int32_t blocks,
+                                       int32_t from_edges, double t, double* __restrict__ scalars) {
+  __shared__ double s_len[kBlock];
+  __shared__ double s_cnt[kBlock];
+  double len = 0.0, cnt = 0.0;
+  for (int32_t b = threadIdx.x; b < blocks; b += kBlock) {
+    len += partials[2 * b];
+    cnt += partials[2 * b + 1];
+  }
+  s_len[threadIdx.x] = len;
+  s_cnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      s_len[threadIdx.x] += s_len[threadIdx.x + stride];
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + stride];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    scalars[0] = s_len[0];
+    scalars[1] = s_cnt[0];
+    if (from_edges) {
+      double h = s_cnt[0] > 0.0 ? s_len[0] / s_cnt[0] : 0.0;
+      scalars[2] = -1.0 * (h * h);
+      scalars[3] = h * h;
+    } else {
+      scalars[2] = -t;
+      scalars[3] = t;
+    }
+  }
+}
+
+// The three operators over one pattern from one walk of each row (kernels/heat.py:298-465):
+// L_ij = sum of the entry's float64-widened half-cotangents in ascending corner order
+// (kernels/laplacian.py:970-989); L_ii = -(sum of the row's off-diagonal entries in column order);
+// heat = factor L (+ m_i on the diagonal, kernels/linalg.py:1612-1621); Poisson = -L. Also the
+// heat values narrowed to float32 (the settled CG's stream, linalg.py:2516-2521) and both
+// systems' inverse diagonals (Jacobi). One thread a row.
+__global__ void operator_rows_kernel(
+    const int32_t* __restrict__ offsets, const int32_t* __restrict__ columns,
+    const int32_t* __restrict__ corner_ptr, const int32_t* __restrict__ corner_slots,
+    const float* __restrict__ cot, const double* __restrict__ mass,
+    const double* __restrict__ scalars, int64_t n, double* __restrict__ lap,
+    double* __restrict__ heat, double* __restrict__ poisson, float* __restrict__ heat32,
+    double* __restrict__ heat_inv_diag, double* __restrict__ poisson_inv_diag) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double factor = scalars[2];
+  double total = 0.0;
+  int32_t slot = -1;
+  for (int32_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+    if (columns[k] == i) {
+      slot = k;
+      continue;
+    }
+    double value = 0.0;
+    for (int32_t p = corner_ptr[k]; p < corner_ptr[k + 1]; ++p)
+      value += static_cast<double>(cot[corner_slots[p]]);
+    lap[k] = value;
+    total += value;
+    double h = factor * value;
+    heat[k] = h;
+    heat32[k] = static_cast<float>(h);
+    poisson[k] = -value;
+  }
+  double hd = 0.0, pd = 0.0;
+  if (slot >= 0) {
+    double diagonal = -total;
+    lap[slot] = diagonal;
+    hd = factor * diagonal + mass[i];
+    heat[slot] = hd;
+    heat32[slot] = static_cast<float>(hd);
+    pd = -diagonal;
+    poisson[slot] = pd;
+  }
+  heat_inv_diag[i] = hd != 0.0 ? 1.0 / hd : 0.0;
+  poisson_inv_diag[i] = pd != 0.0 ? 1.0 / pd : 0.0;
+}
+
+// u0[q, s] = 1 at each source of column q (duplicates allowed); an index outside [0, n) writes
+// nothing and raises the flag (kernels/heat.py:599-623). One thread a source entry.
+__global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* __restrict__ owner,
+                            int64_t n_entries, int64_t n, double* __restrict__ u0,
+                            int32_t* __restrict__ invalid) {
+  int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_entries) return;
+  int32_t v = sources[e];
+  if (v < 0 || v >= n) {
+    atomicOr(invalid, 1);
+    return;
+  }
+  u0[owner[e] * n + v] = 1.0;
+}
+
+// Per face and column: the unit gradient direction x of the heat (sum_k u_k (N x e_k) / (2A) in
+// float64 from float32 N, A, positions; scaled by its largest component before normalizing,
+// kernels/triangles.py:834-894, kernels/predicates.py:718-724), integrated onto the three corners
+// as c_opp (e . x) (kernels/heat.py:626-660). Terms stored per corner; the vertex sums follow.
+__global__ void divergence_corners_kernel(const float* __restrict__ xyz,
+                                          const int32_t* __restrict__ faces,
+                                          const float* __restrict__ normals,
+                                          const float* __restrict__ areas,
+                                          const float* __restrict__ cot,
+                                          const double* __restrict__ heat, int64_t n,
+                                          int64_t n_faces, int64_t n_columns,
+                                          double* __restrict__ corner_terms) {
+  int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t q = blockIdx.y;
+  if (f >= n_faces || q >= n_columns) return;
+  int32_t i0 = faces[3 * f], i1 = faces[3 * f + 1], i2 = faces[3 * f + 2];
+  D3 v0 = widen(load3(xyz, i0)), v1 = widen(load3(xyz, i1)), v2 = widen(load3(xyz, i2));
+  D3 nrm = widen(load3(normals, f));
+  double area = static_cast<double>(areas[f]);
+  const double* u = heat + q * n;
+  D3 grad{0.0, 0.0, 0.0};
+  if (area > 0.0) {
+    D3 sum = add(add(scale(u[i0], cross(nrm, sub(v2, v1))), scale(u[i1], cross(nrm, sub(v0, v2)))),
+                 scale(u[i2], cross(nrm, sub(v1, v0))));
+    double twice = 2.0 * area;
+    grad = D3{sum.x / twice, sum.y / twice, sum.z / twice};
+  }
+  D3 x{0.0, 0.0, 0.0};
+  double largest = fmax(fmax(fabs(grad.x), fabs(grad.y)), fabs(grad.z));
+  if (largest != 0.0) {
+    D3 w{grad.x / largest, grad.y / largest, grad.z / largest};
+    double length = sqrt(dot(w, w));
+    x = D3{w.x / length, w.y / length, w.z / length};
+  }
+  double c0 = cot[3 * f], c1 = cot[3 * f + 1], c2 = cot[3 * f + 2];
+  double* out = corner_terms + q * 3 * n_faces + 3 * f;
+  out[0] = c2 * dot(sub(v1, v0), x) + c1 * dot(sub(v2, v0), x);
+  out[1] = c0 * dot(sub(v2, v1), x) + c2 * dot(sub(v0, v1), x);
+  out[2] = c1 * dot(sub(v0, v2), x) + c0 * dot(sub(v1, v2), x);
+}
+
+// Each vertex's corner terms summed in ascending corner order (kernels/scatter.py's
+// sum_sorted_corner_values, heat.py:3218-3232).
+__global__ void corner_sums_kernel(const int32_t* __restrict__ vertex_ptr,
+                                   const int32_t* __restrict__ vertex_corners,
+                                   const double* __restrict__ corner_terms, int64_t n,
+                                   int64_t n_faces, int64_t n_columns, double* __restrict__ out) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t q = blockIdx.y;
+  if (i >= n || q >= n_columns) return;
+  const double* terms = corner_terms + q * 3 * n_faces;
+  double sum = 0.0;
+  for (int32_t k = vertex_ptr[i]; k < vertex_ptr[i + 1]; ++k) sum += terms[vertex_corners[k]];
+  out[q * n + i] = sum;
+}
+
+// First stage of each column's (sum over its sources, sum over every vertex) of phi
+// (kernels/heat.py:3383-3404): block partials of the global sum; the source sum is folded by the
+// final stage, sources being few.
+__global__ void column_sum_kernel(const double* __restrict__ phi, int64_t n, int64_t n_columns,
+                                  double* __restrict__ partials) {
+  __shared__ double s[kBlock];
+  int64_t q = blockIdx.y;
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  s[threadIdx.x] = (i < n) ? phi[q * n + i] : 0.0;
+  __syncthreads();
+  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) s[threadIdx.x] += s[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[q * gridDim.x + blockIdx.x] = s[0];
+}
+
+// One block a column: fold its partials in block order and its sources in order, and decide the
+// shift: offset = mean over the sources, orient when the shifted mean is negative
+// (kernels/heat.py:3407-3424). shift[2q] = offset, shift[2q+1] = 1 to negate.
+__global__ void shift_decide_kernel(const double* __restrict__ phi,
+                                    const double* __restrict__ partials, int32_t blocks,
+                                    const int32_t* __restrict__ sources,
+                                    const int64_t* __restrict__ source_offsets, int64_t n,
+                                    double* __restrict__ shift) {
+  __shared__ double s[kBlock];
+  __shared__ double s_src[kBlock];
+  int64_t q = blockIdx.x;
+  double total = 0.0;
+  for (int32_t b = threadIdx.x; b < blocks; b += kBlock) total += partials[q * blocks + b];
+  int64_t first = source_offsets[q], last = source_offsets[q + 1];
+  double src = 0.0;
+  for (int64_t e = first + threadIdx.x; e < last; e += kBlock) src += phi[q * n + sources[e]];
+  s[threadIdx.x] = total;
+  s_src[threadIdx.x] = src;
+  __syncthreads();
+  for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      s[threadIdx.x] += s[threadIdx.x + stride];
+      s_src[threadIdx.x] += s_src[threadIdx.x + stride];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int64_t count = last - first;
+    double offset = count > 0 ? s_src[0] / static_cast<double>(count) : 0.0;
+    shift[2 * q] = offset;
+    shift[2 * q + 1] = (count > 0 && s[0] / static_cast<double>(n) - offset < 0.0) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void shift_apply_kernel(double* __restrict__ phi, const double* __restrict__ shift,
+                                   int64_t n, int64_t n_columns) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t q = blockIdx.y;
+  if (i >= n || q >= n_columns) return;
+  double offset = shift[2 * q];
+  double value = phi[q * n + i];
+  phi[q * n + i] = shift[2 * q + 1] != 0.0 ? offset - value : value - offset;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------------------
+
+static inline unsigned blocks_for(int64_t n) {
+  return static_cast<unsigned>((n + kBlock - 1) / kBlock);
+}
+
+void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, float* cot,
+                       float* normals, float* areas, hipStream_t stream) {
+  if (n_faces == 0) return;
+  face_terms_kernel<<<blocks_for(n_faces), kBlock, 0, stream>>>(xyz, faces, n_faces, cot, normals,
+                                                                 areas);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_mass(const int32_t* vertex_ptr, const int32_t* vertex_corners, const float* areas,
+                 int64_t n, double* mass, hipStream_t stream) {
+  if (n == 0) return;
+  mass_kernel<<<blocks_for(n), kBlock, 0, stream>>>(vertex_ptr, vertex_corners, areas, n, mass);
+  HG_HIP(hipGetLastError());
+}
+
+int32_t edge_sum_blocks(int64_t n) { return static_cast<int32_t>(blocks_for(std::max<int64_t>(n, 1))); }
+
+void launch_edge_sums(const int32_t* offsets, const int32_t* columns, const int32_t* corner_ptr,
+                      const float* xyz, int64_t n, int32_t face_edges, double* partials,
+                      int32_t from_edges, double t, double* scalars, hipStream_t stream) {
+  int32_t blocks = edge_sum_blocks(n);
+  if (n > 0)
+    edge_sums_kernel<<<blocks, kBlock, 0, stream>>>(offsets, columns, corner_ptr, xyz, n,
+                                                    face_edges, partials);
+  else
+    HG_HIP(hipMemsetAsync(partials, 0, sizeof(double) * 2 * blocks, stream));
+  edge_sums_final_kernel<<<1, kBlock, 0, stream>>>(partials, blocks, from_edges, t, scalars);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
+                          const int32_t* corner_ptr, const int32_t* corner_slots, const float* cot,
+                          const double* mass, const double* scalars, int64_t n, double* lap,
+                          double* heat, double* poisson, float* heat32, double* heat_inv_diag,
+                          double* poisson_inv_diag, hipStream_t stream) {
+  if (n == 0) return;
+  operator_rows_kernel<<<blocks_for(n), kBlock, 0, stream>>>(
+      offsets, columns, corner_ptr, corner_slots, cot, mass, scalars, n, lap, heat, poisson,
+      heat32, heat_inv_diag, poisson_inv_diag);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
+                 double* u0, int32_t* invalid, hipStream_t stream) {
+  if (n_entries == 0) return;
+  seed_kernel<<<blocks_for(n_entries), kBlock, 0, stream>>>(sources, owner, n_entries, n, u0,
+                                                            invalid);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_divergence(const float* xyz, const int32_t* faces, const float* normals,
+                       const float* areas, const float* cot, const int32_t* vertex_ptr,
+                       const int32_t* vertex_corners, const double* heat, int64_t n,
+                       int64_t n_faces, int64_t n_columns, double* corner_terms, double* out,
+                       hipStream_t stream) {
+  if (n == 0 || n_columns == 0) return;
+  if (n_faces > 0) {
+    dim3 grid(blocks_for(n_faces), static_cast<unsigned>(n_columns));
+    divergence_corners_kernel<<<grid, kBlock, 0, stream>>>(xyz, faces, normals, areas, cot, heat,
+                                                           n, n_faces, n_columns, corner_terms);
+  }
+  dim3 grid_v(blocks_for(n), static_cast<unsigned>(n_columns));
+  corner_sums_kernel<<<grid_v, kBlock, 0, stream>>>(vertex_ptr, vertex_corners, corner_terms, n,
+                                                    n_faces, n_columns, out);
+  HG_HIP(hipGetLastError());
+}
+
+int32_t shift_partial_blocks(int64_t n) { return static_cast<int32_t>(blocks_for(n)); }
+
+void launch_shift(double* phi, const int32_t* sources, const int64_t* source_offsets, int64_t n,
+                  int64_t n_columns, double* partials, double* shift, hipStream_t stream) {
+  if (n == 0 || n_columns == 0) return;
+  int32_t blocks = shift_partial_blocks(n);
+  dim3 grid(blocks, static_cast<unsigned>(n_columns));
+  column_sum_kernel<<<grid, kBlock, 0, stream>>>(phi, n, n_columns, partials);
+  shift_decide_kernel<<<static_cast<unsigned>(n_columns), kBlock, 0, stream>>>(
+      phi, partials, blocks, sources, source_offsets, n, shift);
+  shift_apply_kernel<<<grid, kBlock, 0, stream>>>(phi, shift, n, n_columns);
+  HG_HIP(hipGetLastError());
+}
+
+}  // namespace hg

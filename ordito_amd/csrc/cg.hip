@@ -1,0 +1,473 @@
+// Persistent conjugate-gradient solves: the settled Jacobi-CG of the heat step and the
+// Jacobi-Chebyshev preconditioned CG of the Poisson step, each one cooperative launch.
+//
+// The iteration is ordito's Chronopoulos-Gear form (kernels/algorithms/conjugate_gradient.py:
+// 381-866): a round is w = A u with the three dots r.u, w.u, r.r in one pass, then
+// beta = gamma/gamma_old, alpha = gamma / (delta - beta gamma / alpha_old) and the update
+// p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r. Where ordito records
+// two launches a round into a conditional CUDA graph, this runs the whole loop -- rounds, the
+// settle monitor (conjugate_gradient.py:1491-1580) and the stop decisions -- inside one kernel
+// with grid-wide barriers, so nothing returns to the host until the solve ends. Every block folds
+// the per-block partials in the same fixed order, so all blocks agree on alpha, beta and the stop
+// decision bit for bit (the redundant fold ordito uses instead of a barrier, here beside one).
+#include <hip/hip_cooperative_groups.h>
+
+#include "kernels.h"
+
+namespace hg {
+namespace {
+
+constexpr int kMaxColumns = 64;
+
+struct CgArgs {
+  int64_t n;
+  int32_t k;
+  const int32_t* offsets;
+  const int32_t* columns;
+  const float* values32;
+  const double* values64;
+  const double* inv_diag;
+  const double* rhs;
+  double* x;
+  double* r;
+  double* u;
+  double* p;
+  double* s;
+  double* w;
+  double* prev;
+  double* res;
+  double* da;
+  double* db;
+  double* partials;  // [k][G][3], then [G][2] for the settle check, then [k][G] for |b|^2
+  int32_t* counters;  // [0] stepped rounds
+  int32_t settle;
+  int32_t check_rounds;
+  double change_tolerance;
+  int32_t settle_rounds;
+  double tol_sq;
+  int64_t maxiter;
+  int32_t cheb_steps;
+  double theta;
+  double delta;
+};
+
+template <int kThreads, bool kSingle>
+__device__ __forceinline__ void barrier() {
+  if constexpr (kSingle) {
+    __syncthreads();
+  } else {
+    cooperative_groups::this_grid().sync();
+  }
+}
+
+// Fixed-tree block sum of up to three values; every thread gets the result.
+template <int kThreads, int kWidth>
+__device__ __forceinline__ void block_sum(double (&v)[kWidth], double* scratch) {
+  for (int c = 0; c < kWidth; ++c) scratch[c * kThreads + threadIdx.x] = v[c];
+  __syncthreads();
+  for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride)
+      for (int c = 0; c < kWidth; ++c)
+        scratch[c * kThreads + threadIdx.x] += scratch[c * kThreads + threadIdx.x + stride];
+    __syncthreads();
+  }
+  for (int c = 0; c < kWidth; ++c) v[c] = scratch[c * kThreads];
+  __syncthreads();
+}
+
+template <int kThreads>
+__device__ __forceinline__ double block_max(double v, double* scratch) {
+  scratch[threadIdx.x] = v;
+  __syncthreads();
+  for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride)
+      scratch[threadIdx.x] = fmax(scratch[threadIdx.x], scratch[threadIdx.x + stride]);
+    __syncthreads();
+  }
+  double out = scratch[0];
+  __syncthreads();
+  return out;
+}
+
+__device__ __forceinline__ double row_dot(const CgArgs& a, int64_t i, const double* vec) {
+  double sum = 0.0;
+  const int32_t begin = a.offsets[i], end = a.offsets[i + 1];
+  if (a.values32) {
+    for (int32_t e = begin; e < end; ++e)
+      sum += static_cast<double>(a.values32[e]) * vec[a.columns[e]];
+  } else {
+    for (int32_t e = begin; e < end; ++e) sum += a.values64[e] * vec[a.columns[e]];
+  }
+  return sum;
+}
+
+// u = p(D^-1 A) D^-1 r for every column: the Chebyshev semi-iteration on D^-1 A z = D^-1 r from
+// z = 0 (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed., Algorithm 12.1), cheb_steps
+// steps, cheb_steps - 1 products; the search direction is double-buffered so each step is one
+// barrier. Leaves every thread past a barrier.
+template <int kThreads, bool kSingle>
+__device__ void chebyshev_apply(const CgArgs& a) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  const int64_t first = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const double sigma = a.theta / a.delta;
+  double rho = 1.0 / sigma;
+  for (int32_t q = 0; q < a.k; ++q) {
+    const int64_t o = static_cast<int64_t>(q) * a.n;
+    for (int64_t i = first; i < a.n; i += stride) {
+      double c = a.inv_diag[i] * a.r[o + i];
+      double d = c / a.theta;
+      a.res[o + i] = c;
+      a.da[o + i] = d;
+      a.u[o + i] = d;
+    }
+  }
+  barrier<kThreads, kSingle>();
+  double* d_old = a.da;
+  double* d_new = a.db;
+  for (int32_t step = 1; step < a.cheb_steps; ++step) {
+    const double rho_new = 1.0 / (2.0 * sigma - rho);
+    const double keep = rho_new * rho;
+    const double push = 2.0 * rho_new / a.delta;
+    for (int32_t q = 0; q < a.k; ++q) {
+      const int64_t o = static_cast<int64_t>(q) * a.n;
+      for (int64_t i = first; i < a.n; i += stride) {
+        double ad = row_dot(a, i, d_old + o);
+        double res = a.res[o + i] - a.inv_diag[i] * ad;
+        a.res[o + i] = res;
+        double d = keep * d_old[o + i] + push * res;
+        d_new[o + i] = d;
+        a.u[o + i] += d;
+      }
+    }
+    rho = rho_new;
+    barrier<kThreads, kSingle>();
+    double* t = d_old;
+    d_old = d_new;
+    d_new = t;
+  }
+}
+
+template <int kThreads, bool kSingle>
+__global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
+  __shared__ double scratch[3 * kThreads];
+  __shared__ double gamma_old[kMaxColumns];
+  __shared__ double alpha_old[kMaxColumns];
+  __shared__ double atol_sq[kMaxColumns];
+  __shared__ double coeff[2 * kMaxColumns];
+  __shared__ int stepping[kMaxColumns];
+
+  const int32_t G = gridDim.x;
+  const int64_t stride = static_cast<int64_t>(G) * kThreads;
+  const int64_t first = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  double* round_partials = a.partials;
+  double* settle_partials = a.partials + static_cast<int64_t>(a.k) * G * 3;
+  double* norm_partials = settle_partials + 2 * G;
+
+  // Initial state from x = 0: r = b, p = s = 0, |b|^2 for the tolerance.
+  for (int32_t q = 0; q < a.k; ++q) {
+    const int64_t o = static_cast<int64_t>(q) * a.n;
+    double acc[1] = {0.0};
+    for (int64_t i = first; i < a.n; i += stride) {
+      double b = a.rhs[o + i];
+      a.x[o + i] = 0.0;
+      a.r[o + i] = b;
+      a.p[o + i] = 0.0;
+      a.s[o + i] = 0.0;
+      a.prev[o + i] = 0.0;
+      if (a.cheb_steps == 0) a.u[o + i] = a.inv_diag[i] * b;
+      acc[0] += b * b;
+    }
+    block_sum<kThreads, 1>(acc, scratch);
+    if (threadIdx.x == 0) norm_partials[static_cast<int64_t>(q) * G + blockIdx.x] = acc[0];
+  }
+  barrier<kThreads, kSingle>();
+  if (a.cheb_steps > 0) chebyshev_apply<kThreads, kSingle>(a);
+  for (int32_t q = 0; q < a.k; ++q) {
+    double acc[1] = {0.0};
+    for (int32_t b = threadIdx.x; b < G; b += kThreads)
+      acc[0] += norm_partials[static_cast<int64_t>(q) * G + b];
+    block_sum<kThreads, 1>(acc, scratch);
+    if (threadIdx.x == 0) {
+      atol_sq[q] = a.tol_sq * acc[0];
+      gamma_old[q] = INFINITY;
+      alpha_old[q] = 1.0;
+    }
+  }
+  __syncthreads();
+
+  int32_t stepped_rounds = 0;
+  int64_t round = 0;
+  double settle_previous = 0.0, reach_check = 0.0, checks = 0.0;
+  bool running = true;
+  while (running) {
+    ++round;
+    // w = A u and the round's three dots.
+    for (int32_t q = 0; q < a.k; ++q) {
+      const int64_t o = static_cast<int64_t>(q) * a.n;
+      double acc[3] = {0.0, 0.0, 0.0};
+      for (int64_t i = first; i < a.n; i += stride) {
+        double wi = row_dot(a, i, a.u + o);
+        a.w[o + i] = wi;
+        double ri = a.r[o + i], ui = a.u[o + i];
+        acc[0] += ri * ui;
+        acc[1] += wi * ui;
+        acc[2] += ri * ri;
+      }
+      block_sum<kThreads, 3>(acc, scratch);
+      if (threadIdx.x == 0) {
+        double* dst = round_partials + (static_cast<int64_t>(q) * G + blockIdx.x) * 3;
+        dst[0] = acc[0];
+        dst[1] = acc[1];
+        dst[2] = acc[2];
+      }
+    }
+    barrier<kThreads, kSingle>();
+    // Every block folds every column's partials in block order: alpha, beta, whether it steps.
+    for (int32_t q = 0; q < a.k; ++q) {
+      double acc[3] = {0.0, 0.0, 0.0};
+      for (int32_t b = threadIdx.x; b < G; b += kThreads) {
+        const double* src = round_partials + (static_cast<int64_t>(q) * G + b) * 3;
+        acc[0] += src[0];
+        acc[1] += src[1];
+        acc[2] += src[2];
+      }
+      block_sum<kThreads, 3>(acc, scratch);
+      if (threadIdx.x == 0) {
+        int step = 0;
+        double alpha = 0.0, beta = 0.0;
+        if (acc[2] > atol_sq[q] && round <= a.maxiter && acc[0] != 0.0) {
+          beta = acc[0] / gamma_old[q];
+          double denominator = acc[1] - beta * acc[0] / alpha_old[q];
+          if (denominator != 0.0) {
+            alpha = acc[0] / denominator;
+            step = 1;
+          }
+        }
+        stepping[q] = step;
+        coeff[2 * q] = alpha;
+        coeff[2 * q + 1] = beta;
+        if (step) {
+          gamma_old[q] = acc[0];
+          alpha_old[q] = alpha;
+        }
+      }
+    }
+    __syncthreads();
+    bool any = false;
+    for (int32_t q = 0; q < a.k; ++q) any = any || stepping[q];
+    if (!any) break;
+    ++stepped_rounds;
+    for (int32_t q = 0; q < a.k; ++q) {
+      if (!stepping[q]) continue;
+      const double alpha = coeff[2 * q], beta = coeff[2 * q + 1];
+      const int64_t o = static_cast<int64_t>(q) * a.n;
+      for (int64_t i = first; i < a.n; i += stride) {
+        double pk = a.u[o + i] + beta * a.p[o + i];
+        double sk = a.w[o + i] + beta * a.s[o + i];
+        a.p[o + i] = pk;
+        a.s[o + i] = sk;
+        a.x[o + i] += alpha * pk;
+        double rk = a.r[o + i] - alpha * sk;
+        a.r[o + i] = rk;
+        if (a.cheb_steps == 0) a.u[o + i] = a.inv_diag[i] * rk;
+      }
+    }
+    if (round >= a.maxiter) running = false;
+    if (a.cheb_steps > 0) {
+      barrier<kThreads, kSingle>();
+      chebyshev_apply<kThreads, kSingle>(a);
+    }
+    if (a.settle && round % a.check_rounds == 0) {
+      // Per-entry relative change since the last check and the count of reached entries.
+      double change = 0.0;
+      double reached[1] = {0.0};
+      for (int32_t q = 0; q < a.k; ++q) {
+        const int64_t o = static_cast<int64_t>(q) * a.n;
+        for (int64_t i = first; i < a.n; i += stride) {
+          double xv = a.x[o + i];
+          if (xv != 0.0) {
+            change = fmax(change, fabs(xv - a.prev[o + i]) / fabs(xv));
+            reached[0] += 1.0;
+          }
+          a.prev[o + i] = xv;
+        }
+      }
+      change = block_max<kThreads>(change, scratch);
+      block_sum<kThreads, 1>(reached, scratch);
+      if (threadIdx.x == 0) {
+        settle_partials[2 * blockIdx.x] = change;
+        settle_partials[2 * blockIdx.x + 1] = reached[0];
+      }
+      barrier<kThreads, kSingle>();
+      double c = 0.0;
+      double count[1] = {0.0};
+      for (int32_t b = threadIdx.x; b < G; b += kThreads) {
+        c = fmax(c, settle_partials[2 * b]);
+        count[0] += settle_partials[2 * b + 1];
+      }
+      c = block_max<kThreads>(c, scratch);
+      block_sum<kThreads, 1>(count, scratch);
+      checks += 1.0;
+      double total = count[0] + 1.0;
+      if (total != settle_previous) {
+        settle_previous = total;
+        reach_check = checks;
+      } else {
+        double idle = (checks - reach_check) * static_cast<double>(a.check_rounds);
+        if (c < a.change_tolerance || idle >= static_cast<double>(a.settle_rounds)) running = false;
+      }
+    } else if (a.cheb_steps == 0) {
+      barrier<kThreads, kSingle>();
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = stepped_rounds;
+}
+
+// Componentwise backward error flag (kernels/linalg.py:1128-1230): "not within" so NaN flags.
+__global__ void backward_error_kernel(const int32_t* __restrict__ offsets,
+                                      const int32_t* __restrict__ columns,
+                                      const double* __restrict__ values,
+                                      const double* __restrict__ x, const double* __restrict__ b,
+                                      int64_t n, double tolerance, int32_t* __restrict__ flag) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const double* xq = x + q * n;
+  double bi = b[q * n + i];
+  double total = bi, scale = fabs(bi);
+  for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+    double term = values[e] * xq[columns[e]];
+    total -= term;
+    scale += fabs(term);
+  }
+  if (!(fabs(total) <= tolerance * scale)) atomicOr(flag, 1);
+}
+
+__global__ void dominance_partial_kernel(const int32_t* __restrict__ offsets,
+                                         const int32_t* __restrict__ columns,
+                                         const double* __restrict__ values, int64_t n,
+                                         double* __restrict__ partials) {
+  __shared__ double scratch[kBlock];
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double ratio = 0.0;
+  if (i < n) {
+    double diag = 0.0, off = 0.0;
+    for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+      if (columns[e] == i) diag = fabs(values[e]);
+      else off += fabs(values[e]);
+    }
+    if (diag > 0.0) ratio = off / diag;
+  }
+  ratio = block_max<kBlock>(ratio, scratch);
+  if (threadIdx.x == 0) partials[blockIdx.x] = ratio;
+}
+
+__global__ void dominance_final_kernel(const double* __restrict__ partials, int32_t blocks,
+                                       double* __restrict__ out) {
+  __shared__ double scratch[kBlock];
+  double v = 0.0;
+  for (int32_t b = threadIdx.x; b < blocks; b += kBlock) v = fmax(v, partials[b]);
+  v = block_max<kBlock>(v, scratch);
+  if (threadIdx.x == 0) out[0] = v;
+}
+
+template <int kThreads, bool kSingle>
+int32_t cooperative_grid(int64_t n) {
+  if (kSingle) return 1;
+  int device = 0;
+  HG_HIP(hipGetDevice(&device));
+  int cus = 0;
+  HG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  int per_cu = 0;
+  HG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_kernel<kThreads, kSingle>,
+                                                      kThreads, 0));
+  int64_t resident = static_cast<int64_t>(cus) * std::max(per_cu, 1);
+  int64_t needed = (n + kThreads - 1) / kThreads;
+  return static_cast<int32_t>(std::max<int64_t>(1, std::min(resident, needed)));
+}
+
+}  // namespace
+
+int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
+  const int64_t n = problem.n;
+  const int64_t k = problem.n_columns;
+  HG_REQUIRE(k >= 1 && k <= kMaxColumns, HG_ERR_VALUE, "conjugate gradient takes 1-64 columns");
+  if (n == 0) return 0;
+  // Small systems run in one block of 1024 lanes with block barriers only.
+  const bool single = n <= 8192;
+  const int32_t G = single ? 1 : cooperative_grid<256, false>(n);
+  const int64_t entries = n * k;
+  if (ws.capacity_entries < entries) {
+    ws.vectors.allocate(9 * entries);
+    ws.capacity_entries = entries;
+  }
+  const int64_t partial_size = k * G * 3 + 2 * G + k * G;
+  if (ws.partials.size < partial_size) ws.partials.allocate(partial_size);
+  if (ws.counters.size < 4) ws.counters.allocate(4);
+  CgArgs a{};
+  a.n = n;
+  a.k = static_cast<int32_t>(k);
+  a.offsets = problem.offsets;
+  a.columns = problem.columns;
+  a.values32 = problem.values32;
+  a.values64 = problem.values64;
+  a.inv_diag = problem.inv_diag;
+  a.rhs = problem.rhs;
+  a.x = problem.x;
+  double* base = ws.vectors.ptr;
+  a.r = base;
+  a.u = base + entries;
+  a.p = base + 2 * entries;
+  a.s = base + 3 * entries;
+  a.w = base + 4 * entries;
+  a.prev = base + 5 * entries;
+  a.res = base + 6 * entries;
+  a.da = base + 7 * entries;
+  a.db = base + 8 * entries;
+  a.partials = ws.partials.ptr;
+  a.counters = ws.counters.ptr;
+  a.settle = problem.settle;
+  a.check_rounds = std::max(problem.check_rounds, 1);
+  a.change_tolerance = problem.change_tolerance;
+  a.settle_rounds = problem.settle_rounds;
+  a.tol_sq = problem.settle ? 0.0 : problem.tolerance * problem.tolerance;
+  a.maxiter = problem.maxiter > 0 ? problem.maxiter : 10 * n;
+  a.cheb_steps = problem.chebyshev_steps;
+  a.theta = 0.5 * (problem.lambda_max + problem.lambda_min);
+  a.delta = 0.5 * (problem.lambda_max - problem.lambda_min);
+  void* args[] = {&a};
+  if (single) {
+    hipLaunchKernelGGL((cg_kernel<1024, true>), dim3(1), dim3(1024), 0, stream, a);
+    HG_HIP(hipGetLastError());
+  } else {
+    HG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&cg_kernel<256, false>),
+                                      dim3(G), dim3(256), args, 0, stream));
+  }
+  int32_t rounds = 0;
+  HG_HIP(hipMemcpyAsync(&rounds, ws.counters.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+  HG_HIP(hipStreamSynchronize(stream));
+  return rounds;
+}
+
+void launch_backward_error(const int32_t* offsets, const int32_t* columns, const double* values,
+                           const double* x, const double* b, int64_t n, int64_t n_columns,
+                           double tolerance, int32_t* flag, hipStream_t stream) {
+  if (n == 0) return;
+  dim3 grid(static_cast<unsigned>((n + kBlock - 1) / kBlock), static_cast<unsigned>(n_columns));
+  backward_error_kernel<<<grid, kBlock, 0, stream>>>(offsets, columns, values, x, b, n, tolerance,
+                                                     flag);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_dominance_bound(const int32_t* offsets, const int32_t* columns, const double* values,
+                            int64_t n, double* partials, double* out, hipStream_t stream) {
+  int32_t blocks = static_cast<int32_t>((std::max<int64_t>(n, 1) + kBlock - 1) / kBlock);
+  if (n > 0)
+    dominance_partial_kernel<<<blocks, kBlock, 0, stream>>>(offsets, columns, values, n, partials);
+  else
+    HG_HIP(hipMemsetAsync(partials, 0, sizeof(double) * blocks, stream));
+  dominance_final_kernel<<<1, kBlock, 0, stream>>>(partials, blocks, out);
+  HG_HIP(hipGetLastError());
+}
+
+}  // namespace hg

@@ -1,0 +1,142 @@
+// Internal declarations of libheatgeo: mesh state, the Cholesky plan and error plumbing.
+// Everything here is private to the library; the public C ABI is include/heatgeo.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/heatgeo.h"
+
+namespace hg {
+
+// ------------------------------------------------------------------------------------------------
+// Errors: every entry point returns an hg_status and leaves a message for hg_last_error().
+// ------------------------------------------------------------------------------------------------
+
+void set_error(const std::string& message);
+
+struct Error {
+  int status;
+  std::string message;
+};
+
+#define HG_HIP(call)                                                                          \
+  do {                                                                                        \
+    hipError_t hg_err_ = (call);                                                              \
+    if (hg_err_ != hipSuccess) {                                                              \
+      throw ::hg::Error{HG_ERR_RUNTIME, std::string(#call) + ": " + hipGetErrorString(hg_err_)}; \
+    }                                                                                         \
+  } while (0)
+
+#define HG_REQUIRE(cond, status, message)          \
+  do {                                             \
+    if (!(cond)) throw ::hg::Error{(status), (message)}; \
+  } while (0)
+
+// A device allocation owned by the library.
+template <typename T>
+struct DeviceArray {
+  T* ptr = nullptr;
+  int64_t size = 0;
+  DeviceArray() = default;
+  explicit DeviceArray(int64_t n) { allocate(n); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : ptr(o.ptr), size(o.size) { o.ptr = nullptr; o.size = 0; }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) { reset(); ptr = o.ptr; size = o.size; o.ptr = nullptr; o.size = 0; }
+    return *this;
+  }
+  ~DeviceArray() { reset(); }
+  void allocate(int64_t n) {
+    reset();
+    size = n;
+    if (n > 0) HG_HIP(hipMalloc(&ptr, sizeof(T) * static_cast<size_t>(n)));
+  }
+  void reset() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    size = 0;
+  }
+  int64_t bytes() const { return size * static_cast<int64_t>(sizeof(T)); }
+  void upload(const T* host, int64_t n, hipStream_t stream) {
+    if (n > 0) HG_HIP(hipMemcpyAsync(ptr, host, sizeof(T) * n, hipMemcpyHostToDevice, stream));
+  }
+  void upload(const std::vector<T>& host, hipStream_t stream) {
+    allocate(static_cast<int64_t>(host.size()));
+    upload(host.data(), size, stream);
+  }
+};
+
+// ------------------------------------------------------------------------------------------------
+// Host analysis (pattern.cpp, analysis.cpp)
+// ------------------------------------------------------------------------------------------------
+
+// The cotangent operator's sparsity: every face edge in both directions plus a diagonal per
+// referenced vertex, rows sorted by column, and for each stored entry its run of contributing
+// corner slots 3f+e (corner e opposite the edge) in ascending order.
+struct HostPattern {
+  int64_t n = 0;
+  std::vector<int32_t> offsets;       // n+1
+  std::vector<int32_t> columns;       // nnz
+  std::vector<int32_t> corner_ptr;    // nnz+1
+  std::vector<int32_t> corner_slots;  // <= 6F
+  std::vector<int32_t> vertex_ptr;    // n+1: each vertex's corners, ascending
+  std::vector<int32_t> vertex_corners;
+  std::vector<int32_t> diagonal;      // n: slot of the diagonal entry, -1 for an empty row
+};
+
+HostPattern build_pattern(int64_t n_vertices, const int32_t* faces, int64_t n_faces);
+
+// Connected components of the pattern's graph.
+std::vector<int32_t> component_labels(const HostPattern& pattern, int32_t* n_components);
+
+// A supernodal Cholesky plan: nested-dissection order, supernode tree, level schedule, the maps
+// the numeric factorization and the multifrontal solves use.
+struct Plan {
+  int64_t n = 0;
+  int32_t n_super = 0;
+  int32_t n_levels = 0;
+  std::vector<int32_t> perm;        // new -> old
+  std::vector<int32_t> iperm;       // old -> new
+  std::vector<int32_t> col_begin;   // per supernode, new index of its first column
+  std::vector<int32_t> ncol;
+  std::vector<int32_t> nrow;
+  std::vector<int32_t> parent;
+  std::vector<int32_t> level;
+  std::vector<int32_t> row_ptr;     // CSR over supernodes: the structure rows (new indices)
+  std::vector<int32_t> rows;
+  std::vector<int32_t> child_ptr;   // CSR over supernodes: children, ascending
+  std::vector<int32_t> children;
+  std::vector<int32_t> rel;         // per supernode s, nrow[s] positions of rows(s) in parent's front
+  std::vector<int32_t> rel_ptr;     // = row_ptr
+  std::vector<int32_t> level_ptr;   // CSR over levels: supernodes at each level (bottom-up)
+  std::vector<int32_t> level_nodes;
+  std::vector<int64_t> front_off;   // arena offset (doubles) of each m x m front
+  std::vector<int64_t> w_off;       // offset of each m x ncol solve block
+  std::vector<int64_t> vec_off;     // offset (per column) of each front's m-vector in solve scratch
+  int64_t arena_size = 0;
+  int64_t w_size = 0;
+  int64_t vec_size = 0;
+  // Lower-triangle entries of the operator in front coordinates: value slot -> arena offset.
+  std::vector<int32_t> entry_src;
+  std::vector<int64_t> entry_dst;
+  std::vector<int32_t> entry_row;   // new row index of the entry
+  std::vector<int32_t> entry_col;   // new column index
+  double flops = 0.0;               // dense partial-factorization flops
+  int64_t nnz_w = 0;                // stored solve-block entries
+  int32_t max_front = 0;
+};
+
+struct AnalysisOptions {
+  int32_t leaf_rows = 48;
+  int32_t top_rows = 512;
+};
+
+Plan analyse(const HostPattern& pattern, const float* coordinates, const AnalysisOptions& options);
+
+}  // namespace hg

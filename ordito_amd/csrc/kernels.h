@@ -1,0 +1,81 @@
+// Launchers of the device kernels, called by the C ABI layer (api.hip).
+#pragma once
+
+#include "hg_internal.h"
+
+namespace hg {
+
+constexpr int kBlock = 256;
+
+// assembly.hip
+void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, float* cot,
+                       float* normals, float* areas, hipStream_t stream);
+void launch_mass(const int32_t* vertex_ptr, const int32_t* vertex_corners, const float* areas,
+                 int64_t n, double* mass, hipStream_t stream);
+int32_t edge_sum_blocks(int64_t n);
+void launch_edge_sums(const int32_t* offsets, const int32_t* columns, const int32_t* corner_ptr,
+                      const float* xyz, int64_t n, int32_t face_edges, double* partials, int32_t from_edges, double t,
+                      double* scalars, hipStream_t stream);
+void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
+                          const int32_t* corner_ptr, const int32_t* corner_slots, const float* cot,
+                          const double* mass, const double* scalars, int64_t n, double* lap,
+                          double* heat, double* poisson, float* heat32, double* heat_inv_diag,
+                          double* poisson_inv_diag, hipStream_t stream);
+void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
+                 double* u0, int32_t* invalid, hipStream_t stream);
+void launch_divergence(const float* xyz, const int32_t* faces, const float* normals,
+                       const float* areas, const float* cot, const int32_t* vertex_ptr,
+                       const int32_t* vertex_corners, const double* heat, int64_t n,
+                       int64_t n_faces, int64_t n_columns, double* corner_terms, double* out,
+                       hipStream_t stream);
+int32_t shift_partial_blocks(int64_t n);
+void launch_shift(double* phi, const int32_t* sources, const int64_t* source_offsets, int64_t n,
+                  int64_t n_columns, double* partials, double* shift, hipStream_t stream);
+
+// cg.hip: the persistent conjugate-gradient solves.
+struct CgProblem {
+  int64_t n = 0;
+  int64_t n_columns = 1;
+  const int32_t* offsets = nullptr;
+  const int32_t* columns = nullptr;
+  const float* values32 = nullptr;   // narrowed values, or null to read values64
+  const double* values64 = nullptr;
+  const double* inv_diag = nullptr;
+  const double* rhs = nullptr;
+  double* x = nullptr;
+  // Stop rule: settle (the heat step) or relative residual tol (the Poisson step).
+  int32_t settle = 1;
+  int32_t check_rounds = 16;
+  double change_tolerance = 1e-6;
+  int32_t settle_rounds = 192;
+  double tolerance = 1e-8;
+  int64_t maxiter = 0;
+  // Jacobi-Chebyshev polynomial preconditioner: steps > 0 applies it, with its interval.
+  int32_t chebyshev_steps = 0;
+  double lambda_min = 0.0;
+  double lambda_max = 0.0;
+};
+
+struct CgWorkspace {
+  DeviceArray<double> vectors;   // r, u, p, s, w, prev, and two polynomial vectors
+  DeviceArray<double> partials;
+  DeviceArray<int32_t> counters;
+  int64_t capacity_entries = 0;
+  int32_t capacity_blocks = 0;
+  int32_t capacity_columns = 0;
+};
+
+// Runs the solve; returns the rounds that stepped.
+int32_t run_cg(const CgProblem& problem, CgWorkspace& workspace, hipStream_t stream);
+
+// Componentwise backward error |b - Ax|_i <= tol (|A||x| + |b|)_i everywhere, as a flag.
+void launch_backward_error(const int32_t* offsets, const int32_t* columns, const double* values,
+                           const double* x, const double* b, int64_t n, int64_t n_columns,
+                           double tolerance, int32_t* flag, hipStream_t stream);
+
+// Largest Gershgorin ratio sum_{j != i} |A_ij| / |A_ii| over the rows (for the polynomial's
+// interval), reduced into *out (one block).
+void launch_dominance_bound(const int32_t* offsets, const int32_t* columns, const double* values,
+                            int64_t n, double* partials, double* out, hipStream_t stream);
+
+}  // namespace hg
