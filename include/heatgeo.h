@@ -177,15 +177,26 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
                 int64_t n_queries, double* d_out, int32_t mode, int32_t precision,
                 hg_solve_stats* out_stats);
 
+/*
+ * Device-time instrumentation (no counterpart in the reference, which quotes measurements in
+ * comments): with timing enabled every phase of a solve is bracketed by HIP events on the mesh's
+ * stream. hg_timing_read returns, per phase, the accumulated milliseconds and the span count:
+ * 0 numeric factorization, 1 heat triangular sweep (forward + backward), 2 Poisson sweep,
+ * 3 refinement residual, 4 divergence, 5 shift, 6 conjugate gradient, 7 assembly.
+ */
+int hg_timing_enable(hg_mesh* mesh, int32_t enable);
+int hg_timing_read(hg_mesh* mesh, double* out_ms8, int64_t* out_counts8, int32_t reset);
+
 /* Device bytes the held factorizations take (HeatSolver.nbytes, heat.py:3838-3847). */
 int hg_nbytes(hg_mesh* mesh, int64_t* out_bytes);
 
 /* Drop every held factorization; the operators stay (HeatSolver.release, heat.py:3849-3861). */
 int hg_release(hg_mesh* mesh);
 
-/* Analysis statistics of the held plan: supernodes, levels, stored solve-block entries, flops,
- * arena entries, largest front. */
-int hg_plan_stats(hg_mesh* mesh, int64_t* out_stats6);
+/* Analysis statistics of the mesh's plan (analysing it if needed): supernodes, levels, stored
+ * solve-block entries, factorization flops, arena entries, largest front, front-vector entries
+ * (n + total structure rows), rows. */
+int hg_plan_stats(hg_mesh* mesh, int64_t* out_stats8);
 
 /*
  * Host-only access to the Cholesky analysis (sparse_cholesky's ordering and symbolic phase,

@@ -411,10 +411,48 @@ def components(pattern: Pattern) -> tuple[int, np.ndarray]:
     return connected_components(graph, directed=False)
 
 
+def factorize(matrix: sp.csr_matrix):
+    """FP64 sparse LU (``scipy.sparse.linalg.splu``) under a symmetric minimum-degree ordering."""
+    return spla.splu(matrix.tocsc(), permc_spec="MMD_AT_PLUS_A")
+
+
 def solve_direct(matrix: sp.csr_matrix, rhs: np.ndarray) -> np.ndarray:
-    """FP64 sparse LU solve (``scipy.sparse.linalg.splu``), for ``rhs`` of shape (n,) or (n, k)."""
-    lu = spla.splu(matrix.tocsc())
-    return lu.solve(rhs)
+    """FP64 sparse LU solve, for ``rhs`` of shape (n,) or (n, k)."""
+    return factorize(matrix).solve(rhs)
+
+
+class FactoredOracle:
+    """
+    Both systems of one mesh factored once, for many source sets: the reference's amortized
+    (``Trimesh``, factored) form (``heat.py:4394-4423``), solved directly in FP64.
+    """
+
+    def __init__(self, vertices: np.ndarray, faces: np.ndarray, t: float | None = None) -> None:
+        self.vertices = _f32(vertices)
+        self.faces = np.asarray(faces, dtype=np.int64).reshape(-1)
+        t_resolved, _ = resolve_time(self.vertices, self.faces, t)
+        self.ops = assemble(self.vertices, self.faces, t_resolved)
+        self.heat_lu = factorize(self.ops.matrix(self.ops.heat))
+        n = self.ops.pattern.n
+        n_comp, labels = components(self.ops.pattern)
+        self.members = [np.flatnonzero(labels == c) for c in range(n_comp)]
+        pinned = np.zeros(n, dtype=bool)
+        for members in self.members:
+            pinned[members[-1]] = True
+        self.keep = ~pinned
+        reduced = self.ops.matrix(self.ops.poisson)[self.keep][:, self.keep]
+        self.poisson_lu = factorize(reduced)
+
+    def distance(self, sources: np.ndarray) -> np.ndarray:
+        n = self.ops.pattern.n
+        u0 = seed_indicator(n, sources)
+        heat = self.heat_lu.solve(u0)
+        neg_div = face_unit_divergence(self.vertices, self.faces, self.ops.terms, heat)
+        phi = np.zeros(n)
+        phi[self.keep] = self.poisson_lu.solve(neg_div[self.keep])
+        for members in self.members:
+            phi[members] -= phi[members].mean()
+        return shift_and_orient(phi, sources)
 
 
 def solve_poisson(ops: Operators, rhs: np.ndarray) -> np.ndarray:
@@ -489,20 +527,10 @@ def geodesic_columns(
     vertices: np.ndarray, faces: np.ndarray, source_sets: list[np.ndarray], t: float | None = None
 ) -> np.ndarray:
     """One distance field per source set, sharing one factorization of each system: (k, V)."""
-    vertices = _f32(vertices)
-    n = vertices.shape[0]
-    t_resolved, _ = resolve_time(vertices, faces, t)
-    ops = assemble(vertices, faces, t_resolved)
-    u0 = np.stack([seed_indicator(n, s) for s in source_sets], axis=1)
-    heat_lu = spla.splu(ops.matrix(ops.heat).tocsc())
-    heat = heat_lu.solve(u0)
-    if heat.ndim == 1:
-        heat = heat[:, None]
-    out = np.zeros((len(source_sets), n))
-    for k, s in enumerate(source_sets):
-        neg_div = face_unit_divergence(vertices, faces, ops.terms, heat[:, k])
-        out[k] = shift_and_orient(solve_poisson(ops, neg_div), s)
-    return out
+    oracle = FactoredOracle(vertices, faces, t)
+    return np.stack([oracle.distance(s) for s in source_sets]) if source_sets else np.zeros(
+        (0, _f32(vertices).shape[0])
+    )
 
 
 def relative_error(d: np.ndarray, reference: np.ndarray) -> tuple[float, float]:

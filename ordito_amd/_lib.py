@@ -54,6 +54,8 @@ EXPORTED = (
     "hg_plan_sizes",
     "hg_plan_copy",
     "hg_plan_destroy",
+    "hg_timing_enable",
+    "hg_timing_read",
 )
 
 
@@ -127,7 +129,21 @@ _SIGNATURES = {
     "hg_plan_sizes": ([_P, _PI64], ctypes.c_int),
     "hg_plan_copy": ([_P, _I32, _P], ctypes.c_int),
     "hg_plan_destroy": ([_P], ctypes.c_int),
+    "hg_timing_enable": ([_P, _I32], ctypes.c_int),
+    "hg_timing_read": ([_P, ctypes.POINTER(ctypes.c_double), _PI64, _I32], ctypes.c_int),
 }
+
+# hg_timing_read's phases, in order.
+PHASES = (
+    "factor",
+    "heat_sweep",
+    "poisson_sweep",
+    "residual",
+    "divergence",
+    "shift",
+    "cg",
+    "assembly",
+)
 
 
 def load() -> ctypes.CDLL:

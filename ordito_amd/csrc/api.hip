@@ -67,6 +67,7 @@ struct hg_mesh {
   DeviceArray<int64_t> d_owner, d_offsets;
   int64_t field_capacity = 0, corner_capacity = 0;
   hg_solve_stats stats{};
+  PhaseTimer timer;
 };
 
 namespace {
@@ -247,6 +248,7 @@ void ensure_plan(hg_mesh& m) {
 void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
   ensure_operators(m);
   ensure_plan(m);
+  PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
   if (system == HG_SYSTEM_HEAT) {
     if (!m.heat_factor.held())
       factor_numeric(*m.plan, m.heat.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
@@ -264,6 +266,7 @@ void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
 // ---- solves -----------------------------------------------------------------------------------
 
 int32_t diffuse_iterated(hg_mesh& m, const double* rhs, double* x, int64_t k) {
+  PhaseScope scope(&m.timer, kPhaseCg, m.stream);
   int32_t total = 0;
   for (int64_t q0 = 0; q0 < k; q0 += 64) {
     int64_t kk = std::min<int64_t>(64, k - q0);
@@ -304,7 +307,8 @@ int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, in
                        rhs, x, k, mixed ? kMixedHeatTolerance : kSettleDirectTolerance, true,
                        mixed ? kMixedRefineRounds : kRefineRounds,
                        m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
-                       m.has_heat_pins ? m.heat_dropped_old.ptr : nullptr, m.solve, m.stream);
+                       m.has_heat_pins ? m.heat_dropped_old.ptr : nullptr, m.solve, m.stream,
+                       &m.timer, kPhaseHeatSweep);
 }
 
 // The settled iterate, verified, falling back to the factorization (solve_spd_settled,
@@ -354,7 +358,8 @@ void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t 
     m.stats.poisson_refinements = refined_solve(
         *m.plan, m.poisson_factor, mixed, m.offsets.ptr, m.columns.ptr, m.poisson.ptr, rhs, x, k,
         kPoissonTolerance, false, mixed ? kMixedRefineRounds : kRefineRounds,
-        m.poisson_pinned_new.ptr, m.poisson_dropped_old.ptr, m.solve, m.stream);
+        m.poisson_pinned_new.ptr, m.poisson_dropped_old.ptr, m.solve, m.stream, &m.timer,
+        kPhasePoissonSweep);
     m.stats.poisson_rounds = 0;
   } else {
     // The Jacobi-Chebyshev interval (linalg.py:6074-6090): [max(min(80/n, 0.5), 1 - r), 1 + r],
@@ -362,6 +367,7 @@ void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t 
     double r = std::max(poisson_bound(m), 1e-3);
     double lo = std::max(std::min(kChebyshevInterval / static_cast<double>(m.n), kChebyshevCap), 1.0 - r);
     double hi = 1.0 + r;
+    PhaseScope scope(&m.timer, kPhaseCg, m.stream);
     int32_t rounds = 0;
     for (int64_t q0 = 0; q0 < k; q0 += 64) {
       int64_t kk = std::min<int64_t>(64, k - q0);
@@ -420,6 +426,7 @@ void divergence(hg_mesh& m, const double* heat, double* out, int64_t k) {
     m.corner_terms.allocate(std::max<int64_t>(3 * m.f * k, 1));
     m.corner_capacity = 3 * m.f * k;
   }
+  PhaseScope scope(&m.timer, kPhaseDivergence, m.stream);
   launch_divergence(m.xyz.ptr, m.faces.ptr, m.normals.ptr, m.areas.ptr, m.cot.ptr,
                     m.vertex_ptr.ptr, m.vertex_corners.ptr, heat, m.n, m.f, k, m.corner_terms.ptr,
                     out, m.stream);
@@ -429,6 +436,7 @@ void shift(hg_mesh& m, double* phi, int64_t k) {
   int32_t blocks = shift_partial_blocks(m.n);
   if (m.shift_partials.size < blocks * k) m.shift_partials.allocate(blocks * k);
   if (m.shift_values.size < 2 * k) m.shift_values.allocate(2 * k);
+  PhaseScope scope(&m.timer, kPhaseShift, m.stream);
   launch_shift(phi, m.d_sources.ptr, m.d_offsets.ptr, m.n, k, m.shift_partials.ptr,
                m.shift_values.ptr, m.stream);
 }
@@ -641,7 +649,30 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     solve_poisson(m, negdiv, d_out, k, mode, precision);
     shift(m, d_out, k);
     HG_HIP(hipStreamSynchronize(m.stream));
+    m.timer.resolve();
     if (out_stats) *out_stats = m.stats;
+  });
+}
+
+int hg_timing_enable(hg_mesh* mesh, int32_t enable) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+    mesh->timer.resolve();
+    mesh->timer.enabled = enable != 0;
+  });
+}
+
+int hg_timing_read(hg_mesh* mesh, double* out_ms, int64_t* out_counts, int32_t reset) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+    mesh->timer.resolve();
+    for (int i = 0; i < kPhaseCount; ++i) {
+      if (out_ms) out_ms[i] = mesh->timer.ms[i];
+      if (out_counts) out_counts[i] = mesh->timer.count[i];
+    }
+    if (reset) mesh->timer.reset();
   });
 }
 
@@ -676,6 +707,8 @@ int hg_plan_stats(hg_mesh* mesh, int64_t* out) {
     out[3] = static_cast<int64_t>(p.flops);
     out[4] = p.arena_size;
     out[5] = p.max_front;
+    out[6] = p.vec_size;
+    out[7] = p.n;
   });
 }
 

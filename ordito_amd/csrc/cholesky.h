@@ -57,7 +57,8 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
                       const int32_t* offsets, const int32_t* columns, const double* values,
                       const double* b, double* x, int64_t k, double tolerance, bool componentwise,
                       int32_t max_rounds, const int32_t* pinned_new, const int32_t* dropped_old,
-                      SolveWorkspace& ws, hipStream_t stream);
+                      SolveWorkspace& ws, hipStream_t stream, PhaseTimer* timer = nullptr,
+                      int sweep_phase = kPhaseHeatSweep);
 
 // Each listed component's constant set to zero mean, per column.
 void remove_null_space(const int32_t* comp_ptr, const int32_t* members, int32_t n_comp, int64_t n,

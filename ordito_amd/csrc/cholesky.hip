@@ -459,7 +459,8 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
                       const int32_t* offsets, const int32_t* columns, const double* values,
                       const double* b, double* x, int64_t k, double tolerance,
                       bool componentwise, int32_t max_rounds, const int32_t* pinned_new,
-                      const int32_t* dropped_old, SolveWorkspace& ws, hipStream_t stream) {
+                      const int32_t* dropped_old, SolveWorkspace& ws, hipStream_t stream,
+                      PhaseTimer* timer, int sweep_phase) {
   const int64_t n = plan.n;
   if (n == 0 || k == 0) return 0;
   ws.reserve(plan, k);
@@ -471,26 +472,33 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
   }
   int32_t rounds = 0;
   for (; rounds < max_rounds;) {
-    gather_perm_kernel<<<grid, 256, 0, stream>>>(ws.r.ptr, plan.perm.ptr, pinned_new, n, k,
-                                                 ws.rp.ptr);
-    if (use_fp32) {
-      triangular_solve<float>(plan, factor.w32.ptr, ws.rp.ptr, ws.y.ptr, ws.xp.ptr, ws.fvec.ptr, k,
-                              stream);
-    } else {
-      triangular_solve<double>(plan, factor.w64.ptr, ws.rp.ptr, ws.y.ptr, ws.xp.ptr, ws.fvec.ptr,
-                               k, stream);
+    {
+      PhaseScope scope(timer, sweep_phase, stream);
+      gather_perm_kernel<<<grid, 256, 0, stream>>>(ws.r.ptr, plan.perm.ptr, pinned_new, n, k,
+                                                   ws.rp.ptr);
+      if (use_fp32) {
+        triangular_solve<float>(plan, factor.w32.ptr, ws.rp.ptr, ws.y.ptr, ws.xp.ptr, ws.fvec.ptr,
+                                k, stream);
+      } else {
+        triangular_solve<double>(plan, factor.w64.ptr, ws.rp.ptr, ws.y.ptr, ws.xp.ptr,
+                                 ws.fvec.ptr, k, stream);
+      }
+      scatter_add_perm_kernel<<<grid, 256, 0, stream>>>(ws.xp.ptr, plan.perm.ptr, n, k, x);
     }
-    scatter_add_perm_kernel<<<grid, 256, 0, stream>>>(ws.xp.ptr, plan.perm.ptr, n, k, x);
     ++rounds;
-    HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
-    residual_test_kernel<<<grid, 256, 0, stream>>>(offsets, columns, values, x, b, dropped_old,
-                                                   ws.bmax.ptr, n, k, tolerance,
-                                                   componentwise ? 1 : 0, ws.r.ptr,
-                                                   ws.pending.ptr);
+    {
+      PhaseScope scope(timer, kPhaseResidual, stream);
+      HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
+      residual_test_kernel<<<grid, 256, 0, stream>>>(offsets, columns, values, x, b, dropped_old,
+                                                     ws.bmax.ptr, n, k, tolerance,
+                                                     componentwise ? 1 : 0, ws.r.ptr,
+                                                     ws.pending.ptr);
+    }
     HG_HIP(hipGetLastError());
     int32_t pending = 0;
     HG_HIP(hipMemcpyAsync(&pending, ws.pending.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     HG_HIP(hipStreamSynchronize(stream));
+    if (timer) timer->resolve();
     if (!pending) break;
   }
   return rounds;

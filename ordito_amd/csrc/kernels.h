@@ -7,6 +7,81 @@ namespace hg {
 
 constexpr int kBlock = 256;
 
+// Device time of the phases of a solve, from HIP events on the stream the work is issued on.
+enum Phase : int {
+  kPhaseFactor = 0,
+  kPhaseHeatSweep = 1,     // one forward + backward triangular sweep of the heat system
+  kPhasePoissonSweep = 2,  // the same for the Poisson system
+  kPhaseResidual = 3,      // a refinement residual and its test
+  kPhaseDivergence = 4,
+  kPhaseShift = 5,
+  kPhaseCg = 6,
+  kPhaseAssembly = 7,
+  kPhaseCount = 8
+};
+
+struct PhaseTimer {
+  bool enabled = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Span {
+    int phase;
+    hipEvent_t start, stop;
+  };
+  std::vector<Span> pending;
+  double ms[kPhaseCount] = {};
+  int64_t count[kPhaseCount] = {};
+
+  hipEvent_t mark(hipStream_t stream) {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      HG_HIP(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    hipEvent_t e = pool[used++];
+    HG_HIP(hipEventRecord(e, stream));
+    return e;
+  }
+  void add(int phase, hipEvent_t start, hipEvent_t stop) { pending.push_back({phase, start, stop}); }
+  // Call after the stream is synchronized: folds the recorded spans into the totals.
+  void resolve() {
+    for (const Span& s : pending) {
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, s.start, s.stop) == hipSuccess) {
+        ms[s.phase] += t;
+        count[s.phase] += 1;
+      }
+    }
+    pending.clear();
+    used = 0;
+  }
+  void reset() {
+    pending.clear();
+    used = 0;
+    for (int i = 0; i < kPhaseCount; ++i) {
+      ms[i] = 0.0;
+      count[i] = 0;
+    }
+  }
+  ~PhaseTimer() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// Records a phase's span when the timer is enabled.
+struct PhaseScope {
+  PhaseTimer* timer;
+  int phase;
+  hipStream_t stream;
+  hipEvent_t start = nullptr;
+  PhaseScope(PhaseTimer* t, int p, hipStream_t s) : timer(t), phase(p), stream(s) {
+    if (timer && timer->enabled) start = timer->mark(stream);
+  }
+  ~PhaseScope() {
+    if (start) timer->add(phase, start, timer->mark(stream));
+  }
+};
+
 // assembly.hip
 void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, float* cot,
                        float* normals, float* areas, hipStream_t stream);

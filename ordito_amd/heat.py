@@ -349,11 +349,31 @@ class HeatSolver:
         )
         return bool(out.value)
 
+    def enable_timing(self, enable: bool = True) -> None:
+        """Bracket every phase of later solves with HIP events on the solver's stream."""
+        _lib.call("hg_timing_enable", self._native().ptr, int(bool(enable)))
+
+    def read_timing(self, reset: bool = False) -> dict[str, tuple[float, int]]:
+        """Accumulated device milliseconds and span counts per phase since the last reset."""
+        ms = (ctypes.c_double * 8)()
+        counts = (ctypes.c_int64 * 8)()
+        _lib.call("hg_timing_read", self._native().ptr, ms, counts, int(bool(reset)))
+        return {name: (float(ms[i]), int(counts[i])) for i, name in enumerate(_lib.PHASES)}
+
     def plan_stats(self) -> dict[str, int]:
         """Statistics of the Cholesky analysis: supernodes, levels, solve-block entries, flops."""
-        out = (ctypes.c_int64 * 6)()
+        out = (ctypes.c_int64 * 8)()
         _lib.call("hg_plan_stats", self._native().ptr, out)
-        keys = ("supernodes", "levels", "solve_block_entries", "flops", "arena_entries", "max_front")
+        keys = (
+            "supernodes",
+            "levels",
+            "solve_block_entries",
+            "flops",
+            "arena_entries",
+            "max_front",
+            "front_vector_entries",
+            "rows",
+        )
         return dict(zip(keys, (int(v) for v in out)))
 
     # -- solves ----------------------------------------------------------------------------------
