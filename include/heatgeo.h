@@ -213,6 +213,9 @@ int hg_plan_sizes(hg_plan* plan, int64_t* out8);
  * 8 rel, 9 child_ptr, 10 children, 11 level_ptr, 12 level_nodes (int32); 13 front_off, 14 w_off,
  * 15 vec_off, 16 entry_dst (int64); 17 entry_src, 18 entry_row, 19 entry_col (int32). */
 int hg_plan_copy(hg_plan* plan, int32_t field, void* out);
+/* Checks every index the device sweeps form from this plan (task ranges, partial slots, gather
+ * sources) against its buffer; HG_ERR_RUNTIME names the first violation. */
+int hg_plan_validate(hg_plan* plan);
 int hg_plan_destroy(hg_plan* plan);
 
 #ifdef __cplusplus

@@ -353,6 +353,30 @@ def stable_normalize(v: np.ndarray) -> np.ndarray:
     return out
 
 
+def face_gradients(
+    vertices: np.ndarray, faces: np.ndarray, terms: FaceTerms, u: np.ndarray
+) -> np.ndarray:
+    """
+    Per-face gradient of a vertex field: ``sum_k u_k (N x e_k) / (2A)`` in float64 from the float32
+    unit normal and area, ``e_k`` the edge opposite corner k; zero for a zero-area face.
+
+    Follows ``kernels/triangles.py:834-864`` (``face_gradient_given``).
+    """
+    p = _f32(vertices).astype(np.float64)
+    tri = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    v0, v1, v2 = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
+    n = terms.normals.astype(np.float64)
+    area = terms.areas.astype(np.float64)
+    u0, u1, u2 = u[tri[:, 0]], u[tri[:, 1]], u[tri[:, 2]]
+    grad = (
+        u0[:, None] * _cross3(n, v2 - v1)
+        + u1[:, None] * _cross3(n, v0 - v2)
+        + u2[:, None] * _cross3(n, v1 - v0)
+    ) / (2.0 * area[:, None] + (area[:, None] <= 0.0))
+    grad[area <= 0.0] = 0.0
+    return grad
+
+
 def face_unit_divergence(
     vertices: np.ndarray, faces: np.ndarray, terms: FaceTerms, heat: np.ndarray
 ) -> np.ndarray:
@@ -368,16 +392,7 @@ def face_unit_divergence(
     p = _f32(vertices).astype(np.float64)
     tri = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
     v0, v1, v2 = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
-    n = terms.normals.astype(np.float64)
-    area = terms.areas.astype(np.float64)
-    u0, u1, u2 = heat[tri[:, 0]], heat[tri[:, 1]], heat[tri[:, 2]]
-    grad = (
-        u0[:, None] * _cross3(n, v2 - v1)
-        + u1[:, None] * _cross3(n, v0 - v2)
-        + u2[:, None] * _cross3(n, v1 - v0)
-    ) / (2.0 * area[:, None] + (area[:, None] <= 0.0))
-    grad[area <= 0.0] = 0.0
-    x = stable_normalize(grad)
+    x = stable_normalize(face_gradients(vertices, faces, terms, heat))
     c = terms.cot.astype(np.float64)
     c0, c1, c2 = c[:, 0], c[:, 1], c[:, 2]
     d0 = c2 * _dot3(v1 - v0, x) + c1 * _dot3(v2 - v0, x)

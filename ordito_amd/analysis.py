@@ -52,8 +52,13 @@ class HostPlan:
         raise AttributeError(name)
 
 
-def analyse(offsets, columns, xyz, *, leaf_rows: int = 0, top_rows: int = 0) -> HostPlan:
-    """Run the library's analysis on a host CSR pattern (sorted rows) and float32 (n, 3) xyz."""
+def analyse(
+    offsets, columns, xyz, *, leaf_rows: int = 0, top_rows: int = 0, validate: bool = False
+) -> HostPlan:
+    """
+    Run the library's analysis on a host CSR pattern (sorted rows) and float32 (n, 3) xyz. With
+    ``validate`` the library also checks every index its device sweeps form from the plan.
+    """
     lib = _lib.load()
     offsets = np.ascontiguousarray(offsets, dtype=np.int32)
     columns = np.ascontiguousarray(columns, dtype=np.int32)
@@ -67,6 +72,8 @@ def analyse(offsets, columns, xyz, *, leaf_rows: int = 0, top_rows: int = 0) -> 
         )
     )
     try:
+        if validate:
+            _lib.check(lib.hg_plan_validate(handle))
         sizes = (ctypes.c_int64 * 8)()
         _lib.check(lib.hg_plan_sizes(handle, sizes))
         n_super, n_levels, n_rows, n_children, n_entries, arena, w_size, vec = (int(v) for v in sizes)

@@ -62,7 +62,8 @@ struct hg_mesh {
   // Workspaces.
   CgWorkspace cg;
   SolveWorkspace solve;
-  DeviceArray<double> fields, corner_terms, shift_partials, shift_values;
+  DeviceArray<double> fields, corner_terms, shift_partials, shift_values, null_partials;
+  NullSpace null_space;
   DeviceArray<int32_t> d_sources, invalid;
   DeviceArray<int64_t> d_owner, d_offsets;
   int64_t field_capacity = 0, corner_capacity = 0;
@@ -202,7 +203,7 @@ void ensure_plan(hg_mesh& m) {
   AnalysisOptions options;
   Plan plan = analyse(m.pattern, m.h_xyz.data(), options);
   auto device = std::make_unique<DevicePlan>();
-  device->upload(plan, m.stream);
+  device->upload(plan, m.pattern, m.stream);
   // Pins: the Poisson system's one row per component, its last-eliminated vertex
   // (kernels/cholesky.py:195-214); the heat system's empty rows (unreferenced vertices).
   const int64_t n = m.n;
@@ -303,11 +304,10 @@ bool backward_error_fails(hg_mesh& m, const double* x, const double* b, int64_t 
 int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t precision) {
   const bool mixed = precision == HG_PRECISION_MIXED;
   factor_system(m, HG_SYSTEM_HEAT, mixed);
-  return refined_solve(*m.plan, m.heat_factor, mixed, m.offsets.ptr, m.columns.ptr, m.heat.ptr,
-                       rhs, x, k, mixed ? kMixedHeatTolerance : kSettleDirectTolerance, true,
+  return refined_solve(*m.plan, m.heat_factor, mixed, rhs, x, k,
+                       mixed ? kMixedHeatTolerance : kSettleDirectTolerance, true,
                        mixed ? kMixedRefineRounds : kRefineRounds,
-                       m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
-                       m.has_heat_pins ? m.heat_dropped_old.ptr : nullptr, m.solve, m.stream,
+                       m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr, m.solve, m.stream,
                        &m.timer, kPhaseHeatSweep);
 }
 
@@ -356,10 +356,9 @@ void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t 
     const bool mixed = precision == HG_PRECISION_MIXED;
     factor_system(m, HG_SYSTEM_POISSON, mixed);
     m.stats.poisson_refinements = refined_solve(
-        *m.plan, m.poisson_factor, mixed, m.offsets.ptr, m.columns.ptr, m.poisson.ptr, rhs, x, k,
-        kPoissonTolerance, false, mixed ? kMixedRefineRounds : kRefineRounds,
-        m.poisson_pinned_new.ptr, m.poisson_dropped_old.ptr, m.solve, m.stream, &m.timer,
-        kPhasePoissonSweep);
+        *m.plan, m.poisson_factor, mixed, rhs, x, k, kPoissonTolerance, false,
+        mixed ? kMixedRefineRounds : kRefineRounds, m.poisson_pinned_new.ptr, m.solve, m.stream,
+        &m.timer, kPhasePoissonSweep);
     m.stats.poisson_rounds = 0;
   } else {
     // The Jacobi-Chebyshev interval (linalg.py:6074-6090): [max(min(80/n, 0.5), 1 - r), 1 + r],
@@ -390,7 +389,8 @@ void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t 
     m.stats.poisson_rounds = rounds;
     m.stats.poisson_refinements = 0;
   }
-  remove_null_space(m.comp_ptr.ptr, m.comp_members.ptr, m.n_comp, m.n, k, x, m.stream);
+  remove_null_space(m.null_space, m.comp_ptr.ptr, m.comp_members.ptr, m.n, k, x, m.null_partials,
+                    m.stream);
 }
 
 void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets, int64_t k) {
@@ -493,6 +493,7 @@ int hg_mesh_create(const float* d_vertices, int64_t n_vertices, const int32_t* d
     m->vertex_ptr.upload(m->pattern.vertex_ptr, st);
     m->vertex_corners.upload(m->pattern.vertex_corners, st);
     m->comp_ptr.upload(comp_ptr, st);
+    m->null_space.build(comp_ptr, st);
     m->comp_members.upload(members, st);
     HG_HIP(hipStreamSynchronize(st));
     *out_mesh = m.release();
@@ -781,6 +782,13 @@ int hg_plan_copy(hg_plan* plan, int32_t field, void* out) {
       case 19: put32(p.entry_col); break;
       default: throw Error{HG_ERR_VALUE, "unknown plan field"};
     }
+  });
+}
+
+int hg_plan_validate(hg_plan* plan) {
+  return guarded([&] {
+    HG_REQUIRE(plan != nullptr, HG_ERR_VALUE, "null plan");
+    validate_sweeps(plan->plan);
   });
 }
 

@@ -331,16 +331,23 @@ __global__ void backward_error_kernel(const int32_t* __restrict__ offsets,
                                       int64_t n, double tolerance, int32_t* __restrict__ flag) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t q = blockIdx.y;
-  if (i >= n) return;
-  const double* xq = x + q * n;
-  double bi = b[q * n + i];
-  double total = bi, scale = fabs(bi);
-  for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
-    double term = values[e] * xq[columns[e]];
-    total -= term;
-    scale += fabs(term);
+  bool fail = false;
+  if (i < n) {
+    const double* xq = x + q * n;
+    double bi = b[q * n + i];
+    double total = bi, scale = fabs(bi);
+    for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+      double term = values[e] * xq[columns[e]];
+      total -= term;
+      scale += fabs(term);
+    }
+    fail = !(fabs(total) <= tolerance * scale);
   }
-  if (!(fabs(total) <= tolerance * scale)) atomicOr(flag, 1);
+  // One atomic a failing block, none once the flag reads set.
+  const int any = __syncthreads_or(fail ? 1 : 0);
+  if (any && threadIdx.x == 0 &&
+      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    atomicOr(flag, 1);
 }
 
 __global__ void dominance_partial_kernel(const int32_t* __restrict__ offsets,

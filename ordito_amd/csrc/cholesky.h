@@ -1,4 +1,4 @@
-// Device-side Cholesky plan, factors and solve workspace (cholesky.hip).
+// Device-side Cholesky plan, factors and solve workspace (cholesky.hip, solve.hip).
 #pragma once
 
 #include "kernels.h"
@@ -8,9 +8,10 @@ namespace hg {
 // The analysis' maps on the device (uploaded once per mesh, shared by both systems).
 struct DevicePlan {
   int64_t n = 0;
+  int64_t nnz = 0;
   int32_t n_super = 0;
   int32_t n_levels = 0;
-  std::vector<int32_t> level_ptr;  // host copy: the launches are per level
+  std::vector<int32_t> level_ptr;  // host copies: the launches are per level
   std::vector<int32_t> host_col_begin, host_ncol, host_nrow;
   std::vector<int64_t> host_front_off;
   int64_t arena_size = 0, w_size = 0, vec_size = 0, n_entries = 0, nnz_w = 0;
@@ -21,28 +22,51 @@ struct DevicePlan {
   DeviceArray<int64_t> front_off, w_off, vec_off;
   DeviceArray<int32_t> entry_src, entry_row, entry_col;
   DeviceArray<int64_t> entry_dst;
+  // Sweep tasks, grouped by level. A tile is 64 rows (forward) or 64 solve-block columns
+  // (backward) of one front; a product task is a tile and a range of the inner dimension. Where a
+  // level has too few tiles to fill the device its products are split along the inner dimension
+  // into partial slots that a reduction adds in slot order.
+  struct SweepTasks {
+    std::vector<int32_t> level_ptr;   // product tasks per level
+    std::vector<int32_t> reduce_ptr;  // reduction groups per level (empty level: no split)
+    std::vector<int32_t> slots;       // partial slots each level uses
+    DeviceArray<int32_t> s, tile, k0, k1, slot;  // per product task; slot -1 writes directly
+    DeviceArray<int32_t> rs, rtile, rfirst, rcount;  // per reduction group
+  };
+  SweepTasks fwd, bwd;
+  int32_t max_slots = 0;
+  // Forward gather lists: each front position's children contributions (absolute rows of the
+  // front-vector arena), in ascending child order.
+  DeviceArray<int32_t> gather_ptr, gather_src;
+  std::vector<int32_t> gather_task_ptr;  // = forward tiles per level
+  DeviceArray<int32_t> gather_s, gather_p0;
+  // The operator's pattern in the elimination order (P A P^T) and each entry's original slot.
+  DeviceArray<int32_t> p_offsets, p_columns, p_src;
 
-  void upload(const Plan& plan, hipStream_t stream);
+  void upload(const Plan& plan, const HostPattern& pattern, hipStream_t stream);
   int64_t bytes() const;
 };
 
-// One system's solve blocks [L11^-1; L21 L11^-1] per supernode, FP64 and optionally FP32.
+// One system's solve blocks [L11^-1; L21 L11^-1] per supernode, FP64 and optionally FP32, and the
+// operator's values in the elimination order for the refinement residual.
 struct Factor {
   DeviceArray<double> w64;
   DeviceArray<float> w32;
+  DeviceArray<double> pvalues;
   bool held() const { return w64.ptr != nullptr; }
-  int64_t bytes() const { return w64.bytes() + w32.bytes(); }
+  int64_t bytes() const { return w64.bytes() + w32.bytes() + pvalues.bytes(); }
   void reset() {
     w64.reset();
     w32.reset();
+    pvalues.reset();
   }
 };
 
 struct SolveWorkspace {
-  DeviceArray<double> r, rp, y, xp, fvec, bmax;
+  DeviceArray<double> bp, xa, xb, r, y, d, fvec, bmax, partials;
   DeviceArray<int32_t> pending;
-  int64_t capacity = 0, fcap = 0;
-  void reserve(const DevicePlan& plan, int64_t k);
+  int64_t capacity = 0, fcap = 0, pcap = 0;
+  void reserve(const DevicePlan& plan, int64_t kk);
 };
 
 // Numeric factorization of `values` (the operator over the mesh pattern) with the rows flagged in
@@ -51,17 +75,25 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
                     const std::vector<int64_t>& pin_slots, Factor& factor, bool keep_fp32,
                     hipStream_t stream);
 
-// x = A^-1 b refined against the operator until every row passes (componentwise or normwise) or
-// max_rounds solves ran; returns the solves. k columns of n entries each.
+// x = A^-1 b (k columns of n entries, user layout) refined against the operator (the factor's
+// permuted values) until every row passes the test (componentwise or normwise) or max_rounds
+// sweeps ran; returns the most sweeps any 64-column chunk took. Pinned rows are dropped.
 int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp32,
-                      const int32_t* offsets, const int32_t* columns, const double* values,
                       const double* b, double* x, int64_t k, double tolerance, bool componentwise,
-                      int32_t max_rounds, const int32_t* pinned_new, const int32_t* dropped_old,
-                      SolveWorkspace& ws, hipStream_t stream, PhaseTimer* timer = nullptr,
+                      int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
+                      hipStream_t stream, PhaseTimer* timer = nullptr,
                       int sweep_phase = kPhaseHeatSweep);
 
-// Each listed component's constant set to zero mean, per column.
-void remove_null_space(const int32_t* comp_ptr, const int32_t* members, int32_t n_comp, int64_t n,
-                       int64_t k, double* x, hipStream_t stream);
+// The components' members cut into 256-member chunks, for the null-space projection's reductions.
+struct NullSpace {
+  int32_t n_comp = 0, n_chunks = 0;
+  DeviceArray<int32_t> chunk_comp, chunk_first, comp_chunk_ptr;
+  void build(const std::vector<int32_t>& comp_ptr_host, hipStream_t stream);
+};
+
+// Each component's constant set to zero mean, per column (fixed-order chunk sums).
+void remove_null_space(const NullSpace& ns, const int32_t* comp_ptr, const int32_t* members,
+                       int64_t n, int64_t k, double* x, DeviceArray<double>& scratch,
+                       hipStream_t stream);
 
 }  // namespace hg

@@ -146,165 +146,12 @@ __global__ void narrow_kernel(const double* __restrict__ src, float* __restrict_
   if (i < n) dst[i] = static_cast<float>(src[i]);
 }
 
-// ---- solves -----------------------------------------------------------------------------------
-
-// Forward sweep of one level: each front gathers [b(cols); 0] plus its children's update vectors,
-// then y(cols) = W11 f(cols) and its own update vector f(rows) -= W21 f(cols).
-template <typename T>
-__global__ void forward_level_kernel(const int32_t* __restrict__ nodes,
-                                     const int32_t* __restrict__ col_begin,
-                                     const int32_t* __restrict__ ncol,
-                                     const int32_t* __restrict__ nrow,
-                                     const int32_t* __restrict__ child_ptr,
-                                     const int32_t* __restrict__ children,
-                                     const int32_t* __restrict__ row_ptr,
-                                     const int32_t* __restrict__ rel,
-                                     const int64_t* __restrict__ w_off,
-                                     const int64_t* __restrict__ vec_off, const T* __restrict__ w,
-                                     const double* __restrict__ b, double* __restrict__ y,
-                                     double* __restrict__ fvec, int64_t n, int32_t k) {
-  const int32_t s = nodes[blockIdx.x];
-  const int32_t nc = ncol[s], nr = nrow[s], c0 = col_begin[s];
-  const int64_t m = nc + nr;
-  double* f = fvec + vec_off[s] * k;
-  for (int64_t t = threadIdx.x; t < m * k; t += blockDim.x) {
-    const int64_t q = t / m, i = t % m;
-    f[t] = i < nc ? b[q * n + c0 + i] : 0.0;
-  }
-  __syncthreads();
-  for (int32_t e = child_ptr[s]; e < child_ptr[s + 1]; ++e) {
-    const int32_t c = children[e];
-    const int64_t mc = ncol[c] + nrow[c];
-    const double* fc = fvec + vec_off[c] * k + ncol[c];
-    const int32_t* map = rel + row_ptr[c];
-    const int32_t rc = nrow[c];
-    for (int64_t t = threadIdx.x; t < static_cast<int64_t>(rc) * k; t += blockDim.x) {
-      const int64_t q = t / rc, i = t % rc;
-      f[q * m + map[i]] += fc[q * mc + i];
-    }
-    __syncthreads();
-  }
-  const T* W = w + w_off[s];
-  for (int64_t t = threadIdx.x; t < m * k; t += blockDim.x) {
-    const int64_t q = t / m, i = t % m;
-    const double* fq = f + q * m;
-    double sum = 0.0;
-    const int32_t jmax = i < nc ? static_cast<int32_t>(i) + 1 : nc;
-    for (int32_t j = 0; j < jmax; ++j) sum += static_cast<double>(W[j * m + i]) * fq[j];
-    if (i < nc) y[q * n + c0 + i] = sum;
-    else f[q * m + i] -= sum;
-  }
-}
-
-// Backward sweep of one level: x(cols) = W11^T y(cols) - W21^T x(rows).
-template <typename T>
-__global__ void backward_level_kernel(const int32_t* __restrict__ nodes,
-                                      const int32_t* __restrict__ col_begin,
-                                      const int32_t* __restrict__ ncol,
-                                      const int32_t* __restrict__ nrow,
-                                      const int32_t* __restrict__ row_ptr,
-                                      const int32_t* __restrict__ rows,
-                                      const int64_t* __restrict__ w_off, const T* __restrict__ w,
-                                      const double* __restrict__ y, double* __restrict__ x,
-                                      int64_t n, int32_t k) {
-  const int32_t s = nodes[blockIdx.x];
-  const int32_t nc = ncol[s], nr = nrow[s], c0 = col_begin[s];
-  const int64_t m = nc + nr;
-  const T* W = w + w_off[s];
-  const int32_t* r = rows + row_ptr[s];
-  for (int64_t t = threadIdx.x; t < static_cast<int64_t>(nc) * k; t += blockDim.x) {
-    const int64_t q = t / nc;
-    const int32_t j = static_cast<int32_t>(t % nc);
-    const T* wj = W + j * m;
-    const double* yq = y + q * n + c0;
-    const double* xq = x + q * n;
-    double sum = 0.0;
-    for (int32_t i = j; i < nc; ++i) sum += static_cast<double>(wj[i]) * yq[i];
-    for (int32_t i = 0; i < nr; ++i) sum -= static_cast<double>(wj[nc + i]) * xq[r[i]];
-    x[q * n + c0 + j] = sum;
-  }
-}
-
-// rp[new] = r[perm[new]] (zero at pinned rows: their equations are dropped).
-__global__ void gather_perm_kernel(const double* __restrict__ r, const int32_t* __restrict__ perm,
-                                   const int32_t* __restrict__ pinned, int64_t n, int64_t k,
-                                   double* __restrict__ rp) {
-  int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n * k) return;
-  int64_t q = t / n, i = t % n;
-  rp[t] = (pinned && pinned[i]) ? 0.0 : r[q * n + perm[i]];
-}
-
-// x[perm[new]] += xp[new].
-__global__ void scatter_add_perm_kernel(const double* __restrict__ xp,
-                                        const int32_t* __restrict__ perm, int64_t n, int64_t k,
-                                        double* __restrict__ x) {
-  int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n * k) return;
-  int64_t q = t / n, i = t % n;
-  x[q * n + perm[i]] += xp[t];
-}
-
-// Per column, the largest |b| (the normwise test's scale, kernels/cholesky.py:1357-1390).
-__global__ void column_absmax_kernel(const double* __restrict__ b, int64_t n,
-                                     double* __restrict__ out) {
-  __shared__ double scratch[kFrontThreads];
-  const int64_t q = blockIdx.x;
-  double v = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kFrontThreads) v = fmax(v, fabs(b[q * n + i]));
-  scratch[threadIdx.x] = v;
-  __syncthreads();
-  for (int stride = kFrontThreads / 2; stride > 0; stride >>= 1) {
-    if (threadIdx.x < stride) scratch[threadIdx.x] = fmax(scratch[threadIdx.x], scratch[threadIdx.x + stride]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[q] = scratch[0];
-}
-
-// r = b - A x and whether any row is still above tolerance: componentwise against
-// (|A||x| + |b|)_i, normwise against the larger of that and the column's largest |b|
-// (kernels/cholesky.py:1393-1440). Dropped (pinned) rows are not tested. "Not within" so NaN fails.
-__global__ void residual_test_kernel(const int32_t* __restrict__ offsets,
-                                     const int32_t* __restrict__ columns,
-                                     const double* __restrict__ values, const double* __restrict__ x,
-                                     const double* __restrict__ b, const int32_t* __restrict__ dropped,
-                                     const double* __restrict__ bmax, int64_t n, int64_t k,
-                                     double tolerance, int32_t componentwise,
-                                     double* __restrict__ r, int32_t* __restrict__ pending) {
-  int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n * k) return;
-  int64_t q = t / n, i = t % n;
-  const double* xq = x + q * n;
-  double bi = b[t];
-  double total = bi, scale = fabs(bi);
-  for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
-    double term = values[e] * xq[columns[e]];
-    total -= term;
-    scale += fabs(term);
-  }
-  r[t] = total;
-  if (!componentwise) scale = fmax(scale, bmax[q]);
-  if (!(dropped && dropped[i]) && !(fabs(total) <= tolerance * scale)) atomicOr(pending, 1);
-}
-
-// Each singular component's constant set to zero mean: one block a (component, column).
-__global__ void null_space_kernel(const int32_t* __restrict__ comp_ptr,
-                                  const int32_t* __restrict__ members, int64_t n,
-                                  double* __restrict__ x) {
-  __shared__ double scratch[kFrontThreads];
-  const int32_t c = blockIdx.x;
-  const int64_t q = blockIdx.y;
-  const int32_t first = comp_ptr[c], last = comp_ptr[c + 1];
-  double sum = 0.0;
-  for (int32_t e = first + threadIdx.x; e < last; e += kFrontThreads) sum += x[q * n + members[e]];
-  scratch[threadIdx.x] = sum;
-  __syncthreads();
-  for (int stride = kFrontThreads / 2; stride > 0; stride >>= 1) {
-    if (threadIdx.x < stride) scratch[threadIdx.x] += scratch[threadIdx.x + stride];
-    __syncthreads();
-  }
-  const double mean = scratch[0] / static_cast<double>(last - first);
-  for (int32_t e = first + threadIdx.x; e < last; e += kFrontThreads) x[q * n + members[e]] -= mean;
+// The operator's values in the elimination order's CSR (P A P^T), for the refinement residuals.
+__global__ void permute_values_kernel(const double* __restrict__ values,
+                                      const int32_t* __restrict__ src, int64_t nnz,
+                                      double* __restrict__ out) {
+  int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < nnz) out[e] = values[src[e]];
 }
 
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
@@ -313,7 +160,212 @@ inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) /
 
 // ---- host side --------------------------------------------------------------------------------
 
-void DevicePlan::upload(const Plan& plan, hipStream_t stream) {
+// Product tasks a level is split into at least, where its tiles allow: a few per compute unit.
+constexpr int32_t kTargetTasks = 1024;
+
+HostSweep sweep_tasks(const Plan& plan, bool forward) {
+  HostSweep out;
+  out.level_ptr.assign(plan.n_levels + 1, 0);
+  out.reduce_ptr.assign(plan.n_levels + 1, 0);
+  out.slots.assign(plan.n_levels, 0);
+  for (int32_t level = 0; level < plan.n_levels; ++level) {
+    int64_t tiles = 0;
+    for (int32_t k = plan.level_ptr[level]; k < plan.level_ptr[level + 1]; ++k) {
+      const int32_t s = plan.level_nodes[k];
+      const int32_t extent = forward ? plan.ncol[s] + plan.nrow[s] : plan.ncol[s];
+      tiles += (extent + 63) / 64;
+    }
+    const int64_t per = std::max<int64_t>(tiles, 1);
+    const int32_t split = static_cast<int32_t>(std::max<int64_t>(1, (kTargetTasks + per - 1) / per));
+    int32_t slots = 0;
+    for (int32_t k = plan.level_ptr[level]; k < plan.level_ptr[level + 1]; ++k) {
+      const int32_t s = plan.level_nodes[k];
+      const int32_t nc = plan.ncol[s], m = nc + plan.nrow[s];
+      const int32_t extent = forward ? m : nc;
+      for (int32_t tile = 0; tile < extent; tile += 64) {
+        // Forward rows below nc need j <= i (W11 is lower triangular); backward columns j need
+        // i >= j.
+        const int32_t begin = forward ? 0 : tile;
+        const int32_t end = forward ? ((tile + 64 <= nc) ? tile + 64 : nc) : m;
+        const int32_t inner = end - begin;
+        const int32_t chunks = std::max(1, std::min(split, (inner + 63) / 64));
+        if (chunks == 1) {
+          out.s.push_back(s);
+          out.tile.push_back(tile);
+          out.k0.push_back(begin);
+          out.k1.push_back(end);
+          out.slot.push_back(-1);
+          continue;
+        }
+        const int32_t width = ((inner + chunks - 1) / chunks + 31) / 32 * 32;
+        const int32_t first = slots;
+        for (int32_t k0 = begin; k0 < end; k0 += width) {
+          out.s.push_back(s);
+          out.tile.push_back(tile);
+          out.k0.push_back(k0);
+          out.k1.push_back(std::min(end, k0 + width));
+          out.slot.push_back(slots++);
+        }
+        out.rs.push_back(s);
+        out.rtile.push_back(tile);
+        out.rfirst.push_back(first);
+        out.rcount.push_back(slots - first);
+      }
+    }
+    out.level_ptr[level + 1] = static_cast<int32_t>(out.s.size());
+    out.reduce_ptr[level + 1] = static_cast<int32_t>(out.rs.size());
+    out.slots[level] = slots;
+    out.max_slots = std::max(out.max_slots, slots);
+  }
+  return out;
+}
+
+HostGather gather_lists(const Plan& plan) {
+  HostGather g;
+  g.ptr.assign(plan.vec_size + 1, 0);
+  for (int32_t c = 0; c < plan.n_super; ++c) {
+    const int32_t p = plan.parent[c];
+    if (p < 0) continue;
+    for (int32_t i = 0; i < plan.nrow[c]; ++i) g.ptr[plan.vec_off[p] + plan.rel[plan.row_ptr[c] + i] + 1]++;
+  }
+  for (int64_t v = 0; v < plan.vec_size; ++v) g.ptr[v + 1] += g.ptr[v];
+  g.src.resize(g.ptr[plan.vec_size]);
+  std::vector<int32_t> at(g.ptr.begin(), g.ptr.end() - 1);
+  // Children are numbered before their parents and visited in ascending order, so each position's
+  // list comes out in ascending child order.
+  for (int32_t c = 0; c < plan.n_super; ++c) {
+    const int32_t p = plan.parent[c];
+    if (p < 0) continue;
+    for (int32_t i = 0; i < plan.nrow[c]; ++i) {
+      const int64_t target = plan.vec_off[p] + plan.rel[plan.row_ptr[c] + i];
+      g.src[at[target]++] = static_cast<int32_t>(plan.vec_off[c] + plan.ncol[c] + i);
+    }
+  }
+  g.task_ptr.assign(plan.n_levels + 1, 0);
+  for (int32_t level = 0; level < plan.n_levels; ++level) {
+    for (int32_t k = plan.level_ptr[level]; k < plan.level_ptr[level + 1]; ++k) {
+      const int32_t s = plan.level_nodes[k];
+      for (int32_t p0 = 0; p0 < plan.ncol[s] + plan.nrow[s]; p0 += 64) {
+        g.task_s.push_back(s);
+        g.task_p0.push_back(p0);
+      }
+    }
+    g.task_ptr[level + 1] = static_cast<int32_t>(g.task_s.size());
+  }
+  return g;
+}
+
+void validate_sweeps(const Plan& plan) {
+  auto fail = [](const std::string& what) { throw Error{HG_ERR_RUNTIME, "plan check: " + what}; };
+  const int64_t S = plan.n_super;
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool forward = dir == 0;
+    const HostSweep h = sweep_tasks(plan, forward);
+    std::vector<int64_t> covered(S, 0);
+    for (size_t t = 0; t < h.s.size(); ++t) {
+      const int32_t s = h.s[t];
+      if (s < 0 || s >= S) fail("task front out of range");
+      const int32_t nc = plan.ncol[s], m = nc + plan.nrow[s];
+      const int32_t extent = forward ? m : nc;
+      if (h.tile[t] < 0 || h.tile[t] >= extent || h.tile[t] % 64) fail("task tile out of range");
+      if (h.k0[t] < 0 || h.k1[t] <= h.k0[t]) fail("empty or negative inner range");
+      if (forward && h.k1[t] > nc) fail("forward inner range past the front's columns");
+      if (!forward && (h.k1[t] > m || h.k0[t] < h.tile[t])) fail("backward inner range out of range");
+      if (h.slot[t] >= 0) {
+        int32_t level = plan.level[s];
+        if (h.slot[t] >= h.slots[level]) fail("slot past the level's partials");
+      }
+      covered[s] += h.k1[t] - h.k0[t];
+    }
+    // Each tile's inner range is covered exactly once.
+    for (int64_t s = 0; s < S; ++s) {
+      const int64_t nc = plan.ncol[s], m = nc + plan.nrow[s];
+      int64_t expect = 0;
+      const int64_t extent = forward ? m : nc;
+      for (int64_t tile = 0; tile < extent; tile += 64)
+        expect += forward ? ((tile + 64 <= nc) ? tile + 64 : nc) : (m - tile);
+      if (covered[s] != expect) fail("inner ranges do not tile the products");
+    }
+    for (size_t g = 0; g < h.rs.size(); ++g) {
+      const int32_t s = h.rs[g];
+      if (s < 0 || s >= S) fail("reduction front out of range");
+      if (h.rcount[g] < 2 || h.rfirst[g] < 0 || h.rfirst[g] + h.rcount[g] > h.slots[plan.level[s]])
+        fail("reduction slots out of range");
+    }
+  }
+  const HostGather g = gather_lists(plan);
+  if (static_cast<int64_t>(g.ptr.size()) != plan.vec_size + 1) fail("gather offsets size");
+  for (int32_t v : g.src)
+    if (v < 0 || v >= plan.vec_size) fail("gather source out of the front-vector arena");
+  for (size_t t = 0; t < g.task_s.size(); ++t) {
+    const int32_t s = g.task_s[t];
+    if (s < 0 || s >= S || g.task_p0[t] >= plan.ncol[s] + plan.nrow[s]) fail("gather task range");
+  }
+  // Every structure row of every child lands in exactly one parent position.
+  int64_t rows_with_parent = 0;
+  for (int64_t s = 0; s < S; ++s)
+    if (plan.parent[s] >= 0) rows_with_parent += plan.nrow[s];
+  if (static_cast<int64_t>(g.src.size()) != rows_with_parent) fail("gather list count");
+}
+
+namespace {
+
+void upload_sweep(const HostSweep& h, DevicePlan::SweepTasks& out, hipStream_t stream) {
+  out.level_ptr = h.level_ptr;
+  out.reduce_ptr = h.reduce_ptr;
+  out.slots = h.slots;
+  out.s.upload(h.s, stream);
+  out.tile.upload(h.tile, stream);
+  out.k0.upload(h.k0, stream);
+  out.k1.upload(h.k1, stream);
+  out.slot.upload(h.slot, stream);
+  out.rs.upload(h.rs, stream);
+  out.rtile.upload(h.rtile, stream);
+  out.rfirst.upload(h.rfirst, stream);
+  out.rcount.upload(h.rcount, stream);
+}
+
+}  // namespace
+
+void DevicePlan::upload(const Plan& plan, const HostPattern& pattern, hipStream_t stream) {
+  const HostSweep hf = sweep_tasks(plan, true);
+  const HostSweep hb = sweep_tasks(plan, false);
+  max_slots = std::max(hf.max_slots, hb.max_slots);
+  upload_sweep(hf, fwd, stream);
+  upload_sweep(hb, bwd, stream);
+  {
+    const HostGather g = gather_lists(plan);
+    gather_ptr.upload(g.ptr, stream);
+    gather_src.upload(g.src, stream);
+    gather_task_ptr = g.task_ptr;
+    gather_s.upload(g.task_s, stream);
+    gather_p0.upload(g.task_p0, stream);
+  }
+  // The operator's pattern in the elimination order: row i is old row perm[i], its columns
+  // renumbered and sorted; src names each entry's slot in the original values.
+  {
+    const int64_t nn = plan.n;
+    std::vector<int32_t> offs(nn + 1, 0), cols, src;
+    cols.reserve(pattern.columns.size());
+    src.reserve(pattern.columns.size());
+    std::vector<std::pair<int32_t, int32_t>> row;
+    for (int64_t i = 0; i < nn; ++i) {
+      const int32_t old = plan.perm[i];
+      row.clear();
+      for (int32_t e = pattern.offsets[old]; e < pattern.offsets[old + 1]; ++e)
+        row.push_back({plan.iperm[pattern.columns[e]], e});
+      std::sort(row.begin(), row.end());
+      for (const auto& entry : row) {
+        cols.push_back(entry.first);
+        src.push_back(entry.second);
+      }
+      offs[i + 1] = static_cast<int32_t>(cols.size());
+    }
+    nnz = static_cast<int64_t>(cols.size());
+    p_offsets.upload(offs, stream);
+    p_columns.upload(cols, stream);
+    p_src.upload(src, stream);
+  }
   n = plan.n;
   n_super = plan.n_super;
   n_levels = plan.n_levels;
@@ -353,7 +405,10 @@ int64_t DevicePlan::bytes() const {
   return perm.bytes() + iperm.bytes() + col_begin.bytes() + ncol.bytes() + nrow.bytes() +
          row_ptr.bytes() + rows.bytes() + child_ptr.bytes() + children.bytes() + rel.bytes() +
          level_nodes.bytes() + front_off.bytes() + w_off.bytes() + vec_off.bytes() +
-         entry_src.bytes() + entry_dst.bytes() + entry_row.bytes() + entry_col.bytes();
+         entry_src.bytes() + entry_dst.bytes() + entry_row.bytes() + entry_col.bytes() +
+         fwd.s.bytes() * 5 + fwd.rs.bytes() * 4 + bwd.s.bytes() * 5 + bwd.rs.bytes() * 4 +
+         gather_ptr.bytes() + gather_src.bytes() + gather_s.bytes() + gather_p0.bytes() +
+         p_offsets.bytes() + p_columns.bytes() + p_src.bytes();
 }
 
 void factor_numeric(const DevicePlan& plan, const double* values, const int32_t* pinned_new,
@@ -402,6 +457,12 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
     factor.w64.reset();
     throw Error{HG_ERR_NOT_PD, "the operator met a pivot that is not positive"};
   }
+  factor.pvalues.allocate(plan.nnz);
+  if (plan.nnz > 0) {
+    permute_values_kernel<<<blocks_for(plan.nnz), 256, 0, stream>>>(values, plan.p_src.ptr,
+                                                                   plan.nnz, factor.pvalues.ptr);
+    HG_HIP(hipGetLastError());
+  }
   if (keep_fp32) {
     factor.w32.allocate(plan.w_size);
     if (plan.w_size > 0) {
@@ -412,104 +473,6 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
   } else {
     factor.w32.reset();
   }
-}
-
-void SolveWorkspace::reserve(const DevicePlan& plan, int64_t k) {
-  const int64_t entries = plan.n * k;
-  if (capacity < entries) {
-    r.allocate(entries);
-    rp.allocate(entries);
-    y.allocate(entries);
-    xp.allocate(entries);
-    capacity = entries;
-  }
-  if (fcap < plan.vec_size * k) {
-    fvec.allocate(std::max<int64_t>(plan.vec_size * k, 1));
-    fcap = plan.vec_size * k;
-  }
-  if (bmax.size < k) bmax.allocate(std::max<int64_t>(k, 64));
-  if (pending.size < 1) pending.allocate(1);
-}
-
-template <typename T>
-static void triangular_solve(const DevicePlan& plan, const T* w, const double* rp, double* y,
-                             double* xp, double* fvec, int64_t k, hipStream_t stream) {
-  const int64_t n = plan.n;
-  for (int32_t level = 0; level < plan.n_levels; ++level) {
-    const int32_t first = plan.level_ptr[level];
-    const int32_t count = plan.level_ptr[level + 1] - first;
-    if (count == 0) continue;
-    forward_level_kernel<T><<<count, kFrontThreads, 0, stream>>>(
-        plan.level_nodes.ptr + first, plan.col_begin.ptr, plan.ncol.ptr, plan.nrow.ptr,
-        plan.child_ptr.ptr, plan.children.ptr, plan.row_ptr.ptr, plan.rel.ptr, plan.w_off.ptr,
-        plan.vec_off.ptr, w, rp, y, fvec, n, static_cast<int32_t>(k));
-  }
-  for (int32_t level = plan.n_levels - 1; level >= 0; --level) {
-    const int32_t first = plan.level_ptr[level];
-    const int32_t count = plan.level_ptr[level + 1] - first;
-    if (count == 0) continue;
-    backward_level_kernel<T><<<count, kFrontThreads, 0, stream>>>(
-        plan.level_nodes.ptr + first, plan.col_begin.ptr, plan.ncol.ptr, plan.nrow.ptr,
-        plan.row_ptr.ptr, plan.rows.ptr, plan.w_off.ptr, w, y, xp, n, static_cast<int32_t>(k));
-  }
-  HG_HIP(hipGetLastError());
-}
-
-int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp32,
-                      const int32_t* offsets, const int32_t* columns, const double* values,
-                      const double* b, double* x, int64_t k, double tolerance,
-                      bool componentwise, int32_t max_rounds, const int32_t* pinned_new,
-                      const int32_t* dropped_old, SolveWorkspace& ws, hipStream_t stream,
-                      PhaseTimer* timer, int sweep_phase) {
-  const int64_t n = plan.n;
-  if (n == 0 || k == 0) return 0;
-  ws.reserve(plan, k);
-  const unsigned grid = blocks_for(n * k);
-  HG_HIP(hipMemsetAsync(x, 0, sizeof(double) * n * k, stream));
-  HG_HIP(hipMemcpyAsync(ws.r.ptr, b, sizeof(double) * n * k, hipMemcpyDeviceToDevice, stream));
-  if (!componentwise) {
-    column_absmax_kernel<<<static_cast<unsigned>(k), kFrontThreads, 0, stream>>>(b, n, ws.bmax.ptr);
-  }
-  int32_t rounds = 0;
-  for (; rounds < max_rounds;) {
-    {
-      PhaseScope scope(timer, sweep_phase, stream);
-      gather_perm_kernel<<<grid, 256, 0, stream>>>(ws.r.ptr, plan.perm.ptr, pinned_new, n, k,
-                                                   ws.rp.ptr);
-      if (use_fp32) {
-        triangular_solve<float>(plan, factor.w32.ptr, ws.rp.ptr, ws.y.ptr, ws.xp.ptr, ws.fvec.ptr,
-                                k, stream);
-      } else {
-        triangular_solve<double>(plan, factor.w64.ptr, ws.rp.ptr, ws.y.ptr, ws.xp.ptr,
-                                 ws.fvec.ptr, k, stream);
-      }
-      scatter_add_perm_kernel<<<grid, 256, 0, stream>>>(ws.xp.ptr, plan.perm.ptr, n, k, x);
-    }
-    ++rounds;
-    {
-      PhaseScope scope(timer, kPhaseResidual, stream);
-      HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
-      residual_test_kernel<<<grid, 256, 0, stream>>>(offsets, columns, values, x, b, dropped_old,
-                                                     ws.bmax.ptr, n, k, tolerance,
-                                                     componentwise ? 1 : 0, ws.r.ptr,
-                                                     ws.pending.ptr);
-    }
-    HG_HIP(hipGetLastError());
-    int32_t pending = 0;
-    HG_HIP(hipMemcpyAsync(&pending, ws.pending.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-    HG_HIP(hipStreamSynchronize(stream));
-    if (timer) timer->resolve();
-    if (!pending) break;
-  }
-  return rounds;
-}
-
-void remove_null_space(const int32_t* comp_ptr, const int32_t* members, int32_t n_comp, int64_t n,
-                       int64_t k, double* x, hipStream_t stream) {
-  if (n_comp == 0 || k == 0) return;
-  dim3 grid(static_cast<unsigned>(n_comp), static_cast<unsigned>(k));
-  null_space_kernel<<<grid, kFrontThreads, 0, stream>>>(comp_ptr, members, n, x);
-  HG_HIP(hipGetLastError());
 }
 
 }  // namespace hg

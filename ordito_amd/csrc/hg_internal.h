@@ -139,4 +139,26 @@ struct AnalysisOptions {
 
 Plan analyse(const HostPattern& pattern, const float* coordinates, const AnalysisOptions& options);
 
+// The triangular sweeps' product tasks of one direction (cholesky.hip): per level, every 64-wide
+// tile of every front and its inner range [k0, k1); slot >= 0 names a partial that the level's
+// reduction groups (rs, rtile, rfirst, rcount) add in slot order.
+struct HostSweep {
+  std::vector<int32_t> level_ptr, reduce_ptr, slots;
+  std::vector<int32_t> s, tile, k0, k1, slot;
+  std::vector<int32_t> rs, rtile, rfirst, rcount;
+  int32_t max_slots = 0;
+};
+HostSweep sweep_tasks(const Plan& plan, bool forward);
+
+// The forward sweep's gather lists: for each front-vector row (vec_off[s] + position) the rows of
+// the children's update vectors that add into it, and the gather tasks (front, 64-position chunk).
+struct HostGather {
+  std::vector<int32_t> ptr, src, task_ptr, task_s, task_p0;
+};
+HostGather gather_lists(const Plan& plan);
+
+// Checks every index the device sweeps form from the plan against its buffer; throws on the first
+// violation.
+void validate_sweeps(const Plan& plan);
+
 }  // namespace hg

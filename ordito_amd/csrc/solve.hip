@@ -1,0 +1,690 @@
+// Refined multi-column solves through the partitioned-inverse factor, level by level.
+//
+// The whole refinement (ordito kernels/cholesky.py:1357-1463, cholesky.py:1606-1902) runs in the
+// elimination order with the k right-hand sides of a chunk interleaved ([row][k]): the operator is
+// held permuted (P A P^T), so the right-hand side is permuted in once, the solution out once, and
+// every round is forward sweep, backward sweep, residual -- no per-round permutation. Each level of
+// a sweep is a batch of dense tile products over every front of the level:
+//   forward : y(cols) = W11 f(cols) and the front's update vector u = f(rows) - W21 f(cols), where
+//             f = [r(cols); 0] + the children's update vectors, gathered as the tile loads it
+//             (children in ascending order);
+//   backward: d(cols) = W11^T y(cols) - W21^T d(rows), and the next iterate x + d beside it.
+// A product task is one 64-wide tile of one front and a range of the inner dimension; a level with
+// too few tiles to fill the device splits the inner dimension into partial slots, added in slot
+// order by a reduction, so every sum keeps one fixed order. Tiles stage the solve block and the
+// right-hand sides through LDS and accumulate in FP64 (FP32 solve blocks widen as they load).
+#include "cholesky.h"
+
+namespace hg {
+namespace {
+
+constexpr int kTile = 64;   // rows (forward) or columns (backward) of W per tile
+constexpr int kChunk = 32;  // inner dimension staged per LDS round
+constexpr int kCols = 64;   // right-hand sides per solve chunk
+constexpr int kReduceSplit = 4;  // blocks a reduction group is spread over (16 rows each)
+
+template <typename T>
+__device__ __forceinline__ double widen(T v) {
+  return static_cast<double>(v);
+}
+
+// Gathered front-vector entry: base plus the children's update rows adding into this position.
+__device__ __forceinline__ double gathered(double base, int64_t row, int32_t q, int32_t kk,
+                                           const int32_t* __restrict__ gather_ptr,
+                                           const int32_t* __restrict__ gather_src,
+                                           const double* __restrict__ fvec) {
+  double v = base;
+  for (int32_t g = gather_ptr[row]; g < gather_ptr[row + 1]; ++g)
+    v += fvec[static_cast<int64_t>(gather_src[g]) * kk + q];
+  return v;
+}
+
+// b (user layout, column q at q*n) -> bp[i*kk + q] in elimination order through an LDS transpose
+// of 64 rows; a pinned row's equation is dropped (zero). Also starts x = 0 and r = bp.
+__global__ __launch_bounds__(256) void perm_in_kernel(const double* __restrict__ b,
+                                                      const int32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ pinned,
+                                                      int64_t n, int32_t kk,
+                                                      double* __restrict__ bp,
+                                                      double* __restrict__ x,
+                                                      double* __restrict__ r) {
+  __shared__ double tile[kTile][kCols + 1];
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTile;
+  const int ii = threadIdx.x % kTile, qq = threadIdx.x / kTile;
+  const int64_t i = i0 + ii;
+  const bool live = i < n;
+  const int64_t src = live ? perm[i] : 0;
+  const bool drop = !live || (pinned && pinned[i]);
+  for (int q = qq; q < kk; q += 4) tile[ii][q] = drop ? 0.0 : b[q * n + src];
+  __syncthreads();
+  const int32_t rows = static_cast<int32_t>(min<int64_t>(kTile, n - i0));
+  for (int e = threadIdx.x; e < rows * kk; e += 256) {
+    const int r_ = e / kk, q = e % kk;
+    const int64_t slot = (i0 + r_) * kk + q;
+    const double v = tile[r_][q];
+    bp[slot] = v;
+    r[slot] = v;
+    x[slot] = 0.0;
+  }
+}
+
+// out (user layout) [q*n + perm[i]] = x[i*kk + q], through the same transpose.
+__global__ __launch_bounds__(256) void perm_out_kernel(const double* __restrict__ x,
+                                                       const int32_t* __restrict__ perm,
+                                                       int64_t n, int32_t kk,
+                                                       double* __restrict__ out) {
+  __shared__ double tile[kTile][kCols + 1];
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTile;
+  const int32_t rows = static_cast<int32_t>(min<int64_t>(kTile, n - i0));
+  for (int e = threadIdx.x; e < rows * kk; e += 256) tile[e / kk][e % kk] = x[i0 * kk + e];
+  __syncthreads();
+  const int ii = threadIdx.x % kTile, qq = threadIdx.x / kTile;
+  if (ii >= rows) return;
+  const int64_t dst = perm[i0 + ii];
+  for (int q = qq; q < kk; q += 4) out[q * n + dst] = tile[ii][q];
+}
+
+struct SweepArgs {
+  const int32_t* col_begin;
+  const int32_t* ncol;
+  const int32_t* nrow;
+  const int32_t* row_ptr;
+  const int32_t* rows;
+  const int64_t* w_off;
+  const int64_t* vec_off;
+  const int32_t* gather_ptr;
+  const int32_t* gather_src;
+  const double* r;   // residual, the sweep's right-hand side
+  double* fvec;      // update vectors (rows part of each front's vector)
+  double* y;         // forward result
+  double* d;         // backward result (the correction)
+  const double* x;   // current iterate
+  double* x_next;    // x + d
+  double* partials;
+  int32_t kk;
+};
+
+// Forward tile product for up to 64 right-hand sides: rows [r0, r0+64) of the front's solve block
+// times f over the inner range [k0, k1), f gathered as it is staged. Thread (tc, tr) =
+// (t % 16, t / 16) owns rows tr + 16a and right-hand sides tc + 16b. slot < 0 applies the product;
+// otherwise it is stored as a partial for the reduction.
+template <typename TW>
+__global__ __launch_bounds__(256) void fwd_tile_kernel(
+    const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_r0,
+    const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
+    const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
+  __shared__ double Ws[kChunk][kTile];
+  __shared__ double Fs[kChunk][kCols];
+  const int32_t s = task_s[blockIdx.x];
+  const int32_t r0 = task_r0[blockIdx.x];
+  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
+  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
+  const int32_t m = nc + a.nrow[s];
+  const int32_t kk = a.kk;
+  const TW* W = w + a.w_off[s];
+  const int64_t vbase = a.vec_off[s];
+  const int t = threadIdx.x;
+  const int tc = t % 16, tr = t / 16;
+  double acc[4][4] = {};
+  for (int32_t j0 = k0; j0 < k1; j0 += kChunk) {
+    for (int u = 0; u < (kChunk * kTile) / 256; ++u) {
+      const int e = t + 256 * u;
+      const int ii = e % kTile, jj = e / kTile;
+      const int32_t i = r0 + ii, j = j0 + jj;
+      Ws[jj][ii] = (i < m && j < k1) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
+    }
+    for (int u = 0; u < (kChunk * kCols) / 256; ++u) {
+      const int e = t + 256 * u;
+      const int q = e % kCols, jj = e / kCols;
+      const int32_t j = j0 + jj;
+      double v = 0.0;
+      if (q < kk && j < k1)
+        v = gathered(a.r[static_cast<int64_t>(c0 + j) * kk + q], vbase + j, q, kk, a.gather_ptr,
+                     a.gather_src, a.fvec);
+      Fs[jj][q] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int jj = 0; jj < kChunk; ++jj) {
+      double av[4], bv[4];
+      for (int x = 0; x < 4; ++x) av[x] = Ws[jj][tr + 16 * x];
+      for (int x = 0; x < 4; ++x) bv[x] = Fs[jj][tc + 16 * x];
+      for (int x = 0; x < 4; ++x)
+        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
+    }
+    __syncthreads();
+  }
+  for (int x = 0; x < 4; ++x) {
+    const int32_t ii = tr + 16 * x;
+    const int32_t i = r0 + ii;
+    if (i >= m) continue;
+    for (int z = 0; z < 4; ++z) {
+      const int q = tc + 16 * z;
+      if (q >= kk) continue;
+      if (slot >= 0) {
+        a.partials[(static_cast<int64_t>(slot) * kTile + ii) * kk + q] = acc[x][z];
+      } else if (i < nc) {
+        a.y[static_cast<int64_t>(c0 + i) * kk + q] = acc[x][z];
+      } else {
+        const int64_t row = vbase + i;
+        a.fvec[row * kk + q] =
+            gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) - acc[x][z];
+      }
+    }
+  }
+}
+
+// Adds a split forward tile's partials in slot order and applies the sum; blockIdx.y picks 16 of
+// the tile's 64 rows.
+__global__ __launch_bounds__(256) void fwd_reduce_kernel(
+    const int32_t* __restrict__ grp_s, const int32_t* __restrict__ grp_r0,
+    const int32_t* __restrict__ grp_first, const int32_t* __restrict__ grp_count, SweepArgs a) {
+  const int32_t s = grp_s[blockIdx.x], r0 = grp_r0[blockIdx.x];
+  const int32_t first = grp_first[blockIdx.x], count = grp_count[blockIdx.x];
+  const int32_t nc = a.ncol[s], m = nc + a.nrow[s], c0 = a.col_begin[s];
+  const int32_t kk = a.kk;
+  const int64_t vbase = a.vec_off[s];
+  constexpr int kRows = kTile / kReduceSplit;
+  const int32_t lo = blockIdx.y * kRows;
+  const int32_t rows = min(kRows, m - r0 - lo);
+  for (int32_t e = threadIdx.x; e < rows * kk; e += blockDim.x) {
+    const int32_t ii = lo + e / kk, q = e % kk;
+    double sum = 0.0;
+    for (int32_t c = 0; c < count; ++c)
+      sum += a.partials[(static_cast<int64_t>(first + c) * kTile + ii) * kk + q];
+    const int32_t i = r0 + ii;
+    if (i < nc) {
+      a.y[static_cast<int64_t>(c0 + i) * kk + q] = sum;
+    } else {
+      const int64_t row = vbase + i;
+      a.fvec[row * kk + q] = gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) - sum;
+    }
+  }
+}
+
+// Backward tile product: solve-block columns [j0, j0+64) transposed, times
+// g = [y(cols); -d(rows)] over the inner range [k0, k1). Thread (tc, tr) owns solve-block columns
+// tr + 16a and right-hand sides tc + 16b. Applied results also give x_next = x + d.
+template <typename TW>
+__global__ __launch_bounds__(256) void bwd_tile_kernel(
+    const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_j0,
+    const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
+    const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
+  __shared__ double Ws[kChunk][kTile];
+  __shared__ double Gs[kChunk][kCols];
+  const int32_t s = task_s[blockIdx.x];
+  const int32_t j0 = task_j0[blockIdx.x];
+  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
+  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
+  const int32_t m = nc + a.nrow[s];
+  const int32_t kk = a.kk;
+  const TW* W = w + a.w_off[s];
+  const int32_t* srows = a.rows + a.row_ptr[s];
+  const int t = threadIdx.x;
+  const int tc = t % 16, tr = t / 16;
+  double acc[4][4] = {};
+  for (int32_t i0 = k0; i0 < k1; i0 += kChunk) {
+    for (int u = 0; u < (kChunk * kTile) / 256; ++u) {
+      const int e = t + 256 * u;
+      const int ii = e % kChunk, jj = e / kChunk;
+      const int32_t i = i0 + ii, j = j0 + jj;
+      Ws[ii][jj] = (i < k1 && j < nc) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
+    }
+    for (int u = 0; u < (kChunk * kCols) / 256; ++u) {
+      const int e = t + 256 * u;
+      const int q = e % kCols, ii = e / kCols;
+      const int32_t i = i0 + ii;
+      double g = 0.0;
+      if (q < kk && i < k1) {
+        g = i < nc ? a.y[static_cast<int64_t>(c0 + i) * kk + q]
+                   : -a.d[static_cast<int64_t>(srows[i - nc]) * kk + q];
+      }
+      Gs[ii][q] = g;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int ii = 0; ii < kChunk; ++ii) {
+      double av[4], bv[4];
+      for (int x = 0; x < 4; ++x) av[x] = Ws[ii][tr + 16 * x];
+      for (int x = 0; x < 4; ++x) bv[x] = Gs[ii][tc + 16 * x];
+      for (int x = 0; x < 4; ++x)
+        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
+    }
+    __syncthreads();
+  }
+  for (int x = 0; x < 4; ++x) {
+    const int32_t jj = tr + 16 * x;
+    const int32_t j = j0 + jj;
+    if (j >= nc) continue;
+    for (int z = 0; z < 4; ++z) {
+      const int q = tc + 16 * z;
+      if (q >= kk) continue;
+      if (slot >= 0) {
+        a.partials[(static_cast<int64_t>(slot) * kTile + jj) * kk + q] = acc[x][z];
+      } else {
+        const int64_t at = static_cast<int64_t>(c0 + j) * kk + q;
+        a.d[at] = acc[x][z];
+        a.x_next[at] = a.x[at] + acc[x][z];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void bwd_reduce_kernel(
+    const int32_t* __restrict__ grp_s, const int32_t* __restrict__ grp_j0,
+    const int32_t* __restrict__ grp_first, const int32_t* __restrict__ grp_count, SweepArgs a) {
+  const int32_t s = grp_s[blockIdx.x], j0 = grp_j0[blockIdx.x];
+  const int32_t first = grp_first[blockIdx.x], count = grp_count[blockIdx.x];
+  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
+  const int32_t kk = a.kk;
+  constexpr int kRows = kTile / kReduceSplit;
+  const int32_t lo = blockIdx.y * kRows;
+  const int32_t cols = min(kRows, nc - j0 - lo);
+  for (int32_t e = threadIdx.x; e < cols * kk; e += blockDim.x) {
+    const int32_t jj = lo + e / kk, q = e % kk;
+    double sum = 0.0;
+    for (int32_t c = 0; c < count; ++c)
+      sum += a.partials[(static_cast<int64_t>(first + c) * kTile + jj) * kk + q];
+    const int64_t at = static_cast<int64_t>(c0 + j0 + jj) * kk + q;
+    a.d[at] = sum;
+    a.x_next[at] = a.x[at] + sum;
+  }
+}
+
+// One right-hand side: f gathered into LDS 256 entries at a time, then four lanes a row split the
+// inner range (j = lane mod 4), folded in lane order.
+template <typename TW>
+__global__ __launch_bounds__(256) void fwd_tile1_kernel(
+    const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_r0,
+    const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
+    const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
+  __shared__ double Fs[256];
+  __shared__ double part[4][kTile];
+  const int32_t s = task_s[blockIdx.x];
+  const int32_t r0 = task_r0[blockIdx.x];
+  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
+  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
+  const int32_t m = nc + a.nrow[s];
+  const TW* W = w + a.w_off[s];
+  const int64_t vbase = a.vec_off[s];
+  const int ii = threadIdx.x % kTile, lane = threadIdx.x / kTile;
+  const int32_t i = r0 + ii;
+  double acc = 0.0;
+  for (int32_t j0 = k0; j0 < k1; j0 += 256) {
+    const int32_t j = j0 + threadIdx.x;
+    Fs[threadIdx.x] =
+        j < k1 ? gathered(a.r[c0 + j], vbase + j, 0, 1, a.gather_ptr, a.gather_src, a.fvec) : 0.0;
+    __syncthreads();
+    const int32_t jend = min(k1 - j0, 256);
+    if (i < m)
+      for (int32_t jj = lane; jj < jend; jj += 4)
+        acc += widen(W[static_cast<int64_t>(j0 + jj) * m + i]) * Fs[jj];
+    __syncthreads();
+  }
+  part[lane][ii] = acc;
+  __syncthreads();
+  if (lane == 0 && i < m) {
+    const double sum = ((part[0][ii] + part[1][ii]) + part[2][ii]) + part[3][ii];
+    if (slot >= 0) {
+      a.partials[static_cast<int64_t>(slot) * kTile + ii] = sum;
+    } else if (i < nc) {
+      a.y[c0 + i] = sum;
+    } else {
+      const int64_t row = vbase + i;
+      a.fvec[row] = gathered(0.0, row, 0, 1, a.gather_ptr, a.gather_src, a.fvec) - sum;
+    }
+  }
+}
+
+template <typename TW>
+__global__ __launch_bounds__(256) void bwd_tile1_kernel(
+    const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_j0,
+    const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
+    const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
+  __shared__ double part[kTile][5];
+  const int32_t s = task_s[blockIdx.x];
+  const int32_t j0 = task_j0[blockIdx.x];
+  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
+  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
+  const int32_t m = nc + a.nrow[s];
+  const TW* W = w + a.w_off[s];
+  const int32_t* srows = a.rows + a.row_ptr[s];
+  // Four lanes a solve-block column, each striding the column's rows (contiguous in memory).
+  const int lane = threadIdx.x % 4, jj = threadIdx.x / 4;
+  const int32_t j = j0 + jj;
+  double acc = 0.0;
+  if (j < nc) {
+    const TW* wj = W + static_cast<int64_t>(j) * m;
+    for (int32_t i = max(k0, j) + lane; i < k1; i += 4) {
+      const double g = i < nc ? a.y[c0 + i] : -a.d[srows[i - nc]];
+      acc += widen(wj[i]) * g;
+    }
+  }
+  part[jj][lane] = acc;
+  __syncthreads();
+  if (lane == 0 && j < nc) {
+    const double sum = ((part[jj][0] + part[jj][1]) + part[jj][2]) + part[jj][3];
+    if (slot >= 0) {
+      a.partials[static_cast<int64_t>(slot) * kTile + jj] = sum;
+    } else {
+      a.d[c0 + j] = sum;
+      a.x_next[c0 + j] = a.x[c0 + j] + sum;
+    }
+  }
+}
+
+// Raises *flag once per block that fails, and not at all once the flag reads set: with every row
+// failing in a solve's first rounds, an atomic a wave on one word serializes the whole launch.
+// Block-collective.
+__device__ __forceinline__ void raise_flag(bool fail, int32_t* flag) {
+  const int any = __syncthreads_or(fail ? 1 : 0);
+  if (any && threadIdx.x == 0 &&
+      __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    atomicOr(flag, 1);
+}
+
+// r = b - A x in the elimination order and whether any row is still above tolerance
+// (componentwise against (|A||x| + |b|)_i, normwise against the larger of that and the column's
+// largest |b|; kernels/cholesky.py:1393-1440). Pinned rows are dropped. One thread a (row,
+// right-hand side); the row's operator entries are shared by its kk threads.
+__global__ void residual_kernel(const int32_t* __restrict__ offsets,
+                                const int32_t* __restrict__ columns,
+                                const double* __restrict__ values, const double* __restrict__ b,
+                                const double* __restrict__ x, const int32_t* __restrict__ pinned,
+                                const double* __restrict__ bmax, int64_t n, int32_t kk,
+                                double tolerance, int32_t componentwise, double* __restrict__ r,
+                                int32_t* __restrict__ pending) {
+  int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool fail = false;
+  if (t < n * kk) {
+    const int64_t i = t / kk, q = t % kk;
+    const double bi = b[t];
+    double total = bi, scale = fabs(bi);
+    for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+      const double term = values[e] * x[static_cast<int64_t>(columns[e]) * kk + q];
+      total -= term;
+      scale += fabs(term);
+    }
+    const bool dropped = pinned && pinned[i];
+    r[t] = dropped ? 0.0 : total;
+    if (!componentwise) scale = fmax(scale, bmax[q]);
+    fail = !dropped && !(fabs(total) <= tolerance * scale);
+  }
+  raise_flag(fail, pending);
+}
+
+// Per-column partial maxima of |b| (user layout), one block a (chunk, column).
+__global__ void absmax_partial_kernel(const double* __restrict__ b, int64_t n,
+                                      double* __restrict__ partials) {
+  __shared__ double scratch[256];
+  const int64_t q = blockIdx.y;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  scratch[threadIdx.x] = i < n ? fabs(b[q * n + i]) : 0.0;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) scratch[threadIdx.x] = fmax(scratch[threadIdx.x], scratch[threadIdx.x + stride]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[q * gridDim.x + blockIdx.x] = scratch[0];
+}
+
+__global__ void absmax_final_kernel(const double* __restrict__ partials, int32_t blocks,
+                                    double* __restrict__ out) {
+  __shared__ double scratch[256];
+  const int64_t q = blockIdx.x;
+  double v = 0.0;
+  for (int32_t b = threadIdx.x; b < blocks; b += 256) v = fmax(v, partials[q * blocks + b]);
+  scratch[threadIdx.x] = v;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) scratch[threadIdx.x] = fmax(scratch[threadIdx.x], scratch[threadIdx.x + stride]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = scratch[0];
+}
+
+// Component sums for the null-space projection: block partials per (component chunk, column).
+__global__ void component_partial_kernel(const int32_t* __restrict__ chunk_comp,
+                                         const int32_t* __restrict__ chunk_first,
+                                         const int32_t* __restrict__ comp_ptr,
+                                         const int32_t* __restrict__ members, int64_t n,
+                                         const double* __restrict__ x,
+                                         double* __restrict__ partials) {
+  __shared__ double scratch[256];
+  const int32_t chunk = blockIdx.x;
+  const int64_t q = blockIdx.y;
+  const int32_t c = chunk_comp[chunk];
+  const int32_t e = chunk_first[chunk] + threadIdx.x;
+  scratch[threadIdx.x] = e < comp_ptr[c + 1] ? x[q * n + members[e]] : 0.0;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) scratch[threadIdx.x] += scratch[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[q * gridDim.x + chunk] = scratch[0];
+}
+
+// Each (component, column)'s mean, from its chunks' partials in chunk order.
+__global__ void component_mean_kernel(const int32_t* __restrict__ comp_chunk_ptr,
+                                      const int32_t* __restrict__ comp_ptr, int32_t n_chunks,
+                                      const double* __restrict__ partials,
+                                      double* __restrict__ means) {
+  __shared__ double scratch[256];
+  const int32_t c = blockIdx.x;
+  const int64_t q = blockIdx.y;
+  double sum = 0.0;
+  for (int32_t k = comp_chunk_ptr[c] + threadIdx.x; k < comp_chunk_ptr[c + 1]; k += 256)
+    sum += partials[q * n_chunks + k];
+  scratch[threadIdx.x] = sum;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) scratch[threadIdx.x] += scratch[threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    means[q * gridDim.x + c] = scratch[0] / static_cast<double>(comp_ptr[c + 1] - comp_ptr[c]);
+}
+
+__global__ void component_shift_kernel(const int32_t* __restrict__ chunk_comp,
+                                       const int32_t* __restrict__ chunk_first,
+                                       const int32_t* __restrict__ comp_ptr,
+                                       const int32_t* __restrict__ members, int32_t n_comp,
+                                       int64_t n, const double* __restrict__ means,
+                                       double* __restrict__ x) {
+  const int32_t chunk = blockIdx.x;
+  const int64_t q = blockIdx.y;
+  const int32_t c = chunk_comp[chunk];
+  const int32_t e = chunk_first[chunk] + threadIdx.x;
+  if (e < comp_ptr[c + 1]) x[q * n + members[e]] -= means[q * n_comp + c];
+}
+
+inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+template <typename TW>
+void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream) {
+  const DevicePlan::SweepTasks& F = plan.fwd;
+  const DevicePlan::SweepTasks& B = plan.bwd;
+  for (int32_t level = 0; level < plan.n_levels; ++level) {
+    const int32_t t0 = F.level_ptr[level];
+    const int32_t tasks = F.level_ptr[level + 1] - t0;
+    if (tasks == 0) continue;
+    if (a.kk == 1) {
+      fwd_tile1_kernel<TW><<<tasks, 256, 0, stream>>>(F.s.ptr + t0, F.tile.ptr + t0,
+                                                      F.k0.ptr + t0, F.k1.ptr + t0,
+                                                      F.slot.ptr + t0, w, a);
+    } else {
+      fwd_tile_kernel<TW><<<tasks, 256, 0, stream>>>(F.s.ptr + t0, F.tile.ptr + t0,
+                                                     F.k0.ptr + t0, F.k1.ptr + t0,
+                                                     F.slot.ptr + t0, w, a);
+    }
+    const int32_t r0 = F.reduce_ptr[level];
+    const int32_t groups = F.reduce_ptr[level + 1] - r0;
+    if (groups > 0) {
+      fwd_reduce_kernel<<<dim3(groups, kReduceSplit), 256, 0, stream>>>(
+          F.rs.ptr + r0, F.rtile.ptr + r0, F.rfirst.ptr + r0, F.rcount.ptr + r0, a);
+    }
+  }
+  for (int32_t level = plan.n_levels - 1; level >= 0; --level) {
+    const int32_t t0 = B.level_ptr[level];
+    const int32_t tasks = B.level_ptr[level + 1] - t0;
+    if (tasks == 0) continue;
+    if (a.kk == 1) {
+      bwd_tile1_kernel<TW><<<tasks, 256, 0, stream>>>(B.s.ptr + t0, B.tile.ptr + t0,
+                                                      B.k0.ptr + t0, B.k1.ptr + t0,
+                                                      B.slot.ptr + t0, w, a);
+    } else {
+      bwd_tile_kernel<TW><<<tasks, 256, 0, stream>>>(B.s.ptr + t0, B.tile.ptr + t0,
+                                                     B.k0.ptr + t0, B.k1.ptr + t0,
+                                                     B.slot.ptr + t0, w, a);
+    }
+    const int32_t r0 = B.reduce_ptr[level];
+    const int32_t groups = B.reduce_ptr[level + 1] - r0;
+    if (groups > 0) {
+      bwd_reduce_kernel<<<dim3(groups, kReduceSplit), 256, 0, stream>>>(
+          B.rs.ptr + r0, B.rtile.ptr + r0, B.rfirst.ptr + r0, B.rcount.ptr + r0, a);
+    }
+  }
+  HG_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void SolveWorkspace::reserve(const DevicePlan& plan, int64_t kk) {
+  const int64_t entries = plan.n * kk;
+  if (capacity < entries) {
+    bp.allocate(entries);
+    xa.allocate(entries);
+    xb.allocate(entries);
+    r.allocate(entries);
+    y.allocate(entries);
+    d.allocate(entries);
+    capacity = entries;
+  }
+  if (fcap < plan.vec_size * kk) {
+    fvec.allocate(std::max<int64_t>(plan.vec_size * kk, 1));
+    fcap = plan.vec_size * kk;
+  }
+  const int64_t pentries = static_cast<int64_t>(plan.max_slots) * kTile * kk;
+  if (pcap < pentries) {
+    partials.allocate(std::max<int64_t>(pentries, 1));
+    pcap = pentries;
+  }
+  const int64_t absmax = static_cast<int64_t>(blocks_for(plan.n)) * kCols + kCols;
+  if (bmax.size < absmax) bmax.allocate(absmax);
+  if (pending.size < 1) pending.allocate(1);
+}
+
+int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp32,
+                      const double* b, double* x, int64_t k, double tolerance, bool componentwise,
+                      int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
+                      hipStream_t stream, PhaseTimer* timer, int sweep_phase) {
+  const int64_t n = plan.n;
+  if (n == 0 || k == 0) return 0;
+  int32_t max_used = 0;
+  for (int64_t q0 = 0; q0 < k; q0 += kCols) {
+    const int32_t kk = static_cast<int32_t>(std::min<int64_t>(kCols, k - q0));
+    ws.reserve(plan, kk);
+    const double* bq = b + q0 * n;
+    double* xq = x + q0 * n;
+    const unsigned grid = blocks_for(n * kk);
+    const unsigned row_tiles = static_cast<unsigned>((n + kTile - 1) / kTile);
+    perm_in_kernel<<<row_tiles, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk, ws.bp.ptr,
+                                                  ws.xa.ptr, ws.r.ptr);
+    double* bmax = ws.bmax.ptr + static_cast<int64_t>(blocks_for(n)) * kCols;
+    if (!componentwise) {
+      const unsigned blocks = blocks_for(n);
+      absmax_partial_kernel<<<dim3(blocks, kk), 256, 0, stream>>>(bq, n, ws.bmax.ptr);
+      absmax_final_kernel<<<kk, 256, 0, stream>>>(ws.bmax.ptr, static_cast<int32_t>(blocks), bmax);
+    }
+    SweepArgs a{};
+    a.col_begin = plan.col_begin.ptr;
+    a.ncol = plan.ncol.ptr;
+    a.nrow = plan.nrow.ptr;
+    a.row_ptr = plan.row_ptr.ptr;
+    a.rows = plan.rows.ptr;
+    a.w_off = plan.w_off.ptr;
+    a.vec_off = plan.vec_off.ptr;
+    a.gather_ptr = plan.gather_ptr.ptr;
+    a.gather_src = plan.gather_src.ptr;
+    a.r = ws.r.ptr;
+    a.fvec = ws.fvec.ptr;
+    a.y = ws.y.ptr;
+    a.d = ws.d.ptr;
+    a.partials = ws.partials.ptr;
+    a.kk = kk;
+    double* x_cur = ws.xa.ptr;
+    double* x_next = ws.xb.ptr;
+    int32_t rounds = 0;
+    while (rounds < max_rounds) {
+      a.x = x_cur;
+      a.x_next = x_next;
+      {
+        PhaseScope scope(timer, sweep_phase, stream);
+        if (use_fp32) sweep<float>(plan, factor.w32.ptr, a, stream);
+        else sweep<double>(plan, factor.w64.ptr, a, stream);
+      }
+      ++rounds;
+      std::swap(x_cur, x_next);
+      {
+        PhaseScope scope(timer, kPhaseResidual, stream);
+        HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
+        residual_kernel<<<grid, 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
+                                                  factor.pvalues.ptr, ws.bp.ptr, x_cur,
+                                                  pinned_new, bmax, n, kk, tolerance,
+                                                  componentwise ? 1 : 0, ws.r.ptr,
+                                                  ws.pending.ptr);
+      }
+      HG_HIP(hipGetLastError());
+      int32_t pending = 0;
+      HG_HIP(hipMemcpyAsync(&pending, ws.pending.ptr, sizeof(int32_t), hipMemcpyDeviceToHost,
+                            stream));
+      HG_HIP(hipStreamSynchronize(stream));
+      if (timer) timer->resolve();
+      if (!pending) break;
+    }
+    perm_out_kernel<<<row_tiles, 256, 0, stream>>>(x_cur, plan.perm.ptr, n, kk, xq);
+    HG_HIP(hipGetLastError());
+    max_used = std::max(max_used, rounds);
+  }
+  return max_used;
+}
+
+void NullSpace::build(const std::vector<int32_t>& comp_ptr_host, hipStream_t stream) {
+  n_comp = static_cast<int32_t>(comp_ptr_host.size()) - 1;
+  std::vector<int32_t> cc, cf, ccp(n_comp + 1, 0);
+  for (int32_t c = 0; c < n_comp; ++c) {
+    for (int32_t e = comp_ptr_host[c]; e < comp_ptr_host[c + 1]; e += 256) {
+      cc.push_back(c);
+      cf.push_back(e);
+    }
+    ccp[c + 1] = static_cast<int32_t>(cc.size());
+  }
+  n_chunks = static_cast<int32_t>(cc.size());
+  chunk_comp.upload(cc, stream);
+  chunk_first.upload(cf, stream);
+  comp_chunk_ptr.upload(ccp, stream);
+}
+
+void remove_null_space(const NullSpace& ns, const int32_t* comp_ptr, const int32_t* members,
+                       int64_t n, int64_t k, double* x, DeviceArray<double>& scratch,
+                       hipStream_t stream) {
+  if (ns.n_chunks == 0 || k == 0) return;
+  const int64_t need = static_cast<int64_t>(ns.n_chunks) * k + static_cast<int64_t>(ns.n_comp) * k;
+  if (scratch.size < need) scratch.allocate(need);
+  double* partials = scratch.ptr;
+  double* means = scratch.ptr + static_cast<int64_t>(ns.n_chunks) * k;
+  dim3 grid(static_cast<unsigned>(ns.n_chunks), static_cast<unsigned>(k));
+  component_partial_kernel<<<grid, 256, 0, stream>>>(ns.chunk_comp.ptr, ns.chunk_first.ptr,
+                                                     comp_ptr, members, n, x, partials);
+  component_mean_kernel<<<dim3(static_cast<unsigned>(ns.n_comp), static_cast<unsigned>(k)), 256, 0,
+                          stream>>>(ns.comp_chunk_ptr.ptr, comp_ptr, ns.n_chunks, partials, means);
+  component_shift_kernel<<<grid, 256, 0, stream>>>(ns.chunk_comp.ptr, ns.chunk_first.ptr,
+                                                   comp_ptr, members, ns.n_comp, n, means, x);
+  HG_HIP(hipGetLastError());
+}
+
+}  // namespace hg

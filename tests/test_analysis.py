@@ -193,3 +193,27 @@ def test_emulated_pinned_poisson_solve_matches_scipy():
     expected[keep] = spla.spsolve(reduced.tocsc(), b[keep])
     assert abs(x[pin_old]) == 0.0
     assert np.max(np.abs(x - expected)) <= 1e-8 * np.max(np.abs(expected))
+
+
+@pytest.mark.parametrize("name", ["icosphere3", "torus", "grid", "bumpy", "saddle_graded"])
+def test_sweep_tasks_stay_in_bounds(name):
+    """The library's own check of every device sweep index (task ranges, partial slots, gather
+    sources) against its buffers, run on the host."""
+    _plan_for_validated(name)
+
+
+def test_sweep_tasks_stay_in_bounds_at_benchmark_scale():
+    from ordito_amd import analysis, creation
+
+    vertices, faces = creation.torus(1024, 512, 1.0, 0.3, 0.1, 0)
+    pattern = ho.operator_pattern(len(vertices), faces)
+    analysis.analyse(pattern.indptr, pattern.indices, vertices, validate=True)
+
+
+def _plan_for_validated(name):
+    from ordito_amd import analysis
+
+    vertices, faces = mesh(name)
+    pattern = ho.operator_pattern(len(vertices), faces)
+    for leaf in (8, 48):
+        analysis.analyse(pattern.indptr, pattern.indices, vertices, leaf_rows=leaf, validate=True)
