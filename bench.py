@@ -11,7 +11,10 @@ solve, unit-gradient divergence, Poisson solve, shift -- all on the GPU, factors
 - ``value``: served (mixed-precision) queries/s over the timed steps.
 - ``plain_fp64``: the plain FP64 GPU path (FP64 factors, one query per call), timed beside it.
 - ``roofline``: the triangular sweep (forward + backward, the dominant kernel), algorithmic bytes
-  per sweep over its HIP-event-measured mean duration against the 8 TB/s HBM peak.
+  per sweep over its HIP-event-measured mean duration against the 8 TB/s HBM peak; ``traffic`` is
+  the HBM bytes per sweep that rocprofv3's FETCH_SIZE / WRITE_SIZE passes measured
+  (profiles/triangular_sweep_torus1024x512_traffic.json, scaled by a 1 GiB copy:
+  ``bench.py --calibrate-copy``), null when the sweep kernels' source has changed since.
 - ``cpu_baseline``: the FP64 oracle (SciPy SuperLU, one thread) on a bounded sample of the same
   queries on this host, factorization excluded as for the GPU; its results also give the error
   of the GPU fields.
@@ -48,7 +51,48 @@ def parse() -> argparse.Namespace:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--major", type=int, default=1024)
     parser.add_argument("--minor", type=int, default=512)
+    parser.add_argument(
+        "--calibrate-copy",
+        action="store_true",
+        help="only run three 1 GiB device copies (the reference kernel for scaling the FETCH_SIZE "
+        "and WRITE_SIZE counters) and exit",
+    )
     return parser.parse_args()
+
+
+def calibrate_copy() -> int:
+    import torch
+
+    x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 1 GiB
+    x.fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 device copies of 1 GiB")
+    return 0
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "triangular_sweep_torus1024x512_traffic.json"
+
+
+def measured_traffic(columns: int, solve_block_entries: int) -> float | None:
+    """HBM bytes per sweep from the committed FETCH_SIZE / WRITE_SIZE passes, when they were taken
+    on this workload with the sweep kernels' current source; otherwise None."""
+    import hashlib
+
+    try:
+        record = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    source = (ROOT / "ordito_amd" / "csrc" / "solve.hip").read_bytes()
+    if (
+        record.get("solve_hip_sha256_16") != hashlib.sha256(source).hexdigest()[:16]
+        or record.get("columns") != columns
+        or record.get("solve_block_entries") != solve_block_entries
+    ):
+        return None
+    return float(record["traffic_bytes_per_sweep"])
 
 
 def log(message: str) -> None:
@@ -83,6 +127,8 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
     torch.cuda.init()
+    if args.calibrate_copy:
+        return calibrate_copy()
     log(f"== generating torus {args.major}x{args.minor}")
     vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)
     n = vertices.shape[0]
@@ -155,6 +201,7 @@ def main() -> int:
     mean_sweep_s = sweep_ms / max(sweep_count, 1) / 1e3
     bytes_per_sweep = sweep_bytes(stats, len(source_sets), 4)
     achieved = bytes_per_sweep / mean_sweep_s / 1e9 if mean_sweep_s > 0 else 0.0
+    traffic = measured_traffic(len(source_sets), stats["solve_block_entries"])
     roofline = {
         "bound": "hbm",
         "kernel": "triangular sweep (forward + backward level sweeps, mixed FP32 solve blocks)",
@@ -162,7 +209,7 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
         "bytes_per_launch": bytes_per_sweep,
         "mean_launch_ms": round(mean_sweep_s * 1e3, 4),
         "launches": sweep_count,

@@ -20,7 +20,8 @@ BUILD = PACKAGE.parent / "_build"
 LIBRARY = PACKAGE / "libheatgeo.so"
 ARCH = os.environ.get("HEATGEO_ARCH", "gfx950")
 
-# Packed FP32 instructions stay off; -O3 for device and host.
+# Packed FP32 instructions (v_pk_*_f32) stay off: some of their forms have given silently wrong
+# results on this hardware. -O3 for device and host.
 COMMON_FLAGS = [
     "-O3",
     "-std=c++17",
