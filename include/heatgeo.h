@@ -43,7 +43,8 @@ extern "C" {
 
 /* Precision of a factored solve. */
 #define HG_PRECISION_FP64 0  /* FP64 solve blocks and vectors, FP64 refinement */
-#define HG_PRECISION_MIXED 1 /* FP32 solve blocks, FP64 residuals and refinement */
+#define HG_PRECISION_MIXED 1 /* FP32 solve blocks, FP64 residuals and refinement to a served
+                                accuracy: componentwise 1e-6 (heat), normwise 1e-6 (Poisson) */
 
 /* Which systems hg_factor builds. */
 #define HG_SYSTEM_HEAT 1
@@ -213,9 +214,15 @@ int hg_plan_sizes(hg_plan* plan, int64_t* out8);
  * 8 rel, 9 child_ptr, 10 children, 11 level_ptr, 12 level_nodes (int32); 13 front_off, 14 w_off,
  * 15 vec_off, 16 entry_dst (int64); 17 entry_src, 18 entry_row, 19 entry_col (int32). */
 int hg_plan_copy(hg_plan* plan, int32_t field, void* out);
-/* Checks every index the device sweeps form from this plan (task ranges, partial slots, gather
- * sources) against its buffer; HG_ERR_RUNTIME names the first violation. */
+/* Checks every index the device sweeps and the numeric factorization form from this plan (task
+ * ranges, partial slots, gather sources, tile tasks) against its buffer; HG_ERR_RUNTIME names the
+ * first violation. */
 int hg_plan_validate(hg_plan* plan);
+/* The numeric factorization's tile launches in execution order (the blocked form of _factor,
+ * cholesky.py:1399-1540, kernels/cholesky.py:289-768): out_counts = [launches, tasks];
+ * when the arrays are non-null, launches receives 4 int32 per launch (kind 0 extend-add, 1 panel,
+ * 2 update; level; first task; task count) and tasks 4 int32 per task. */
+int hg_plan_factor_tasks(hg_plan* plan, int64_t* out_counts, int32_t* launches, int32_t* tasks);
 int hg_plan_destroy(hg_plan* plan);
 
 #ifdef __cplusplus

@@ -54,6 +54,7 @@ EXPORTED = (
     "hg_plan_sizes",
     "hg_plan_copy",
     "hg_plan_validate",
+    "hg_plan_factor_tasks",
     "hg_plan_destroy",
     "hg_timing_enable",
     "hg_timing_read",
@@ -130,6 +131,7 @@ _SIGNATURES = {
     "hg_plan_sizes": ([_P, _PI64], ctypes.c_int),
     "hg_plan_copy": ([_P, _I32, _P], ctypes.c_int),
     "hg_plan_validate": ([_P], ctypes.c_int),
+    "hg_plan_factor_tasks": ([_P, _PI64, _P, _P], ctypes.c_int),
     "hg_plan_destroy": ([_P], ctypes.c_int),
     "hg_timing_enable": ([_P, _I32], ctypes.c_int),
     "hg_timing_read": ([_P, ctypes.POINTER(ctypes.c_double), _PI64, _I32], ctypes.c_int),

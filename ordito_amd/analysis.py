@@ -95,6 +95,14 @@ def analyse(
             out = np.zeros(max(count[size], 1), dtype=np.int64)
             _lib.check(lib.hg_plan_copy(handle, field, out.ctypes.data))
             arrays[name] = out[: count[size]]
+        # The numeric factorization's tile launches (kind, level, first, count) and tasks.
+        counts = (ctypes.c_int64 * 2)()
+        _lib.check(lib.hg_plan_factor_tasks(handle, counts, None, None))
+        launches = np.zeros((max(int(counts[0]), 1), 4), dtype=np.int32)
+        tasks = np.zeros((max(int(counts[1]), 1), 4), dtype=np.int32)
+        _lib.check(lib.hg_plan_factor_tasks(handle, counts, launches.ctypes.data, tasks.ctypes.data))
+        arrays["factor_launches"] = launches[: int(counts[0])]
+        arrays["factor_tasks"] = tasks[: int(counts[1])]
         return HostPlan(n, n_super, n_levels, arena, w_size, vec, arrays)
     finally:
         lib.hg_plan_destroy(handle)

@@ -42,13 +42,16 @@ struct DevicePlan {
   DeviceArray<int32_t> gather_s, gather_p0;
   // The operator's pattern in the elimination order (P A P^T) and each entry's original slot.
   DeviceArray<int32_t> p_offsets, p_columns, p_src;
+  // The numeric factorization's tile launches (host list) and tasks (device, 4 int32 each).
+  std::vector<FactorLaunch> factor_launches;
+  DeviceArray<int32_t> factor_tasks;
 
   void upload(const Plan& plan, const HostPattern& pattern, hipStream_t stream);
   int64_t bytes() const;
 };
 
-// One system's solve blocks [L11^-1; L21 L11^-1] per supernode, FP64 and optionally FP32, and the
-// operator's values in the elimination order for the refinement residual.
+// One system's inverse factors Z = [L11^-1; -L21 L11^-1] per supernode, FP64 and optionally FP32,
+// and the operator's values in the elimination order for the refinement residual.
 struct Factor {
   DeviceArray<double> w64;
   DeviceArray<float> w32;

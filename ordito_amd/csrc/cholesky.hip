@@ -3,17 +3,24 @@
 //
 // Follows ordito's design (cholesky.py:1-35, kernels/cholesky.py:1-31): each supernode's front is
 // a dense matrix assembled from the operator's entries and its children's Schur complements
-// (extend-add), partially factored, and its solve block [L11^-1; L21 L11^-1] stored so both
-// triangular sweeps are level-scheduled matrix products (the partitioned inverse of Alvarado,
-// Pothen and Schreiber 1993). The forward sweep here is multifrontal: each front gathers its
-// children's update vectors through the same relative index maps the factorization's extend-add
-// uses, so no sum is atomic and every sum has one fixed order.
+// (extend-add), partially factored, and its solve block stored so both triangular sweeps are
+// level-scheduled matrix products (the partitioned inverse of Alvarado, Pothen and Schreiber 1993).
+// The forward sweep here is multifrontal: each front gathers its children's update vectors through
+// the same relative index maps the factorization's extend-add uses, so no sum is atomic and every
+// sum has one fixed order.
+//
+// The partial factorization is blocked right-looking over 64-column panels, every front of a level
+// in the same launches. The solve block is the inverse factor Z = [L11^-1; -L21 L11^-1], the first
+// ncol columns of the inverse of the unit-extended lower factor [L11 0; L21 I]: that inverse is
+// the product of the panels' elementary inverses [I 0 0; 0 D^-1 0; 0 -P D^-1 I] (D the panel's
+// diagonal block factor, P the panel below it), so each panel step updates Z with the same tile
+// product that updates the trailing matrix, and no separate triangular inversion is needed.
 #include "cholesky.h"
 
 namespace hg {
 namespace {
 
-constexpr int kFrontThreads = 256;
+constexpr int kFt = kFactorTile;
 
 // ---- numeric factorization ---------------------------------------------------------------------
 
@@ -39,105 +46,210 @@ __global__ void pin_diagonal_kernel(const int32_t* __restrict__ pin_front_slot, 
   if (k < n_pins) arena[pin_front_slot[k]] = 1.0;
 }
 
-// Children's Schur complements into this level's fronts, children in ascending order.
-__global__ void extend_add_kernel(const int32_t* __restrict__ nodes, const int32_t* __restrict__ ncol,
-                                  const int32_t* __restrict__ nrow,
-                                  const int32_t* __restrict__ child_ptr,
-                                  const int32_t* __restrict__ children,
-                                  const int32_t* __restrict__ row_ptr,
-                                  const int32_t* __restrict__ rel,
-                                  const int64_t* __restrict__ front_off,
-                                  double* __restrict__ arena) {
-  const int32_t s = nodes[blockIdx.x];
-  const int64_t mp = ncol[s] + nrow[s];
-  double* parent = arena + front_off[s];
-  for (int32_t k = child_ptr[s]; k < child_ptr[s + 1]; ++k) {
-    const int32_t c = children[k];
-    const int32_t nc = ncol[c], rc = nrow[c];
-    const int64_t mc = nc + rc;
-    const double* child = arena + front_off[c];
-    const int32_t* map = rel + row_ptr[c];
-    for (int32_t j = 0; j < rc; ++j) {
-      const int64_t pj = map[j];
-      const double* col = child + (nc + j) * mc + nc;
-      for (int32_t i = j + threadIdx.x; i < rc; i += blockDim.x)
-        parent[pj * mp + map[i]] += col[i];
-    }
-    __syncthreads();
+struct FactorArgs {
+  const int32_t* ncol;
+  const int32_t* nrow;
+  const int32_t* row_ptr;
+  const int32_t* rel;
+  const int64_t* front_off;
+  const int64_t* w_off;
+  double* arena;   // fronts, column-major m x m
+  double* z;       // inverse factors, column-major m x ncol, zero on entry
+  int32_t* status;
+};
+
+// One child's Schur-complement tile added into its parent through the relative map (lower part).
+// Tasks of one launch are one child rank, so no two of them write the same parent entry.
+__global__ __launch_bounds__(256) void extend_add_kernel(const int32_t* __restrict__ tasks,
+                                                         FactorArgs a) {
+  const int32_t* task = tasks + 4 * static_cast<int64_t>(blockIdx.x);
+  const int32_t s = task[0], c = task[1], i0 = task[2], k0 = task[3];
+  const int32_t ncc = a.ncol[c], rc = a.nrow[c];
+  const int64_t mc = ncc + rc;
+  const int64_t ms = a.ncol[s] + a.nrow[s];
+  const double* child = a.arena + a.front_off[c];
+  double* parent = a.arena + a.front_off[s];
+  const int32_t* map = a.rel + a.row_ptr[c];
+  for (int e = threadIdx.x; e < kFt * kFt; e += 256) {
+    const int32_t i = i0 + e % kFt, k = k0 + e / kFt;
+    if (i < rc && k < rc && i >= k)
+      parent[static_cast<int64_t>(map[k]) * ms + map[i]] += child[(ncc + k) * mc + ncc + i];
   }
 }
 
-// Right-looking dense partial Cholesky of each front's first ncol columns, in place
-// (column-major), leaving the Schur complement in the trailing block. A non-positive pivot raises
-// status[0] (cholesky.py:260-266).
-__global__ void factor_fronts_kernel(const int32_t* __restrict__ nodes,
-                                     const int32_t* __restrict__ ncol,
-                                     const int32_t* __restrict__ nrow,
-                                     const int64_t* __restrict__ front_off,
-                                     double* __restrict__ arena, int32_t* __restrict__ status) {
-  __shared__ double pivot;
-  const int32_t s = nodes[blockIdx.x];
-  const int32_t nc = ncol[s];
-  const int64_t m = nc + nrow[s];
-  double* F = arena + front_off[s];
-  for (int32_t j = 0; j < nc; ++j) {
-    if (threadIdx.x == 0) {
-      double d = F[j * m + j];
-      if (!(d > 0.0)) {
-        atomicOr(status, 1);
-        d = 1.0;
-      }
-      d = sqrt(d);
-      F[j * m + j] = d;
-      pivot = d;
-    }
-    __syncthreads();
-    const double d = pivot;
-    double* colj = F + j * m;
-    for (int64_t i = j + 1 + threadIdx.x; i < m; i += blockDim.x) colj[i] /= d;
-    __syncthreads();
-    for (int64_t k = j + 1 + threadIdx.x; k < m; k += blockDim.x) {
-      const double lkj = colj[k];
-      double* colk = F + k * m;
-      for (int64_t i = k; i < m; ++i) colk[i] -= colj[i] * lkj;
-    }
-    __syncthreads();
-  }
-}
-
-// Solve block of each front: W11 = L11^-1 by forward substitution a column a thread, then
-// W21 = L21 W11; stored column-major (leading dimension m) at w_off.
-template <typename T>
-__global__ void solve_blocks_kernel(const int32_t* __restrict__ nodes,
-                                    const int32_t* __restrict__ ncol,
-                                    const int32_t* __restrict__ nrow,
-                                    const int64_t* __restrict__ front_off,
-                                    const int64_t* __restrict__ w_off,
-                                    const double* __restrict__ arena, double* __restrict__ w) {
-  const int32_t s = nodes[blockIdx.x];
-  const int32_t nc = ncol[s], nr = nrow[s];
-  const int64_t m = nc + nr;
-  const double* L = arena + front_off[s];
-  double* W = w + w_off[s];
-  for (int32_t j = threadIdx.x; j < nc; j += blockDim.x) {
-    double* wj = W + j * m;
-    for (int32_t i = 0; i < j; ++i) wj[i] = 0.0;
-    wj[j] = 1.0 / L[j * m + j];
-    for (int32_t i = j + 1; i < nc; ++i) {
-      double sum = 0.0;
-      for (int32_t k = j; k < i; ++k) sum += L[k * m + i] * wj[k];
-      wj[i] = -sum / L[i * m + i];
+// Panel step at column j0 of front s. Every task first eliminates the b x b diagonal block D in
+// LDS while applying the same eliminations to the identity, which leaves Zd = L_D^-1 (right-looking:
+// at pivot k, rows i > k lose D(i,k)/D(k,k) times row k of D's trailing part and of Zd, then Zd's
+// row k is divided by sqrt(D(k,k))). Then, by kind: 0 solves panel rows [t0, t0+64):
+// X = F(rows, J) Zd^T; 1 applies Zd to Z's block rows J at columns [t0, t0+64); 2 stores Zd as Z's
+// diagonal block. A non-positive pivot raises status[0] (cholesky.py:260-266).
+__global__ __launch_bounds__(256) void panel_kernel(const int32_t* __restrict__ tasks,
+                                                    FactorArgs a) {
+  __shared__ double Ds[kFt][kFt + 1];
+  __shared__ double Zs[kFt][kFt + 1];
+  __shared__ double Ts[kFt][kFt + 1];
+  const int32_t* task = tasks + 4 * static_cast<int64_t>(blockIdx.x);
+  const int32_t s = task[0], j0 = task[1], kind = task[2], t0 = task[3];
+  const int32_t nc = a.ncol[s];
+  const int64_t m = nc + a.nrow[s];
+  const int32_t b = min(kFt, nc - j0);
+  double* F = a.arena + a.front_off[s];
+  double* Z = a.z + a.w_off[s];
+  const int t = threadIdx.x;
+  for (int e = t; e < kFt * kFt; e += 256) {
+    const int i = e % kFt, j = e / kFt;
+    Ds[i][j] = (i < b && j < b && i >= j) ? F[(j0 + j) * m + j0 + i] : 0.0;
+    Zs[i][j] = (i == j && i < b) ? 1.0 : 0.0;
+    if (kind == 0) {
+      const int64_t r = t0 + i;
+      Ts[i][j] = (r < m && j < b) ? F[(j0 + j) * m + r] : 0.0;
+    } else if (kind == 1) {
+      // Ts[k][cc] = Z(j0 + k, t0 + cc), read along k (contiguous).
+      Ts[i][j] = (i < b) ? Z[static_cast<int64_t>(t0 + j) * m + j0 + i] : 0.0;
     }
   }
   __syncthreads();
-  const int64_t total = static_cast<int64_t>(nr) * nc;
-  for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
-    const int32_t i = static_cast<int32_t>(t % nr);
-    const int32_t j = static_cast<int32_t>(t / nr);
-    const double* wj = W + j * m;
-    double sum = 0.0;
-    for (int32_t k = j; k < nc; ++k) sum += L[k * m + nc + i] * wj[k];
-    W[j * m + nc + i] = sum;
+  // Thread (row i, column group cg) owns entries (i, cg + 4v), v < 16.
+  const int i = t % kFt, cg = t / kFt;
+  bool bad = false;
+  for (int k = 0; k < b; ++k) {
+    double d = Ds[k][k];
+    if (!(d > 0.0)) {
+      bad = true;
+      d = 1.0;
+    }
+    if (i > k && i < b) {
+      const double lik = Ds[i][k] / d;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int j = cg + 4 * v;
+        if (j > k && j <= i) Ds[i][j] -= lik * Ds[j][k];
+        else if (j <= k) Zs[i][j] -= lik * Zs[k][j];
+      }
+    }
+    __syncthreads();
+    // Row k is read by no later pivot's updates, so its scaling needs no barrier of its own.
+    if (i == k) {
+      const double inv = 1.0 / sqrt(d);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int j = cg + 4 * v;
+        if (j <= k) Zs[k][j] *= inv;
+      }
+    }
+  }
+  __syncthreads();
+  if (kind == 2) {
+    if (bad && t == 0) atomicOr(a.status, 1);
+    for (int e = t; e < kFt * kFt; e += 256) {
+      const int r = e % kFt, c = e / kFt;
+      if (r < b && c < b && r >= c) Z[static_cast<int64_t>(j0 + c) * m + j0 + r] = Zs[r][c];
+    }
+    return;
+  }
+  // 64 x 64 product, thread (tr, tc) = (t % 16, t / 16) owns rows tr + 16x, columns tc + 16z.
+  const int tr = t % 16, tc = t / 16;
+  double acc[4][4] = {};
+  if (kind == 0) {
+    for (int k = 0; k < b; ++k) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) av[x] = Ts[tr + 16 * x][k];
+#pragma unroll
+      for (int z = 0; z < 4; ++z) bv[z] = Zs[tc + 16 * z][k];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int64_t r = t0 + tr + 16 * x;
+#pragma unroll
+      for (int z = 0; z < 4; ++z) {
+        const int c = tc + 16 * z;
+        if (r < m && c < b) F[(j0 + c) * m + r] = acc[x][z];
+      }
+    }
+  } else {
+    for (int k = 0; k < b; ++k) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) av[x] = Zs[tr + 16 * x][k];
+#pragma unroll
+      for (int z = 0; z < 4; ++z) bv[z] = Ts[k][tc + 16 * z];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int r = tr + 16 * x;
+#pragma unroll
+      for (int z = 0; z < 4; ++z) {
+        const int64_t c = t0 + tc + 16 * z;
+        if (r < b) Z[c * m + j0 + r] = acc[x][z];
+      }
+    }
+  }
+}
+
+// Update after the panel step at j0: rows [i0, i0+64) of the trailing matrix (c0 >= j0 + b, lower
+// part) lose P(i0..) P(c0..)^T, or of Z (c0 < j0 + b) lose P(i0..) Z(J, c0..), where P is the
+// solved panel (the front's columns J = [j0, j0+b)).
+__global__ __launch_bounds__(256) void update_kernel(const int32_t* __restrict__ tasks,
+                                                     FactorArgs a) {
+  __shared__ double As[kFt][kFt + 1];  // [k][row]
+  __shared__ double Bs[kFt][kFt + 1];  // [k][column]
+  const int32_t* task = tasks + 4 * static_cast<int64_t>(blockIdx.x);
+  const int32_t s = task[0], j0 = task[1], i0 = task[2], c0 = task[3];
+  const int32_t nc = a.ncol[s];
+  const int64_t m = nc + a.nrow[s];
+  const int32_t b = min(kFt, nc - j0);
+  const bool trailing = c0 >= j0 + b;
+  double* F = a.arena + a.front_off[s];
+  double* Z = a.z + a.w_off[s];
+  const int t = threadIdx.x;
+  for (int e = t; e < kFt * kFt; e += 256) {
+    const int r = e % kFt, k = e / kFt;
+    As[k][r] = (i0 + r < m && k < b) ? F[(j0 + k) * m + i0 + r] : 0.0;
+    if (trailing) {
+      Bs[k][r] = (c0 + r < m && k < b) ? F[(j0 + k) * m + c0 + r] : 0.0;
+    } else {
+      // Bs[k][cc] = Z(j0 + k, c0 + cc): e % 64 runs along k here, contiguous in Z.
+      const int kk = e % kFt, cc = e / kFt;
+      Bs[kk][cc] = (kk < b && c0 + cc < j0 + b) ? Z[static_cast<int64_t>(c0 + cc) * m + j0 + kk]
+                                                 : 0.0;
+    }
+  }
+  __syncthreads();
+  const int tr = t % 16, tc = t / 16;
+  double acc[4][4] = {};
+  for (int k = 0; k < b; ++k) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) av[x] = As[k][tr + 16 * x];
+#pragma unroll
+    for (int z = 0; z < 4; ++z) bv[z] = Bs[k][tc + 16 * z];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int64_t r = i0 + tr + 16 * x;
+#pragma unroll
+    for (int z = 0; z < 4; ++z) {
+      const int64_t c = c0 + tc + 16 * z;
+      if (r >= m) continue;
+      if (trailing) {
+        if (c < m && r >= c) F[c * m + r] -= acc[x][z];
+      } else if (c < j0 + b) {
+        Z[c * m + r] -= acc[x][z];
+      }
+    }
   }
 }
 
@@ -308,6 +420,129 @@ void validate_sweeps(const Plan& plan) {
   if (static_cast<int64_t>(g.src.size()) != rows_with_parent) fail("gather list count");
 }
 
+HostFactorTasks factor_tasks(const Plan& plan) {
+  HostFactorTasks out;
+  auto push = [&](int32_t a, int32_t b, int32_t c, int32_t d) {
+    out.tasks.insert(out.tasks.end(), {a, b, c, d});
+  };
+  auto task_count = [&]() { return static_cast<int32_t>(out.tasks.size() / 4); };
+  for (int32_t level = 0; level < plan.n_levels; ++level) {
+    const int32_t lo = plan.level_ptr[level], hi = plan.level_ptr[level + 1];
+    int32_t max_children = 0, max_ncol = 0;
+    for (int32_t k = lo; k < hi; ++k) {
+      const int32_t s = plan.level_nodes[k];
+      max_children = std::max(max_children, plan.child_ptr[s + 1] - plan.child_ptr[s]);
+      max_ncol = std::max(max_ncol, plan.ncol[s]);
+    }
+    for (int32_t rank = 0; rank < max_children; ++rank) {
+      const int32_t first = task_count();
+      for (int32_t k = lo; k < hi; ++k) {
+        const int32_t s = plan.level_nodes[k];
+        if (plan.child_ptr[s] + rank >= plan.child_ptr[s + 1]) continue;
+        const int32_t c = plan.children[plan.child_ptr[s] + rank];
+        for (int32_t k0 = 0; k0 < plan.nrow[c]; k0 += kFt)
+          for (int32_t i0 = k0; i0 < plan.nrow[c]; i0 += kFt) push(s, c, i0, k0);
+      }
+      if (task_count() > first) out.launches.push_back({kFactorExtend, level, first, task_count() - first});
+    }
+    for (int32_t j0 = 0; j0 < max_ncol; j0 += kFt) {
+      int32_t first = task_count();
+      for (int32_t k = lo; k < hi; ++k) {
+        const int32_t s = plan.level_nodes[k];
+        const int32_t nc = plan.ncol[s], m = nc + plan.nrow[s];
+        if (j0 >= nc) continue;
+        const int32_t b = std::min(kFt, nc - j0);
+        push(s, j0, 2, j0);
+        for (int32_t r0 = j0 + b; r0 < m; r0 += kFt) push(s, j0, 0, r0);
+        for (int32_t c0 = 0; c0 < j0; c0 += kFt) push(s, j0, 1, c0);
+      }
+      if (task_count() > first) out.launches.push_back({kFactorPanel, level, first, task_count() - first});
+      first = task_count();
+      for (int32_t k = lo; k < hi; ++k) {
+        const int32_t s = plan.level_nodes[k];
+        const int32_t nc = plan.ncol[s], m = nc + plan.nrow[s];
+        if (j0 >= nc) continue;
+        const int32_t b = std::min(kFt, nc - j0);
+        for (int32_t i0 = j0 + b; i0 < m; i0 += kFt) {
+          for (int32_t c0 = j0 + b; c0 <= i0; c0 += kFt) push(s, j0, i0, c0);
+          for (int32_t c0 = 0; c0 < j0 + b; c0 += kFt) push(s, j0, i0, c0);
+        }
+      }
+      if (task_count() > first) out.launches.push_back({kFactorUpdate, level, first, task_count() - first});
+    }
+  }
+  return out;
+}
+
+void validate_factor_tasks(const Plan& plan, const HostFactorTasks& h) {
+  auto fail = [](const std::string& what) { throw Error{HG_ERR_RUNTIME, "factor plan check: " + what}; };
+  const int32_t S = plan.n_super;
+  const int64_t n_tasks = static_cast<int64_t>(h.tasks.size() / 4);
+  if (static_cast<int64_t>(h.tasks.size()) != 4 * n_tasks) fail("task array not a multiple of 4");
+  int64_t expect_first = 0;
+  int32_t last_level = 0;
+  // Per front: the panel steps seen, in order, and the extend-add tiles per child.
+  std::vector<int32_t> next_j0(S, 0);
+  std::vector<int64_t> ext_tiles(S, 0);
+  for (const FactorLaunch& L : h.launches) {
+    if (L.first != expect_first || L.count <= 0) fail("launches do not tile the task list");
+    expect_first += L.count;
+    if (L.level < last_level || L.level >= plan.n_levels) fail("launch level out of order");
+    last_level = L.level;
+    for (int64_t t = L.first; t < L.first + L.count; ++t) {
+      const int32_t* f = h.tasks.data() + 4 * t;
+      const int32_t s = f[0];
+      if (s < 0 || s >= S) fail("task front out of range");
+      if (plan.level[s] != L.level) fail("task front not on the launch's level");
+      const int32_t nc = plan.ncol[s], m = nc + plan.nrow[s];
+      if (L.kind == kFactorExtend) {
+        const int32_t c = f[1];
+        if (c < 0 || c >= S || plan.parent[c] != s) fail("extend-add child is not a child of the front");
+        const int32_t rc = plan.nrow[c];
+        if (f[2] < 0 || f[3] < 0 || f[2] >= rc || f[3] >= rc || f[2] < f[3] || f[2] % kFt ||
+            f[3] % kFt)
+          fail("extend-add tile out of range");
+        ext_tiles[c] += 1;
+        continue;
+      }
+      const int32_t j0 = f[1];
+      if (j0 < 0 || j0 >= nc || j0 % kFt) fail("panel column out of range");
+      const int32_t b = std::min(kFt, nc - j0);
+      if (L.kind == kFactorPanel) {
+        const int32_t kind = f[2], t0 = f[3];
+        if (kind == 2) {
+          if (t0 != j0 || next_j0[s] != j0) fail("diagonal task out of step order");
+          next_j0[s] = j0 + b;
+        } else if (kind == 0) {
+          if (t0 < j0 + b || t0 >= m || (t0 - j0 - b) % kFt) fail("panel row tile out of range");
+        } else if (kind == 1) {
+          if (t0 < 0 || t0 + kFt > j0 || t0 % kFt) fail("inverse-factor tile out of range");
+        } else {
+          fail("unknown panel task kind");
+        }
+      } else if (L.kind == kFactorUpdate) {
+        const int32_t i0 = f[2], c0 = f[3];
+        if (next_j0[s] != j0 + b) fail("update before its panel");
+        if (i0 < j0 + b || i0 >= m || (i0 - j0 - b) % kFt) fail("update row tile out of range");
+        if (c0 >= j0 + b) {
+          if (c0 > i0 || (c0 - j0 - b) % kFt) fail("trailing tile out of range");
+        } else if (c0 < 0 || c0 % kFt) {
+          fail("inverse-factor update tile out of range");
+        }
+      } else {
+        fail("unknown launch kind");
+      }
+    }
+  }
+  if (expect_first != n_tasks) fail("tasks outside every launch");
+  for (int32_t s = 0; s < S; ++s) {
+    if (next_j0[s] != plan.ncol[s]) fail("a front's panels do not cover its columns");
+    const int64_t tiles = (plan.nrow[s] + kFt - 1) / kFt;
+    if (plan.parent[s] >= 0 && ext_tiles[s] != tiles * (tiles + 1) / 2)
+      fail("extend-add tiles do not cover a child's Schur complement");
+  }
+}
+
 namespace {
 
 void upload_sweep(const HostSweep& h, DevicePlan::SweepTasks& out, hipStream_t stream) {
@@ -328,6 +563,11 @@ void upload_sweep(const HostSweep& h, DevicePlan::SweepTasks& out, hipStream_t s
 }  // namespace
 
 void DevicePlan::upload(const Plan& plan, const HostPattern& pattern, hipStream_t stream) {
+  {
+    HostFactorTasks ft = hg::factor_tasks(plan);
+    factor_launches = std::move(ft.launches);
+    factor_tasks.upload(ft.tasks, stream);
+  }
   const HostSweep hf = sweep_tasks(plan, true);
   const HostSweep hb = sweep_tasks(plan, false);
   max_slots = std::max(hf.max_slots, hb.max_slots);
@@ -408,7 +648,7 @@ int64_t DevicePlan::bytes() const {
          entry_src.bytes() + entry_dst.bytes() + entry_row.bytes() + entry_col.bytes() +
          fwd.s.bytes() * 5 + fwd.rs.bytes() * 4 + bwd.s.bytes() * 5 + bwd.rs.bytes() * 4 +
          gather_ptr.bytes() + gather_src.bytes() + gather_s.bytes() + gather_p0.bytes() +
-         p_offsets.bytes() + p_columns.bytes() + p_src.bytes();
+         p_offsets.bytes() + p_columns.bytes() + p_src.bytes() + factor_tasks.bytes();
 }
 
 void factor_numeric(const DevicePlan& plan, const double* values, const int32_t* pinned_new,
@@ -434,20 +674,24 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
     HG_HIP(hipStreamSynchronize(stream));
   }
   factor.w64.allocate(plan.w_size);
-  for (int32_t level = 0; level < plan.n_levels; ++level) {
-    const int32_t first = plan.level_ptr[level];
-    const int32_t count = plan.level_ptr[level + 1] - first;
-    if (count == 0) continue;
-    const int32_t* nodes = plan.level_nodes.ptr + first;
-    extend_add_kernel<<<count, kFrontThreads, 0, stream>>>(
-        nodes, plan.ncol.ptr, plan.nrow.ptr, plan.child_ptr.ptr, plan.children.ptr,
-        plan.row_ptr.ptr, plan.rel.ptr, plan.front_off.ptr, arena.ptr);
-    factor_fronts_kernel<<<count, kFrontThreads, 0, stream>>>(nodes, plan.ncol.ptr, plan.nrow.ptr,
-                                                              plan.front_off.ptr, arena.ptr,
-                                                              status.ptr);
-    solve_blocks_kernel<double><<<count, kFrontThreads, 0, stream>>>(
-        nodes, plan.ncol.ptr, plan.nrow.ptr, plan.front_off.ptr, plan.w_off.ptr, arena.ptr,
-        factor.w64.ptr);
+  if (plan.w_size > 0)
+    HG_HIP(hipMemsetAsync(factor.w64.ptr, 0, sizeof(double) * plan.w_size, stream));
+  FactorArgs a{};
+  a.ncol = plan.ncol.ptr;
+  a.nrow = plan.nrow.ptr;
+  a.row_ptr = plan.row_ptr.ptr;
+  a.rel = plan.rel.ptr;
+  a.front_off = plan.front_off.ptr;
+  a.w_off = plan.w_off.ptr;
+  a.arena = arena.ptr;
+  a.z = factor.w64.ptr;
+  a.status = status.ptr;
+  for (const FactorLaunch& L : plan.factor_launches) {
+    const int32_t* tasks = plan.factor_tasks.ptr + 4 * static_cast<int64_t>(L.first);
+    const unsigned grid = static_cast<unsigned>(L.count);
+    if (L.kind == kFactorExtend) extend_add_kernel<<<grid, 256, 0, stream>>>(tasks, a);
+    else if (L.kind == kFactorPanel) panel_kernel<<<grid, 256, 0, stream>>>(tasks, a);
+    else update_kernel<<<grid, 256, 0, stream>>>(tasks, a);
     HG_HIP(hipGetLastError());
   }
   int32_t host_status = 0;

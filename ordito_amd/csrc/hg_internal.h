@@ -161,4 +161,33 @@ HostGather gather_lists(const Plan& plan);
 // violation.
 void validate_sweeps(const Plan& plan);
 
+// The numeric factorization as launches of 64 x 64 tile tasks (cholesky.hip), in execution order.
+// A front's inverse factor Z is the first ncol columns of the inverse of its unit-extended lower
+// factor [L11 0; L21 I], i.e. [L11^-1; -L21 L11^-1] (m x ncol, column-major, leading dimension
+// m); it is what the sweeps multiply by. Task fields (4 int32 each):
+//   kFactorExtend (parent s, child c, i0, k0): the child's Schur-complement tile rows
+//       [i0, i0+64) x columns [k0, k0+64), i0 >= k0, added into the parent through the child's
+//       relative map. One launch per child rank, so siblings add in ascending order.
+//   kFactorPanel (s, j0, kind, t0): the diagonal block at j0 (width b = min(64, ncol - j0)) is
+//       eliminated in LDS together with its inverse; kind 0 solves the panel rows [t0, t0+64)
+//       below it, kind 1 applies the inverse to Z's block rows at columns [t0, t0+64) (t0 < j0),
+//       kind 2 stores the inverse as Z's diagonal block.
+//   kFactorUpdate (s, j0, i0, c0): rows [i0, i0+64) minus the panel's rows [i0, i0+64) times
+//       the panel rows [c0, c0+64) transposed (the trailing matrix, c0 >= j0 + b, c0 <= i0) or
+//       times Z's block rows at columns [c0, c0+64) (c0 < j0 + b).
+enum FactorKind : int32_t { kFactorExtend = 0, kFactorPanel = 1, kFactorUpdate = 2 };
+struct FactorLaunch {
+  int32_t kind, level, first, count;
+};
+struct HostFactorTasks {
+  std::vector<FactorLaunch> launches;
+  std::vector<int32_t> tasks;  // 4 per task
+};
+constexpr int32_t kFactorTile = 64;
+HostFactorTasks factor_tasks(const Plan& plan);
+
+// Checks every factorization task against the plan (fronts, children, tile ranges, the step
+// order); throws on the first violation.
+void validate_factor_tasks(const Plan& plan, const HostFactorTasks& tasks);
+
 }  // namespace hg

@@ -5,10 +5,11 @@
 // held permuted (P A P^T), so the right-hand side is permuted in once, the solution out once, and
 // every round is forward sweep, backward sweep, residual -- no per-round permutation. Each level of
 // a sweep is a batch of dense tile products over every front of the level:
-//   forward : y(cols) = W11 f(cols) and the front's update vector u = f(rows) - W21 f(cols), where
+//   forward : y(cols) = Z11 f(cols) and the front's update vector u = f(rows) + Z21 f(cols), where
 //             f = [r(cols); 0] + the children's update vectors, gathered as the tile loads it
 //             (children in ascending order);
-//   backward: d(cols) = W11^T y(cols) - W21^T d(rows), and the next iterate x + d beside it.
+//   backward: d(cols) = Z11^T y(cols) + Z21^T d(rows), and the next iterate x + d beside it;
+// Z = [L11^-1; -L21 L11^-1] is the front's inverse factor (cholesky.hip).
 // A product task is one 64-wide tile of one front and a range of the inner dimension; a level with
 // too few tiles to fill the device splits the inner dimension into partial slots, added in slot
 // order by a reduction, so every sum keeps one fixed order. Tiles stage the solve block and the
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
       } else {
         const int64_t row = vbase + i;
         a.fvec[row * kk + q] =
-            gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) - acc[x][z];
+            gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) + acc[x][z];
       }
     }
   }
@@ -198,13 +199,13 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
       a.y[static_cast<int64_t>(c0 + i) * kk + q] = sum;
     } else {
       const int64_t row = vbase + i;
-      a.fvec[row * kk + q] = gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) - sum;
+      a.fvec[row * kk + q] = gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) + sum;
     }
   }
 }
 
 // Backward tile product: solve-block columns [j0, j0+64) transposed, times
-// g = [y(cols); -d(rows)] over the inner range [k0, k1). Thread (tc, tr) owns solve-block columns
+// g = [y(cols); d(rows)] over the inner range [k0, k1). Thread (tc, tr) owns solve-block columns
 // tr + 16a and right-hand sides tc + 16b. Applied results also give x_next = x + d.
 template <typename TW>
 __global__ __launch_bounds__(256) void bwd_tile_kernel(
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
       double g = 0.0;
       if (q < kk && i < k1) {
         g = i < nc ? a.y[static_cast<int64_t>(c0 + i) * kk + q]
-                   : -a.d[static_cast<int64_t>(srows[i - nc]) * kk + q];
+                   : a.d[static_cast<int64_t>(srows[i - nc]) * kk + q];
       }
       Gs[ii][q] = g;
     }
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(256) void fwd_tile1_kernel(
       a.y[c0 + i] = sum;
     } else {
       const int64_t row = vbase + i;
-      a.fvec[row] = gathered(0.0, row, 0, 1, a.gather_ptr, a.gather_src, a.fvec) - sum;
+      a.fvec[row] = gathered(0.0, row, 0, 1, a.gather_ptr, a.gather_src, a.fvec) + sum;
     }
   }
 }
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(256) void bwd_tile1_kernel(
   if (j < nc) {
     const TW* wj = W + static_cast<int64_t>(j) * m;
     for (int32_t i = max(k0, j) + lane; i < k1; i += 4) {
-      const double g = i < nc ? a.y[c0 + i] : -a.d[srows[i - nc]];
+      const double g = i < nc ? a.y[c0 + i] : a.d[srows[i - nc]];
       acc += widen(wj[i]) * g;
     }
   }

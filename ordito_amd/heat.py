@@ -246,7 +246,8 @@ class HeatSolver:
     (ordito ``heat.py:3491-4456``): operators assembled on first use; a system factored on its
     second solve, or at once when the heat step's settled iterate fails its backward-error check;
     the Poisson system factored whenever the heat system is. ``precision`` ("fp64", the reference's,
-    or "mixed": FP32 solve blocks with FP64 refinement) selects the factored solves' precision.
+    or "mixed": FP32 solve blocks with FP64 refinement to componentwise 1e-6 for the heat step and
+    normwise 1e-6 for the Poisson step, a served accuracy) selects the factored solves' precision.
     """
 
     def __init__(

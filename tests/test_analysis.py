@@ -108,10 +108,147 @@ def emulate_factor(plan, values, pinned=None):
             L11 = np.tril(F[:nc, :nc])
             W11 = np.linalg.inv(L11)
             W21 = F[nc:, :nc] @ W11
-            block = np.vstack([W11, W21])
+            block = np.vstack([W11, -W21])  # the inverse factor Z
             base = plan.w_off[s]
             w[base : base + m * nc] = block.T.reshape(-1)  # column-major, leading dimension m
     return w
+
+
+TILE = 64
+
+
+class _LaunchRecord:
+    """Reads and writes of one launch's tasks, for the check that no task reads or writes an entry
+    another task of the same launch writes (the device runs them concurrently)."""
+
+    def __init__(self):
+        self.writes = {"arena": [], "z": []}
+        self.reads = {"arena": [], "z": []}
+
+    def task(self, reads, writes):
+        for space in ("arena", "z"):
+            self.reads[space].append(np.concatenate(reads.get(space, [np.zeros(0, np.int64)])))
+            self.writes[space].append(np.concatenate(writes.get(space, [np.zeros(0, np.int64)])))
+
+    def check(self, sizes):
+        for space, size in sizes.items():
+            writes = self.writes[space]
+            if not writes:
+                continue
+            allw = np.concatenate(writes)
+            assert np.unique(allw).size == allw.size, f"two tasks write one {space} entry"
+            owner = np.full(size, -1, dtype=np.int64)
+            for t, wt in enumerate(writes):
+                owner[wt] = t
+            for t, rt in enumerate(self.reads[space]):
+                o = owner[rt]
+                assert np.all((o == -1) | (o == t)), f"a task reads a {space} entry another writes"
+
+
+def _idx(off, m, rows, cols):
+    """Flat column-major indices (leading dimension m) of rows x cols at off."""
+    return (off + np.asarray(cols, np.int64)[None, :] * m + np.asarray(rows, np.int64)[:, None])
+
+
+def eliminate_block(D):
+    """The panel kernel's in-LDS elimination: returns L_D^-1 of the lower triangle of D, built by
+    applying each pivot's row operations to the identity."""
+    b = D.shape[0]
+    D = np.tril(D).copy()
+    Z = np.eye(b)
+    for k in range(b):
+        d = D[k, k]
+        assert d > 0.0
+        lik = D[k + 1 :, k] / d
+        for i in range(k + 1, b):
+            D[i, k + 1 : i + 1] -= lik[i - k - 1] * D[k + 1 : i + 1, k]
+        Z[k + 1 :, : k + 1] -= np.outer(lik, Z[k, : k + 1])
+        Z[k, : k + 1] /= np.sqrt(d)
+    return Z
+
+
+def emulate_factor_tasks(plan, values, pinned=None, check_races=False):
+    """
+    Runs the library's factorization launches (``hg_plan_factor_tasks``) tile by tile with the
+    device kernels' arithmetic, every task of a launch against the state before the launch, and
+    returns the inverse factors Z. With ``check_races`` each launch is also checked for two tasks
+    touching one entry that either writes.
+    """
+    arena = np.zeros(plan.arena_size)
+    v = values[plan.entry_src].copy()
+    if pinned is not None:
+        hit = pinned[plan.entry_row] | pinned[plan.entry_col]
+        v[hit] = np.where(plan.entry_row[hit] == plan.entry_col[hit], 1.0, 0.0)
+    arena[plan.entry_dst] = v
+    z = np.zeros(plan.w_size)
+    sizes = {"arena": arena.size, "z": z.size}
+    for kind, level, first, count in plan.factor_launches:
+        record = _LaunchRecord()
+        updates = []  # (space, flat indices, new values), applied after every task ran
+        for s, f1, f2, f3 in plan.factor_tasks[first : first + count]:
+            assert plan.level[s] == level
+            nc = int(plan.ncol[s])
+            m = nc + int(plan.nrow[s])
+            fo, zo = int(plan.front_off[s]), int(plan.w_off[s])
+            if kind == 0:  # extend-add of child f1's tile (f2, f3)
+                c = int(f1)
+                ncc, rc = int(plan.ncol[c]), int(plan.nrow[c])
+                mc = ncc + rc
+                mp = plan.rel[plan.row_ptr[c] : plan.row_ptr[c] + rc]
+                rows = np.arange(f2, min(f2 + TILE, rc))
+                cols = np.arange(f3, min(f3 + TILE, rc))
+                ii, kk = np.meshgrid(rows, cols, indexing="ij")
+                keep = ii >= kk
+                ii, kk = ii[keep], kk[keep]
+                src = int(plan.front_off[c]) + (ncc + kk) * mc + ncc + ii
+                dst = fo + mp[kk].astype(np.int64) * m + mp[ii]
+                updates.append(("arena", dst, arena[dst] + arena[src]))
+                record.task({"arena": [src, dst]}, {"arena": [dst]})
+                continue
+            j0 = int(f1)
+            b = min(TILE, nc - j0)
+            J = np.arange(j0, j0 + b)
+            if kind == 1:  # panel
+                dblock = _idx(fo, m, J, J)
+                Zd = eliminate_block(arena[dblock])
+                low = np.tril(np.ones((b, b), dtype=bool))
+                if f2 == 2:
+                    dst = _idx(zo, m, J, J)[low]
+                    updates.append(("z", dst, Zd[low]))
+                    record.task({"arena": [dblock.ravel()]}, {"z": [dst]})
+                elif f2 == 0:
+                    rows = np.arange(f3, min(f3 + TILE, m))
+                    at = _idx(fo, m, rows, J)
+                    updates.append(("arena", at.ravel(), (arena[at] @ Zd.T).ravel()))
+                    record.task({"arena": [dblock.ravel(), at.ravel()]}, {"arena": [at.ravel()]})
+                else:
+                    cols = np.arange(f3, f3 + TILE)
+                    at = _idx(zo, m, J, cols)
+                    updates.append(("z", at.ravel(), (Zd @ z[at]).ravel()))
+                    record.task({"arena": [dblock.ravel()], "z": [at.ravel()]}, {"z": [at.ravel()]})
+            else:  # update
+                rows = np.arange(f2, min(f2 + TILE, m))
+                P = _idx(fo, m, rows, J)
+                if f3 >= j0 + b:
+                    cols = np.arange(f3, min(f3 + TILE, m))
+                    K = _idx(fo, m, cols, J)
+                    at = _idx(fo, m, rows, cols)
+                    low = rows[:, None] >= cols[None, :]
+                    new = arena[at] - arena[P] @ arena[K].T
+                    updates.append(("arena", at[low], new[low]))
+                    record.task({"arena": [P.ravel(), K.ravel(), at[low]]}, {"arena": [at[low]]})
+                else:
+                    cols = np.arange(f3, min(f3 + TILE, j0 + b))
+                    B = _idx(zo, m, J, cols)
+                    at = _idx(zo, m, rows, cols)
+                    new = z[at] - arena[P] @ z[B]
+                    updates.append(("z", at.ravel(), new.ravel()))
+                    record.task({"arena": [P.ravel()], "z": [B.ravel(), at.ravel()]}, {"z": [at.ravel()]})
+        if check_races:
+            record.check(sizes)
+        for space, dst, val in updates:
+            (arena if space == "arena" else z)[dst] = val
+    return z
 
 
 def emulate_solve(plan, w, b_perm):
@@ -130,17 +267,17 @@ def emulate_solve(plan, w, b_perm):
                 rel = plan.rel[plan.row_ptr[c] : plan.row_ptr[c + 1]]
                 fc = f[plan.vec_off[c] + plan.ncol[c] : plan.vec_off[c] + plan.ncol[c] + plan.nrow[c]]
                 fs[rel] += fc
-            W = w[plan.w_off[s] : plan.w_off[s] + m * nc].reshape(nc, m).T
+            Z = w[plan.w_off[s] : plan.w_off[s] + m * nc].reshape(nc, m).T
             head = fs[:nc].copy()
-            y[c0 : c0 + nc] = W[:nc] @ head
-            fs[nc:] -= W[nc:] @ head
+            y[c0 : c0 + nc] = Z[:nc] @ head
+            fs[nc:] += Z[nc:] @ head
     for level in range(plan.n_levels - 1, -1, -1):
         for s in plan.level_nodes[plan.level_ptr[level] : plan.level_ptr[level + 1]]:
             nc, nr, c0 = plan.ncol[s], plan.nrow[s], plan.col_begin[s]
             m = nc + nr
-            W = w[plan.w_off[s] : plan.w_off[s] + m * nc].reshape(nc, m).T
+            Z = w[plan.w_off[s] : plan.w_off[s] + m * nc].reshape(nc, m).T
             rows = plan.rows[plan.row_ptr[s] : plan.row_ptr[s + 1]]
-            x[c0 : c0 + nc] = W[:nc].T @ y[c0 : c0 + nc] - W[nc:].T @ x[rows]
+            x[c0 : c0 + nc] = Z[:nc].T @ y[c0 : c0 + nc] + Z[nc:].T @ x[rows]
     return x
 
 
@@ -193,6 +330,68 @@ def test_emulated_pinned_poisson_solve_matches_scipy():
     expected[keep] = spla.spsolve(reduced.tocsc(), b[keep])
     assert abs(x[pin_old]) == 0.0
     assert np.max(np.abs(x - expected)) <= 1e-8 * np.max(np.abs(expected))
+
+
+@pytest.mark.parametrize(
+    "name,leaf,top",
+    [("icosphere3", 0, 0), ("icosphere4", 0, 0), ("torus", 0, 0), ("torus", 8, 16), ("grid", 0, 0),
+     ("bumpy", 8, 0), ("saddle_graded", 0, 0)],
+)
+def test_factor_tasks_give_the_inverse_factor(name, leaf, top):
+    """The blocked factorization's launches, run task by task (each launch against the state before
+    it, and checked for two tasks of one launch touching an entry either writes), give the inverse
+    factors the column-by-column factorization gives, and a solve through them matches SciPy."""
+    vertices, _, ops, plan = _plan_for(name, leaf_rows=leaf, top_rows=top)
+    z = emulate_factor_tasks(plan, ops.heat, check_races=True)
+    z_ref = emulate_factor(plan, ops.heat)
+    assert np.max(np.abs(z - z_ref)) <= 1e-12 * np.max(np.abs(z_ref))
+    b = np.random.default_rng(2).random(plan.n)
+    x = np.empty(plan.n)
+    x[plan.perm] = emulate_solve(plan, z, b[plan.perm])
+    expected = spla.spsolve(ops.matrix(ops.heat).tocsc(), b)
+    assert np.max(np.abs(x - expected)) <= 1e-10 * np.max(np.abs(expected))
+
+
+def test_factor_tasks_cover_multi_panel_fronts():
+    """The fronts the tests above factor include ones wider than a panel (several steps, a last
+    panel narrower than 64), so every task kind runs."""
+    _, _, _, plan = _plan_for("icosphere4")
+    kinds = {int(k) for k in plan.factor_launches[:, 0]}
+    assert kinds == {0, 1, 2}
+    assert np.any(plan.ncol > TILE) and np.any((plan.ncol > TILE) & (plan.ncol % TILE != 0))
+    panel = plan.factor_tasks[
+        np.concatenate([np.arange(f, f + c) for k, _, f, c in plan.factor_launches if k == 1])
+    ]
+    assert {0, 1, 2} <= set(panel[:, 2].tolist())
+
+
+def test_factor_tasks_pinned_poisson_matches_scipy():
+    vertices, faces, ops, plan = _plan_for("torus")
+    n = plan.n
+    pinned_new = np.zeros(n, dtype=bool)
+    pinned_new[n - 1] = True
+    z = emulate_factor_tasks(plan, ops.poisson, pinned=pinned_new)
+    b = np.random.default_rng(3).random(n)
+    b -= b.mean()
+    rp = b[plan.perm].copy()
+    rp[n - 1] = 0.0
+    x = np.empty(n)
+    x[plan.perm] = emulate_solve(plan, z, rp)
+    pin_old = int(plan.perm[-1])
+    keep = np.ones(n, dtype=bool)
+    keep[pin_old] = False
+    expected = np.zeros(n)
+    expected[keep] = spla.spsolve(ops.matrix(ops.poisson)[keep][:, keep].tocsc(), b[keep])
+    assert np.max(np.abs(x - expected)) <= 1e-8 * np.max(np.abs(expected))
+
+
+def test_eliminate_block_is_the_inverse_cholesky_factor():
+    rng = np.random.default_rng(4)
+    for b in (1, 2, 5, 64):
+        a = rng.standard_normal((b, b))
+        spd = a @ a.T + b * np.eye(b)
+        L = np.linalg.cholesky(spd)
+        assert np.allclose(eliminate_block(spd), np.linalg.inv(L), rtol=0, atol=1e-12)
 
 
 @pytest.mark.parametrize("name", ["icosphere3", "torus", "grid", "bumpy", "saddle_graded"])
