@@ -4,16 +4,20 @@ Benchmark of the heat-method geodesic path on one MI355X.
 Workload (the single-GPU configuration the headline metric is quoted on): a torus of 1024 x 512
 quads (524,288 vertices, 1,048,576 faces), R = 1, r = 0.3, normal jitter N(0, 0.1 h), FP32
 positions, 64 source vertices drawn from a seed; the diffusion time is the solver's default rule
-(h^2 raised to (bbox diagonal / 400)^2). Both systems are factored once (setup, reported apart),
-then a *step* is one batch of the 64 single-source queries through the served path: seed, heat
-solve, unit-gradient divergence, Poisson solve, shift -- all on the GPU, factors resident in HBM.
+(h^2 raised to (bbox diagonal / 400)^2). The systems are factored once (setup, reported apart) and
+stay resident in HBM; a *step* is one batch of the 64 single-source queries through the served
+path: seed, heat solve, unit-gradient divergence, Poisson solve, shift -- all on the GPU.
 
-- ``value``: served (mixed-precision) queries/s over the timed steps.
-- ``plain_fp64``: the plain FP64 GPU path (FP64 factors, one query per call), timed beside it.
-- ``roofline``: the triangular sweep (forward + backward, the dominant kernel), algorithmic bytes
-  per sweep over its HIP-event-measured mean duration against the 8 TB/s HBM peak; ``traffic`` is
-  the HBM bytes per sweep that rocprofv3's FETCH_SIZE / WRITE_SIZE passes measured
-  (profiles/triangular_sweep_torus1024x512_traffic.json, scaled by a 1 GiB copy:
+- ``value``: served queries/s over the timed steps: the 64 queries of a step as one batch, one
+  64-column refined solve per system through the FP64 inverse factors.
+- ``served_mixed``: the same batch through FP32 inverse factors with FP64 refinement (the
+  mixed-precision form), timed beside it, with its refinement rounds and its error against the
+  served fields.
+- ``plain_fp64``: the reference's form, one query per call through the FP64 factors.
+- ``roofline``: the triangular sweep of the served path (forward + backward, the dominant kernel),
+  algorithmic bytes per sweep over its HIP-event-measured mean duration against the 8 TB/s HBM
+  peak; ``traffic`` is the HBM bytes per sweep that rocprofv3's FETCH_SIZE / WRITE_SIZE passes
+  measured (profiles/triangular_sweep_torus1024x512_traffic.json, scaled by a 1 GiB copy:
   ``bench.py --calibrate-copy``), null when the sweep kernels' source has changed since.
 - ``cpu_baseline``: the FP64 oracle (SciPy SuperLU, one thread) on a bounded sample of the same
   queries on this host, factorization excluded as for the GPU; its results also give the error
@@ -49,6 +53,7 @@ def parse() -> argparse.Namespace:
     parser.add_argument("--plain-queries", type=int, default=8)
     parser.add_argument("--cpu-sample", type=int, default=4)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-mixed", action="store_true", help="skip the served_mixed line")
     parser.add_argument("--major", type=int, default=1024)
     parser.add_argument("--minor", type=int, default=512)
     parser.add_argument(
@@ -58,6 +63,10 @@ def parse() -> argparse.Namespace:
         "and WRITE_SIZE counters) and exit",
     )
     return parser.parse_args()
+
+
+def log(message: str) -> None:
+    print(message, file=sys.stderr, flush=True)
 
 
 def calibrate_copy() -> int:
@@ -76,7 +85,7 @@ def calibrate_copy() -> int:
 TRAFFIC_FILE = ROOT / "profiles" / "triangular_sweep_torus1024x512_traffic.json"
 
 
-def measured_traffic(columns: int, solve_block_entries: int) -> float | None:
+def measured_traffic(columns: int, solve_block_entries: int, factor_bytes: int) -> float | None:
     """HBM bytes per sweep from the committed FETCH_SIZE / WRITE_SIZE passes, when they were taken
     on this workload with the sweep kernels' current source; otherwise None."""
     import hashlib
@@ -90,24 +99,49 @@ def measured_traffic(columns: int, solve_block_entries: int) -> float | None:
         record.get("solve_hip_sha256_16") != hashlib.sha256(source).hexdigest()[:16]
         or record.get("columns") != columns
         or record.get("solve_block_entries") != solve_block_entries
+        or record.get("factor_bytes") != factor_bytes
     ):
         return None
     return float(record["traffic_bytes_per_sweep"])
 
 
-def log(message: str) -> None:
-    print(message, file=sys.stderr, flush=True)
-
-
 def sweep_bytes(stats: dict[str, int], columns: int, factor_bytes: int) -> float:
-    """Algorithmic bytes of one forward + backward sweep over `columns` right-hand sides:
-    the solve blocks read twice, and per column 9 n + 3 (sum of structure rows) float64 vector
-    touches (permute in/out, b read, y write/read, update vectors write/read, x gather, x write,
-    scatter-add read/write)."""
+    """Algorithmic bytes of one forward + backward sweep over `columns` right-hand sides: the
+    inverse factors read twice, and per column these float64 vector touches -- forward: r read,
+    y write (n each), update vectors written and gathered by the parents (structure rows each);
+    backward: y read, x read, d write, x + d write (n each), d gathered at the structure rows."""
     n = stats["rows"]
     structure_rows = stats["front_vector_entries"] - n
     w = stats["solve_block_entries"]
-    return 2.0 * w * factor_bytes + columns * (9.0 * n + 3.0 * structure_rows) * 8.0
+    return 2.0 * w * factor_bytes + columns * (6.0 * n + 3.0 * structure_rows) * 8.0
+
+
+def timed_steps(solver, source_sets, steps: int, warmup: int):
+    """Warm up, then time `steps` batches with HIP-event phase timing on; returns (seconds,
+    per-step latencies, phase timing, last output, last stats)."""
+    import torch
+
+    for _ in range(warmup):
+        solver.geodesic(source_sets)
+    solver.enable_timing(True)
+    solver.read_timing(reset=True)
+    torch.cuda.synchronize()
+    latencies = []
+    out = None
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        out = solver.geodesic(source_sets)
+        torch.cuda.synchronize()
+        latencies.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - t_start
+    timing = solver.read_timing(reset=True)
+    solver.enable_timing(False)
+    return elapsed, latencies, timing, out, dict(solver.last_stats)
+
+
+def ms(values, q) -> float:
+    return round(1e3 * float(np.percentile(values, q)), 3)
 
 
 def main() -> int:
@@ -136,80 +170,55 @@ def main() -> int:
     v_gpu = torch.from_numpy(vertices).cuda()
     f_gpu = torch.from_numpy(faces).cuda()
     source_sets = [torch.tensor([s], dtype=torch.int32, device="cuda") for s in sources]
-
-    mesh_served = heat.Trimesh(v_gpu, f_gpu)
-    served = mesh_served.heat_solver(method="meshlab", precision="mixed")
-    mesh_plain = heat.Trimesh(v_gpu, f_gpu)
-    plain = mesh_plain.heat_solver(method="meshlab", precision="fp64")
+    k = len(source_sets)
 
     log("== factoring (setup)")
+    mesh = heat.Trimesh(v_gpu, f_gpu)
+    solver = mesh.heat_solver(method="meshlab", precision="fp64")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    assert served.factor()
+    assert solver.factor()
     torch.cuda.synchronize()
-    setup_served = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    assert plain.factor()
-    torch.cuda.synchronize()
-    setup_plain = time.perf_counter() - t0
-    stats = served.plan_stats()
-    log(f"   factor mixed {setup_served:.2f}s fp64 {setup_plain:.2f}s plan {stats}")
-    # The first solve after factoring warms the solve-side allocations; keep it off the clock.
-    served._solves = {"heat": 2, "poisson": 2}
-    plain._solves = {"heat": 2, "poisson": 2}
+    setup_fp64 = time.perf_counter() - t0
+    stats = solver.plan_stats()
+    log(f"   factor fp64 {setup_fp64:.3f}s plan {stats}")
+    # Factored: every later solve goes through the held factors (the rent-or-buy count is past).
+    solver._solves = {"heat": 2, "poisson": 2}
 
-    log(f"== served warmup {args.warmup}")
-    for _ in range(args.warmup):
-        served.geodesic(source_sets)
-    served.enable_timing(True)
-    served.read_timing(reset=True)
-    torch.cuda.synchronize()
-    log(f"== served timed {args.steps} steps of {len(source_sets)} queries")
-    latencies = []
-    out = None
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        out = served.geodesic(source_sets)
-        torch.cuda.synchronize()
-        latencies.append(time.perf_counter() - s0)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    timing = served.read_timing(reset=True)
-    served.enable_timing(False)
-    last_stats = dict(served.last_stats)
-    value = len(source_sets) * args.steps / elapsed
-    log(f"   served {value:.1f} queries/s, step {1e3 * elapsed / args.steps:.2f} ms, stats {last_stats}")
+    log(f"== served: {args.steps} steps of {k} queries, FP64 inverse factors")
+    elapsed, latencies, timing, out, served_stats = timed_steps(solver, source_sets, args.steps, args.warmup)
+    value = k * args.steps / elapsed
+    log(f"   served {value:.1f} queries/s, step {1e3 * elapsed / args.steps:.2f} ms, stats {served_stats}")
+    served_host = out.cpu().numpy()
 
     log(f"== plain FP64, one query per call, {args.plain_queries} queries")
-    plain.geodesic([source_sets[0]])
+    solver.geodesic([source_sets[0]])
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     plain_lat = []
     plain_out = []
+    t0 = time.perf_counter()
     for q in range(args.plain_queries):
         s0 = time.perf_counter()
-        plain_out.append(plain.geodesic([source_sets[q % len(source_sets)]])[0])
+        plain_out.append(solver.geodesic([source_sets[q % k]])[0])
         torch.cuda.synchronize()
         plain_lat.append(time.perf_counter() - s0)
-    plain_elapsed = time.perf_counter() - t0
-    plain_value = args.plain_queries / plain_elapsed
+    plain_value = args.plain_queries / (time.perf_counter() - t0)
+    plain_stats = dict(solver.last_stats)
 
-    # Roofline of the dominant kernel: the heat and Poisson triangular sweeps.
+    # Roofline of the dominant kernel: the served path's triangular sweeps (heat and Poisson).
     sweep_ms = timing["heat_sweep"][0] + timing["poisson_sweep"][0]
     sweep_count = timing["heat_sweep"][1] + timing["poisson_sweep"][1]
     mean_sweep_s = sweep_ms / max(sweep_count, 1) / 1e3
-    bytes_per_sweep = sweep_bytes(stats, len(source_sets), 4)
+    bytes_per_sweep = sweep_bytes(stats, k, 8)
     achieved = bytes_per_sweep / mean_sweep_s / 1e9 if mean_sweep_s > 0 else 0.0
-    traffic = measured_traffic(len(source_sets), stats["solve_block_entries"])
     roofline = {
         "bound": "hbm",
-        "kernel": "triangular sweep (forward + backward level sweeps, mixed FP32 solve blocks)",
+        "kernel": "triangular sweep (forward + backward level sweeps, FP64 inverse factors, 64 columns)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
+        "traffic": measured_traffic(k, stats["solve_block_entries"], 8),
         "bytes_per_launch": bytes_per_sweep,
         "mean_launch_ms": round(mean_sweep_s * 1e3, 4),
         "launches": sweep_count,
@@ -226,34 +235,63 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 solve blocks, f64 vectors/refinement",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": f"torus {args.major}x{args.minor} R=1 r=0.3 normal jitter 0.1h, "
-            f"{len(source_sets)} single-source queries per step, factors resident",
+            f"{k} single-source queries per step as one batch, factors resident",
             "n_vertices": int(n),
             "n_faces": int(faces.size // 3),
-            "queries_per_step": len(source_sets),
-            "t": served.time,
+            "queries_per_step": k,
+            "t": solver.time,
             "supernodes": stats["supernodes"],
             "levels": stats["levels"],
             "solve_block_entries": stats["solve_block_entries"],
         },
-        "p50_ms": round(1e3 * float(np.percentile(latencies, 50)), 3),
-        "p99_ms": round(1e3 * float(np.percentile(latencies, 99)), 3),
+        "p50_ms": ms(latencies, 50),
+        "p99_ms": ms(latencies, 99),
+        "latency_note": "a query's latency is its step's: all 64 complete together",
         "plain_fp64": {
             "value": round(plain_value, 2),
             "unit": "queries/s",
-            "p50_ms": round(1e3 * float(np.percentile(plain_lat, 50)), 3),
-            "p99_ms": round(1e3 * float(np.percentile(plain_lat, 99)), 3),
+            "p50_ms": ms(plain_lat, 50),
+            "p99_ms": ms(plain_lat, 99),
             "queries": args.plain_queries,
+            "refinements": plain_stats,
         },
         "served_vs_plain": round(value / plain_value, 2) if plain_value > 0 else None,
-        "setup_s": {"factor_mixed": round(setup_served, 3), "factor_fp64": round(setup_plain, 3)},
-        "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timing.items() if v[1]},
-        "refinements": last_stats,
+        "setup_s": {"factor_fp64": round(setup_fp64, 3)},
+        "phase_ms_per_step": {key: round(v[0] / args.steps, 3) for key, v in timing.items() if v[1]},
+        "refinements": served_stats,
         "roofline": roofline,
     }
+
+    if not args.no_mixed:
+        log("== served_mixed: the same batches through FP32 inverse factors")
+        mesh_mixed = heat.Trimesh(v_gpu, f_gpu)
+        mixed = mesh_mixed.heat_solver(method="meshlab", precision="mixed")
+        t0 = time.perf_counter()
+        assert mixed.factor()
+        torch.cuda.synchronize()
+        result["setup_s"]["factor_mixed"] = round(time.perf_counter() - t0, 3)
+        mixed._solves = {"heat": 2, "poisson": 2}
+        m_elapsed, m_lat, m_timing, m_out, m_stats = timed_steps(mixed, source_sets, args.steps, 1)
+        mixed_host = m_out.cpu().numpy()
+        from_served = [
+            float(np.max(np.abs(mixed_host[q] - served_host[q])) / np.ptp(served_host[q]))
+            for q in range(k)
+        ]
+        result["served_mixed"] = {
+            "value": round(k * args.steps / m_elapsed, 2),
+            "unit": "queries/s",
+            "ms_per_step": round(1e3 * m_elapsed / args.steps, 3),
+            "p50_ms": ms(m_lat, 50),
+            "p99_ms": ms(m_lat, 99),
+            "refinements": m_stats,
+            "max_rel_diff_from_served": max(from_served),
+            "phase_ms_per_step": {key: round(v[0] / args.steps, 3) for key, v in m_timing.items() if v[1]},
+        }
+        del mixed, mesh_mixed
 
     if not args.no_cpu_baseline:
         from oracle import heat_oracle as ho
@@ -262,17 +300,16 @@ def main() -> int:
         t0 = time.perf_counter()
         oracle = ho.FactoredOracle(vertices, faces)
         oracle_setup = time.perf_counter() - t0
-        sample = list(range(min(args.cpu_sample, len(source_sets))))
-        errs = []
-        plain_errs = []
+        sample = list(range(min(args.cpu_sample, k)))
         t0 = time.perf_counter()
         expected = [oracle.distance(np.array([sources[q]])) for q in sample]
         cpu_elapsed = time.perf_counter() - t0
-        served_host = out.cpu().numpy()
-        for i, q in enumerate(sample):
-            errs.append(ho.relative_error(served_host[q], expected[i]))
-            if q < len(plain_out):
-                plain_errs.append(ho.relative_error(plain_out[q].cpu().numpy(), expected[i]))
+        errs = [ho.relative_error(served_host[q], expected[i]) for i, q in enumerate(sample)]
+        plain_errs = [
+            ho.relative_error(plain_out[q].cpu().numpy(), expected[i])
+            for i, q in enumerate(sample)
+            if q < len(plain_out)
+        ]
         result["cpu_baseline"] = {
             "value": round(len(sample) / cpu_elapsed, 4),
             "unit": "queries/s",

@@ -125,6 +125,16 @@ int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_fa
 int hg_factored(hg_mesh* mesh, int32_t* out_systems);
 
 /*
+ * Refinement targets of the factored solves at one precision: the heat step's componentwise
+ * tolerance and the Poisson step's normwise one, each in (0, 1); NaN keeps the current value. No
+ * counterpart in the reference, whose targets are fixed (linalg.py:181, heat.py:108); the
+ * defaults are those for HG_PRECISION_FP64 and 1e-6 / 1e-6 for HG_PRECISION_MIXED. A serving
+ * layer sets them from the accuracy its queries ask for.
+ */
+int hg_set_refinement(hg_mesh* mesh, int32_t precision, double heat_tolerance,
+                      double poisson_tolerance);
+
+/*
  * Diffuse: solve (M - tL) u = rhs for n_columns columns of V entries (HeatSolver.diffuse,
  * heat.py:3890-3914). HG_MODE_ITERATE runs the settled Jacobi-CG (solve_spd_settled,
  * linalg.py:2490-2727) and its componentwise backward-error check; on failure it factors the
@@ -177,6 +187,33 @@ int hg_shift_to_sources(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
 int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_offsets,
                 int64_t n_queries, double* d_out, int32_t mode, int32_t precision,
                 hg_solve_stats* out_stats);
+
+/*
+ * Packed meshes: many meshes bound as one block-diagonal system, so every solve, level sweep and
+ * factorization launch covers all of them at once. No counterpart in the reference, which solves
+ * one mesh per call (heat.py:370-609); each mesh keeps what it would have alone (its own diffusion
+ * time rule, components, pins and shift). Mesh s has vertices
+ * d_vertices[vertex_offsets[s] .. vertex_offsets[s+1]) and faces
+ * d_faces[3 face_offsets[s] .. 3 face_offsets[s+1]) indexing its own vertices from 0; the offset
+ * arrays are host arrays of n_meshes + 1 entries starting at 0. Solves run in HG_MODE_FACTORED;
+ * hg_geodesic, hg_shift_to_sources and HG_MODE_ITERATE refuse a packed mesh (HG_ERR_VALUE).
+ */
+int hg_mesh_create_packed(const float* d_vertices, const int64_t* vertex_offsets,
+                          const int32_t* d_faces, const int64_t* face_offsets, int64_t n_meshes,
+                          void* stream, hg_mesh** out_mesh);
+
+/*
+ * n_queries source sets on a packed mesh: query q asks for mesh query_mesh[q] (host array) from
+ * the sources d_sources[off[q]..off[q+1]) (indices of that mesh's vertices; source_offsets is a
+ * host array). A mesh's queries fill columns 0, 1, ... of one batched solve. Query q's distance
+ * (its mesh's V_q entries, float64) goes to d_out at the sum of the earlier queries' V.
+ */
+int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_offsets,
+                       const int32_t* query_mesh, int64_t n_queries, double* d_out,
+                       int32_t precision, hg_solve_stats* out_stats);
+
+/* Each packed mesh's diffusion time (n_meshes entries; assembles at the default time if needed). */
+int hg_packed_times(hg_mesh* mesh, double* out_times);
 
 /*
  * Device-time instrumentation (no counterpart in the reference, which quotes measurements in

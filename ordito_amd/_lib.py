@@ -41,12 +41,16 @@ EXPORTED = (
     "hg_operator_views",
     "hg_factor",
     "hg_factored",
+    "hg_set_refinement",
     "hg_diffuse",
     "hg_solve_poisson",
     "hg_unit_gradient_divergence",
     "hg_seed_sources",
     "hg_shift_to_sources",
     "hg_geodesic",
+    "hg_mesh_create_packed",
+    "hg_geodesic_packed",
+    "hg_packed_times",
     "hg_nbytes",
     "hg_release",
     "hg_plan_stats",
@@ -118,12 +122,19 @@ _SIGNATURES = {
     "hg_operator_views": ([_P, ctypes.POINTER(OperatorView)], ctypes.c_int),
     "hg_factor": ([_P, _I32, _I32, _PI32], ctypes.c_int),
     "hg_factored": ([_P, _PI32], ctypes.c_int),
+    "hg_set_refinement": ([_P, _I32, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "hg_diffuse": ([_P, _P, _P, _I64, _I32, _I32, _PI32], ctypes.c_int),
     "hg_solve_poisson": ([_P, _P, _P, _I64, _I32, _I32], ctypes.c_int),
     "hg_unit_gradient_divergence": ([_P, _P, _P, _I64], ctypes.c_int),
     "hg_seed_sources": ([_P, _P, _PI64, _I64, _P], ctypes.c_int),
     "hg_shift_to_sources": ([_P, _P, _P, _PI64, _I64], ctypes.c_int),
     "hg_geodesic": ([_P, _P, _PI64, _I64, _P, _I32, _I32, ctypes.POINTER(SolveStats)], ctypes.c_int),
+    "hg_mesh_create_packed": ([_P, _PI64, _P, _PI64, _I64, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "hg_geodesic_packed": (
+        [_P, _P, _PI64, _PI32, _I64, _P, _I32, ctypes.POINTER(SolveStats)],
+        ctypes.c_int,
+    ),
+    "hg_packed_times": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "hg_nbytes": ([_P, _PI64], ctypes.c_int),
     "hg_release": ([_P], ctypes.c_int),
     "hg_plan_stats": ([_P, _PI64], ctypes.c_int),

@@ -151,9 +151,19 @@ Plan analyse(const HostPattern& pattern, const float* coordinates, const Analysi
   const int64_t n = pattern.n;
   plan.n = n;
   Dissector dissector(pattern, coordinates, std::max(options.leaf_rows, 1));
-  std::vector<int32_t> all(static_cast<size_t>(n));
-  std::iota(all.begin(), all.end(), 0);
-  std::vector<int32_t> roots = dissector.dissect(std::move(all));
+  // Each connected component is dissected on its own, so a block-diagonal system (several meshes
+  // packed into one) gets one tree per block and every block's leaves sit on level 0.
+  std::vector<int32_t> roots;
+  {
+    int32_t n_comp = 0;
+    const std::vector<int32_t> label = component_labels(pattern, &n_comp);
+    std::vector<std::vector<int32_t>> members(static_cast<size_t>(n_comp));
+    for (int64_t v = 0; v < n; ++v) members[label[v]].push_back(static_cast<int32_t>(v));
+    for (std::vector<int32_t>& set : members) {
+      std::vector<int32_t> r = dissector.dissect(std::move(set));
+      roots.insert(roots.end(), r.begin(), r.end());
+    }
+  }
   std::vector<Node>& nodes = dissector.nodes;
 
   // Merge the top of each tree into its root while the root holds at most top_rows rows.

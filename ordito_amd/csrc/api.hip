@@ -44,6 +44,12 @@ struct hg_mesh {
   int64_t n = 0, f = 0;
   std::vector<float> h_xyz;
   std::vector<int32_t> h_faces;
+  // Segments: meshes packed block-diagonally into this one (hg_mesh_create_packed); segment s owns
+  // vertices [seg_v[s], seg_v[s+1]) and faces [seg_f[s], seg_f[s+1]). A plain mesh is one segment.
+  std::vector<int64_t> seg_v{0, 0}, seg_f{0, 0};
+  std::vector<double> seg_t;
+  DeviceArray<int32_t> row_segment;  // packed meshes only
+  int64_t segments() const { return static_cast<int64_t>(seg_v.size()) - 1; }
   HostPattern pattern;
   DeviceArray<float> xyz;
   DeviceArray<int32_t> faces, offsets, columns, corner_ptr, corner_slots, vertex_ptr,
@@ -66,6 +72,10 @@ struct hg_mesh {
   // Cholesky.
   std::unique_ptr<DevicePlan> plan;
   Factor heat_factor, poisson_factor;
+  // Refinement targets of the factored solves, [precision][0 heat (componentwise), 1 Poisson
+  // (normwise)].
+  double refine_tolerance[2][2] = {{kSettleDirectTolerance, kPoissonTolerance},
+                                   {kMixedHeatTolerance, kMixedPoissonTolerance}};
   // Workspaces.
   CgWorkspace cg;
   SolveWorkspace solve;
@@ -74,6 +84,8 @@ struct hg_mesh {
   DeviceArray<int32_t> d_sources, invalid;
   DeviceArray<int64_t> d_owner, d_offsets;
   int64_t field_capacity = 0, corner_capacity = 0;
+  DeviceArray<double> packed_phi;  // packed meshes: the solved slot columns before extraction
+  int64_t packed_capacity = 0;
   hg_solve_stats stats{};
   PhaseTimer timer;
 };
@@ -101,13 +113,21 @@ void require_mesh(hg_mesh* mesh) {
   HG_REQUIRE(mesh != nullptr, HG_ERR_RUNTIME, "the mesh handle is null (released or never created)");
 }
 
+// Packed meshes solve through the factors only: the settled iteration's window and stopping rule
+// are per mesh (heat.py:5234-5243), which one packed iteration cannot honour.
+void require_single_or_factored(const hg_mesh& m, int32_t mode) {
+  HG_REQUIRE(m.segments() == 1 || mode == HG_MODE_FACTORED, HG_ERR_VALUE,
+             "packed meshes solve in HG_MODE_FACTORED only");
+}
+
 // ---- operators --------------------------------------------------------------------------------
 
-// The mean unique-edge length: each edge once, over the pattern's strict upper triangle.
-double host_mean_unique_edge(const hg_mesh& m) {
+// The mean unique-edge length of segment s: each edge once, over the pattern's strict upper
+// triangle.
+double host_mean_unique_edge(const hg_mesh& m, int64_t s) {
   double total = 0.0;
   int64_t count = 0;
-  for (int64_t i = 0; i < m.n; ++i) {
+  for (int64_t i = m.seg_v[s]; i < m.seg_v[s + 1]; ++i) {
     for (int32_t k = m.pattern.offsets[i]; k < m.pattern.offsets[i + 1]; ++k) {
       int32_t j = m.pattern.columns[k];
       if (j <= i) continue;
@@ -121,10 +141,11 @@ double host_mean_unique_edge(const hg_mesh& m) {
   return count > 0 ? total / static_cast<double>(count) : 0.0;
 }
 
-double host_mean_face_edge(const hg_mesh& m) {
-  if (m.f == 0) return 0.0;
+double host_mean_face_edge(const hg_mesh& m, int64_t s) {
+  const int64_t f0 = m.seg_f[s], f1 = m.seg_f[s + 1];
+  if (f1 == f0) return 0.0;
   double total = 0.0;
-  for (int64_t f = 0; f < m.f; ++f) {
+  for (int64_t f = f0; f < f1; ++f) {
     for (int e = 0; e < 3; ++e) {
       const float* a = &m.h_xyz[3 * m.h_faces[3 * f + e]];
       const float* b = &m.h_xyz[3 * m.h_faces[3 * f + (e + 1) % 3]];
@@ -132,23 +153,23 @@ double host_mean_face_edge(const hg_mesh& m) {
       total += static_cast<double>(sqrtf(dx * dx + dy * dy + dz * dz));
     }
   }
-  return total / static_cast<double>(3 * m.f);
+  return total / static_cast<double>(3 * (f1 - f0));
 }
 
-double host_diffusion_floor(const hg_mesh& m) {
-  if (m.n == 0) return 0.0;
+double host_diffusion_floor(const hg_mesh& m, int64_t s) {
+  if (m.seg_v[s + 1] == m.seg_v[s]) return 0.0;
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t v = 0; v < m.n; ++v)
+  for (int64_t v = m.seg_v[s]; v < m.seg_v[s + 1]; ++v)
     for (int d = 0; d < 3; ++d) {
       lo[d] = std::min(lo[d], m.h_xyz[3 * v + d]);
       hi[d] = std::max(hi[d], m.h_xyz[3 * v + d]);
     }
-  double s = 0.0;
+  double squares = 0.0;
   for (int d = 0; d < 3; ++d) {
     double e = static_cast<double>(hi[d]) - static_cast<double>(lo[d]);
-    s += e * e;
+    squares += e * e;
   }
-  double diag = std::sqrt(s);
+  double diag = std::sqrt(squares);
   return (diag / kDiffusionDecay) * (diag / kDiffusionDecay);
 }
 
@@ -156,19 +177,29 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   const hipStream_t st = m.stream;
   const int64_t n = m.n, nnz = static_cast<int64_t>(m.pattern.columns.size());
   const int32_t face_edges = method == HG_METHOD_MESHLAB ? 1 : 0;
-  // The time rule of HeatSolver._vertex_time (heat.py:3714-3736) and _settle_window (5234-5243).
-  const double mean = face_edges ? host_mean_face_edge(m) : host_mean_unique_edge(m);
-  const double def = mean * mean;
-  double t_value = t_request;
-  int32_t from_edges = 0;
-  if (std::isnan(t_request)) {
-    const double floor = solver_default ? host_diffusion_floor(m) : 0.0;
-    if (floor > def) t_value = floor;
-    else from_edges = 1;
+  // The time rule of HeatSolver._vertex_time (heat.py:3714-3736) and _settle_window (5234-5243),
+  // per segment: each packed mesh keeps the time it would have alone.
+  const int64_t n_seg = m.segments();
+  std::vector<double> t_values(n_seg);
+  std::vector<int32_t> from_edges(n_seg, 0);
+  for (int64_t s = 0; s < n_seg; ++s) {
+    const double mean = face_edges ? host_mean_face_edge(m, s) : host_mean_unique_edge(m, s);
+    const double def = mean * mean;
+    double t_value = t_request;
+    if (std::isnan(t_request)) {
+      const double floor = solver_default ? host_diffusion_floor(m, s) : 0.0;
+      if (floor > def) t_value = floor;
+      else from_edges[s] = 1;
+    }
+    t_values[s] = t_value;
+    if (s == 0) {
+      if (from_edges[s] || !(t_value > def && def > 0.0)) m.settle_rounds = kHeatSettleRounds;
+      else
+        m.settle_rounds =
+            static_cast<int32_t>(std::ceil(kHeatSettleRounds * std::sqrt(t_value / def)));
+      m.time_from_edges = from_edges[s];
+    }
   }
-  if (from_edges || !(t_value > def && def > 0.0)) m.settle_rounds = kHeatSettleRounds;
-  else m.settle_rounds = static_cast<int32_t>(std::ceil(kHeatSettleRounds * std::sqrt(t_value / def)));
-  m.time_from_edges = from_edges;
 
   m.cot.allocate(3 * m.f);
   m.normals.allocate(3 * m.f);
@@ -180,19 +211,30 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   m.heat32.allocate(nnz);
   m.heat_inv.allocate(n);
   m.poisson_inv.allocate(n);
-  m.scalars.allocate(4);
-  m.edge_partials.allocate(2 * edge_sum_blocks(n));
+  m.scalars.allocate(4 * n_seg);
+  int64_t longest = 0;
+  for (int64_t s = 0; s < n_seg; ++s) longest = std::max(longest, m.seg_v[s + 1] - m.seg_v[s]);
+  m.edge_partials.allocate(2 * edge_sum_blocks(longest));
   launch_face_terms(m.xyz.ptr, m.faces.ptr, m.f, m.cot.ptr, m.normals.ptr, m.areas.ptr, st);
   launch_mass(m.vertex_ptr.ptr, m.vertex_corners.ptr, m.areas.ptr, n, m.mass.ptr, st);
-  launch_edge_sums(m.offsets.ptr, m.columns.ptr, m.corner_ptr.ptr, m.xyz.ptr, n, face_edges,
-                   m.edge_partials.ptr, from_edges, from_edges ? 0.0 : t_value, m.scalars.ptr, st);
+  // One edge-sum reduction per segment, its blocks counted from the segment's first row: the sums
+  // (and so the times) are those the segment's mesh forms alone. The launches share the partials
+  // buffer in stream order.
+  for (int64_t s = 0; s < n_seg; ++s)
+    launch_edge_sums(m.offsets.ptr, m.columns.ptr, m.corner_ptr.ptr, m.xyz.ptr, m.seg_v[s],
+                     m.seg_v[s + 1] - m.seg_v[s], face_edges, m.edge_partials.ptr, from_edges[s],
+                     from_edges[s] ? 0.0 : t_values[s], m.scalars.ptr + 4 * s, st);
   launch_operator_rows(m.offsets.ptr, m.columns.ptr, m.corner_ptr.ptr, m.corner_slots.ptr,
-                       m.cot.ptr, m.mass.ptr, m.scalars.ptr, n, m.lap.ptr, m.heat.ptr,
-                       m.poisson.ptr, m.heat32.ptr, m.heat_inv.ptr, m.poisson_inv.ptr, st);
-  double host_scalars[4] = {0, 0, 0, 0};
-  HG_HIP(hipMemcpyAsync(host_scalars, m.scalars.ptr, sizeof(host_scalars), hipMemcpyDeviceToHost, st));
+                       m.cot.ptr, m.mass.ptr, m.scalars.ptr, n_seg > 1 ? m.row_segment.ptr : nullptr,
+                       n, m.lap.ptr, m.heat.ptr, m.poisson.ptr, m.heat32.ptr, m.heat_inv.ptr,
+                       m.poisson_inv.ptr, st);
+  std::vector<double> host_scalars(4 * n_seg, 0.0);
+  HG_HIP(hipMemcpyAsync(host_scalars.data(), m.scalars.ptr, sizeof(double) * 4 * n_seg,
+                        hipMemcpyDeviceToHost, st));
   HG_HIP(hipStreamSynchronize(st));
-  m.t = host_scalars[3];
+  m.seg_t.resize(n_seg);
+  for (int64_t s = 0; s < n_seg; ++s) m.seg_t[s] = host_scalars[4 * s + 3];
+  m.t = m.seg_t[0];
   m.assembled = true;
   m.poisson_bound = -1.0;
   m.heat_factor.reset();
@@ -312,7 +354,7 @@ int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, in
   const bool mixed = precision == HG_PRECISION_MIXED;
   factor_system(m, HG_SYSTEM_HEAT, mixed);
   return refined_solve(*m.plan, m.heat_factor, mixed, rhs, x, k,
-                       mixed ? kMixedHeatTolerance : kSettleDirectTolerance, true,
+                       m.refine_tolerance[mixed ? 1 : 0][0], true,
                        mixed ? kMixedRefineRounds : kRefineRounds,
                        m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr, m.solve, m.stream,
                        &m.timer, kPhaseHeatSweep);
@@ -363,8 +405,7 @@ void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t 
     const bool mixed = precision == HG_PRECISION_MIXED;
     factor_system(m, HG_SYSTEM_POISSON, mixed);
     m.stats.poisson_refinements = refined_solve(
-        *m.plan, m.poisson_factor, mixed, rhs, x, k,
-        mixed ? kMixedPoissonTolerance : kPoissonTolerance, false,
+        *m.plan, m.poisson_factor, mixed, rhs, x, k, m.refine_tolerance[mixed ? 1 : 0][1], false,
         mixed ? kMixedRefineRounds : kRefineRounds, m.poisson_pinned_new.ptr, m.solve, m.stream,
         &m.timer, kPhasePoissonSweep);
     m.stats.poisson_rounds = 0;
@@ -440,21 +481,70 @@ void divergence(hg_mesh& m, const double* heat, double* out, int64_t k) {
                     out, m.stream);
 }
 
-void shift(hg_mesh& m, double* phi, int64_t k) {
-  int32_t blocks = shift_partial_blocks(m.n);
-  if (m.shift_partials.size < blocks * k) m.shift_partials.allocate(blocks * k);
-  if (m.shift_values.size < 2 * k) m.shift_values.allocate(2 * k);
-  PhaseScope scope(&m.timer, kPhaseShift, m.stream);
-  launch_shift(phi, m.d_sources.ptr, m.d_offsets.ptr, m.n, k, m.shift_partials.ptr,
-               m.shift_values.ptr, m.stream);
-}
-
 double* reserve_fields(hg_mesh& m, int64_t k) {
   if (m.field_capacity < 3 * m.n * k) {
     m.fields.allocate(std::max<int64_t>(3 * m.n * k, 1));
     m.field_capacity = 3 * m.n * k;
   }
   return m.fields.ptr;
+}
+
+void shift(hg_mesh& m, double* phi, int64_t k) {
+  int32_t blocks = shift_partial_blocks(m.n);
+  if (m.shift_partials.size < blocks * k) m.shift_partials.allocate(blocks * k);
+  if (m.shift_values.size < 2 * k) m.shift_values.allocate(2 * k);
+  PhaseScope scope(&m.timer, kPhaseShift, m.stream);
+  launch_shift(phi, m.d_sources.ptr, m.d_offsets.ptr, m.n, k, nullptr, nullptr,
+               m.shift_partials.ptr, m.shift_values.ptr, m.stream);
+}
+
+// Seed, heat step, divergence and Poisson step over k columns of the whole (possibly packed)
+// system: the unshifted distance into phi. The sources must be uploaded.
+void solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, int32_t mode,
+                   int32_t precision) {
+  const int64_t n = m.n;
+  double* fields = reserve_fields(m, k);
+  double* u0 = fields;
+  double* heat = fields + n * k;
+  double* negdiv = fields + 2 * n * k;
+  seed(m, offsets, k, u0);
+  int32_t flags = diffuse(m, u0, heat, k, mode, precision);
+  m.stats.backward_error_failed = flags & 1;
+  divergence(m, heat, negdiv, k);
+  solve_poisson(m, negdiv, phi, k, mode, precision);
+}
+
+// The host and device state of a mesh from its host positions and faces (global indices).
+void bind(hg_mesh& m) {
+  const int64_t n_vertices = m.n;
+  m.pattern = build_pattern(n_vertices, m.h_faces.data(), m.f);
+  m.labels = component_labels(m.pattern, &m.n_comp);
+  std::vector<int32_t> comp_ptr(m.n_comp + 1, 0), members(n_vertices);
+  for (int64_t v = 0; v < n_vertices; ++v) comp_ptr[m.labels[v] + 1]++;
+  for (int32_t c = 0; c < m.n_comp; ++c) comp_ptr[c + 1] += comp_ptr[c];
+  {
+    std::vector<int32_t> at(comp_ptr.begin(), comp_ptr.end() - 1);
+    for (int64_t v = 0; v < n_vertices; ++v) members[at[m.labels[v]]++] = static_cast<int32_t>(v);
+  }
+  hipStream_t st = m.stream;
+  m.xyz.upload(m.h_xyz, st);
+  m.faces.upload(m.h_faces, st);
+  m.offsets.upload(m.pattern.offsets, st);
+  m.columns.upload(m.pattern.columns, st);
+  m.corner_ptr.upload(m.pattern.corner_ptr, st);
+  m.corner_slots.upload(m.pattern.corner_slots, st);
+  m.vertex_ptr.upload(m.pattern.vertex_ptr, st);
+  m.vertex_corners.upload(m.pattern.vertex_corners, st);
+  m.comp_ptr.upload(comp_ptr, st);
+  m.null_space.build(comp_ptr, st);
+  m.comp_members.upload(members, st);
+  if (m.segments() > 1) {
+    std::vector<int32_t> seg(static_cast<size_t>(n_vertices));
+    for (int64_t s = 0; s < m.segments(); ++s)
+      std::fill(seg.begin() + m.seg_v[s], seg.begin() + m.seg_v[s + 1], static_cast<int32_t>(s));
+    m.row_segment.upload(seg, st);
+  }
+  HG_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -476,34 +566,58 @@ int hg_mesh_create(const float* d_vertices, int64_t n_vertices, const int32_t* d
     m->stream = static_cast<hipStream_t>(stream);
     m->n = n_vertices;
     m->f = n_faces;
+    m->seg_v = {0, n_vertices};
+    m->seg_f = {0, n_faces};
     m->h_xyz.resize(3 * n_vertices);
     m->h_faces.resize(3 * n_faces);
     if (n_vertices > 0)
       HG_HIP(hipMemcpy(m->h_xyz.data(), d_vertices, sizeof(float) * 3 * n_vertices, hipMemcpyDefault));
     if (n_faces > 0)
       HG_HIP(hipMemcpy(m->h_faces.data(), d_faces, sizeof(int32_t) * 3 * n_faces, hipMemcpyDefault));
-    m->pattern = build_pattern(n_vertices, m->h_faces.data(), n_faces);
-    m->labels = component_labels(m->pattern, &m->n_comp);
-    std::vector<int32_t> comp_ptr(m->n_comp + 1, 0), members(n_vertices);
-    for (int64_t v = 0; v < n_vertices; ++v) comp_ptr[m->labels[v] + 1]++;
-    for (int32_t c = 0; c < m->n_comp; ++c) comp_ptr[c + 1] += comp_ptr[c];
-    {
-      std::vector<int32_t> at(comp_ptr.begin(), comp_ptr.end() - 1);
-      for (int64_t v = 0; v < n_vertices; ++v) members[at[m->labels[v]]++] = static_cast<int32_t>(v);
+    bind(*m);
+    *out_mesh = m.release();
+  });
+}
+
+int hg_mesh_create_packed(const float* d_vertices, const int64_t* vertex_offsets,
+                          const int32_t* d_faces, const int64_t* face_offsets, int64_t n_meshes,
+                          void* stream, hg_mesh** out_mesh) {
+  return guarded([&] {
+    HG_REQUIRE(out_mesh != nullptr && vertex_offsets != nullptr && face_offsets != nullptr,
+               HG_ERR_VALUE, "null argument");
+    HG_REQUIRE(n_meshes >= 1, HG_ERR_VALUE, "a packed mesh needs at least one mesh");
+    HG_REQUIRE(vertex_offsets[0] == 0 && face_offsets[0] == 0, HG_ERR_VALUE,
+               "offsets must start at 0");
+    for (int64_t s = 0; s < n_meshes; ++s)
+      HG_REQUIRE(vertex_offsets[s + 1] >= vertex_offsets[s] && face_offsets[s + 1] >= face_offsets[s],
+                 HG_ERR_VALUE, "offsets must not decrease");
+    const int64_t n_vertices = vertex_offsets[n_meshes], n_faces = face_offsets[n_meshes];
+    HG_REQUIRE(n_vertices < (1ll << 31) && 3 * n_faces < (1ll << 31), HG_ERR_VALUE,
+               "packed meshes too large for int32 indices");
+    auto m = std::make_unique<hg_mesh>();
+    m->stream = static_cast<hipStream_t>(stream);
+    m->n = n_vertices;
+    m->f = n_faces;
+    m->seg_v.assign(vertex_offsets, vertex_offsets + n_meshes + 1);
+    m->seg_f.assign(face_offsets, face_offsets + n_meshes + 1);
+    m->h_xyz.resize(3 * n_vertices);
+    m->h_faces.resize(3 * n_faces);
+    if (n_vertices > 0)
+      HG_HIP(hipMemcpy(m->h_xyz.data(), d_vertices, sizeof(float) * 3 * n_vertices, hipMemcpyDefault));
+    if (n_faces > 0)
+      HG_HIP(hipMemcpy(m->h_faces.data(), d_faces, sizeof(int32_t) * 3 * n_faces, hipMemcpyDefault));
+    // Each mesh's faces index its own vertices; shift them to the packed numbering.
+    for (int64_t s = 0; s < n_meshes; ++s) {
+      const int64_t base = vertex_offsets[s], count = vertex_offsets[s + 1] - base;
+      for (int64_t e = 3 * face_offsets[s]; e < 3 * face_offsets[s + 1]; ++e) {
+        const int32_t v = m->h_faces[e];
+        HG_REQUIRE(v >= 0 && v < count, HG_ERR_VALUE,
+                   "mesh " + std::to_string(s) + " has a face index outside [0, " +
+                       std::to_string(count) + ")");
+        m->h_faces[e] = static_cast<int32_t>(v + base);
+      }
     }
-    hipStream_t st = m->stream;
-    m->xyz.upload(m->h_xyz, st);
-    m->faces.upload(m->h_faces, st);
-    m->offsets.upload(m->pattern.offsets, st);
-    m->columns.upload(m->pattern.columns, st);
-    m->corner_ptr.upload(m->pattern.corner_ptr, st);
-    m->corner_slots.upload(m->pattern.corner_slots, st);
-    m->vertex_ptr.upload(m->pattern.vertex_ptr, st);
-    m->vertex_corners.upload(m->pattern.vertex_corners, st);
-    m->comp_ptr.upload(comp_ptr, st);
-    m->null_space.build(comp_ptr, st);
-    m->comp_members.upload(members, st);
-    HG_HIP(hipStreamSynchronize(st));
+    bind(*m);
     *out_mesh = m.release();
   });
 }
@@ -578,11 +692,28 @@ int hg_factored(hg_mesh* mesh, int32_t* out_systems) {
   });
 }
 
+int hg_set_refinement(hg_mesh* mesh, int32_t precision, double heat_tolerance,
+                      double poisson_tolerance) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(precision == HG_PRECISION_FP64 || precision == HG_PRECISION_MIXED, HG_ERR_VALUE,
+               "unknown precision");
+    const bool keep_heat = std::isnan(heat_tolerance), keep_poisson = std::isnan(poisson_tolerance);
+    HG_REQUIRE(keep_heat || (heat_tolerance > 0.0 && heat_tolerance < 1.0), HG_ERR_VALUE,
+               "the heat refinement tolerance must lie in (0, 1)");
+    HG_REQUIRE(keep_poisson || (poisson_tolerance > 0.0 && poisson_tolerance < 1.0), HG_ERR_VALUE,
+               "the Poisson refinement tolerance must lie in (0, 1)");
+    if (!keep_heat) mesh->refine_tolerance[precision][0] = heat_tolerance;
+    if (!keep_poisson) mesh->refine_tolerance[precision][1] = poisson_tolerance;
+  });
+}
+
 int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
                int32_t mode, int32_t precision, int32_t* out_flags) {
   return guarded([&] {
     require_mesh(mesh);
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
+    require_single_or_factored(*mesh, mode);
     mesh->stats = hg_solve_stats{};
     int32_t flags = diffuse(*mesh, d_rhs, d_solution, n_columns, mode, precision);
     HG_HIP(hipStreamSynchronize(mesh->stream));
@@ -595,6 +726,7 @@ int hg_solve_poisson(hg_mesh* mesh, const double* d_rhs, double* d_solution, int
   return guarded([&] {
     require_mesh(mesh);
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
+    require_single_or_factored(*mesh, mode);
     solve_poisson(*mesh, d_rhs, d_solution, n_columns, mode, precision);
     HG_HIP(hipStreamSynchronize(mesh->stream));
   });
@@ -623,6 +755,8 @@ int hg_shift_to_sources(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
                         const int64_t* source_offsets, int64_t n_columns) {
   return guarded([&] {
     require_mesh(mesh);
+    HG_REQUIRE(mesh->segments() == 1, HG_ERR_VALUE,
+               "packed meshes shift each query in hg_geodesic_packed");
     upload_sources(*mesh, d_sources, source_offsets, n_columns);
     shift(*mesh, d_phi, n_columns);
     HG_HIP(hipStreamSynchronize(mesh->stream));
@@ -636,6 +770,8 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     require_mesh(mesh);
     HG_REQUIRE(n_queries >= 0, HG_ERR_VALUE, "negative query count");
     HG_REQUIRE(source_offsets != nullptr, HG_ERR_VALUE, "source_offsets is null");
+    HG_REQUIRE(mesh->segments() == 1, HG_ERR_VALUE,
+               "a packed mesh answers queries through hg_geodesic_packed");
     hg_mesh& m = *mesh;
     m.stats = hg_solve_stats{};
     const int64_t n = m.n, k = n_queries;
@@ -647,19 +783,115 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     }
     ensure_operators(m);
     upload_sources(m, d_sources, source_offsets, k);
-    double* fields = reserve_fields(m, k);
-    double* u0 = fields;
-    double* heat = fields + n * k;
-    double* negdiv = fields + 2 * n * k;
-    seed(m, source_offsets, k, u0);
-    int32_t flags = diffuse(m, u0, heat, k, mode, precision);
-    m.stats.backward_error_failed = flags & 1;
-    divergence(m, heat, negdiv, k);
-    solve_poisson(m, negdiv, d_out, k, mode, precision);
+    solve_columns(m, source_offsets, k, d_out, mode, precision);
     shift(m, d_out, k);
     HG_HIP(hipStreamSynchronize(m.stream));
     m.timer.resolve();
     if (out_stats) *out_stats = m.stats;
+  });
+}
+
+int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_offsets,
+                       const int32_t* query_mesh, int64_t n_queries, double* d_out,
+                       int32_t precision, hg_solve_stats* out_stats) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(n_queries >= 0, HG_ERR_VALUE, "negative query count");
+    HG_REQUIRE(source_offsets != nullptr && query_mesh != nullptr, HG_ERR_VALUE,
+               "source_offsets or query_mesh is null");
+    hg_mesh& m = *mesh;
+    m.stats = hg_solve_stats{};
+    if (n_queries == 0) return;
+    const int64_t n_seg = m.segments();
+    // Slots: a mesh's queries take columns 0, 1, ... of the packed system in their order, so one
+    // column carries one query of every mesh that has one; the batch is as wide as the mesh with
+    // the most queries.
+    std::vector<int64_t> next_slot(n_seg, 0), slot(n_queries);
+    int64_t width = 0, max_len = 0;
+    std::vector<int64_t> src_off(n_queries), dst_off(n_queries), len(n_queries);
+    int64_t out_at = 0;
+    for (int64_t q = 0; q < n_queries; ++q) {
+      const int32_t s = query_mesh[q];
+      HG_REQUIRE(s >= 0 && s < n_seg, HG_ERR_VALUE,
+                 "query " + std::to_string(q) + " names mesh " + std::to_string(s) + " of " +
+                     std::to_string(n_seg));
+      slot[q] = next_slot[s]++;
+      width = std::max(width, slot[q] + 1);
+      len[q] = m.seg_v[s + 1] - m.seg_v[s];
+      max_len = std::max(max_len, len[q]);
+      dst_off[q] = out_at;
+      out_at += len[q];
+    }
+    // Sources: each query's indices are its own mesh's; check them on the host copy, then shift
+    // them into the packed numbering and give each the column of its query's slot.
+    const int64_t total = source_offsets[n_queries] - source_offsets[0];
+    std::vector<int32_t> local(static_cast<size_t>(std::max<int64_t>(total, 1)));
+    if (total > 0)
+      HG_HIP(hipMemcpy(local.data(), d_sources + source_offsets[0], sizeof(int32_t) * total,
+                       hipMemcpyDefault));
+    std::vector<int32_t> packed(local.size());
+    std::vector<int64_t> owner(local.size()), packed_offsets(width + 1, 0), local_offsets(n_queries + 1);
+    for (int64_t q = 0; q < n_queries; ++q) {
+      const int32_t s = query_mesh[q];
+      for (int64_t e = source_offsets[q]; e < source_offsets[q + 1]; ++e) {
+        const int64_t at = e - source_offsets[0];
+        HG_REQUIRE(local[at] >= 0 && local[at] < len[q], HG_ERR_VALUE,
+                   "query " + std::to_string(q) + ": every entry of sources must be a vertex index "
+                   "in [0, " + std::to_string(len[q]) + ")");
+        packed[at] = static_cast<int32_t>(local[at] + m.seg_v[s]);
+        owner[at] = slot[q];
+      }
+      local_offsets[q] = source_offsets[q] - source_offsets[0];
+      src_off[q] = slot[q] * m.n + m.seg_v[s];
+    }
+    local_offsets[n_queries] = total;
+    if (m.n == 0 || m.f == 0 || max_len == 0) {
+      if (out_at > 0) HG_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * out_at, m.stream));
+      HG_HIP(hipStreamSynchronize(m.stream));
+      return;
+    }
+    ensure_operators(m);
+    // The seed reads the packed sources with their slot owners; the offsets are unused there.
+    if (m.d_sources.size < std::max<int64_t>(total, 1)) m.d_sources.allocate(std::max<int64_t>(total, 1));
+    m.d_sources.upload(packed.data(), total, m.stream);
+    m.d_owner.upload(owner, m.stream);
+    std::vector<int64_t> seed_offsets(width + 1, 0);
+    seed_offsets[width] = total;
+    if (m.packed_capacity < m.n * width) {
+      m.packed_phi.allocate(m.n * width);
+      m.packed_capacity = m.n * width;
+    }
+    solve_columns(m, seed_offsets.data(), width, m.packed_phi.ptr, HG_MODE_FACTORED, precision);
+    // Each query's segment of its slot's column, then its shift against its own sources.
+    DeviceArray<int64_t> d_src_off, d_dst_off, d_len, d_local_offsets;
+    d_src_off.upload(src_off, m.stream);
+    d_dst_off.upload(dst_off, m.stream);
+    d_len.upload(len, m.stream);
+    d_local_offsets.upload(local_offsets, m.stream);
+    DeviceArray<int32_t> d_local;
+    d_local.upload(local, m.stream);
+    launch_extract_fields(m.packed_phi.ptr, d_src_off.ptr, d_dst_off.ptr, d_len.ptr, max_len,
+                          n_queries, d_out, m.stream);
+    const int32_t blocks = shift_partial_blocks(max_len);
+    if (m.shift_partials.size < blocks * n_queries) m.shift_partials.allocate(blocks * n_queries);
+    if (m.shift_values.size < 2 * n_queries) m.shift_values.allocate(2 * n_queries);
+    {
+      PhaseScope scope(&m.timer, kPhaseShift, m.stream);
+      launch_shift(d_out, d_local.ptr, d_local_offsets.ptr, max_len, n_queries, d_dst_off.ptr,
+                   d_len.ptr, m.shift_partials.ptr, m.shift_values.ptr, m.stream);
+    }
+    HG_HIP(hipStreamSynchronize(m.stream));
+    m.timer.resolve();
+    if (out_stats) *out_stats = m.stats;
+  });
+}
+
+int hg_packed_times(hg_mesh* mesh, double* out_times) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(out_times != nullptr, HG_ERR_VALUE, "out_times is null");
+    ensure_operators(*mesh);
+    for (int64_t s = 0; s < mesh->segments(); ++s) out_times[s] = mesh->seg_t[s];
   });
 }
 

@@ -93,19 +93,20 @@ __global__ void mass_kernel(const int32_t* __restrict__ vertex_ptr,
   mass[i] = sum / 3.0;
 }
 
-// First stage of (sum of w |v_j - v_i|, sum of w) over the strict upper triangle, w the faces on
-// the edge (MeshLab's face-edge mean, kernels/heat.py:143-235): one thread a row, a fixed block
-// tree, one partial pair a block.
+// First stage of (sum of w |v_j - v_i|, sum of w) over the strict upper triangle of rows
+// [row0, row0 + n), w the faces on the edge (MeshLab's face-edge mean, kernels/heat.py:143-235):
+// one thread a row, a fixed block tree counted from row0, one partial pair a block.
 __global__ void edge_sums_kernel(const int32_t* __restrict__ offsets,
                                  const int32_t* __restrict__ columns,
                                  const int32_t* __restrict__ corner_ptr,
-                                 const float* __restrict__ xyz, int64_t n, int32_t face_edges,
-                                 double* __restrict__ partials) {
+                                 const float* __restrict__ xyz, int64_t row0, int64_t n,
+                                 int32_t face_edges, double* __restrict__ partials) {
   __shared__ double s_len[kBlock];
   __shared__ double s_cnt[kBlock];
-  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t local = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = row0 + local;
   double len = 0.0, cnt = 0.0;
-  if (i < n) {
+  if (local < n) {
     F3 vi = load3(xyz, i);
     for (int32_t k = offsets[i]; k < offsets[i + 1]; ++k) {
       int32_t j = columns[k];
@@ -172,17 +173,19 @@ __global__ void edge_sums_final_kernel(const double* __restrict__ partials, int3
 // (kernels/laplacian.py:970-989); L_ii = -(sum of the row's off-diagonal entries in column order);
 // heat = factor L (+ m_i on the diagonal, kernels/linalg.py:1612-1621); Poisson = -L. Also the
 // heat values narrowed to float32 (the settled CG's stream, linalg.py:2516-2521) and both
-// systems' inverse diagonals (Jacobi). One thread a row.
+// systems' inverse diagonals (Jacobi). One thread a row; packed meshes read their own segment's
+// factor (row_segment non-null).
 __global__ void operator_rows_kernel(
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ columns,
     const int32_t* __restrict__ corner_ptr, const int32_t* __restrict__ corner_slots,
     const float* __restrict__ cot, const double* __restrict__ mass,
-    const double* __restrict__ scalars, int64_t n, double* __restrict__ lap,
-    double* __restrict__ heat, double* __restrict__ poisson, float* __restrict__ heat32,
-    double* __restrict__ heat_inv_diag, double* __restrict__ poisson_inv_diag) {
+    const double* __restrict__ scalars, const int32_t* __restrict__ row_segment, int64_t n,
+    double* __restrict__ lap, double* __restrict__ heat, double* __restrict__ poisson,
+    float* __restrict__ heat32, double* __restrict__ heat_inv_diag,
+    double* __restrict__ poisson_inv_diag) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double factor = scalars[2];
+  const double factor = scalars[4 * (row_segment ? row_segment[i] : 0) + 2];
   double total = 0.0;
   int32_t slot = -1;
   for (int32_t k = offsets[i]; k < offsets[i + 1]; ++k) {
@@ -285,15 +288,26 @@ __global__ void corner_sums_kernel(const int32_t* __restrict__ vertex_ptr,
   out[q * n + i] = sum;
 }
 
+// Columns of the shift: column q starts at col_off[q] and holds col_len[q] entries (packed meshes'
+// fields, each its own length), or at q * n with n entries when col_off is null.
+struct Columns {
+  const int64_t* off;
+  const int64_t* len;
+  int64_t n;
+  __device__ __forceinline__ int64_t start(int64_t q) const { return off ? off[q] : q * n; }
+  __device__ __forceinline__ int64_t size(int64_t q) const { return len ? len[q] : n; }
+};
+
 // First stage of each column's (sum over its sources, sum over every vertex) of phi
 // (kernels/heat.py:3383-3404): block partials of the global sum; the source sum is folded by the
-// final stage, sources being few.
-__global__ void column_sum_kernel(const double* __restrict__ phi, int64_t n, int64_t n_columns,
+// final stage, sources being few. A column shorter than the grid adds zero partials, so its sum
+// is the one a grid of its own length forms.
+__global__ void column_sum_kernel(const double* __restrict__ phi, Columns cols, int64_t n_columns,
                                   double* __restrict__ partials) {
   __shared__ double s[kBlock];
   int64_t q = blockIdx.y;
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  s[threadIdx.x] = (i < n) ? phi[q * n + i] : 0.0;
+  s[threadIdx.x] = (i < cols.size(q)) ? phi[cols.start(q) + i] : 0.0;
   __syncthreads();
   for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
     if (threadIdx.x < stride) s[threadIdx.x] += s[threadIdx.x + stride];
@@ -308,16 +322,17 @@ __global__ void column_sum_kernel(const double* __restrict__ phi, int64_t n, int
 __global__ void shift_decide_kernel(const double* __restrict__ phi,
                                     const double* __restrict__ partials, int32_t blocks,
                                     const int32_t* __restrict__ sources,
-                                    const int64_t* __restrict__ source_offsets, int64_t n,
+                                    const int64_t* __restrict__ source_offsets, Columns cols,
                                     double* __restrict__ shift) {
   __shared__ double s[kBlock];
   __shared__ double s_src[kBlock];
   int64_t q = blockIdx.x;
+  const int64_t base = cols.start(q), n = cols.size(q);
   double total = 0.0;
   for (int32_t b = threadIdx.x; b < blocks; b += kBlock) total += partials[q * blocks + b];
   int64_t first = source_offsets[q], last = source_offsets[q + 1];
   double src = 0.0;
-  for (int64_t e = first + threadIdx.x; e < last; e += kBlock) src += phi[q * n + sources[e]];
+  for (int64_t e = first + threadIdx.x; e < last; e += kBlock) src += phi[base + sources[e]];
   s[threadIdx.x] = total;
   s_src[threadIdx.x] = src;
   __syncthreads();
@@ -337,13 +352,26 @@ __global__ void shift_decide_kernel(const double* __restrict__ phi,
 }
 
 __global__ void shift_apply_kernel(double* __restrict__ phi, const double* __restrict__ shift,
-                                   int64_t n, int64_t n_columns) {
+                                   Columns cols, int64_t n_columns) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t q = blockIdx.y;
-  if (i >= n || q >= n_columns) return;
+  if (q >= n_columns || i >= cols.size(q)) return;
+  const int64_t at = cols.start(q) + i;
   double offset = shift[2 * q];
-  double value = phi[q * n + i];
-  phi[q * n + i] = shift[2 * q + 1] != 0.0 ? offset - value : value - offset;
+  double value = phi[at];
+  phi[at] = shift[2 * q + 1] != 0.0 ? offset - value : value - offset;
+}
+
+// Packed meshes: query q's field is rows [src_off[q], src_off[q] + len[q]) of the solved columns
+// (its mesh's segment of its slot's column), copied to out[dst_off[q] ...].
+__global__ void extract_fields_kernel(const double* __restrict__ phi,
+                                      const int64_t* __restrict__ src_off,
+                                      const int64_t* __restrict__ dst_off,
+                                      const int64_t* __restrict__ len, double* __restrict__ out) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t q = blockIdx.y;
+  if (i >= len[q]) return;
+  out[dst_off[q] + i] = phi[src_off[q] + i];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -372,11 +400,12 @@ void launch_mass(const int32_t* vertex_ptr, const int32_t* vertex_corners, const
 int32_t edge_sum_blocks(int64_t n) { return static_cast<int32_t>(blocks_for(std::max<int64_t>(n, 1))); }
 
 void launch_edge_sums(const int32_t* offsets, const int32_t* columns, const int32_t* corner_ptr,
-                      const float* xyz, int64_t n, int32_t face_edges, double* partials,
-                      int32_t from_edges, double t, double* scalars, hipStream_t stream) {
+                      const float* xyz, int64_t row0, int64_t n, int32_t face_edges,
+                      double* partials, int32_t from_edges, double t, double* scalars,
+                      hipStream_t stream) {
   int32_t blocks = edge_sum_blocks(n);
   if (n > 0)
-    edge_sums_kernel<<<blocks, kBlock, 0, stream>>>(offsets, columns, corner_ptr, xyz, n,
+    edge_sums_kernel<<<blocks, kBlock, 0, stream>>>(offsets, columns, corner_ptr, xyz, row0, n,
                                                     face_edges, partials);
   else
     HG_HIP(hipMemsetAsync(partials, 0, sizeof(double) * 2 * blocks, stream));
@@ -386,13 +415,13 @@ void launch_edge_sums(const int32_t* offsets, const int32_t* columns, const int3
 
 void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
                           const int32_t* corner_ptr, const int32_t* corner_slots, const float* cot,
-                          const double* mass, const double* scalars, int64_t n, double* lap,
-                          double* heat, double* poisson, float* heat32, double* heat_inv_diag,
-                          double* poisson_inv_diag, hipStream_t stream) {
+                          const double* mass, const double* scalars, const int32_t* row_segment,
+                          int64_t n, double* lap, double* heat, double* poisson, float* heat32,
+                          double* heat_inv_diag, double* poisson_inv_diag, hipStream_t stream) {
   if (n == 0) return;
   operator_rows_kernel<<<blocks_for(n), kBlock, 0, stream>>>(
-      offsets, columns, corner_ptr, corner_slots, cot, mass, scalars, n, lap, heat, poisson,
-      heat32, heat_inv_diag, poisson_inv_diag);
+      offsets, columns, corner_ptr, corner_slots, cot, mass, scalars, row_segment, n, lap, heat,
+      poisson, heat32, heat_inv_diag, poisson_inv_diag);
   HG_HIP(hipGetLastError());
 }
 
@@ -424,14 +453,25 @@ void launch_divergence(const float* xyz, const int32_t* faces, const float* norm
 int32_t shift_partial_blocks(int64_t n) { return static_cast<int32_t>(blocks_for(n)); }
 
 void launch_shift(double* phi, const int32_t* sources, const int64_t* source_offsets, int64_t n,
-                  int64_t n_columns, double* partials, double* shift, hipStream_t stream) {
+                  int64_t n_columns, const int64_t* col_off, const int64_t* col_len,
+                  double* partials, double* shift, hipStream_t stream) {
   if (n == 0 || n_columns == 0) return;
   int32_t blocks = shift_partial_blocks(n);
   dim3 grid(blocks, static_cast<unsigned>(n_columns));
-  column_sum_kernel<<<grid, kBlock, 0, stream>>>(phi, n, n_columns, partials);
+  const Columns cols{col_off, col_len, n};
+  column_sum_kernel<<<grid, kBlock, 0, stream>>>(phi, cols, n_columns, partials);
   shift_decide_kernel<<<static_cast<unsigned>(n_columns), kBlock, 0, stream>>>(
-      phi, partials, blocks, sources, source_offsets, n, shift);
-  shift_apply_kernel<<<grid, kBlock, 0, stream>>>(phi, shift, n, n_columns);
+      phi, partials, blocks, sources, source_offsets, cols, shift);
+  shift_apply_kernel<<<grid, kBlock, 0, stream>>>(phi, shift, cols, n_columns);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_extract_fields(const double* phi, const int64_t* src_off, const int64_t* dst_off,
+                           const int64_t* len, int64_t max_len, int64_t n_queries, double* out,
+                           hipStream_t stream) {
+  if (max_len == 0 || n_queries == 0) return;
+  dim3 grid(blocks_for(max_len), static_cast<unsigned>(n_queries));
+  extract_fields_kernel<<<grid, kBlock, 0, stream>>>(phi, src_off, dst_off, len, out);
   HG_HIP(hipGetLastError());
 }
 

@@ -88,14 +88,17 @@ void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, 
 void launch_mass(const int32_t* vertex_ptr, const int32_t* vertex_corners, const float* areas,
                  int64_t n, double* mass, hipStream_t stream);
 int32_t edge_sum_blocks(int64_t n);
+// Edge-length sums over rows [row0, row0 + n) into scalars[0..4).
 void launch_edge_sums(const int32_t* offsets, const int32_t* columns, const int32_t* corner_ptr,
-                      const float* xyz, int64_t n, int32_t face_edges, double* partials, int32_t from_edges, double t,
-                      double* scalars, hipStream_t stream);
+                      const float* xyz, int64_t row0, int64_t n, int32_t face_edges,
+                      double* partials, int32_t from_edges, double t, double* scalars,
+                      hipStream_t stream);
+// row_segment (nullable): each row's segment s, whose heat factor is scalars[4 s + 2].
 void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
                           const int32_t* corner_ptr, const int32_t* corner_slots, const float* cot,
-                          const double* mass, const double* scalars, int64_t n, double* lap,
-                          double* heat, double* poisson, float* heat32, double* heat_inv_diag,
-                          double* poisson_inv_diag, hipStream_t stream);
+                          const double* mass, const double* scalars, const int32_t* row_segment,
+                          int64_t n, double* lap, double* heat, double* poisson, float* heat32,
+                          double* heat_inv_diag, double* poisson_inv_diag, hipStream_t stream);
 void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
                  double* u0, int32_t* invalid, hipStream_t stream);
 void launch_divergence(const float* xyz, const int32_t* faces, const float* normals,
@@ -104,8 +107,14 @@ void launch_divergence(const float* xyz, const int32_t* faces, const float* norm
                        int64_t n_faces, int64_t n_columns, double* corner_terms, double* out,
                        hipStream_t stream);
 int32_t shift_partial_blocks(int64_t n);
+// Columns start at col_off[q] with col_len[q] entries (device; both null: q * n and n), n the
+// longest.
 void launch_shift(double* phi, const int32_t* sources, const int64_t* source_offsets, int64_t n,
-                  int64_t n_columns, double* partials, double* shift, hipStream_t stream);
+                  int64_t n_columns, const int64_t* col_off, const int64_t* col_len,
+                  double* partials, double* shift, hipStream_t stream);
+void launch_extract_fields(const double* phi, const int64_t* src_off, const int64_t* dst_off,
+                           const int64_t* len, int64_t max_len, int64_t n_queries, double* out,
+                           hipStream_t stream);
 
 // cg.hip: the persistent conjugate-gradient solves.
 struct CgProblem {

@@ -20,7 +20,6 @@ namespace hg {
 namespace {
 
 constexpr int kTile = 64;   // rows (forward) or columns (backward) of W per tile
-constexpr int kChunk = 32;  // inner dimension staged per LDS round
 constexpr int kCols = 64;   // right-hand sides per solve chunk
 constexpr int kReduceSplit = 4;  // blocks a reduction group is spread over (16 rows each)
 
@@ -105,17 +104,44 @@ struct SweepArgs {
   int32_t kk;
 };
 
-// Forward tile product for up to 64 right-hand sides: rows [r0, r0+64) of the front's solve block
-// times f over the inner range [k0, k1), f gathered as it is staged. Thread (tc, tr) =
-// (t % 16, t / 16) owns rows tr + 16a and right-hand sides tc + 16b. slot < 0 applies the product;
-// otherwise it is stored as a partial for the reduction.
+// The tile products run on the FP64 matrix cores: v_mfma_f64_16x16x4 takes a 16 x 4 block of A
+// (lane l holds A[l % 16][l / 16]) and a 4 x 16 block of B (lane l holds B[l / 16][l % 16]) and
+// accumulates the 16 x 16 product, lane l holding rows l / 16 + 4v (v < 4) of column l % 16. A
+// tile's four waves each own 16 of its 64 rows and accumulate all 64 right-hand sides as four
+// 16-column blocks. Both operands are staged through LDS kInner inner indices at a time; the next
+// stage's global loads are issued before the current stage's products, so they overlap.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kInner = 16;
+
+__device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Four 4-deep steps over one stage: A from As[k][16 * wave + l % 16] (the wave's 16 output rows),
+// B from Bs[k][16 n + l % 16] for each 16-column block n holding right-hand sides.
+__device__ __forceinline__ void stage_products(const double (*As)[kTile + 1],
+                                               const double (*Bs)[kCols + 1], int wave, int lane,
+                                               int blocks, f64x4 acc[4]) {
+  const int li = lane % 16, lk = lane / 16;
+#pragma unroll 1
+  for (int kb = 0; kb < kInner; kb += 4) {
+    const double av = As[kb + lk][16 * wave + li];
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+      if (n < blocks) acc[n] = mfma16x16x4(av, Bs[kb + lk][16 * n + li], acc[n]);
+  }
+}
+
+// Forward tile product for up to 64 right-hand sides: rows [r0, r0+64) of the front's inverse
+// factor times f over the inner range [k0, k1), f gathered as it is staged. slot < 0 applies the
+// product; otherwise it is stored as a partial for the reduction.
 template <typename TW>
 __global__ __launch_bounds__(256) void fwd_tile_kernel(
     const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_r0,
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
-  __shared__ double Ws[kChunk][kTile];
-  __shared__ double Fs[kChunk][kCols];
+  __shared__ double Ws[kInner][kTile + 1];  // [inner][row]
+  __shared__ double Fs[kInner][kCols + 1];  // [inner][right-hand side]
   const int32_t s = task_s[blockIdx.x];
   const int32_t r0 = task_r0[blockIdx.x];
   const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
@@ -125,52 +151,59 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
   const int32_t kk = a.kk;
   const TW* W = w + a.w_off[s];
   const int64_t vbase = a.vec_off[s];
-  const int t = threadIdx.x;
-  const int tc = t % 16, tr = t / 16;
-  double acc[4][4] = {};
-  for (int32_t j0 = k0; j0 < k1; j0 += kChunk) {
-    for (int u = 0; u < (kChunk * kTile) / 256; ++u) {
+  const int t = threadIdx.x, wave = t / 64, lane = t % 64;
+  const int blocks = (kk + 15) / 16;
+  const bool rows_live = r0 + 16 * wave < m;  // wave-uniform
+  // Staging: entry e = t + 256u of a stage is W row e % 64 / f column e % 64 at inner e / 64.
+  double wreg[4], freg[4];
+  auto load = [&](int32_t j0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
       const int ii = e % kTile, jj = e / kTile;
       const int32_t i = r0 + ii, j = j0 + jj;
-      Ws[jj][ii] = (i < m && j < k1) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
+      wreg[u] = (i < m && j < k1) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
+      const int q = e % kCols;
+      freg[u] = (q < kk && j < k1)
+                    ? gathered(a.r[static_cast<int64_t>(c0 + j) * kk + q], vbase + j, q, kk,
+                               a.gather_ptr, a.gather_src, a.fvec)
+                    : 0.0;
     }
-    for (int u = 0; u < (kChunk * kCols) / 256; ++u) {
+  };
+  f64x4 acc[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
+  load(k0);
+  for (int32_t j0 = k0; j0 < k1; j0 += kInner) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
-      const int q = e % kCols, jj = e / kCols;
-      const int32_t j = j0 + jj;
-      double v = 0.0;
-      if (q < kk && j < k1)
-        v = gathered(a.r[static_cast<int64_t>(c0 + j) * kk + q], vbase + j, q, kk, a.gather_ptr,
-                     a.gather_src, a.fvec);
-      Fs[jj][q] = v;
+      Ws[e / kTile][e % kTile] = wreg[u];
+      Fs[e / kCols][e % kCols] = freg[u];
     }
     __syncthreads();
-#pragma unroll 4
-    for (int jj = 0; jj < kChunk; ++jj) {
-      double av[4], bv[4];
-      for (int x = 0; x < 4; ++x) av[x] = Ws[jj][tr + 16 * x];
-      for (int x = 0; x < 4; ++x) bv[x] = Fs[jj][tc + 16 * x];
-      for (int x = 0; x < 4; ++x)
-        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
-    }
+    if (j0 + kInner < k1) load(j0 + kInner);
+    if (rows_live) stage_products(Ws, Fs, wave, lane, blocks, acc);
     __syncthreads();
   }
-  for (int x = 0; x < 4; ++x) {
-    const int32_t ii = tr + 16 * x;
-    const int32_t i = r0 + ii;
-    if (i >= m) continue;
-    for (int z = 0; z < 4; ++z) {
-      const int q = tc + 16 * z;
-      if (q >= kk) continue;
+  if (!rows_live) return;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int q = 16 * n + lane % 16;
+    if (q >= kk) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int32_t ii = 16 * wave + lane / 16 + 4 * v;
+      const int32_t i = r0 + ii;
+      if (i >= m) continue;
+      const double value = acc[n][v];
       if (slot >= 0) {
-        a.partials[(static_cast<int64_t>(slot) * kTile + ii) * kk + q] = acc[x][z];
+        a.partials[(static_cast<int64_t>(slot) * kTile + ii) * kk + q] = value;
       } else if (i < nc) {
-        a.y[static_cast<int64_t>(c0 + i) * kk + q] = acc[x][z];
+        a.y[static_cast<int64_t>(c0 + i) * kk + q] = value;
       } else {
         const int64_t row = vbase + i;
-        a.fvec[row * kk + q] =
-            gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) + acc[x][z];
+        a.fvec[row * kk + q] = gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) + value;
       }
     }
   }
@@ -204,16 +237,16 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
   }
 }
 
-// Backward tile product: solve-block columns [j0, j0+64) transposed, times
-// g = [y(cols); d(rows)] over the inner range [k0, k1). Thread (tc, tr) owns solve-block columns
-// tr + 16a and right-hand sides tc + 16b. Applied results also give x_next = x + d.
+// Backward tile product: inverse-factor columns [j0, j0+64) transposed, times
+// g = [y(cols); d(rows)] over the inner range [k0, k1); each wave owns 16 of the columns. Applied
+// results also give x_next = x + d.
 template <typename TW>
 __global__ __launch_bounds__(256) void bwd_tile_kernel(
     const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_j0,
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
-  __shared__ double Ws[kChunk][kTile];
-  __shared__ double Gs[kChunk][kCols];
+  __shared__ double Ws[kInner][kTile + 1];  // [inner][inverse-factor column]
+  __shared__ double Gs[kInner][kCols + 1];  // [inner][right-hand side]
   const int32_t s = task_s[blockIdx.x];
   const int32_t j0 = task_j0[blockIdx.x];
   const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
@@ -223,51 +256,62 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
   const int32_t kk = a.kk;
   const TW* W = w + a.w_off[s];
   const int32_t* srows = a.rows + a.row_ptr[s];
-  const int t = threadIdx.x;
-  const int tc = t % 16, tr = t / 16;
-  double acc[4][4] = {};
-  for (int32_t i0 = k0; i0 < k1; i0 += kChunk) {
-    for (int u = 0; u < (kChunk * kTile) / 256; ++u) {
+  const int t = threadIdx.x, wave = t / 64, lane = t % 64;
+  const int blocks = (kk + 15) / 16;
+  const bool cols_live = j0 + 16 * wave < nc;  // wave-uniform
+  // Staging: entry e = t + 256u is inverse-factor column e / 16 at inner e % 16 (contiguous in
+  // memory), and g's right-hand side e % 64 at inner e / 64.
+  double wreg[4], greg[4];
+  auto load = [&](int32_t i0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
-      const int ii = e % kChunk, jj = e / kChunk;
+      const int ii = e % kInner, jj = e / kInner;
       const int32_t i = i0 + ii, j = j0 + jj;
-      Ws[ii][jj] = (i < k1 && j < nc) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
-    }
-    for (int u = 0; u < (kChunk * kCols) / 256; ++u) {
-      const int e = t + 256 * u;
-      const int q = e % kCols, ii = e / kCols;
-      const int32_t i = i0 + ii;
+      wreg[u] = (i < k1 && j < nc) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
+      const int q = e % kCols;
+      const int32_t ig = i0 + e / kCols;
       double g = 0.0;
-      if (q < kk && i < k1) {
-        g = i < nc ? a.y[static_cast<int64_t>(c0 + i) * kk + q]
-                   : a.d[static_cast<int64_t>(srows[i - nc]) * kk + q];
+      if (q < kk && ig < k1) {
+        g = ig < nc ? a.y[static_cast<int64_t>(c0 + ig) * kk + q]
+                    : a.d[static_cast<int64_t>(srows[ig - nc]) * kk + q];
       }
-      Gs[ii][q] = g;
+      greg[u] = g;
+    }
+  };
+  f64x4 acc[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
+  load(k0);
+  for (int32_t i0 = k0; i0 < k1; i0 += kInner) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
+      Ws[e % kInner][e / kInner] = wreg[u];
+      Gs[e / kCols][e % kCols] = greg[u];
     }
     __syncthreads();
-#pragma unroll 4
-    for (int ii = 0; ii < kChunk; ++ii) {
-      double av[4], bv[4];
-      for (int x = 0; x < 4; ++x) av[x] = Ws[ii][tr + 16 * x];
-      for (int x = 0; x < 4; ++x) bv[x] = Gs[ii][tc + 16 * x];
-      for (int x = 0; x < 4; ++x)
-        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
-    }
+    if (i0 + kInner < k1) load(i0 + kInner);
+    if (cols_live) stage_products(Ws, Gs, wave, lane, blocks, acc);
     __syncthreads();
   }
-  for (int x = 0; x < 4; ++x) {
-    const int32_t jj = tr + 16 * x;
-    const int32_t j = j0 + jj;
-    if (j >= nc) continue;
-    for (int z = 0; z < 4; ++z) {
-      const int q = tc + 16 * z;
-      if (q >= kk) continue;
+  if (!cols_live) return;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int q = 16 * n + lane % 16;
+    if (q >= kk) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int32_t jj = 16 * wave + lane / 16 + 4 * v;
+      const int32_t j = j0 + jj;
+      if (j >= nc) continue;
+      const double value = acc[n][v];
       if (slot >= 0) {
-        a.partials[(static_cast<int64_t>(slot) * kTile + jj) * kk + q] = acc[x][z];
+        a.partials[(static_cast<int64_t>(slot) * kTile + jj) * kk + q] = value;
       } else {
         const int64_t at = static_cast<int64_t>(c0 + j) * kk + q;
-        a.d[at] = acc[x][z];
-        a.x_next[at] = a.x[at] + acc[x][z];
+        a.d[at] = value;
+        a.x_next[at] = a.x[at] + value;
       }
     }
   }

@@ -350,6 +350,21 @@ class HeatSolver:
         )
         return bool(out.value)
 
+    def set_refinement(self, heat: float | None = None, poisson: float | None = None) -> None:
+        """
+        Refinement targets of this solver's factored solves at its precision: the heat step's
+        componentwise tolerance and the Poisson step's normwise one, each in (0, 1); None keeps
+        the current value. (No counterpart in the reference, whose targets are fixed at
+        ``linalg.py:181`` and ``heat.py:108``.)
+        """
+        _lib.call(
+            "hg_set_refinement",
+            self._native().ptr,
+            self._precision_code,
+            math.nan if heat is None else float(heat),
+            math.nan if poisson is None else float(poisson),
+        )
+
     def enable_timing(self, enable: bool = True) -> None:
         """Bracket every phase of later solves with HIP events on the solver's stream."""
         _lib.call("hg_timing_enable", self._native().ptr, int(bool(enable)))
