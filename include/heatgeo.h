@@ -135,6 +135,15 @@ int hg_set_refinement(hg_mesh* mesh, int32_t precision, double heat_tolerance,
                       double poisson_tolerance);
 
 /*
+ * Tunables of the sparse Cholesky (0 keeps the default): the nested dissection's leaf size
+ * (default 48 rows) and top merge (512 rows; ordito's leaf 32 / top 512, cholesky.py:538-660),
+ * and the task count a sweep level's products are split towards (1024). A new leaf or top size
+ * drops the plan and every factor (the next solve analyses and factors again); a new split target
+ * only rebuilds the sweep tasks. The autotuner (ordito_amd/tune.py) times candidates through it.
+ */
+int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows, int32_t split_target);
+
+/*
  * Diffuse: solve (M - tL) u = rhs for n_columns columns of V entries (HeatSolver.diffuse,
  * heat.py:3890-3914). HG_MODE_ITERATE runs the settled Jacobi-CG (solve_spd_settled,
  * linalg.py:2490-2727) and its componentwise backward-error check; on failure it factors the

@@ -42,6 +42,7 @@ EXPORTED = (
     "hg_factor",
     "hg_factored",
     "hg_set_refinement",
+    "hg_set_plan_options",
     "hg_diffuse",
     "hg_solve_poisson",
     "hg_unit_gradient_divergence",
@@ -123,6 +124,7 @@ _SIGNATURES = {
     "hg_factor": ([_P, _I32, _I32, _PI32], ctypes.c_int),
     "hg_factored": ([_P, _PI32], ctypes.c_int),
     "hg_set_refinement": ([_P, _I32, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "hg_set_plan_options": ([_P, _I32, _I32, _I32], ctypes.c_int),
     "hg_diffuse": ([_P, _P, _P, _I64, _I32, _I32, _PI32], ctypes.c_int),
     "hg_solve_poisson": ([_P, _P, _P, _I64, _I32, _I32], ctypes.c_int),
     "hg_unit_gradient_divergence": ([_P, _P, _P, _I64], ctypes.c_int),

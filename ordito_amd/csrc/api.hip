@@ -70,6 +70,9 @@ struct hg_mesh {
   std::vector<int64_t> poisson_pin_slots, heat_pin_slots;
   bool has_heat_pins = false;
   // Cholesky.
+  AnalysisOptions options;                  // nested-dissection leaf and top-merge sizes
+  int32_t split_target = kDefaultSplitTarget;  // sweep tasks a level is split towards
+  std::unique_ptr<Plan> host_plan;
   std::unique_ptr<DevicePlan> plan;
   Factor heat_factor, poisson_factor;
   // Refinement targets of the factored solves, [precision][0 heat (componentwise), 1 Poisson
@@ -86,6 +89,8 @@ struct hg_mesh {
   int64_t field_capacity = 0, corner_capacity = 0;
   DeviceArray<double> packed_phi;  // packed meshes: the solved slot columns before extraction
   int64_t packed_capacity = 0;
+  DeviceArray<int64_t> q_src_off, q_dst_off, q_len;  // packed meshes: each query's field
+  DeviceArray<int32_t> q_local;                       // packed meshes: sources in mesh numbering
   hg_solve_stats stats{};
   PhaseTimer timer;
 };
@@ -249,10 +254,10 @@ void ensure_operators(hg_mesh& m) {
 
 void ensure_plan(hg_mesh& m) {
   if (m.plan) return;
-  AnalysisOptions options;
-  Plan plan = analyse(m.pattern, m.h_xyz.data(), options);
+  m.host_plan = std::make_unique<Plan>(analyse(m.pattern, m.h_xyz.data(), m.options));
+  const Plan& plan = *m.host_plan;
   auto device = std::make_unique<DevicePlan>();
-  device->upload(plan, m.pattern, m.stream);
+  device->upload(plan, m.pattern, m.split_target, m.stream);
   // Pins: the Poisson system's one row per component, its last-eliminated vertex
   // (kernels/cholesky.py:195-214); the heat system's empty rows (unreferenced vertices).
   const int64_t n = m.n;
@@ -449,8 +454,8 @@ void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets
     for (int64_t e = offsets[q]; e < offsets[q + 1]; ++e) owner[e - offsets[0]] = q;
   std::vector<int64_t> rel_offsets(k + 1);
   for (int64_t q = 0; q <= k; ++q) rel_offsets[q] = offsets[q] - offsets[0];
-  m.d_owner.upload(owner, m.stream);
-  m.d_offsets.upload(rel_offsets, m.stream);
+  m.d_owner.assign(owner, m.stream);
+  m.d_offsets.assign(rel_offsets, m.stream);
   if (m.d_sources.size < std::max<int64_t>(total, 1)) m.d_sources.allocate(std::max<int64_t>(total, 1));
   if (total > 0)
     HG_HIP(hipMemcpyAsync(m.d_sources.ptr, d_sources + offsets[0], sizeof(int32_t) * total,
@@ -692,6 +697,40 @@ int hg_factored(hg_mesh* mesh, int32_t* out_systems) {
   });
 }
 
+int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows,
+                        int32_t split_target) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(leaf_rows >= 0 && top_rows >= 0 && split_target >= 0, HG_ERR_VALUE,
+               "plan options must not be negative");
+    hg_mesh& m = *mesh;
+    const AnalysisOptions defaults;
+    const int32_t leaf = leaf_rows > 0 ? leaf_rows : defaults.leaf_rows;
+    const int32_t top = top_rows > 0 ? top_rows : defaults.top_rows;
+    const int32_t split = split_target > 0 ? split_target : kDefaultSplitTarget;
+    if (leaf != m.options.leaf_rows || top != m.options.top_rows) {
+      // A new ordering: the plan and every factor built on it go.
+      HG_HIP(hipStreamSynchronize(m.stream));
+      m.heat_factor.reset();
+      m.poisson_factor.reset();
+      m.plan.reset();
+      m.host_plan.reset();
+      m.options.leaf_rows = leaf;
+      m.options.top_rows = top;
+      m.split_target = split;
+      return;
+    }
+    if (split != m.split_target) {
+      m.split_target = split;
+      if (m.plan) {
+        HG_HIP(hipStreamSynchronize(m.stream));
+        m.plan->upload_sweeps(*m.host_plan, split, m.stream);
+        HG_HIP(hipStreamSynchronize(m.stream));
+      }
+    }
+  });
+}
+
 int hg_set_refinement(hg_mesh* mesh, int32_t precision, double heat_tolerance,
                       double poisson_tolerance) {
   return guarded([&] {
@@ -826,11 +865,13 @@ int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* s
     // them into the packed numbering and give each the column of its query's slot.
     const int64_t total = source_offsets[n_queries] - source_offsets[0];
     std::vector<int32_t> local(static_cast<size_t>(std::max<int64_t>(total, 1)));
-    if (total > 0)
-      HG_HIP(hipMemcpy(local.data(), d_sources + source_offsets[0], sizeof(int32_t) * total,
-                       hipMemcpyDefault));
+    if (total > 0) {
+      HG_HIP(hipMemcpyAsync(local.data(), d_sources + source_offsets[0], sizeof(int32_t) * total,
+                            hipMemcpyDefault, m.stream));
+      HG_HIP(hipStreamSynchronize(m.stream));
+    }
     std::vector<int32_t> packed(local.size());
-    std::vector<int64_t> owner(local.size()), packed_offsets(width + 1, 0), local_offsets(n_queries + 1);
+    std::vector<int64_t> owner(local.size()), local_offsets(n_queries + 1);
     for (int64_t q = 0; q < n_queries; ++q) {
       const int32_t s = query_mesh[q];
       for (int64_t e = source_offsets[q]; e < source_offsets[q + 1]; ++e) {
@@ -852,9 +893,8 @@ int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* s
     }
     ensure_operators(m);
     // The seed reads the packed sources with their slot owners; the offsets are unused there.
-    if (m.d_sources.size < std::max<int64_t>(total, 1)) m.d_sources.allocate(std::max<int64_t>(total, 1));
-    m.d_sources.upload(packed.data(), total, m.stream);
-    m.d_owner.upload(owner, m.stream);
+    m.d_sources.assign(packed, m.stream);
+    m.d_owner.assign(owner, m.stream);
     std::vector<int64_t> seed_offsets(width + 1, 0);
     seed_offsets[width] = total;
     if (m.packed_capacity < m.n * width) {
@@ -863,22 +903,20 @@ int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* s
     }
     solve_columns(m, seed_offsets.data(), width, m.packed_phi.ptr, HG_MODE_FACTORED, precision);
     // Each query's segment of its slot's column, then its shift against its own sources.
-    DeviceArray<int64_t> d_src_off, d_dst_off, d_len, d_local_offsets;
-    d_src_off.upload(src_off, m.stream);
-    d_dst_off.upload(dst_off, m.stream);
-    d_len.upload(len, m.stream);
-    d_local_offsets.upload(local_offsets, m.stream);
-    DeviceArray<int32_t> d_local;
-    d_local.upload(local, m.stream);
-    launch_extract_fields(m.packed_phi.ptr, d_src_off.ptr, d_dst_off.ptr, d_len.ptr, max_len,
+    m.q_src_off.assign(src_off, m.stream);
+    m.q_dst_off.assign(dst_off, m.stream);
+    m.q_len.assign(len, m.stream);
+    m.d_offsets.assign(local_offsets, m.stream);
+    m.q_local.assign(local, m.stream);
+    launch_extract_fields(m.packed_phi.ptr, m.q_src_off.ptr, m.q_dst_off.ptr, m.q_len.ptr, max_len,
                           n_queries, d_out, m.stream);
     const int32_t blocks = shift_partial_blocks(max_len);
     if (m.shift_partials.size < blocks * n_queries) m.shift_partials.allocate(blocks * n_queries);
     if (m.shift_values.size < 2 * n_queries) m.shift_values.allocate(2 * n_queries);
     {
       PhaseScope scope(&m.timer, kPhaseShift, m.stream);
-      launch_shift(d_out, d_local.ptr, d_local_offsets.ptr, max_len, n_queries, d_dst_off.ptr,
-                   d_len.ptr, m.shift_partials.ptr, m.shift_values.ptr, m.stream);
+      launch_shift(d_out, m.q_local.ptr, m.d_offsets.ptr, max_len, n_queries, m.q_dst_off.ptr,
+                   m.q_len.ptr, m.shift_partials.ptr, m.shift_values.ptr, m.stream);
     }
     HG_HIP(hipStreamSynchronize(m.stream));
     m.timer.resolve();

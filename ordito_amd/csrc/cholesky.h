@@ -35,18 +35,20 @@ struct DevicePlan {
   };
   SweepTasks fwd, bwd;
   int32_t max_slots = 0;
+  int32_t split_target = kDefaultSplitTarget;
   // Forward gather lists: each front position's children contributions (absolute rows of the
   // front-vector arena), in ascending child order.
   DeviceArray<int32_t> gather_ptr, gather_src;
-  std::vector<int32_t> gather_task_ptr;  // = forward tiles per level
-  DeviceArray<int32_t> gather_s, gather_p0;
   // The operator's pattern in the elimination order (P A P^T) and each entry's original slot.
   DeviceArray<int32_t> p_offsets, p_columns, p_src;
   // The numeric factorization's tile launches (host list) and tasks (device, 4 int32 each).
   std::vector<FactorLaunch> factor_launches;
   DeviceArray<int32_t> factor_tasks;
 
-  void upload(const Plan& plan, const HostPattern& pattern, hipStream_t stream);
+  void upload(const Plan& plan, const HostPattern& pattern, int32_t target_tasks,
+              hipStream_t stream);
+  // Rebuilds the sweep tasks for another split target (the factors are unaffected).
+  void upload_sweeps(const Plan& plan, int32_t target_tasks, hipStream_t stream);
   int64_t bytes() const;
 };
 

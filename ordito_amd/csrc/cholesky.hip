@@ -272,10 +272,8 @@ inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) /
 
 // ---- host side --------------------------------------------------------------------------------
 
-// Product tasks a level is split into at least, where its tiles allow: a few per compute unit.
-constexpr int32_t kTargetTasks = 1024;
-
-HostSweep sweep_tasks(const Plan& plan, bool forward) {
+HostSweep sweep_tasks(const Plan& plan, bool forward, int32_t target_tasks) {
+  const int64_t kTargetTasks = std::max<int32_t>(target_tasks, 1);
   HostSweep out;
   out.level_ptr.assign(plan.n_levels + 1, 0);
   out.reduce_ptr.assign(plan.n_levels + 1, 0);
@@ -353,26 +351,15 @@ HostGather gather_lists(const Plan& plan) {
       g.src[at[target]++] = static_cast<int32_t>(plan.vec_off[c] + plan.ncol[c] + i);
     }
   }
-  g.task_ptr.assign(plan.n_levels + 1, 0);
-  for (int32_t level = 0; level < plan.n_levels; ++level) {
-    for (int32_t k = plan.level_ptr[level]; k < plan.level_ptr[level + 1]; ++k) {
-      const int32_t s = plan.level_nodes[k];
-      for (int32_t p0 = 0; p0 < plan.ncol[s] + plan.nrow[s]; p0 += 64) {
-        g.task_s.push_back(s);
-        g.task_p0.push_back(p0);
-      }
-    }
-    g.task_ptr[level + 1] = static_cast<int32_t>(g.task_s.size());
-  }
   return g;
 }
 
-void validate_sweeps(const Plan& plan) {
+void validate_sweeps(const Plan& plan, int32_t target_tasks) {
   auto fail = [](const std::string& what) { throw Error{HG_ERR_RUNTIME, "plan check: " + what}; };
   const int64_t S = plan.n_super;
   for (int dir = 0; dir < 2; ++dir) {
     const bool forward = dir == 0;
-    const HostSweep h = sweep_tasks(plan, forward);
+    const HostSweep h = sweep_tasks(plan, forward, target_tasks);
     std::vector<int64_t> covered(S, 0);
     for (size_t t = 0; t < h.s.size(); ++t) {
       const int32_t s = h.s[t];
@@ -409,10 +396,6 @@ void validate_sweeps(const Plan& plan) {
   if (static_cast<int64_t>(g.ptr.size()) != plan.vec_size + 1) fail("gather offsets size");
   for (int32_t v : g.src)
     if (v < 0 || v >= plan.vec_size) fail("gather source out of the front-vector arena");
-  for (size_t t = 0; t < g.task_s.size(); ++t) {
-    const int32_t s = g.task_s[t];
-    if (s < 0 || s >= S || g.task_p0[t] >= plan.ncol[s] + plan.nrow[s]) fail("gather task range");
-  }
   // Every structure row of every child lands in exactly one parent position.
   int64_t rows_with_parent = 0;
   for (int64_t s = 0; s < S; ++s)
@@ -562,24 +545,27 @@ void upload_sweep(const HostSweep& h, DevicePlan::SweepTasks& out, hipStream_t s
 
 }  // namespace
 
-void DevicePlan::upload(const Plan& plan, const HostPattern& pattern, hipStream_t stream) {
+void DevicePlan::upload_sweeps(const Plan& plan, int32_t target_tasks, hipStream_t stream) {
+  const HostSweep hf = sweep_tasks(plan, true, target_tasks);
+  const HostSweep hb = sweep_tasks(plan, false, target_tasks);
+  max_slots = std::max(hf.max_slots, hb.max_slots);
+  upload_sweep(hf, fwd, stream);
+  upload_sweep(hb, bwd, stream);
+  split_target = target_tasks;
+}
+
+void DevicePlan::upload(const Plan& plan, const HostPattern& pattern, int32_t target_tasks,
+                        hipStream_t stream) {
   {
     HostFactorTasks ft = hg::factor_tasks(plan);
     factor_launches = std::move(ft.launches);
     factor_tasks.upload(ft.tasks, stream);
   }
-  const HostSweep hf = sweep_tasks(plan, true);
-  const HostSweep hb = sweep_tasks(plan, false);
-  max_slots = std::max(hf.max_slots, hb.max_slots);
-  upload_sweep(hf, fwd, stream);
-  upload_sweep(hb, bwd, stream);
+  upload_sweeps(plan, target_tasks, stream);
   {
     const HostGather g = gather_lists(plan);
     gather_ptr.upload(g.ptr, stream);
     gather_src.upload(g.src, stream);
-    gather_task_ptr = g.task_ptr;
-    gather_s.upload(g.task_s, stream);
-    gather_p0.upload(g.task_p0, stream);
   }
   // The operator's pattern in the elimination order: row i is old row perm[i], its columns
   // renumbered and sorted; src names each entry's slot in the original values.
@@ -647,7 +633,7 @@ int64_t DevicePlan::bytes() const {
          level_nodes.bytes() + front_off.bytes() + w_off.bytes() + vec_off.bytes() +
          entry_src.bytes() + entry_dst.bytes() + entry_row.bytes() + entry_col.bytes() +
          fwd.s.bytes() * 5 + fwd.rs.bytes() * 4 + bwd.s.bytes() * 5 + bwd.rs.bytes() * 4 +
-         gather_ptr.bytes() + gather_src.bytes() + gather_s.bytes() + gather_p0.bytes() +
+         gather_ptr.bytes() + gather_src.bytes() +
          p_offsets.bytes() + p_columns.bytes() + p_src.bytes() + factor_tasks.bytes();
 }
 

@@ -70,6 +70,14 @@ struct DeviceArray {
     allocate(static_cast<int64_t>(host.size()));
     upload(host.data(), size, stream);
   }
+  // Copies host into the array, reallocating only to grow: per-call arrays (sources, offsets)
+  // keep their memory, so a solve allocates nothing once warm (hipMalloc and hipFree synchronize
+  // the device, which would serialize solves on other streams).
+  void assign(const std::vector<T>& host, hipStream_t stream) {
+    const int64_t count = static_cast<int64_t>(host.size());
+    if (size < std::max<int64_t>(count, 1)) allocate(std::max<int64_t>(count, 1));
+    upload(host.data(), count, stream);
+  }
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -148,18 +156,21 @@ struct HostSweep {
   std::vector<int32_t> rs, rtile, rfirst, rcount;
   int32_t max_slots = 0;
 };
-HostSweep sweep_tasks(const Plan& plan, bool forward);
+// A level whose tiles are fewer than `target_tasks` has its products split along the inner
+// dimension until it has about that many tasks (a few per compute unit by default).
+constexpr int32_t kDefaultSplitTarget = 1024;
+HostSweep sweep_tasks(const Plan& plan, bool forward, int32_t target_tasks = kDefaultSplitTarget);
 
 // The forward sweep's gather lists: for each front-vector row (vec_off[s] + position) the rows of
-// the children's update vectors that add into it, and the gather tasks (front, 64-position chunk).
+// the children's update vectors that add into it.
 struct HostGather {
-  std::vector<int32_t> ptr, src, task_ptr, task_s, task_p0;
+  std::vector<int32_t> ptr, src;
 };
 HostGather gather_lists(const Plan& plan);
 
 // Checks every index the device sweeps form from the plan against its buffer; throws on the first
 // violation.
-void validate_sweeps(const Plan& plan);
+void validate_sweeps(const Plan& plan, int32_t target_tasks = kDefaultSplitTarget);
 
 // The numeric factorization as launches of 64 x 64 tile tasks (cholesky.hip), in execution order.
 // A front's inverse factor Z is the first ncol columns of the inverse of its unit-extended lower
