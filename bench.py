@@ -24,6 +24,11 @@ path: seed, heat solve, unit-gradient divergence, Poisson solve, shift -- all on
   of the GPU fields.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Two further workloads print their own JSON line (the default run above is the headline):
+``--workload batch`` (256 meshes of 1e3-2e5 vertices packed into one system, one query each) and
+``--workload serve`` (64 meshes of 1e4-1e6 vertices behind the continuous-batching server,
+Poisson arrivals at 2,000 queries/s for 60 s, 200 GB budget).
 """
 
 from __future__ import annotations
@@ -62,7 +67,222 @@ def parse() -> argparse.Namespace:
         help="only run three 1 GiB device copies (the reference kernel for scaling the FETCH_SIZE "
         "and WRITE_SIZE counters) and exit",
     )
-    return parser.parse_args()
+    parser.add_argument(
+        "--workload",
+        choices=("torus", "batch", "serve"),
+        default="torus",
+        help="torus: the headline configuration (default); batch: many meshes packed into one "
+        "system; serve: the continuous-batching server under Poisson arrivals",
+    )
+    many = parser.add_argument_group("--workload batch / serve")
+    many.add_argument("--meshes", type=int, help="default 256 (batch), 64 (serve)")
+    many.add_argument("--min-vertices", type=float, help="default 1e3 (batch), 1e4 (serve)")
+    many.add_argument("--max-vertices", type=float, help="default 2e5 (batch), 1e6 (serve)")
+    many.add_argument("--seed", type=int, default=0)
+    many.add_argument("--tune", action="store_true", help="batch: autotune first")
+    many.add_argument("--rate", type=float, default=2000.0, help="serve: Poisson arrivals per second")
+    many.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
+    many.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget")
+    many.add_argument("--workers", type=int, default=4, help="serve: batches solved at once")
+    args = parser.parse_args()
+    defaults = {"batch": (256, 1e3, 2e5), "serve": (64, 1e4, 1e6)}.get(args.workload)
+    if defaults:
+        args.meshes = args.meshes or defaults[0]
+        args.min_vertices = args.min_vertices or defaults[1]
+        args.max_vertices = args.max_vertices or defaults[2]
+    return args
+
+
+def _generated_meshes(count: int, low: float, high: float, rng) -> dict:
+    """Seeded meshes with log-uniform vertex counts in [low, high]: jittered icospheres at the
+    nearest level, jittered tori and jittered planar grids (open, with a rim)."""
+    from ordito_amd import creation
+
+    meshes = {}
+    for i in range(count):
+        target = float(np.exp(rng.uniform(np.log(low), np.log(high))))
+        kind = int(rng.integers(3))
+        seed = int(rng.integers(1 << 30))
+        if kind == 0:
+            level = int(np.clip(round(np.log(max(target - 2, 10) / 10) / np.log(4)), 1, 9))
+            meshes[i] = creation.jittered_icosphere(level, 0.01, seed)
+        elif kind == 1:
+            minor = max(8, int(round(np.sqrt(target / 2))))
+            major = max(8, int(round(target / minor)))
+            meshes[i] = creation.torus(major, minor, 1.0, 0.3, jitter=0.1, seed=seed)
+        else:
+            side = max(8, int(round(np.sqrt(target))))
+            meshes[i] = creation.planar_grid(side, side, 1.0, jitter=0.2, seed=seed)
+    return meshes
+
+
+def batch_workload(args) -> int:
+    """
+    The batched configuration: ``--meshes`` (default 256) meshes with log-uniform vertex counts in
+    [``--min-vertices``, ``--max-vertices``] (default 1e3..2e5), one query each with 1-16 random
+    sources, packed into one block-diagonal system (``MeshBatch``): one factorization and one
+    batched solve for all of them, every level sweep one launch over every mesh. ``--tune`` runs
+    the autotuner first. Prints one JSON line: queries/s over the timed batches, setup, and the
+    error of sampled answers against the FP64 oracle.
+    """
+    import torch
+
+    from ordito_amd.batch import MeshBatch
+
+    rng = np.random.default_rng(args.seed)
+    log(f"== generating {args.meshes} meshes")
+    meshes = _generated_meshes(args.meshes, args.min_vertices, args.max_vertices, rng)
+    sizes = [len(meshes[i][0]) for i in range(args.meshes)]
+    queries = []
+    for i in range(args.meshes):
+        count = min(int(rng.integers(1, 17)), sizes[i])
+        queries.append((i, rng.choice(sizes[i], size=count, replace=False).astype(np.int32)))
+    t0 = time.perf_counter()
+    batch = MeshBatch([(torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda()) for v, f in meshes.values()])
+    bind_s = time.perf_counter() - t0
+    tuned = None
+    if args.tune:
+        from ordito_amd.tune import autotune
+
+        log("== autotuning")
+        tuned = autotune(batch, repeats=2)
+        log(f"   best {tuned['best']}")
+    t0 = time.perf_counter()
+    assert batch.factor()
+    torch.cuda.synchronize()
+    factor_s = time.perf_counter() - t0
+    device_queries = [(m, torch.from_numpy(s).cuda()) for m, s in queries]
+    log(f"== {args.steps} batches of {len(queries)} queries on {sum(sizes)} vertices")
+    for _ in range(args.warmup):
+        batch.geodesic(device_queries)
+    torch.cuda.synchronize()
+    latencies = []
+    fields = None
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        fields = batch.geodesic(device_queries)
+        torch.cuda.synchronize()
+        latencies.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - start
+    from oracle import heat_oracle as ho
+
+    errors = []
+    for i in sorted(range(args.meshes), key=lambda i: sizes[i])[:3]:
+        v, f = meshes[i]
+        errors.append(ho.relative_error(fields[i].cpu().numpy(), ho.geodesic(v, f, queries[i][1]).distance)[0])
+    result = {
+        "metric": METRIC,
+        "value": round(len(queries) * args.steps / elapsed, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.meshes} meshes packed block-diagonally, V log-uniform in "
+            f"[{args.min_vertices:.0f}, {args.max_vertices:.0f}] (icosphere / torus / planar "
+            f"grid), one query of 1-16 sources each, all in one batched solve",
+            "total_vertices": int(sum(sizes)),
+        },
+        "p50_ms": ms(latencies, 50),
+        "p99_ms": ms(latencies, 99),
+        "setup_s": {"bind": round(bind_s, 3), "factor": round(factor_s, 3)},
+        "refinements": dict(batch.last_stats),
+        "autotune_best": tuned["best"] if tuned else None,
+        "sampled_rel_err_max": max(errors),
+        "factor_gb": round(batch.nbytes / 1e9, 3),
+    }
+    print(json.dumps(result))
+    return 0
+
+
+def serve_workload(args) -> int:
+    """
+    The serving configuration: ``--meshes`` meshes with log-uniform vertex counts in
+    [min, max] (seeded jittered icospheres at the nearest level, tori and jittered planar grids),
+    queries arriving as a Poisson process at ``--rate`` per second for ``--duration`` seconds, each
+    asking for 1-32 random sources on a uniformly chosen mesh, served by ``GeodesicServer`` under a
+    ``--budget-gb`` HBM budget. Every mesh is warmed (factored) first; that setup is reported
+    apart. Prints one JSON line: throughput, p50/p99 latency, and the error of sampled answers
+    against the FP64 oracle.
+    """
+    from ordito_amd.serve import GeodesicServer
+
+    rng = np.random.default_rng(args.seed)
+    log(f"== generating {args.meshes} meshes")
+    meshes = _generated_meshes(args.meshes, args.min_vertices, args.max_vertices, rng)
+    sizes = {i: len(v) for i, (v, _) in meshes.items()}
+    server = GeodesicServer(meshes, budget_bytes=args.budget_gb * 1e9, workers=args.workers)
+    log(f"== warming {args.meshes} meshes ({sum(sizes.values())} vertices)")
+    t0 = time.perf_counter()
+    for future in [server.submit(i, [0]) for i in meshes]:
+        future.result()
+    setup = time.perf_counter() - t0
+    log(f"   warm in {setup:.1f}s; {args.duration:.0f}s of arrivals at {args.rate:.0f}/s")
+    start = time.perf_counter()
+    arrival = start
+    submitted = []
+    next_report = start + 5.0
+    while True:
+        arrival += rng.exponential(1.0 / args.rate)
+        if arrival - start > args.duration:
+            break
+        delay = arrival - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        m = int(rng.integers(args.meshes))
+        count = min(int(rng.integers(1, 33)), sizes[m])
+        sources = rng.choice(sizes[m], size=count, replace=False).astype(np.int32)
+        submitted.append((m, sources, server.submit(m, sources)))
+        if time.perf_counter() > next_report:
+            log(f"   {len(submitted)} submitted, {server.stats(since=start).get('completed', 0)} done")
+            next_report += 5.0
+    for _, _, future in submitted:
+        future.result()
+    stats = server.stats(since=start)
+    # Accuracy: sampled answers on the smallest meshes against the FP64 oracle.
+    from oracle import heat_oracle as ho
+
+    small = sorted(meshes, key=lambda i: sizes[i])[:3]
+    errors = []
+    for m, sources, future in submitted:
+        if m in small and len(errors) < 3:
+            v, f = meshes[m]
+            errors.append(ho.relative_error(future.result().cpu().numpy(), ho.geodesic(v, f, sources).distance)[0])
+    server.close()
+    result = {
+        "metric": METRIC,
+        "value": round(stats["queries_per_s"], 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.meshes} meshes, V log-uniform in [{args.min_vertices:.0f}, "
+            f"{args.max_vertices:.0f}] (icosphere / torus / planar grid), Poisson arrivals at "
+            f"{args.rate:.0f}/s for {args.duration:.0f}s, 1-32 sources per query, "
+            f"{args.budget_gb:.0f} GB budget, {args.workers} workers",
+            "total_vertices": int(sum(sizes.values())),
+        },
+        "offered_per_s": round(len(submitted) / args.duration, 2),
+        "completed": stats["completed"],
+        "p50_ms": round(stats["p50_ms"], 3),
+        "p99_ms": round(stats["p99_ms"], 3),
+        "mean_ms": round(stats["mean_ms"], 3),
+        "batches": stats["batches"],
+        "evictions": stats["evictions"],
+        "resident_meshes": stats["resident"],
+        "resident_gb": round(stats["resident_bytes"] / 1e9, 3),
+        "setup_s": round(setup, 2),
+        "sampled_rel_err_max": max(errors) if errors else None,
+    }
+    print(json.dumps(result))
+    return 0
 
 
 def log(message: str) -> None:
@@ -163,6 +383,10 @@ def main() -> int:
     torch.cuda.init()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.workload == "serve":
+        return serve_workload(args)
+    if args.workload == "batch":
+        return batch_workload(args)
     log(f"== generating torus {args.major}x{args.minor}")
     vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)
     n = vertices.shape[0]
