@@ -25,10 +25,12 @@ path: seed, heat solve, unit-gradient divergence, Poisson solve, shift -- all on
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Two further workloads print their own JSON line (the default run above is the headline):
-``--workload batch`` (256 meshes of 1e3-2e5 vertices packed into one system, one query each) and
-``--workload serve`` (64 meshes of 1e4-1e6 vertices behind the continuous-batching server,
-Poisson arrivals at 2,000 queries/s for 60 s, 200 GB budget).
+Three further workloads print their own JSON line (the default run above is the headline):
+``--workload sphere`` (icosphere(9) displaced by spherical harmonics, 2.6 M vertices, 16 queries,
+FP64 and mixed refined to 1e-10, errors against the oracle), ``--workload batch`` (256 meshes of
+1e3-2e5 vertices packed into one system, one query each) and ``--workload serve`` (64 meshes of
+1e4-1e6 vertices behind the continuous-batching server, Poisson arrivals at 2,000 queries/s for
+60 s, 200 GB budget).
 """
 
 from __future__ import annotations
@@ -69,10 +71,11 @@ def parse() -> argparse.Namespace:
     )
     parser.add_argument(
         "--workload",
-        choices=("torus", "batch", "serve"),
+        choices=("torus", "sphere", "batch", "serve"),
         default="torus",
-        help="torus: the headline configuration (default); batch: many meshes packed into one "
-        "system; serve: the continuous-batching server under Poisson arrivals",
+        help="torus: the headline configuration (default); sphere: the 2.6 M-vertex displaced "
+        "sphere, mixed and FP64 against the oracle; batch: many meshes packed into one system; "
+        "serve: the continuous-batching server under Poisson arrivals",
     )
     many = parser.add_argument_group("--workload batch / serve")
     many.add_argument("--meshes", type=int, help="default 256 (batch), 64 (serve)")
@@ -91,6 +94,80 @@ def parse() -> argparse.Namespace:
         args.min_vertices = args.min_vertices or defaults[1]
         args.max_vertices = args.max_vertices or defaults[2]
     return args
+
+
+def sphere_workload(args) -> int:
+    """
+    The large-mesh configuration: an icosphere subdivided 9 times (2,621,442 vertices) displaced by
+    real spherical harmonics up to degree 16 with N(0, 0.05 / l) coefficients, 16 single-source
+    queries as one batch, through FP64 factors and through FP32 factors refined to 1e-10
+    (componentwise heat, normwise Poisson); both fields' max and mean error against the FP64
+    oracle (SciPy SuperLU) on a sample of the queries.
+    """
+    import torch
+
+    from ordito_amd import creation, heat
+    from oracle import heat_oracle as ho
+
+    log("== generating the displaced sphere (icosphere(9), SH degree 16)")
+    vertices, faces = creation.displaced_sphere(9, lmax=16, amplitude=0.05, seed=args.seed)
+    n = len(vertices)
+    sources = creation.random_sources(n, 16, seed=args.seed + 1)
+    sets = [torch.tensor([s], dtype=torch.int32, device="cuda") for s in sources]
+    v_gpu, f_gpu = torch.from_numpy(vertices).cuda(), torch.from_numpy(faces).cuda()
+    result = {
+        "metric": METRIC,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "data": "synthetic",
+        "config": {
+            "workload": "icosphere(9) displaced by real SH (l <= 16, N(0, 0.05/l)), 16 "
+            "single-source queries as one batch, factors resident",
+            "n_vertices": int(n),
+            "n_faces": int(faces.size // 3),
+        },
+    }
+    fields = {}
+    for precision in ("fp64", "mixed"):
+        mesh = heat.Trimesh(v_gpu, f_gpu)
+        solver = mesh.heat_solver(method="meshlab", precision=precision)
+        if precision == "mixed":
+            solver.set_refinement(heat=1e-10, poisson=1e-10)
+        t0 = time.perf_counter()
+        assert solver.factor()
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        elapsed, latencies, timing, out, stats = timed_steps(solver, sets, args.steps, args.warmup)
+        fields[precision] = out.cpu().numpy()
+        result[precision] = {
+            "value": round(len(sets) * args.steps / elapsed, 2),
+            "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+            "p50_ms": ms(latencies, 50),
+            "p99_ms": ms(latencies, 99),
+            "factor_s": round(setup, 3),
+            "factor_gb": round(solver.nbytes / 1e9, 3),
+            "refinements": stats,
+            "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timing.items() if v[1]},
+            "t": solver.time,
+        }
+        log(f"   {precision}: {result[precision]['value']} queries/s, {stats}")
+        del solver, mesh
+        torch.cuda.empty_cache()
+    result["value"] = result["fp64"]["value"]
+    result["dtype"] = "f64"
+    log("== oracle (SciPy SuperLU, 1 thread)")
+    t0 = time.perf_counter()
+    oracle = ho.FactoredOracle(vertices, faces)
+    result["oracle_factor_s"] = round(time.perf_counter() - t0, 1)
+    sample = list(range(min(args.cpu_sample, len(sets))))
+    for precision in ("fp64", "mixed"):
+        errs = [ho.relative_error(fields[precision][q], oracle.distance(np.array([sources[q]]))) for q in sample]
+        result[precision]["rel_err_max"] = max(e[0] for e in errs)
+        result[precision]["rel_err_mean"] = float(np.mean([e[1] for e in errs]))
+    result["queries_checked"] = len(sample)
+    print(json.dumps(result))
+    return 0
 
 
 def _generated_meshes(count: int, low: float, high: float, rng) -> dict:
@@ -387,6 +464,8 @@ def main() -> int:
         return serve_workload(args)
     if args.workload == "batch":
         return batch_workload(args)
+    if args.workload == "sphere":
+        return sphere_workload(args)
     log(f"== generating torus {args.major}x{args.minor}")
     vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)
     n = vertices.shape[0]

@@ -183,12 +183,12 @@ def planar_grid(
     return vertices.astype(np.float32), faces.astype(np.int32).reshape(-1)
 
 
-def _real_spherical_harmonics(lmax: int, theta: np.ndarray, phi: np.ndarray) -> list[np.ndarray]:
-    """Orthonormal real spherical harmonics ``Y_lm`` for ``1 <= l <= lmax``, ``-l <= m <= l``."""
+def _real_spherical_harmonics(lmax: int, theta: np.ndarray, phi: np.ndarray):
+    """Orthonormal real spherical harmonics ``Y_lm`` for ``1 <= l <= lmax``, ``-l <= m <= l``, one
+    at a time in that order (a 2.6 M-vertex sphere's 288 of them would take 6 GB at once)."""
     from scipy.special import lpmv
 
     x = np.cos(theta)
-    basis = []
     for degree in range(1, lmax + 1):
         for order in range(-degree, degree + 1):
             k = abs(order)
@@ -199,12 +199,11 @@ def _real_spherical_harmonics(lmax: int, theta: np.ndarray, phi: np.ndarray) -> 
             # scipy's lpmv includes the Condon-Shortley phase; it cancels in a random-sign sum.
             legendre = lpmv(k, degree, x)
             if order == 0:
-                basis.append(norm * legendre)
+                yield norm * legendre
             elif order > 0:
-                basis.append(math.sqrt(2.0) * norm * legendre * np.cos(k * phi))
+                yield math.sqrt(2.0) * norm * legendre * np.cos(k * phi)
             else:
-                basis.append(math.sqrt(2.0) * norm * legendre * np.sin(k * phi))
-    return basis
+                yield math.sqrt(2.0) * norm * legendre * np.sin(k * phi)
 
 
 def displaced_sphere(
@@ -220,12 +219,10 @@ def displaced_sphere(
     phi = np.arctan2(unit[:, 1], unit[:, 0])
     rng = np.random.default_rng(seed)
     radius = np.ones(unit.shape[0])
-    index = 0
     basis = _real_spherical_harmonics(lmax, theta, phi)
     for degree in range(1, lmax + 1):
         for _ in range(2 * degree + 1):
-            radius += rng.normal(0.0, amplitude / degree) * basis[index]
-            index += 1
+            radius += rng.normal(0.0, amplitude / degree) * next(basis)
     return (unit * radius[:, None]).astype(np.float32), faces
 
 

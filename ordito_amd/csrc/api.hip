@@ -355,27 +355,30 @@ bool backward_error_fails(hg_mesh& m, const double* x, const double* b, int64_t 
   return flag != 0;
 }
 
-int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t precision) {
+int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t precision,
+                         FieldLayout layout = FieldLayout::kColumns) {
   const bool mixed = precision == HG_PRECISION_MIXED;
   factor_system(m, HG_SYSTEM_HEAT, mixed);
   return refined_solve(*m.plan, m.heat_factor, mixed, rhs, x, k,
                        m.refine_tolerance[mixed ? 1 : 0][0], true,
                        mixed ? kMixedRefineRounds : kRefineRounds,
                        m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr, m.solve, m.stream,
-                       &m.timer, kPhaseHeatSweep);
+                       &m.timer, kPhaseHeatSweep, layout, layout);
 }
 
 // The settled iterate, verified, falling back to the factorization (solve_spd_settled,
 // linalg.py:2659-2727). Returns the flags of hg_diffuse.
+// Fields stored by rows (FieldLayout::kRows) go through the factors only.
 int32_t diffuse(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t mode,
-                int32_t precision) {
+                int32_t precision, FieldLayout layout = FieldLayout::kColumns) {
   ensure_operators(m);
   int32_t flags = 0;
   if (mode == HG_MODE_FACTORED || m.heat_factor.held()) {
-    m.stats.heat_refinements = diffuse_factored(m, rhs, x, k, precision);
+    m.stats.heat_refinements = diffuse_factored(m, rhs, x, k, precision, layout);
     m.stats.heat_rounds = 0;
     return flags;
   }
+  HG_REQUIRE(layout == FieldLayout::kColumns, HG_ERR_RUNTIME, "an iterated solve takes columns");
   m.stats.heat_rounds = diffuse_iterated(m, rhs, x, k);
   m.stats.heat_refinements = 0;
   if (backward_error_fails(m, x, rhs, k)) {
@@ -402,8 +405,9 @@ double poisson_bound(hg_mesh& m) {
   return m.poisson_bound;
 }
 
+// The solution is stored by columns; the right-hand side may be stored by rows (factored only).
 void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t mode,
-                   int32_t precision) {
+                   int32_t precision, FieldLayout rhs_layout = FieldLayout::kColumns) {
   ensure_operators(m);
   const bool factored = mode == HG_MODE_FACTORED || m.poisson_factor.held() || m.heat_factor.held();
   if (factored) {
@@ -412,9 +416,11 @@ void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t 
     m.stats.poisson_refinements = refined_solve(
         *m.plan, m.poisson_factor, mixed, rhs, x, k, m.refine_tolerance[mixed ? 1 : 0][1], false,
         mixed ? kMixedRefineRounds : kRefineRounds, m.poisson_pinned_new.ptr, m.solve, m.stream,
-        &m.timer, kPhasePoissonSweep);
+        &m.timer, kPhasePoissonSweep, rhs_layout, FieldLayout::kColumns);
     m.stats.poisson_rounds = 0;
   } else {
+    HG_REQUIRE(rhs_layout == FieldLayout::kColumns, HG_ERR_RUNTIME,
+               "an iterated solve takes columns");
     // The Jacobi-Chebyshev interval (linalg.py:6074-6090): [max(min(80/n, 0.5), 1 - r), 1 + r],
     // r the largest Gershgorin ratio of D^-1 A, at least 1e-3.
     double r = std::max(poisson_bound(m), 1e-3);
@@ -462,12 +468,14 @@ void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets
                           hipMemcpyDefault, m.stream));
 }
 
-void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0) {
+void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0,
+          FieldLayout layout = FieldLayout::kColumns) {
   const int64_t total = offsets[k] - offsets[0];
   HG_HIP(hipMemsetAsync(u0, 0, sizeof(double) * m.n * k, m.stream));
   if (m.invalid.size < 1) m.invalid.allocate(1);
   HG_HIP(hipMemsetAsync(m.invalid.ptr, 0, sizeof(int32_t), m.stream));
-  launch_seed(m.d_sources.ptr, m.d_owner.ptr, total, m.n, u0, m.invalid.ptr, m.stream);
+  launch_seed(m.d_sources.ptr, m.d_owner.ptr, total, m.n, layout == FieldLayout::kRows ? k : 0,
+              u0, m.invalid.ptr, m.stream);
   int32_t flag = 0;
   HG_HIP(hipMemcpyAsync(&flag, m.invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
   HG_HIP(hipStreamSynchronize(m.stream));
@@ -475,7 +483,8 @@ void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0) {
              "every entry of sources must be a vertex index in [0, " + std::to_string(m.n) + ")");
 }
 
-void divergence(hg_mesh& m, const double* heat, double* out, int64_t k) {
+void divergence(hg_mesh& m, const double* heat, double* out, int64_t k,
+                FieldLayout layout = FieldLayout::kColumns) {
   if (m.corner_capacity < 3 * m.f * k) {
     m.corner_terms.allocate(std::max<int64_t>(3 * m.f * k, 1));
     m.corner_capacity = 3 * m.f * k;
@@ -483,7 +492,7 @@ void divergence(hg_mesh& m, const double* heat, double* out, int64_t k) {
   PhaseScope scope(&m.timer, kPhaseDivergence, m.stream);
   launch_divergence(m.xyz.ptr, m.faces.ptr, m.normals.ptr, m.areas.ptr, m.cot.ptr,
                     m.vertex_ptr.ptr, m.vertex_corners.ptr, heat, m.n, m.f, k, m.corner_terms.ptr,
-                    out, m.stream);
+                    out, layout, m.stream);
 }
 
 double* reserve_fields(hg_mesh& m, int64_t k) {
@@ -512,11 +521,16 @@ void solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   double* u0 = fields;
   double* heat = fields + n * k;
   double* negdiv = fields + 2 * n * k;
-  seed(m, offsets, k, u0);
-  int32_t flags = diffuse(m, u0, heat, k, mode, precision);
+  // Through the factors the intermediate fields stay in the solves' row layout: the permutations
+  // in and out of the refined solves and the divergence then read and write whole 512-byte rows.
+  const FieldLayout layout = (mode == HG_MODE_FACTORED || m.heat_factor.held())
+                                 ? FieldLayout::kRows
+                                 : FieldLayout::kColumns;
+  seed(m, offsets, k, u0, layout);
+  int32_t flags = diffuse(m, u0, heat, k, mode, precision, layout);
   m.stats.backward_error_failed = flags & 1;
-  divergence(m, heat, negdiv, k);
-  solve_poisson(m, negdiv, phi, k, mode, precision);
+  divergence(m, heat, negdiv, k, layout);
+  solve_poisson(m, negdiv, phi, k, mode, precision, layout);
 }
 
 // The host and device state of a mesh from its host positions and faces (global indices).

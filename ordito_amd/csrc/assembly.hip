@@ -219,8 +219,9 @@ __global__ void operator_rows_kernel(
 
 // u0[q, s] = 1 at each source of column q (duplicates allowed); an index outside [0, n) writes
 // nothing and raises the flag (kernels/heat.py:599-623). One thread a source entry.
+// rows_k > 0: u0 is stored by rows (FieldLayout::kRows) over rows_k columns.
 __global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* __restrict__ owner,
-                            int64_t n_entries, int64_t n, double* __restrict__ u0,
+                            int64_t n_entries, int64_t n, int64_t rows_k, double* __restrict__ u0,
                             int32_t* __restrict__ invalid) {
   int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n_entries) return;
@@ -229,7 +230,14 @@ __global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* 
     atomicOr(invalid, 1);
     return;
   }
-  u0[owner[e] * n + v] = 1.0;
+  const int64_t q = owner[e];
+  if (rows_k > 0) {
+    const int64_t chunk = q / kFieldChunk;
+    const int64_t kk = min(kFieldChunk, rows_k - chunk * kFieldChunk);
+    u0[chunk * kFieldChunk * n + v * kk + q % kFieldChunk] = 1.0;
+  } else {
+    u0[q * n + v] = 1.0;
+  }
 }
 
 // Per face and column: the unit gradient direction x of the heat (sum_k u_k (N x e_k) / (2A) in
@@ -271,6 +279,61 @@ __global__ void divergence_corners_kernel(const float* __restrict__ xyz,
   out[0] = c2 * dot(sub(v1, v0), x) + c1 * dot(sub(v2, v0), x);
   out[1] = c0 * dot(sub(v2, v1), x) + c2 * dot(sub(v0, v1), x);
   out[2] = c1 * dot(sub(v0, v2), x) + c0 * dot(sub(v1, v2), x);
+}
+
+// The same corner terms for a chunk of kk heat columns stored by rows (FieldLayout::kRows): one
+// thread per (face, column), column fastest, so a face's kk heat values at each corner and its kk
+// terms per corner are contiguous runs; terms are stored [corner][kk]. The arithmetic is the
+// per-column kernel's, operation for operation.
+__global__ void divergence_corners_rows_kernel(const float* __restrict__ xyz,
+                                               const int32_t* __restrict__ faces,
+                                               const float* __restrict__ normals,
+                                               const float* __restrict__ areas,
+                                               const float* __restrict__ cot,
+                                               const double* __restrict__ heat, int64_t n_faces,
+                                               int32_t kk, double* __restrict__ corner_terms) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_faces * kk) return;
+  const int64_t f = e / kk, q = e % kk;
+  int32_t i0 = faces[3 * f], i1 = faces[3 * f + 1], i2 = faces[3 * f + 2];
+  D3 v0 = widen(load3(xyz, i0)), v1 = widen(load3(xyz, i1)), v2 = widen(load3(xyz, i2));
+  D3 nrm = widen(load3(normals, f));
+  double area = static_cast<double>(areas[f]);
+  const double u0 = heat[static_cast<int64_t>(i0) * kk + q];
+  const double u1 = heat[static_cast<int64_t>(i1) * kk + q];
+  const double u2 = heat[static_cast<int64_t>(i2) * kk + q];
+  D3 grad{0.0, 0.0, 0.0};
+  if (area > 0.0) {
+    D3 sum = add(add(scale(u0, cross(nrm, sub(v2, v1))), scale(u1, cross(nrm, sub(v0, v2)))),
+                 scale(u2, cross(nrm, sub(v1, v0))));
+    double twice = 2.0 * area;
+    grad = D3{sum.x / twice, sum.y / twice, sum.z / twice};
+  }
+  D3 x{0.0, 0.0, 0.0};
+  double largest = fmax(fmax(fabs(grad.x), fabs(grad.y)), fabs(grad.z));
+  if (largest != 0.0) {
+    D3 w{grad.x / largest, grad.y / largest, grad.z / largest};
+    double length = sqrt(dot(w, w));
+    x = D3{w.x / length, w.y / length, w.z / length};
+  }
+  double c0 = cot[3 * f], c1 = cot[3 * f + 1], c2 = cot[3 * f + 2];
+  double* out = corner_terms + 3 * f * kk + q;
+  out[0] = c2 * dot(sub(v1, v0), x) + c1 * dot(sub(v2, v0), x);
+  out[kk] = c0 * dot(sub(v2, v1), x) + c2 * dot(sub(v0, v1), x);
+  out[2 * kk] = c1 * dot(sub(v0, v2), x) + c0 * dot(sub(v1, v2), x);
+}
+
+__global__ void corner_sums_rows_kernel(const int32_t* __restrict__ vertex_ptr,
+                                        const int32_t* __restrict__ vertex_corners,
+                                        const double* __restrict__ corner_terms, int64_t n,
+                                        int32_t kk, double* __restrict__ out) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * kk) return;
+  const int64_t i = e / kk, q = e % kk;
+  double sum = 0.0;
+  for (int32_t k = vertex_ptr[i]; k < vertex_ptr[i + 1]; ++k)
+    sum += corner_terms[static_cast<int64_t>(vertex_corners[k]) * kk + q];
+  out[e] = sum;
 }
 
 // Each vertex's corner terms summed in ascending corner order (kernels/scatter.py's
@@ -426,10 +489,10 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
 }
 
 void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
-                 double* u0, int32_t* invalid, hipStream_t stream) {
+                 int64_t rows_k, double* u0, int32_t* invalid, hipStream_t stream) {
   if (n_entries == 0) return;
-  seed_kernel<<<blocks_for(n_entries), kBlock, 0, stream>>>(sources, owner, n_entries, n, u0,
-                                                            invalid);
+  seed_kernel<<<blocks_for(n_entries), kBlock, 0, stream>>>(sources, owner, n_entries, n, rows_k,
+                                                            u0, invalid);
   HG_HIP(hipGetLastError());
 }
 
@@ -437,8 +500,20 @@ void launch_divergence(const float* xyz, const int32_t* faces, const float* norm
                        const float* areas, const float* cot, const int32_t* vertex_ptr,
                        const int32_t* vertex_corners, const double* heat, int64_t n,
                        int64_t n_faces, int64_t n_columns, double* corner_terms, double* out,
-                       hipStream_t stream) {
+                       FieldLayout layout, hipStream_t stream) {
   if (n == 0 || n_columns == 0) return;
+  if (layout == FieldLayout::kRows) {
+    for (int64_t c0 = 0; c0 < n_columns; c0 += kFieldChunk) {
+      const int32_t kk = static_cast<int32_t>(std::min<int64_t>(kFieldChunk, n_columns - c0));
+      if (n_faces > 0)
+        divergence_corners_rows_kernel<<<blocks_for(n_faces * kk), kBlock, 0, stream>>>(
+            xyz, faces, normals, areas, cot, heat + c0 * n, n_faces, kk, corner_terms);
+      corner_sums_rows_kernel<<<blocks_for(n * kk), kBlock, 0, stream>>>(
+          vertex_ptr, vertex_corners, corner_terms, n, kk, out + c0 * n);
+    }
+    HG_HIP(hipGetLastError());
+    return;
+  }
   if (n_faces > 0) {
     dim3 grid(blocks_for(n_faces), static_cast<unsigned>(n_columns));
     divergence_corners_kernel<<<grid, kBlock, 0, stream>>>(xyz, faces, normals, areas, cot, heat,

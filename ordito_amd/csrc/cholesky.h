@@ -87,7 +87,8 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
                       const double* b, double* x, int64_t k, double tolerance, bool componentwise,
                       int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
                       hipStream_t stream, PhaseTimer* timer = nullptr,
-                      int sweep_phase = kPhaseHeatSweep);
+                      int sweep_phase = kPhaseHeatSweep, FieldLayout b_layout = FieldLayout::kColumns,
+                      FieldLayout x_layout = FieldLayout::kColumns);
 
 // The components' members cut into 256-member chunks, for the null-space projection's reductions.
 struct NullSpace {

@@ -7,6 +7,13 @@ namespace hg {
 
 constexpr int kBlock = 256;
 
+// How a field of k columns over n vertices is stored. kColumns: the caller's layout, column q at
+// q * n. kRows: in chunks of 64 columns (the solves' width), chunk c at 64 c * n holding its kk
+// columns interleaved per vertex (vertex i's entries at i * kk), the layout the refined solves
+// work in; the heat step hands its field to the divergence and the Poisson step this way.
+enum class FieldLayout : int { kColumns = 0, kRows = 1 };
+constexpr int64_t kFieldChunk = 64;
+
 // Device time of the phases of a solve, from HIP events on the stream the work is issued on.
 enum Phase : int {
   kPhaseFactor = 0,
@@ -99,13 +106,14 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
                           const double* mass, const double* scalars, const int32_t* row_segment,
                           int64_t n, double* lap, double* heat, double* poisson, float* heat32,
                           double* heat_inv_diag, double* poisson_inv_diag, hipStream_t stream);
+// rows_k > 0: u0 stored by rows (FieldLayout::kRows) over rows_k columns; 0: by columns.
 void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
-                 double* u0, int32_t* invalid, hipStream_t stream);
+                 int64_t rows_k, double* u0, int32_t* invalid, hipStream_t stream);
 void launch_divergence(const float* xyz, const int32_t* faces, const float* normals,
                        const float* areas, const float* cot, const int32_t* vertex_ptr,
                        const int32_t* vertex_corners, const double* heat, int64_t n,
                        int64_t n_faces, int64_t n_columns, double* corner_terms, double* out,
-                       hipStream_t stream);
+                       FieldLayout layout, hipStream_t stream);
 int32_t shift_partial_blocks(int64_t n);
 // Columns start at col_off[q] with col_len[q] entries (device; both null: q * n and n), n the
 // longest.

@@ -21,6 +21,7 @@ namespace {
 
 constexpr int kTile = 64;   // rows (forward) or columns (backward) of W per tile
 constexpr int kCols = 64;   // right-hand sides per solve chunk
+static_assert(kCols == kFieldChunk, "a row-layout field chunk is one solve chunk");
 constexpr int kReduceSplit = 4;  // blocks a reduction group is spread over (16 rows each)
 
 template <typename T>
@@ -82,6 +83,34 @@ __global__ __launch_bounds__(256) void perm_out_kernel(const double* __restrict_
   if (ii >= rows) return;
   const int64_t dst = perm[i0 + ii];
   for (int q = qq; q < kk; q += 4) out[q * n + dst] = tile[ii][q];
+}
+
+// The same two permutations for fields stored by rows (FieldLayout::kRows: a chunk's kk columns
+// interleaved per vertex), where a vertex's kk entries are one contiguous run on both sides.
+__global__ __launch_bounds__(256) void perm_in_rows_kernel(const double* __restrict__ b,
+                                                           const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ pinned,
+                                                           int64_t n, int32_t kk,
+                                                           double* __restrict__ bp,
+                                                           double* __restrict__ x,
+                                                           double* __restrict__ r) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= n * kk) return;
+  const int64_t i = e / kk, q = e % kk;
+  const double v = (pinned && pinned[i]) ? 0.0 : b[static_cast<int64_t>(perm[i]) * kk + q];
+  bp[e] = v;
+  r[e] = v;
+  x[e] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void perm_out_rows_kernel(const double* __restrict__ x,
+                                                            const int32_t* __restrict__ perm,
+                                                            int64_t n, int32_t kk,
+                                                            double* __restrict__ out) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= n * kk) return;
+  const int64_t i = e / kk, q = e % kk;
+  out[static_cast<int64_t>(perm[i]) * kk + q] = x[e];
 }
 
 struct SweepArgs {
@@ -187,6 +216,28 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
     __syncthreads();
   }
   if (!rows_live) return;
+  // Epilogue: every load (the children's update rows that add into this tile's update rows)
+  // before any store, so the loads overlap instead of each waiting behind a store it might alias.
+  // A lane's outputs are rows lane / 16 + 4v of its wave's block; the sums keep the gather's order
+  // (0, then each child in ascending order, then the product).
+  double base[4][4] = {};
+  if (slot < 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int32_t i = r0 + 16 * wave + lane / 16 + 4 * v;
+      if (i < nc || i >= m) continue;
+      const int64_t row = vbase + i;
+      const int32_t g1 = a.gather_ptr[row + 1];
+      for (int32_t g = a.gather_ptr[row]; g < g1; ++g) {
+        const int64_t src = static_cast<int64_t>(a.gather_src[g]) * kk;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          const int q = 16 * n + lane % 16;
+          if (q < kk) base[n][v] += a.fvec[src + q];
+        }
+      }
+    }
+  }
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     const int q = 16 * n + lane % 16;
@@ -202,8 +253,7 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
       } else if (i < nc) {
         a.y[static_cast<int64_t>(c0 + i) * kk + q] = value;
       } else {
-        const int64_t row = vbase + i;
-        a.fvec[row * kk + q] = gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) + value;
+        a.fvec[(vbase + i) * kk + q] = base[n][v] + value;
       }
     }
   }
@@ -222,18 +272,25 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
   constexpr int kRows = kTile / kReduceSplit;
   const int32_t lo = blockIdx.y * kRows;
   const int32_t rows = min(kRows, m - r0 - lo);
-  for (int32_t e = threadIdx.x; e < rows * kk; e += blockDim.x) {
+  // At most kRows * kCols = 4 x 256 entries: each thread's four, every load before any store.
+  double sum[4] = {}, base[4] = {};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int32_t e = threadIdx.x + 256 * u;
+    if (e >= rows * kk) continue;
     const int32_t ii = lo + e / kk, q = e % kk;
-    double sum = 0.0;
     for (int32_t c = 0; c < count; ++c)
-      sum += a.partials[(static_cast<int64_t>(first + c) * kTile + ii) * kk + q];
+      sum[u] += a.partials[(static_cast<int64_t>(first + c) * kTile + ii) * kk + q];
+    if (r0 + ii >= nc) base[u] = gathered(0.0, vbase + r0 + ii, q, kk, a.gather_ptr, a.gather_src, a.fvec);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int32_t e = threadIdx.x + 256 * u;
+    if (e >= rows * kk) continue;
+    const int32_t ii = lo + e / kk, q = e % kk;
     const int32_t i = r0 + ii;
-    if (i < nc) {
-      a.y[static_cast<int64_t>(c0 + i) * kk + q] = sum;
-    } else {
-      const int64_t row = vbase + i;
-      a.fvec[row * kk + q] = gathered(0.0, row, q, kk, a.gather_ptr, a.gather_src, a.fvec) + sum;
-    }
+    if (i < nc) a.y[static_cast<int64_t>(c0 + i) * kk + q] = sum[u];
+    else a.fvec[(vbase + i) * kk + q] = base[u] + sum[u];
   }
 }
 
@@ -296,6 +353,19 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
     __syncthreads();
   }
   if (!cols_live) return;
+  // Epilogue: the iterate's entries are all loaded before any store (see the forward tile).
+  double xv[4][4] = {};
+  if (slot < 0) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int q = 16 * n + lane % 16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int32_t j = j0 + 16 * wave + lane / 16 + 4 * v;
+        if (q < kk && j < nc) xv[n][v] = a.x[static_cast<int64_t>(c0 + j) * kk + q];
+      }
+    }
+  }
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     const int q = 16 * n + lane % 16;
@@ -311,7 +381,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
       } else {
         const int64_t at = static_cast<int64_t>(c0 + j) * kk + q;
         a.d[at] = value;
-        a.x_next[at] = a.x[at] + value;
+        a.x_next[at] = xv[n][v] + value;
       }
     }
   }
@@ -327,14 +397,25 @@ __global__ __launch_bounds__(256) void bwd_reduce_kernel(
   constexpr int kRows = kTile / kReduceSplit;
   const int32_t lo = blockIdx.y * kRows;
   const int32_t cols = min(kRows, nc - j0 - lo);
-  for (int32_t e = threadIdx.x; e < cols * kk; e += blockDim.x) {
+  // Each thread's (at most four) entries: every load before any store.
+  double sum[4] = {}, xv[4] = {};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int32_t e = threadIdx.x + 256 * u;
+    if (e >= cols * kk) continue;
     const int32_t jj = lo + e / kk, q = e % kk;
-    double sum = 0.0;
     for (int32_t c = 0; c < count; ++c)
-      sum += a.partials[(static_cast<int64_t>(first + c) * kTile + jj) * kk + q];
+      sum[u] += a.partials[(static_cast<int64_t>(first + c) * kTile + jj) * kk + q];
+    xv[u] = a.x[static_cast<int64_t>(c0 + j0 + jj) * kk + q];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int32_t e = threadIdx.x + 256 * u;
+    if (e >= cols * kk) continue;
+    const int32_t jj = lo + e / kk, q = e % kk;
     const int64_t at = static_cast<int64_t>(c0 + j0 + jj) * kk + q;
-    a.d[at] = sum;
-    a.x_next[at] = a.x[at] + sum;
+    a.d[at] = sum[u];
+    a.x_next[at] = xv[u] + sum[u];
   }
 }
 
@@ -449,7 +530,28 @@ __global__ void residual_kernel(const int32_t* __restrict__ offsets,
     const int64_t i = t / kk, q = t % kk;
     const double bi = b[t];
     double total = bi, scale = fabs(bi);
-    for (int32_t e = offsets[i]; e < offsets[i + 1]; ++e) {
+    // Four entries at a time: their column and value loads, then their iterate loads, issue
+    // together; the sums stay in entry order.
+    int32_t e = offsets[i];
+    const int32_t e1 = offsets[i + 1];
+    for (; e + 4 <= e1; e += 4) {
+      int32_t c[4];
+      double v[4], xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        c[u] = columns[e + u];
+        v[u] = values[e + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[u] = x[static_cast<int64_t>(c[u]) * kk + q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double term = v[u] * xv[u];
+        total -= term;
+        scale += fabs(term);
+      }
+    }
+    for (; e < e1; ++e) {
       const double term = values[e] * x[static_cast<int64_t>(columns[e]) * kk + q];
       total -= term;
       scale += fabs(term);
@@ -475,6 +577,24 @@ __global__ void absmax_partial_kernel(const double* __restrict__ b, int64_t n,
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[q * gridDim.x + blockIdx.x] = scratch[0];
+}
+
+// The same partial maxima for a chunk stored by rows: block b covers rows [256 b, 256 b + 256);
+// thread (q, part) = (t % 64, t / 64) folds every fourth of them for column q. A maximum does not
+// depend on the order it is taken in.
+__global__ void absmax_rows_partial_kernel(const double* __restrict__ b, int64_t n, int32_t kk,
+                                           double* __restrict__ partials) {
+  __shared__ double scratch[4][64];
+  const int q = threadIdx.x % 64, part = threadIdx.x / 64;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * 256;
+  double v = 0.0;
+  if (q < kk)
+    for (int64_t i = i0 + part; i < min<int64_t>(n, i0 + 256); i += 4) v = fmax(v, fabs(b[i * kk + q]));
+  scratch[part][q] = v;
+  __syncthreads();
+  if (part == 0 && q < kk)
+    partials[static_cast<int64_t>(q) * gridDim.x + blockIdx.x] =
+        fmax(fmax(scratch[0][q], scratch[1][q]), fmax(scratch[2][q], scratch[3][q]));
 }
 
 __global__ void absmax_final_kernel(const double* __restrict__ partials, int32_t blocks,
@@ -626,7 +746,8 @@ void SolveWorkspace::reserve(const DevicePlan& plan, int64_t kk) {
 int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp32,
                       const double* b, double* x, int64_t k, double tolerance, bool componentwise,
                       int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
-                      hipStream_t stream, PhaseTimer* timer, int sweep_phase) {
+                      hipStream_t stream, PhaseTimer* timer, int sweep_phase,
+                      FieldLayout b_layout, FieldLayout x_layout) {
   const int64_t n = plan.n;
   if (n == 0 || k == 0) return 0;
   int32_t max_used = 0;
@@ -637,12 +758,19 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
     double* xq = x + q0 * n;
     const unsigned grid = blocks_for(n * kk);
     const unsigned row_tiles = static_cast<unsigned>((n + kTile - 1) / kTile);
-    perm_in_kernel<<<row_tiles, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk, ws.bp.ptr,
-                                                  ws.xa.ptr, ws.r.ptr);
+    if (b_layout == FieldLayout::kRows)
+      perm_in_rows_kernel<<<grid, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk,
+                                                    ws.bp.ptr, ws.xa.ptr, ws.r.ptr);
+    else
+      perm_in_kernel<<<row_tiles, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk,
+                                                    ws.bp.ptr, ws.xa.ptr, ws.r.ptr);
     double* bmax = ws.bmax.ptr + static_cast<int64_t>(blocks_for(n)) * kCols;
     if (!componentwise) {
       const unsigned blocks = blocks_for(n);
-      absmax_partial_kernel<<<dim3(blocks, kk), 256, 0, stream>>>(bq, n, ws.bmax.ptr);
+      if (b_layout == FieldLayout::kRows)
+        absmax_rows_partial_kernel<<<blocks, 256, 0, stream>>>(bq, n, kk, ws.bmax.ptr);
+      else
+        absmax_partial_kernel<<<dim3(blocks, kk), 256, 0, stream>>>(bq, n, ws.bmax.ptr);
       absmax_final_kernel<<<kk, 256, 0, stream>>>(ws.bmax.ptr, static_cast<int32_t>(blocks), bmax);
     }
     SweepArgs a{};
@@ -691,7 +819,10 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
       if (timer) timer->resolve();
       if (!pending) break;
     }
-    perm_out_kernel<<<row_tiles, 256, 0, stream>>>(x_cur, plan.perm.ptr, n, kk, xq);
+    if (x_layout == FieldLayout::kRows)
+      perm_out_rows_kernel<<<grid, 256, 0, stream>>>(x_cur, plan.perm.ptr, n, kk, xq);
+    else
+      perm_out_kernel<<<row_tiles, 256, 0, stream>>>(x_cur, plan.perm.ptr, n, kk, xq);
     HG_HIP(hipGetLastError());
     max_used = std::max(max_used, rounds);
   }
