@@ -61,6 +61,7 @@ struct hg_mesh {
   int32_t time_from_edges = 1;
   DeviceArray<float> cot, normals, areas, heat32;
   DeviceArray<double> mass, lap, heat, poisson, heat_inv, poisson_inv, scalars, edge_partials;
+  DeviceArray<double> face_geom;  // 18 doubles a face: the divergence's per-face quantities
   double poisson_bound = -1.0;
   // Components: the Poisson system pins each component's last-eliminated row.
   int32_t n_comp = 0;
@@ -82,11 +83,11 @@ struct hg_mesh {
   // Workspaces.
   CgWorkspace cg;
   SolveWorkspace solve;
-  DeviceArray<double> fields, corner_terms, shift_partials, shift_values, null_partials;
+  DeviceArray<double> fields, shift_partials, shift_values, null_partials;
   NullSpace null_space;
   DeviceArray<int32_t> d_sources, invalid;
   DeviceArray<int64_t> d_owner, d_offsets;
-  int64_t field_capacity = 0, corner_capacity = 0;
+  int64_t field_capacity = 0;
   DeviceArray<double> packed_phi;  // packed meshes: the solved slot columns before extraction
   int64_t packed_capacity = 0;
   DeviceArray<int64_t> q_src_off, q_dst_off, q_len;  // packed meshes: each query's field
@@ -209,6 +210,7 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   m.cot.allocate(3 * m.f);
   m.normals.allocate(3 * m.f);
   m.areas.allocate(m.f);
+  m.face_geom.allocate(18 * m.f);
   m.mass.allocate(n);
   m.lap.allocate(nnz);
   m.heat.allocate(nnz);
@@ -221,6 +223,8 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   for (int64_t s = 0; s < n_seg; ++s) longest = std::max(longest, m.seg_v[s + 1] - m.seg_v[s]);
   m.edge_partials.allocate(2 * edge_sum_blocks(longest));
   launch_face_terms(m.xyz.ptr, m.faces.ptr, m.f, m.cot.ptr, m.normals.ptr, m.areas.ptr, st);
+  launch_face_geometry(m.xyz.ptr, m.faces.ptr, m.normals.ptr, m.areas.ptr, m.cot.ptr, m.f,
+                       m.face_geom.ptr, st);
   launch_mass(m.vertex_ptr.ptr, m.vertex_corners.ptr, m.areas.ptr, n, m.mass.ptr, st);
   // One edge-sum reduction per segment, its blocks counted from the segment's first row: the sums
   // (and so the times) are those the segment's mesh forms alone. The launches share the partials
@@ -485,14 +489,9 @@ void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0,
 
 void divergence(hg_mesh& m, const double* heat, double* out, int64_t k,
                 FieldLayout layout = FieldLayout::kColumns) {
-  if (m.corner_capacity < 3 * m.f * k) {
-    m.corner_terms.allocate(std::max<int64_t>(3 * m.f * k, 1));
-    m.corner_capacity = 3 * m.f * k;
-  }
   PhaseScope scope(&m.timer, kPhaseDivergence, m.stream);
-  launch_divergence(m.xyz.ptr, m.faces.ptr, m.normals.ptr, m.areas.ptr, m.cot.ptr,
-                    m.vertex_ptr.ptr, m.vertex_corners.ptr, heat, m.n, m.f, k, m.corner_terms.ptr,
-                    out, layout, m.stream);
+  launch_divergence(m.faces.ptr, m.face_geom.ptr, m.vertex_ptr.ptr, m.vertex_corners.ptr, heat,
+                    m.n, k, out, layout, m.stream);
 }
 
 double* reserve_fields(hg_mesh& m, int64_t k) {

@@ -240,115 +240,117 @@ __global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* 
   }
 }
 
-// Per face and column: the unit gradient direction x of the heat (sum_k u_k (N x e_k) / (2A) in
-// float64 from float32 N, A, positions; scaled by its largest component before normalizing,
-// kernels/triangles.py:834-894, kernels/predicates.py:718-724), integrated onto the three corners
-// as c_opp (e . x) (kernels/heat.py:626-660). Terms stored per corner; the vertex sums follow.
-__global__ void divergence_corners_kernel(const float* __restrict__ xyz,
-                                          const int32_t* __restrict__ faces,
-                                          const float* __restrict__ normals,
-                                          const float* __restrict__ areas,
-                                          const float* __restrict__ cot,
-                                          const double* __restrict__ heat, int64_t n,
-                                          int64_t n_faces, int64_t n_columns,
-                                          double* __restrict__ corner_terms) {
-  int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t q = blockIdx.y;
-  if (f >= n_faces || q >= n_columns) return;
-  int32_t i0 = faces[3 * f], i1 = faces[3 * f + 1], i2 = faces[3 * f + 2];
-  D3 v0 = widen(load3(xyz, i0)), v1 = widen(load3(xyz, i1)), v2 = widen(load3(xyz, i2));
-  D3 nrm = widen(load3(normals, f));
-  double area = static_cast<double>(areas[f]);
-  const double* u = heat + q * n;
-  D3 grad{0.0, 0.0, 0.0};
-  if (area > 0.0) {
-    D3 sum = add(add(scale(u[i0], cross(nrm, sub(v2, v1))), scale(u[i1], cross(nrm, sub(v0, v2)))),
-                 scale(u[i2], cross(nrm, sub(v1, v0))));
-    double twice = 2.0 * area;
-    grad = D3{sum.x / twice, sum.y / twice, sum.z / twice};
+// Per face, the float64 quantities every query's divergence reuses (kernels/triangles.py:834-894,
+// kernels/heat.py:626-660), from the float32 positions, unit normal N, area and half-cotangents:
+//   geom[18f + 3k .. +3), k < 3: a_k = N x e_k, e_0 = v2 - v1, e_1 = v0 - v2, e_2 = v1 - v0, so
+//       that sum_k u_k a_k is the heat gradient times 2A (all zero where A = 0: no gradient);
+//   geom[18f + 9 + 3c .. +3), c < 3: corner c's integration vector, corner 0
+//       g_0 = c_2 (v1 - v0) + c_1 (v2 - v0), corner 1 g_1 = c_0 (v2 - v1) + c_2 (v0 - v1),
+//       corner 2 g_2 = c_1 (v0 - v2) + c_0 (v1 - v2): the corner's term is g_c . x for the unit
+//       gradient direction x.
+__global__ void face_geometry_kernel(const float* __restrict__ xyz,
+                                     const int32_t* __restrict__ faces,
+                                     const float* __restrict__ normals,
+                                     const float* __restrict__ areas,
+                                     const float* __restrict__ cot, int64_t n_faces,
+                                     double* __restrict__ geom) {
+  const int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  const D3 v0 = widen(load3(xyz, faces[3 * f])), v1 = widen(load3(xyz, faces[3 * f + 1])),
+           v2 = widen(load3(xyz, faces[3 * f + 2]));
+  const D3 nrm = widen(load3(normals, f));
+  const bool live = static_cast<double>(areas[f]) > 0.0;
+  const D3 zero{0.0, 0.0, 0.0};
+  const D3 a[3] = {live ? cross(nrm, sub(v2, v1)) : zero, live ? cross(nrm, sub(v0, v2)) : zero,
+                   live ? cross(nrm, sub(v1, v0)) : zero};
+  const double c0 = cot[3 * f], c1 = cot[3 * f + 1], c2 = cot[3 * f + 2];
+  const D3 g[3] = {add(scale(c2, sub(v1, v0)), scale(c1, sub(v2, v0))),
+                   add(scale(c0, sub(v2, v1)), scale(c2, sub(v0, v1))),
+                   add(scale(c1, sub(v0, v2)), scale(c0, sub(v1, v2)))};
+  double* out = geom + 18 * f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    out[3 * k] = a[k].x;
+    out[3 * k + 1] = a[k].y;
+    out[3 * k + 2] = a[k].z;
+    out[9 + 3 * k] = g[k].x;
+    out[9 + 3 * k + 1] = g[k].y;
+    out[9 + 3 * k + 2] = g[k].z;
   }
-  D3 x{0.0, 0.0, 0.0};
-  double largest = fmax(fmax(fabs(grad.x), fabs(grad.y)), fabs(grad.z));
-  if (largest != 0.0) {
-    D3 w{grad.x / largest, grad.y / largest, grad.z / largest};
-    double length = sqrt(dot(w, w));
-    x = D3{w.x / length, w.y / length, w.z / length};
-  }
-  double c0 = cot[3 * f], c1 = cot[3 * f + 1], c2 = cot[3 * f + 2];
-  double* out = corner_terms + q * 3 * n_faces + 3 * f;
-  out[0] = c2 * dot(sub(v1, v0), x) + c1 * dot(sub(v2, v0), x);
-  out[1] = c0 * dot(sub(v2, v1), x) + c2 * dot(sub(v0, v1), x);
-  out[2] = c1 * dot(sub(v0, v2), x) + c0 * dot(sub(v1, v2), x);
 }
 
-// The same corner terms for a chunk of kk heat columns stored by rows (FieldLayout::kRows): one
-// thread per (face, column), column fastest, so a face's kk heat values at each corner and its kk
-// terms per corner are contiguous runs; terms are stored [corner][kk]. The arithmetic is the
-// per-column kernel's, operation for operation.
-__global__ void divergence_corners_rows_kernel(const float* __restrict__ xyz,
-                                               const int32_t* __restrict__ faces,
-                                               const float* __restrict__ normals,
-                                               const float* __restrict__ areas,
-                                               const float* __restrict__ cot,
-                                               const double* __restrict__ heat, int64_t n_faces,
-                                               int32_t kk, double* __restrict__ corner_terms) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n_faces * kk) return;
-  const int64_t f = e / kk, q = e % kk;
-  int32_t i0 = faces[3 * f], i1 = faces[3 * f + 1], i2 = faces[3 * f + 2];
-  D3 v0 = widen(load3(xyz, i0)), v1 = widen(load3(xyz, i1)), v2 = widen(load3(xyz, i2));
-  D3 nrm = widen(load3(normals, f));
-  double area = static_cast<double>(areas[f]);
-  const double u0 = heat[static_cast<int64_t>(i0) * kk + q];
-  const double u1 = heat[static_cast<int64_t>(i1) * kk + q];
-  const double u2 = heat[static_cast<int64_t>(i2) * kk + q];
-  D3 grad{0.0, 0.0, 0.0};
-  if (area > 0.0) {
-    D3 sum = add(add(scale(u0, cross(nrm, sub(v2, v1))), scale(u1, cross(nrm, sub(v0, v2)))),
-                 scale(u2, cross(nrm, sub(v1, v0))));
-    double twice = 2.0 * area;
-    grad = D3{sum.x / twice, sum.y / twice, sum.z / twice};
-  }
-  D3 x{0.0, 0.0, 0.0};
-  double largest = fmax(fmax(fabs(grad.x), fabs(grad.y)), fabs(grad.z));
-  if (largest != 0.0) {
-    D3 w{grad.x / largest, grad.y / largest, grad.z / largest};
-    double length = sqrt(dot(w, w));
-    x = D3{w.x / length, w.y / length, w.z / length};
-  }
-  double c0 = cot[3 * f], c1 = cot[3 * f + 1], c2 = cot[3 * f + 2];
-  double* out = corner_terms + 3 * f * kk + q;
-  out[0] = c2 * dot(sub(v1, v0), x) + c1 * dot(sub(v2, v0), x);
-  out[kk] = c0 * dot(sub(v2, v1), x) + c2 * dot(sub(v0, v1), x);
-  out[2 * kk] = c1 * dot(sub(v0, v2), x) + c0 * dot(sub(v1, v2), x);
+// One corner's divergence term for heat values u0, u1, u2 at its face's vertices: the gradient
+// direction s = sum_k u_k a_k is scaled by the power of two that brings its largest component into
+// [0.5, 1) (exact, so tiny heat values neither underflow nor lose bits; the reference divides by
+// the largest component, kernels/predicates.py:718-724), normalized, and dotted with g. A zero
+// gradient contributes zero.
+__device__ __forceinline__ double corner_term(const double* __restrict__ geom, int32_t corner,
+                                              double u0, double u1, double u2) {
+#pragma clang fp contract(fast)
+  const double* a = geom;
+  const double sx = u0 * a[0] + u1 * a[3] + u2 * a[6];
+  const double sy = u0 * a[1] + u1 * a[4] + u2 * a[7];
+  const double sz = u0 * a[2] + u1 * a[5] + u2 * a[8];
+  const double largest = fmax(fmax(fabs(sx), fabs(sy)), fabs(sz));
+  if (largest == 0.0) return 0.0;
+  int e;
+  (void)frexp(largest, &e);
+  const double wx = ldexp(sx, -e), wy = ldexp(sy, -e), wz = ldexp(sz, -e);
+  const double* g = geom + 9 + 3 * corner;
+  return (g[0] * wx + g[1] * wy + g[2] * wz) * rsqrt(wx * wx + wy * wy + wz * wz);
 }
 
-__global__ void corner_sums_rows_kernel(const int32_t* __restrict__ vertex_ptr,
-                                        const int32_t* __restrict__ vertex_corners,
-                                        const double* __restrict__ corner_terms, int64_t n,
-                                        int32_t kk, double* __restrict__ out) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n * kk) return;
-  const int64_t i = e / kk, q = e % kk;
+// -b at vertex i and column q: the terms of i's corners for that column summed in ascending corner
+// order (kernels/heat.py:626-687, heat.py:3218-3232), each term recomputed from the face's heat
+// values (no per-corner intermediate in memory). heat and out share a layout: entry (vertex v,
+// column q) at v * vstride + q * qstride.
+//
+// kWaveVertex: one wave a vertex, its lanes the columns (kk <= 64 columns stored by rows, so the
+// heat rows it reads and the row it writes are contiguous); the vertex, its corners and their face
+// data are wave-uniform. Blocks are numbered so that each XCD's share of the grid (the hardware
+// deals blocks to the eight XCDs in turn) is one contiguous range of vertices, whose neighbours'
+// heat rows then stay in that XCD's L2. Otherwise one thread per (vertex, column), vertex fastest.
+template <bool kWaveVertex>
+__global__ __launch_bounds__(256) void divergence_kernel(
+    const int32_t* __restrict__ faces, const double* __restrict__ geom,
+    const int32_t* __restrict__ vertex_ptr, const int32_t* __restrict__ vertex_corners,
+    const double* __restrict__ heat, int64_t n, int64_t kk, int64_t vstride, int64_t qstride,
+    double* __restrict__ out) {
+  int64_t i, q;
+  if (kWaveVertex) {
+    const int64_t blocks = gridDim.x, per = blocks / 8, extra = blocks % 8;
+    const int64_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int64_t block = xcd * per + min(xcd, extra) + slot;
+    i = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(block * 4 + threadIdx.x / 64));
+    q = threadIdx.x % 64;
+    if (i >= n || q >= kk) return;
+  } else {
+    // Whichever index is contiguous in memory runs fastest across the threads.
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n * kk) return;
+    if (vstride == 1) {
+      q = e / n;
+      i = e - q * n;
+    } else {
+      i = e / kk;
+      q = e - i * kk;
+    }
+  }
+  const double* u = heat + q * qstride;
+  const double ui = u[i * vstride];
   double sum = 0.0;
-  for (int32_t k = vertex_ptr[i]; k < vertex_ptr[i + 1]; ++k)
-    sum += corner_terms[static_cast<int64_t>(vertex_corners[k]) * kk + q];
-  out[e] = sum;
-}
-
-// Each vertex's corner terms summed in ascending corner order (kernels/scatter.py's
-// sum_sorted_corner_values, heat.py:3218-3232).
-__global__ void corner_sums_kernel(const int32_t* __restrict__ vertex_ptr,
-                                   const int32_t* __restrict__ vertex_corners,
-                                   const double* __restrict__ corner_terms, int64_t n,
-                                   int64_t n_faces, int64_t n_columns, double* __restrict__ out) {
-  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t q = blockIdx.y;
-  if (i >= n || q >= n_columns) return;
-  const double* terms = corner_terms + q * 3 * n_faces;
-  double sum = 0.0;
-  for (int32_t k = vertex_ptr[i]; k < vertex_ptr[i + 1]; ++k) sum += terms[vertex_corners[k]];
-  out[q * n + i] = sum;
+  const int32_t k1 = vertex_ptr[i + 1];
+  for (int32_t k = vertex_ptr[i]; k < k1; ++k) {
+    const int32_t c = vertex_corners[k];
+    const int32_t f = c / 3, corner = c - 3 * f;
+    const int32_t* tri = faces + 3 * static_cast<int64_t>(f);
+    // The corner's own vertex is i; the other two are read.
+    const double u0 = corner == 0 ? ui : u[tri[0] * vstride];
+    const double u1 = corner == 1 ? ui : u[tri[1] * vstride];
+    const double u2 = corner == 2 ? ui : u[tri[2] * vstride];
+    sum += corner_term(geom + 18 * static_cast<int64_t>(f), corner, u0, u1, u2);
+  }
+  out[i * vstride + q * qstride] = sum;
 }
 
 // Columns of the shift: column q starts at col_off[q] and holds col_len[q] entries (packed meshes'
@@ -496,32 +498,34 @@ void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries
   HG_HIP(hipGetLastError());
 }
 
-void launch_divergence(const float* xyz, const int32_t* faces, const float* normals,
-                       const float* areas, const float* cot, const int32_t* vertex_ptr,
+void launch_face_geometry(const float* xyz, const int32_t* faces, const float* normals,
+                          const float* areas, const float* cot, int64_t n_faces, double* geom,
+                          hipStream_t stream) {
+  if (n_faces == 0) return;
+  face_geometry_kernel<<<blocks_for(n_faces), kBlock, 0, stream>>>(xyz, faces, normals, areas, cot,
+                                                                    n_faces, geom);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_divergence(const int32_t* faces, const double* geom, const int32_t* vertex_ptr,
                        const int32_t* vertex_corners, const double* heat, int64_t n,
-                       int64_t n_faces, int64_t n_columns, double* corner_terms, double* out,
-                       FieldLayout layout, hipStream_t stream) {
+                       int64_t n_columns, double* out, FieldLayout layout, hipStream_t stream) {
   if (n == 0 || n_columns == 0) return;
   if (layout == FieldLayout::kRows) {
     for (int64_t c0 = 0; c0 < n_columns; c0 += kFieldChunk) {
-      const int32_t kk = static_cast<int32_t>(std::min<int64_t>(kFieldChunk, n_columns - c0));
-      if (n_faces > 0)
-        divergence_corners_rows_kernel<<<blocks_for(n_faces * kk), kBlock, 0, stream>>>(
-            xyz, faces, normals, areas, cot, heat + c0 * n, n_faces, kk, corner_terms);
-      corner_sums_rows_kernel<<<blocks_for(n * kk), kBlock, 0, stream>>>(
-          vertex_ptr, vertex_corners, corner_terms, n, kk, out + c0 * n);
+      const int64_t kk = std::min<int64_t>(kFieldChunk, n_columns - c0);
+      // A wave a vertex while at least a quarter of its lanes hold columns.
+      if (kk >= kFieldChunk / 4)
+        divergence_kernel<true><<<static_cast<unsigned>((n + 3) / 4), kBlock, 0, stream>>>(
+            faces, geom, vertex_ptr, vertex_corners, heat + c0 * n, n, kk, kk, 1, out + c0 * n);
+      else
+        divergence_kernel<false><<<blocks_for(n * kk), kBlock, 0, stream>>>(
+            faces, geom, vertex_ptr, vertex_corners, heat + c0 * n, n, kk, kk, 1, out + c0 * n);
     }
-    HG_HIP(hipGetLastError());
-    return;
+  } else {
+    divergence_kernel<false><<<blocks_for(n * n_columns), kBlock, 0, stream>>>(
+        faces, geom, vertex_ptr, vertex_corners, heat, n, n_columns, 1, n, out);
   }
-  if (n_faces > 0) {
-    dim3 grid(blocks_for(n_faces), static_cast<unsigned>(n_columns));
-    divergence_corners_kernel<<<grid, kBlock, 0, stream>>>(xyz, faces, normals, areas, cot, heat,
-                                                           n, n_faces, n_columns, corner_terms);
-  }
-  dim3 grid_v(blocks_for(n), static_cast<unsigned>(n_columns));
-  corner_sums_kernel<<<grid_v, kBlock, 0, stream>>>(vertex_ptr, vertex_corners, corner_terms, n,
-                                                    n_faces, n_columns, out);
   HG_HIP(hipGetLastError());
 }
 

@@ -38,7 +38,7 @@ struct DevicePlan {
   int32_t split_target = kDefaultSplitTarget;
   // Forward gather lists: each front position's children contributions (absolute rows of the
   // front-vector arena), in ascending child order.
-  DeviceArray<int32_t> gather_ptr, gather_src;
+  DeviceArray<int32_t> gather_ptr, gather_src, gather_pair;
   // The operator's pattern in the elimination order (P A P^T) and each entry's original slot.
   DeviceArray<int32_t> p_offsets, p_columns, p_src;
   // The numeric factorization's tile launches (host list) and tasks (device, 4 int32 each).

@@ -40,7 +40,7 @@ __global__ void scatter_entries_kernel(const double* __restrict__ values,
 }
 
 // A pinned row whose operator stores no diagonal (an empty row) still gets the identity's.
-__global__ void pin_diagonal_kernel(const int32_t* __restrict__ pin_front_slot, int32_t n_pins,
+__global__ void pin_diagonal_kernel(const int64_t* __restrict__ pin_front_slot, int32_t n_pins,
                                     double* __restrict__ arena) {
   int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n_pins) arena[pin_front_slot[k]] = 1.0;
@@ -331,6 +331,9 @@ HostSweep sweep_tasks(const Plan& plan, bool forward, int32_t target_tasks) {
 }
 
 HostGather gather_lists(const Plan& plan) {
+  // The lists hold front-vector rows as int32.
+  HG_REQUIRE(plan.vec_size < (int64_t{1} << 31), HG_ERR_VALUE,
+             "system too large: its front vectors exceed the int32 range");
   HostGather g;
   g.ptr.assign(plan.vec_size + 1, 0);
   for (int32_t c = 0; c < plan.n_super; ++c) {
@@ -350,6 +353,16 @@ HostGather gather_lists(const Plan& plan) {
       const int64_t target = plan.vec_off[p] + plan.rel[plan.row_ptr[c] + i];
       g.src[at[target]++] = static_cast<int32_t>(plan.vec_off[c] + plan.ncol[c] + i);
     }
+  }
+  // The first two sources of each position inline (-1: none), so a gather is one 8-byte load
+  // and the update rows it names; -2 in the second marks a position with more, read on from the
+  // lists.
+  g.pair.assign(2 * plan.vec_size, -1);
+  for (int64_t v = 0; v < plan.vec_size; ++v) {
+    const int32_t count = g.ptr[v + 1] - g.ptr[v];
+    if (count >= 1) g.pair[2 * v] = g.src[g.ptr[v]];
+    if (count == 2) g.pair[2 * v + 1] = g.src[g.ptr[v] + 1];
+    if (count > 2) g.pair[2 * v + 1] = -2;
   }
   return g;
 }
@@ -396,6 +409,14 @@ void validate_sweeps(const Plan& plan, int32_t target_tasks) {
   if (static_cast<int64_t>(g.ptr.size()) != plan.vec_size + 1) fail("gather offsets size");
   for (int32_t v : g.src)
     if (v < 0 || v >= plan.vec_size) fail("gather source out of the front-vector arena");
+  if (static_cast<int64_t>(g.pair.size()) != 2 * plan.vec_size) fail("gather pair size");
+  for (int64_t v = 0; v < plan.vec_size; ++v) {
+    const int32_t count = g.ptr[v + 1] - g.ptr[v];
+    const int32_t first = g.pair[2 * v], second = g.pair[2 * v + 1];
+    if (first != (count >= 1 ? g.src[g.ptr[v]] : -1)) fail("gather pair first source");
+    if (second != (count == 2 ? g.src[g.ptr[v] + 1] : (count > 2 ? -2 : -1)))
+      fail("gather pair second source");
+  }
   // Every structure row of every child lands in exactly one parent position.
   int64_t rows_with_parent = 0;
   for (int64_t s = 0; s < S; ++s)
@@ -566,6 +587,7 @@ void DevicePlan::upload(const Plan& plan, const HostPattern& pattern, int32_t ta
     const HostGather g = gather_lists(plan);
     gather_ptr.upload(g.ptr, stream);
     gather_src.upload(g.src, stream);
+    gather_pair.upload(g.pair, stream);
   }
   // The operator's pattern in the elimination order: row i is old row perm[i], its columns
   // renumbered and sorted; src names each entry's slot in the original values.
@@ -633,7 +655,7 @@ int64_t DevicePlan::bytes() const {
          level_nodes.bytes() + front_off.bytes() + w_off.bytes() + vec_off.bytes() +
          entry_src.bytes() + entry_dst.bytes() + entry_row.bytes() + entry_col.bytes() +
          fwd.s.bytes() * 5 + fwd.rs.bytes() * 4 + bwd.s.bytes() * 5 + bwd.rs.bytes() * 4 +
-         gather_ptr.bytes() + gather_src.bytes() +
+         gather_ptr.bytes() + gather_src.bytes() + gather_pair.bytes() +
          p_offsets.bytes() + p_columns.bytes() + p_src.bytes() + factor_tasks.bytes();
 }
 
@@ -651,11 +673,14 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
     HG_HIP(hipGetLastError());
   }
   if (!pin_slots.empty()) {
-    std::vector<int32_t> slots32(pin_slots.begin(), pin_slots.end());
-    DeviceArray<int32_t> d_slots;
-    d_slots.upload(slots32, stream);
-    pin_diagonal_kernel<<<blocks_for(static_cast<int64_t>(slots32.size())), 256, 0, stream>>>(
-        d_slots.ptr, static_cast<int32_t>(slots32.size()), arena.ptr);
+    // Arena offsets: a large plan's exceed the int32 range.
+    for (int64_t slot : pin_slots)
+      HG_REQUIRE(slot >= 0 && slot < plan.arena_size, HG_ERR_RUNTIME,
+                 "pinned diagonal outside the front arena");
+    DeviceArray<int64_t> d_slots;
+    d_slots.upload(pin_slots, stream);
+    pin_diagonal_kernel<<<blocks_for(static_cast<int64_t>(pin_slots.size())), 256, 0, stream>>>(
+        d_slots.ptr, static_cast<int32_t>(pin_slots.size()), arena.ptr);
     HG_HIP(hipGetLastError());
     HG_HIP(hipStreamSynchronize(stream));
   }

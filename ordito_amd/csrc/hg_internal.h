@@ -162,9 +162,10 @@ constexpr int32_t kDefaultSplitTarget = 1024;
 HostSweep sweep_tasks(const Plan& plan, bool forward, int32_t target_tasks = kDefaultSplitTarget);
 
 // The forward sweep's gather lists: for each front-vector row (vec_off[s] + position) the rows of
-// the children's update vectors that add into it.
+// the children's update vectors that add into it, and the first two of them inline per position
+// (pair[2v], pair[2v+1]; -1 none, -2 in the second: more than two, the rest in the lists).
 struct HostGather {
-  std::vector<int32_t> ptr, src;
+  std::vector<int32_t> ptr, src, pair;
 };
 HostGather gather_lists(const Plan& plan);
 

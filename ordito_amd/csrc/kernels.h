@@ -109,11 +109,14 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
 // rows_k > 0: u0 stored by rows (FieldLayout::kRows) over rows_k columns; 0: by columns.
 void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
                  int64_t rows_k, double* u0, int32_t* invalid, hipStream_t stream);
-void launch_divergence(const float* xyz, const int32_t* faces, const float* normals,
-                       const float* areas, const float* cot, const int32_t* vertex_ptr,
+// Per face, 18 doubles: the gradient basis a_k = N x e_k and the corners' integration vectors.
+void launch_face_geometry(const float* xyz, const int32_t* faces, const float* normals,
+                          const float* areas, const float* cot, int64_t n_faces, double* geom,
+                          hipStream_t stream);
+// -b per vertex and column from the heat (both in `layout`), gathered over each vertex's corners.
+void launch_divergence(const int32_t* faces, const double* geom, const int32_t* vertex_ptr,
                        const int32_t* vertex_corners, const double* heat, int64_t n,
-                       int64_t n_faces, int64_t n_columns, double* corner_terms, double* out,
-                       FieldLayout layout, hipStream_t stream);
+                       int64_t n_columns, double* out, FieldLayout layout, hipStream_t stream);
 int32_t shift_partial_blocks(int64_t n);
 // Columns start at col_off[q] with col_len[q] entries (device; both null: q * n and n), n the
 // longest.

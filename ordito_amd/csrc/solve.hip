@@ -29,26 +29,33 @@ __device__ __forceinline__ double widen(T v) {
   return static_cast<double>(v);
 }
 
-// Gathered front-vector entry: base plus the children's update rows adding into this position.
+// Gathered front-vector entry: base plus the children's update rows adding into this position, in
+// ascending child order. The first two sources sit inline in gather_pair (-1: none; -2 in the
+// second: more than two, read on from the lists after the first).
 __device__ __forceinline__ double gathered(double base, int64_t row, int32_t q, int32_t kk,
+                                           const int32_t* __restrict__ gather_pair,
                                            const int32_t* __restrict__ gather_ptr,
                                            const int32_t* __restrict__ gather_src,
                                            const double* __restrict__ fvec) {
+  const int2 p = reinterpret_cast<const int2*>(gather_pair)[row];
   double v = base;
-  for (int32_t g = gather_ptr[row]; g < gather_ptr[row + 1]; ++g)
-    v += fvec[static_cast<int64_t>(gather_src[g]) * kk + q];
+  if (p.x >= 0) v += fvec[static_cast<int64_t>(p.x) * kk + q];
+  if (p.y >= 0) {
+    v += fvec[static_cast<int64_t>(p.y) * kk + q];
+  } else if (p.y == -2) {
+    for (int32_t g = gather_ptr[row] + 1; g < gather_ptr[row + 1]; ++g)
+      v += fvec[static_cast<int64_t>(gather_src[g]) * kk + q];
+  }
   return v;
 }
 
 // b (user layout, column q at q*n) -> bp[i*kk + q] in elimination order through an LDS transpose
-// of 64 rows; a pinned row's equation is dropped (zero). Also starts x = 0 and r = bp.
+// of 64 rows; a pinned row's equation is dropped (zero).
 __global__ __launch_bounds__(256) void perm_in_kernel(const double* __restrict__ b,
                                                       const int32_t* __restrict__ perm,
                                                       const int32_t* __restrict__ pinned,
                                                       int64_t n, int32_t kk,
-                                                      double* __restrict__ bp,
-                                                      double* __restrict__ x,
-                                                      double* __restrict__ r) {
+                                                      double* __restrict__ bp) {
   __shared__ double tile[kTile][kCols + 1];
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTile;
   const int ii = threadIdx.x % kTile, qq = threadIdx.x / kTile;
@@ -61,11 +68,7 @@ __global__ __launch_bounds__(256) void perm_in_kernel(const double* __restrict__
   const int32_t rows = static_cast<int32_t>(min<int64_t>(kTile, n - i0));
   for (int e = threadIdx.x; e < rows * kk; e += 256) {
     const int r_ = e / kk, q = e % kk;
-    const int64_t slot = (i0 + r_) * kk + q;
-    const double v = tile[r_][q];
-    bp[slot] = v;
-    r[slot] = v;
-    x[slot] = 0.0;
+    bp[(i0 + r_) * kk + q] = tile[r_][q];
   }
 }
 
@@ -91,16 +94,11 @@ __global__ __launch_bounds__(256) void perm_in_rows_kernel(const double* __restr
                                                            const int32_t* __restrict__ perm,
                                                            const int32_t* __restrict__ pinned,
                                                            int64_t n, int32_t kk,
-                                                           double* __restrict__ bp,
-                                                           double* __restrict__ x,
-                                                           double* __restrict__ r) {
+                                                           double* __restrict__ bp) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (e >= n * kk) return;
   const int64_t i = e / kk, q = e % kk;
-  const double v = (pinned && pinned[i]) ? 0.0 : b[static_cast<int64_t>(perm[i]) * kk + q];
-  bp[e] = v;
-  r[e] = v;
-  x[e] = 0.0;
+  bp[e] = (pinned && pinned[i]) ? 0.0 : b[static_cast<int64_t>(perm[i]) * kk + q];
 }
 
 __global__ __launch_bounds__(256) void perm_out_rows_kernel(const double* __restrict__ x,
@@ -121,14 +119,15 @@ struct SweepArgs {
   const int32_t* rows;
   const int64_t* w_off;
   const int64_t* vec_off;
+  const int32_t* gather_pair;
   const int32_t* gather_ptr;
   const int32_t* gather_src;
   const double* r;   // residual, the sweep's right-hand side
   double* fvec;      // update vectors (rows part of each front's vector)
   double* y;         // forward result
   double* d;         // backward result (the correction)
-  const double* x;   // current iterate
-  double* x_next;    // x + d
+  const double* x;   // current iterate; null on the first round (x = 0: d is the next iterate)
+  double* x_next;    // x + d; null on the first round
   double* partials;
   int32_t kk;
 };
@@ -195,7 +194,7 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
       const int q = e % kCols;
       freg[u] = (q < kk && j < k1)
                     ? gathered(a.r[static_cast<int64_t>(c0 + j) * kk + q], vbase + j, q, kk,
-                               a.gather_ptr, a.gather_src, a.fvec)
+                               a.gather_pair, a.gather_ptr, a.gather_src, a.fvec)
                     : 0.0;
     }
   };
@@ -226,15 +225,12 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
     for (int v = 0; v < 4; ++v) {
       const int32_t i = r0 + 16 * wave + lane / 16 + 4 * v;
       if (i < nc || i >= m) continue;
-      const int64_t row = vbase + i;
-      const int32_t g1 = a.gather_ptr[row + 1];
-      for (int32_t g = a.gather_ptr[row]; g < g1; ++g) {
-        const int64_t src = static_cast<int64_t>(a.gather_src[g]) * kk;
 #pragma unroll
-        for (int n = 0; n < 4; ++n) {
-          const int q = 16 * n + lane % 16;
-          if (q < kk) base[n][v] += a.fvec[src + q];
-        }
+      for (int n = 0; n < 4; ++n) {
+        const int q = 16 * n + lane % 16;
+        if (q < kk)
+          base[n][v] = gathered(0.0, vbase + i, q, kk, a.gather_pair, a.gather_ptr, a.gather_src,
+                                a.fvec);
       }
     }
   }
@@ -259,6 +255,30 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
   }
 }
 
+// Adds the reduction partials of entries off[u] (offsets inside a slot of `stride` doubles) over
+// slots [first, first + count) in slot order into sum[u]; four slots' loads are issued before
+// their adds, so the loads overlap while each sum keeps its order.
+__device__ __forceinline__ void add_partials(const double* __restrict__ partials, int64_t stride,
+                                             int32_t first, int32_t count, const int64_t off[4],
+                                             double sum[4]) {
+  const double* p = partials + static_cast<int64_t>(first) * stride;
+  int32_t c = 0;
+  for (; c + 4 <= count; c += 4) {
+    double v[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[cc][u] = p[(c + cc) * stride + off[u]];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) sum[u] += v[cc][u];
+  }
+  for (; c < count; ++c)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sum[u] += p[c * stride + off[u]];
+}
+
 // Adds a split forward tile's partials in slot order and applies the sum; blockIdx.y picks 16 of
 // the tile's 64 rows.
 __global__ __launch_bounds__(256) void fwd_reduce_kernel(
@@ -273,15 +293,23 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
   const int32_t lo = blockIdx.y * kRows;
   const int32_t rows = min(kRows, m - r0 - lo);
   // At most kRows * kCols = 4 x 256 entries: each thread's four, every load before any store.
+  // Entries past the group's rows read slot offset 0 and store nothing.
   double sum[4] = {}, base[4] = {};
+  int64_t off[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int32_t e = threadIdx.x + 256 * u;
+    off[u] = e < rows * kk ? static_cast<int64_t>(lo + e / kk) * kk + e % kk : 0;
+  }
+  add_partials(a.partials, static_cast<int64_t>(kTile) * kk, first, count, off, sum);
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int32_t e = threadIdx.x + 256 * u;
     if (e >= rows * kk) continue;
     const int32_t ii = lo + e / kk, q = e % kk;
-    for (int32_t c = 0; c < count; ++c)
-      sum[u] += a.partials[(static_cast<int64_t>(first + c) * kTile + ii) * kk + q];
-    if (r0 + ii >= nc) base[u] = gathered(0.0, vbase + r0 + ii, q, kk, a.gather_ptr, a.gather_src, a.fvec);
+    if (r0 + ii >= nc)
+      base[u] = gathered(0.0, vbase + r0 + ii, q, kk, a.gather_pair, a.gather_ptr, a.gather_src,
+                         a.fvec);
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -355,7 +383,8 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
   if (!cols_live) return;
   // Epilogue: the iterate's entries are all loaded before any store (see the forward tile).
   double xv[4][4] = {};
-  if (slot < 0) {
+  const bool iterate = a.x != nullptr;  // first round: x = 0 and d is the next iterate itself
+  if (slot < 0 && iterate) {
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int q = 16 * n + lane % 16;
@@ -381,7 +410,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
       } else {
         const int64_t at = static_cast<int64_t>(c0 + j) * kk + q;
         a.d[at] = value;
-        a.x_next[at] = xv[n][v] + value;
+        if (iterate) a.x_next[at] = xv[n][v] + value;
       }
     }
   }
@@ -398,16 +427,17 @@ __global__ __launch_bounds__(256) void bwd_reduce_kernel(
   const int32_t lo = blockIdx.y * kRows;
   const int32_t cols = min(kRows, nc - j0 - lo);
   // Each thread's (at most four) entries: every load before any store.
+  const bool iterate = a.x != nullptr;
   double sum[4] = {}, xv[4] = {};
+  int64_t off[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int32_t e = threadIdx.x + 256 * u;
-    if (e >= cols * kk) continue;
-    const int32_t jj = lo + e / kk, q = e % kk;
-    for (int32_t c = 0; c < count; ++c)
-      sum[u] += a.partials[(static_cast<int64_t>(first + c) * kTile + jj) * kk + q];
-    xv[u] = a.x[static_cast<int64_t>(c0 + j0 + jj) * kk + q];
+    off[u] = e < cols * kk ? static_cast<int64_t>(lo + e / kk) * kk + e % kk : 0;
+    if (iterate && e < cols * kk)
+      xv[u] = a.x[static_cast<int64_t>(c0 + j0 + lo + e / kk) * kk + e % kk];
   }
+  add_partials(a.partials, static_cast<int64_t>(kTile) * kk, first, count, off, sum);
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int32_t e = threadIdx.x + 256 * u;
@@ -415,7 +445,7 @@ __global__ __launch_bounds__(256) void bwd_reduce_kernel(
     const int32_t jj = lo + e / kk, q = e % kk;
     const int64_t at = static_cast<int64_t>(c0 + j0 + jj) * kk + q;
     a.d[at] = sum[u];
-    a.x_next[at] = xv[u] + sum[u];
+    if (iterate) a.x_next[at] = xv[u] + sum[u];
   }
 }
 
@@ -442,7 +472,9 @@ __global__ __launch_bounds__(256) void fwd_tile1_kernel(
   for (int32_t j0 = k0; j0 < k1; j0 += 256) {
     const int32_t j = j0 + threadIdx.x;
     Fs[threadIdx.x] =
-        j < k1 ? gathered(a.r[c0 + j], vbase + j, 0, 1, a.gather_ptr, a.gather_src, a.fvec) : 0.0;
+        j < k1 ? gathered(a.r[c0 + j], vbase + j, 0, 1, a.gather_pair, a.gather_ptr, a.gather_src,
+                          a.fvec)
+               : 0.0;
     __syncthreads();
     const int32_t jend = min(k1 - j0, 256);
     if (i < m)
@@ -460,7 +492,8 @@ __global__ __launch_bounds__(256) void fwd_tile1_kernel(
       a.y[c0 + i] = sum;
     } else {
       const int64_t row = vbase + i;
-      a.fvec[row] = gathered(0.0, row, 0, 1, a.gather_ptr, a.gather_src, a.fvec) + sum;
+      a.fvec[row] =
+          gathered(0.0, row, 0, 1, a.gather_pair, a.gather_ptr, a.gather_src, a.fvec) + sum;
     }
   }
 }
@@ -498,7 +531,7 @@ __global__ __launch_bounds__(256) void bwd_tile1_kernel(
       a.partials[static_cast<int64_t>(slot) * kTile + jj] = sum;
     } else {
       a.d[c0 + j] = sum;
-      a.x_next[c0 + j] = a.x[c0 + j] + sum;
+      if (a.x) a.x_next[c0 + j] = a.x[c0 + j] + sum;
     }
   }
 }
@@ -517,6 +550,9 @@ __device__ __forceinline__ void raise_flag(bool fail, int32_t* flag) {
 // (componentwise against (|A||x| + |b|)_i, normwise against the larger of that and the column's
 // largest |b|; kernels/cholesky.py:1393-1440). Pinned rows are dropped. One thread a (row,
 // right-hand side); the row's operator entries are shared by its kk threads.
+// kWaveRow: kk is the full 64, so each wave holds one row and the row's offsets, columns and
+// values are wave-uniform (read once per wave through the scalar unit).
+template <bool kWaveRow>
 __global__ void residual_kernel(const int32_t* __restrict__ offsets,
                                 const int32_t* __restrict__ columns,
                                 const double* __restrict__ values, const double* __restrict__ b,
@@ -527,7 +563,9 @@ __global__ void residual_kernel(const int32_t* __restrict__ offsets,
   int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool fail = false;
   if (t < n * kk) {
-    const int64_t i = t / kk, q = t % kk;
+    const int64_t i =
+        kWaveRow ? __builtin_amdgcn_readfirstlane(static_cast<int32_t>(t / kCols)) : t / kk;
+    const int64_t q = t % kk;
     const double bi = b[t];
     double total = bi, scale = fabs(bi);
     // Four entries at a time: their column and value loads, then their iterate loads, issue
@@ -760,10 +798,10 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
     const unsigned row_tiles = static_cast<unsigned>((n + kTile - 1) / kTile);
     if (b_layout == FieldLayout::kRows)
       perm_in_rows_kernel<<<grid, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk,
-                                                    ws.bp.ptr, ws.xa.ptr, ws.r.ptr);
+                                                    ws.bp.ptr);
     else
       perm_in_kernel<<<row_tiles, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk,
-                                                    ws.bp.ptr, ws.xa.ptr, ws.r.ptr);
+                                                    ws.bp.ptr);
     double* bmax = ws.bmax.ptr + static_cast<int64_t>(blocks_for(n)) * kCols;
     if (!componentwise) {
       const unsigned blocks = blocks_for(n);
@@ -781,35 +819,46 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
     a.rows = plan.rows.ptr;
     a.w_off = plan.w_off.ptr;
     a.vec_off = plan.vec_off.ptr;
+    a.gather_pair = plan.gather_pair.ptr;
     a.gather_ptr = plan.gather_ptr.ptr;
     a.gather_src = plan.gather_src.ptr;
-    a.r = ws.r.ptr;
     a.fvec = ws.fvec.ptr;
     a.y = ws.y.ptr;
-    a.d = ws.d.ptr;
     a.partials = ws.partials.ptr;
     a.kk = kk;
-    double* x_cur = ws.xa.ptr;
-    double* x_next = ws.xb.ptr;
+    // The first round solves for the iterate itself from x = 0: its right-hand side is b and its
+    // correction is written straight into the iterate. Later rounds correct it: d into ws.d and
+    // x + d into the other iterate buffer.
+    double* x_cur = ws.xb.ptr;
     int32_t rounds = 0;
     while (rounds < max_rounds) {
-      a.x = x_cur;
-      a.x_next = x_next;
+      double* x_next = x_cur == ws.xb.ptr ? ws.xa.ptr : ws.xb.ptr;
+      if (rounds == 0) {
+        a.r = ws.bp.ptr;
+        a.x = nullptr;
+        a.x_next = nullptr;
+        a.d = x_cur;
+      } else {
+        a.r = ws.r.ptr;
+        a.x = x_cur;
+        a.x_next = x_next;
+        a.d = ws.d.ptr;
+      }
       {
         PhaseScope scope(timer, sweep_phase, stream);
         if (use_fp32) sweep<float>(plan, factor.w32.ptr, a, stream);
         else sweep<double>(plan, factor.w64.ptr, a, stream);
       }
+      if (rounds > 0) x_cur = x_next;
       ++rounds;
-      std::swap(x_cur, x_next);
       {
         PhaseScope scope(timer, kPhaseResidual, stream);
         HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
-        residual_kernel<<<grid, 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
-                                                  factor.pvalues.ptr, ws.bp.ptr, x_cur,
-                                                  pinned_new, bmax, n, kk, tolerance,
-                                                  componentwise ? 1 : 0, ws.r.ptr,
-                                                  ws.pending.ptr);
+        auto residual = kk == kCols ? residual_kernel<true> : residual_kernel<false>;
+        residual<<<grid, 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
+                                           factor.pvalues.ptr, ws.bp.ptr, x_cur, pinned_new, bmax,
+                                           n, kk, tolerance, componentwise ? 1 : 0, ws.r.ptr,
+                                           ws.pending.ptr);
       }
       HG_HIP(hipGetLastError());
       int32_t pending = 0;

@@ -41,7 +41,7 @@ def test_oracle_reproduces_golden(name):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gpu_matches_golden(cuda, name, precision):
     """Iterated on the first call, factored from the second (rent or buy), all columns at once on
-    the third: each within 1e-5 of the range (mixed: 1e-4, the north star's bar)."""
+    the third: each within 1e-5 of the range (mixed: 1e-4, the served accuracy bar)."""
     from ordito_amd import heat
 
     vertices, faces, sets, expected = _load(name)

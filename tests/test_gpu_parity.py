@@ -8,7 +8,9 @@ Bars (each written in its test):
 - factored FP64 distance: 1e-9 of the range (both are direct FP64 solves of the same systems);
 - iterated (settled CG) distance: 1e-5 of the range (the settle rule stops at ~1e-6 per entry,
   ordito's iterated-vs-factored agreement is 4e-6 of the range, ``tests/test_heat.py:393-398``);
-- mixed-precision distance: 1e-4 of the range (the north star's bar).
+- mixed-precision distance: 1e-4 of the range (the served accuracy bar);
+- unit-gradient divergence: 1e-12 of each column's largest entry, exact under sign and power-of-two
+  scaling.
 """
 
 from __future__ import annotations
@@ -103,6 +105,44 @@ def test_mixed_precision_matches_oracle(cuda, name):
     for q in range(len(sets)):
         worst, _ = ho.relative_error(got[q], expected[q])
         assert worst <= 1e-4, (q, worst)
+
+
+@pytest.mark.parametrize("name", ["icosphere3", "torus", "grid", "saddle_graded"])
+def test_divergence_matches_oracle(cuda, name):
+    """``hg_unit_gradient_divergence`` against the oracle's face divergence: 1e-12 of the column's
+    largest entry (the kernel normalizes with an exact power-of-two scale and a reciprocal square
+    root, the oracle with divisions, so the two agree to a few ulps per term). The direction is
+    scale-free, so a column scaled by 2^-600 (every product stays a normal number) must give the
+    same bits, a negated column the negated bits, and a zero column zeros."""
+    import math
+
+    from ordito_amd import _lib, heat
+
+    vertices, faces = mesh(name)
+    n = len(vertices)
+    ref = ho.assemble(vertices, faces, None)
+    p = vertices.astype(np.float64)
+    u = np.exp(-np.sum((p - p[0]) ** 2, axis=1) / (0.5 * np.ptp(p, axis=0).max()) ** 2)
+    rng = np.random.default_rng(3)
+    noisy = rng.random(n)
+    columns = np.stack([u, noisy, u * 2.0**-600, -noisy, np.zeros(n)])
+    handle = heat._Handle(_tensor(cuda, vertices), _tensor(cuda, faces))
+    try:
+        _lib.call("hg_operators", handle.ptr, math.nan, 0, 1)
+        d_heat = _tensor(cuda, columns.reshape(-1))
+        d_out = cuda.zeros_like(d_heat)
+        _lib.call("hg_unit_gradient_divergence", handle.ptr, d_heat.data_ptr(), d_out.data_ptr(),
+                  len(columns))
+        got = d_out.cpu().numpy().reshape(len(columns), n)
+    finally:
+        handle.close()
+    for q in (0, 1):
+        expected = ho.face_unit_divergence(vertices, faces, ref.terms, columns[q])
+        err = np.abs(got[q] - expected).max()
+        assert err <= 1e-12 * np.abs(expected).max(), (q, err)
+    assert np.array_equal(got[2], got[0])
+    assert np.array_equal(got[3], -got[1])
+    assert not got[4].any()
 
 
 def test_rent_or_buy_factors_on_the_second_call(cuda):
