@@ -75,6 +75,7 @@ struct hg_mesh {
   int32_t split_target = kDefaultSplitTarget;  // sweep tasks a level is split towards
   std::unique_ptr<Plan> host_plan;
   std::unique_ptr<DevicePlan> plan;
+  DeviceArray<int32_t> faces_rows;  // the faces' corners as rows of the elimination order
   Factor heat_factor, poisson_factor;
   // Refinement targets of the factored solves, [precision][0 heat (componentwise), 1 Poisson
   // (normwise)].
@@ -262,6 +263,11 @@ void ensure_plan(hg_mesh& m) {
   const Plan& plan = *m.host_plan;
   auto device = std::make_unique<DevicePlan>();
   device->upload(plan, m.pattern, m.split_target, m.stream);
+  {
+    std::vector<int32_t> rows(m.h_faces.size());
+    for (size_t e = 0; e < rows.size(); ++e) rows[e] = plan.iperm[m.h_faces[e]];
+    m.faces_rows.upload(rows, m.stream);
+  }
   // Pins: the Poisson system's one row per component, its last-eliminated vertex
   // (kernels/cholesky.py:195-214); the heat system's empty rows (unreferenced vertices).
   const int64_t n = m.n;
@@ -359,7 +365,7 @@ bool backward_error_fails(hg_mesh& m, const double* x, const double* b, int64_t 
   return flag != 0;
 }
 
-int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t precision,
+int32_t diffuse_factored(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t precision,
                          FieldLayout layout = FieldLayout::kColumns) {
   const bool mixed = precision == HG_PRECISION_MIXED;
   factor_system(m, HG_SYSTEM_HEAT, mixed);
@@ -373,7 +379,7 @@ int32_t diffuse_factored(hg_mesh& m, const double* rhs, double* x, int64_t k, in
 // The settled iterate, verified, falling back to the factorization (solve_spd_settled,
 // linalg.py:2659-2727). Returns the flags of hg_diffuse.
 // Fields stored by rows (FieldLayout::kRows) go through the factors only.
-int32_t diffuse(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t mode,
+int32_t diffuse(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
                 int32_t precision, FieldLayout layout = FieldLayout::kColumns) {
   ensure_operators(m);
   int32_t flags = 0;
@@ -410,7 +416,7 @@ double poisson_bound(hg_mesh& m) {
 }
 
 // The solution is stored by columns; the right-hand side may be stored by rows (factored only).
-void solve_poisson(hg_mesh& m, const double* rhs, double* x, int64_t k, int32_t mode,
+void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
                    int32_t precision, FieldLayout rhs_layout = FieldLayout::kColumns) {
   ensure_operators(m);
   const bool factored = mode == HG_MODE_FACTORED || m.poisson_factor.held() || m.heat_factor.held();
@@ -478,8 +484,9 @@ void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0,
   HG_HIP(hipMemsetAsync(u0, 0, sizeof(double) * m.n * k, m.stream));
   if (m.invalid.size < 1) m.invalid.allocate(1);
   HG_HIP(hipMemsetAsync(m.invalid.ptr, 0, sizeof(int32_t), m.stream));
-  launch_seed(m.d_sources.ptr, m.d_owner.ptr, total, m.n, layout == FieldLayout::kRows ? k : 0,
-              u0, m.invalid.ptr, m.stream);
+  const bool rows = layout == FieldLayout::kRows;
+  launch_seed(m.d_sources.ptr, m.d_owner.ptr, total, m.n, rows ? k : 0,
+              rows ? m.plan->iperm.ptr : nullptr, u0, m.invalid.ptr, m.stream);
   int32_t flag = 0;
   HG_HIP(hipMemcpyAsync(&flag, m.invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
   HG_HIP(hipStreamSynchronize(m.stream));
@@ -490,8 +497,10 @@ void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0,
 void divergence(hg_mesh& m, const double* heat, double* out, int64_t k,
                 FieldLayout layout = FieldLayout::kColumns) {
   PhaseScope scope(&m.timer, kPhaseDivergence, m.stream);
-  launch_divergence(m.faces.ptr, m.face_geom.ptr, m.vertex_ptr.ptr, m.vertex_corners.ptr, heat,
-                    m.n, k, out, layout, m.stream);
+  const bool rows = layout == FieldLayout::kRows;
+  launch_divergence(rows ? m.faces_rows.ptr : m.faces.ptr, rows ? m.plan->iperm.ptr : nullptr,
+                    m.face_geom.ptr, m.vertex_ptr.ptr, m.vertex_corners.ptr, heat, m.n, k, out,
+                    layout, m.stream);
 }
 
 double* reserve_fields(hg_mesh& m, int64_t k) {
@@ -520,11 +529,16 @@ void solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   double* u0 = fields;
   double* heat = fields + n * k;
   double* negdiv = fields + 2 * n * k;
-  // Through the factors the intermediate fields stay in the solves' row layout: the permutations
-  // in and out of the refined solves and the divergence then read and write whole 512-byte rows.
+  // Through the factors the intermediate fields stay in the solves' row layout, in the elimination
+  // order: the seeds land there, the heat step's solution stays there for the divergence, which
+  // writes the Poisson step's right-hand side there; only the distance is permuted out.
   const FieldLayout layout = (mode == HG_MODE_FACTORED || m.heat_factor.held())
                                  ? FieldLayout::kRows
                                  : FieldLayout::kColumns;
+  if (layout == FieldLayout::kRows) {
+    ensure_operators(m);
+    ensure_plan(m);
+  }
   seed(m, offsets, k, u0, layout);
   int32_t flags = diffuse(m, u0, heat, k, mode, precision, layout);
   m.stats.backward_error_failed = flags & 1;
@@ -767,7 +781,9 @@ int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
     require_single_or_factored(*mesh, mode);
     mesh->stats = hg_solve_stats{};
-    int32_t flags = diffuse(*mesh, d_rhs, d_solution, n_columns, mode, precision);
+    // By columns (the caller's layout) the solve only reads its right-hand side.
+    int32_t flags = diffuse(*mesh, const_cast<double*>(d_rhs), d_solution, n_columns, mode,
+                            precision);
     HG_HIP(hipStreamSynchronize(mesh->stream));
     if (out_flags) *out_flags = flags;
   });
@@ -779,7 +795,8 @@ int hg_solve_poisson(hg_mesh* mesh, const double* d_rhs, double* d_solution, int
     require_mesh(mesh);
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
     require_single_or_factored(*mesh, mode);
-    solve_poisson(*mesh, d_rhs, d_solution, n_columns, mode, precision);
+    // By columns (the caller's layout) the solve only reads its right-hand side.
+    solve_poisson(*mesh, const_cast<double*>(d_rhs), d_solution, n_columns, mode, precision);
     HG_HIP(hipStreamSynchronize(mesh->stream));
   });
 }

@@ -219,9 +219,11 @@ __global__ void operator_rows_kernel(
 
 // u0[q, s] = 1 at each source of column q (duplicates allowed); an index outside [0, n) writes
 // nothing and raises the flag (kernels/heat.py:599-623). One thread a source entry.
-// rows_k > 0: u0 is stored by rows (FieldLayout::kRows) over rows_k columns.
+// rows_k > 0: u0 is stored by rows (FieldLayout::kRows) over rows_k columns, vertex v at row
+// row_of[v].
 __global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* __restrict__ owner,
-                            int64_t n_entries, int64_t n, int64_t rows_k, double* __restrict__ u0,
+                            int64_t n_entries, int64_t n, int64_t rows_k,
+                            const int32_t* __restrict__ row_of, double* __restrict__ u0,
                             int32_t* __restrict__ invalid) {
   int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n_entries) return;
@@ -234,7 +236,7 @@ __global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* 
   if (rows_k > 0) {
     const int64_t chunk = q / kFieldChunk;
     const int64_t kk = min(kFieldChunk, rows_k - chunk * kFieldChunk);
-    u0[chunk * kFieldChunk * n + v * kk + q % kFieldChunk] = 1.0;
+    u0[chunk * kFieldChunk * n + static_cast<int64_t>(row_of[v]) * kk + q % kFieldChunk] = 1.0;
   } else {
     u0[q * n + v] = 1.0;
   }
@@ -302,26 +304,24 @@ __device__ __forceinline__ double corner_term(const double* __restrict__ geom, i
 
 // -b at vertex i and column q: the terms of i's corners for that column summed in ascending corner
 // order (kernels/heat.py:626-687, heat.py:3218-3232), each term recomputed from the face's heat
-// values (no per-corner intermediate in memory). heat and out share a layout: entry (vertex v,
-// column q) at v * vstride + q * qstride.
+// values (no per-corner intermediate in memory). heat and out share a layout: entry (row r,
+// column q) at r * vstride + q * qstride, vertex i at row row_of[i] (row i when row_of is null),
+// and `faces` name rows.
 //
 // kWaveVertex: one wave a vertex, its lanes the columns (kk <= 64 columns stored by rows, so the
 // heat rows it reads and the row it writes are contiguous); the vertex, its corners and their face
-// data are wave-uniform. Blocks are numbered so that each XCD's share of the grid (the hardware
-// deals blocks to the eight XCDs in turn) is one contiguous range of vertices, whose neighbours'
-// heat rows then stay in that XCD's L2. Otherwise one thread per (vertex, column), vertex fastest.
+// data are wave-uniform. Blocks are numbered so that each XCD's share of the grid is one
+// contiguous range of vertices (xcd_block), whose neighbours' heat rows then stay in that XCD's
+// L2. Otherwise one thread per (vertex, column), the contiguous index fastest.
 template <bool kWaveVertex>
 __global__ __launch_bounds__(256) void divergence_kernel(
     const int32_t* __restrict__ faces, const double* __restrict__ geom,
     const int32_t* __restrict__ vertex_ptr, const int32_t* __restrict__ vertex_corners,
-    const double* __restrict__ heat, int64_t n, int64_t kk, int64_t vstride, int64_t qstride,
-    double* __restrict__ out) {
+    const int32_t* __restrict__ row_of, const double* __restrict__ heat, int64_t n, int64_t kk,
+    int64_t vstride, int64_t qstride, double* __restrict__ out) {
   int64_t i, q;
   if (kWaveVertex) {
-    const int64_t blocks = gridDim.x, per = blocks / 8, extra = blocks % 8;
-    const int64_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-    const int64_t block = xcd * per + min(xcd, extra) + slot;
-    i = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(block * 4 + threadIdx.x / 64));
+    i = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(xcd_block() * 4 + threadIdx.x / 64));
     q = threadIdx.x % 64;
     if (i >= n || q >= kk) return;
   } else {
@@ -337,7 +337,8 @@ __global__ __launch_bounds__(256) void divergence_kernel(
     }
   }
   const double* u = heat + q * qstride;
-  const double ui = u[i * vstride];
+  const int64_t row = row_of ? row_of[i] : i;
+  const double ui = u[row * vstride];
   double sum = 0.0;
   const int32_t k1 = vertex_ptr[i + 1];
   for (int32_t k = vertex_ptr[i]; k < k1; ++k) {
@@ -350,7 +351,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(
     const double u2 = corner == 2 ? ui : u[tri[2] * vstride];
     sum += corner_term(geom + 18 * static_cast<int64_t>(f), corner, u0, u1, u2);
   }
-  out[i * vstride + q * qstride] = sum;
+  out[row * vstride + q * qstride] = sum;
 }
 
 // Columns of the shift: column q starts at col_off[q] and holds col_len[q] entries (packed meshes'
@@ -491,10 +492,11 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
 }
 
 void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
-                 int64_t rows_k, double* u0, int32_t* invalid, hipStream_t stream) {
+                 int64_t rows_k, const int32_t* row_of, double* u0, int32_t* invalid,
+                 hipStream_t stream) {
   if (n_entries == 0) return;
   seed_kernel<<<blocks_for(n_entries), kBlock, 0, stream>>>(sources, owner, n_entries, n, rows_k,
-                                                            u0, invalid);
+                                                            row_of, u0, invalid);
   HG_HIP(hipGetLastError());
 }
 
@@ -507,9 +509,10 @@ void launch_face_geometry(const float* xyz, const int32_t* faces, const float* n
   HG_HIP(hipGetLastError());
 }
 
-void launch_divergence(const int32_t* faces, const double* geom, const int32_t* vertex_ptr,
-                       const int32_t* vertex_corners, const double* heat, int64_t n,
-                       int64_t n_columns, double* out, FieldLayout layout, hipStream_t stream) {
+void launch_divergence(const int32_t* faces, const int32_t* row_of, const double* geom,
+                       const int32_t* vertex_ptr, const int32_t* vertex_corners,
+                       const double* heat, int64_t n, int64_t n_columns, double* out,
+                       FieldLayout layout, hipStream_t stream) {
   if (n == 0 || n_columns == 0) return;
   if (layout == FieldLayout::kRows) {
     for (int64_t c0 = 0; c0 < n_columns; c0 += kFieldChunk) {
@@ -517,14 +520,16 @@ void launch_divergence(const int32_t* faces, const double* geom, const int32_t* 
       // A wave a vertex while at least a quarter of its lanes hold columns.
       if (kk >= kFieldChunk / 4)
         divergence_kernel<true><<<static_cast<unsigned>((n + 3) / 4), kBlock, 0, stream>>>(
-            faces, geom, vertex_ptr, vertex_corners, heat + c0 * n, n, kk, kk, 1, out + c0 * n);
+            faces, geom, vertex_ptr, vertex_corners, row_of, heat + c0 * n, n, kk, kk, 1,
+            out + c0 * n);
       else
         divergence_kernel<false><<<blocks_for(n * kk), kBlock, 0, stream>>>(
-            faces, geom, vertex_ptr, vertex_corners, heat + c0 * n, n, kk, kk, 1, out + c0 * n);
+            faces, geom, vertex_ptr, vertex_corners, row_of, heat + c0 * n, n, kk, kk, 1,
+            out + c0 * n);
     }
   } else {
     divergence_kernel<false><<<blocks_for(n * n_columns), kBlock, 0, stream>>>(
-        faces, geom, vertex_ptr, vertex_corners, heat, n, n_columns, 1, n, out);
+        faces, geom, vertex_ptr, vertex_corners, nullptr, heat, n, n_columns, 1, n, out);
   }
   HG_HIP(hipGetLastError());
 }

@@ -80,11 +80,13 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
                     const std::vector<int64_t>& pin_slots, Factor& factor, bool keep_fp32,
                     hipStream_t stream);
 
-// x = A^-1 b (k columns of n entries, user layout) refined against the operator (the factor's
-// permuted values) until every row passes the test (componentwise or normwise) or max_rounds
-// sweeps ran; returns the most sweeps any 64-column chunk took. Pinned rows are dropped.
+// x = A^-1 b (k columns of n entries in b_layout / x_layout) refined against the operator (the
+// factor's permuted values) until every row passes the test (componentwise or normwise) or
+// max_rounds sweeps ran; returns the most sweeps any 64-column chunk took. Pinned rows are
+// dropped. b stored by columns is only read; b stored by rows is the solve's own right-hand side,
+// its pinned rows zeroed in place.
 int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp32,
-                      const double* b, double* x, int64_t k, double tolerance, bool componentwise,
+                      double* b, double* x, int64_t k, double tolerance, bool componentwise,
                       int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
                       hipStream_t stream, PhaseTimer* timer = nullptr,
                       int sweep_phase = kPhaseHeatSweep, FieldLayout b_layout = FieldLayout::kColumns,

@@ -9,10 +9,21 @@ constexpr int kBlock = 256;
 
 // How a field of k columns over n vertices is stored. kColumns: the caller's layout, column q at
 // q * n. kRows: in chunks of 64 columns (the solves' width), chunk c at 64 c * n holding its kk
-// columns interleaved per vertex (vertex i's entries at i * kk), the layout the refined solves
-// work in; the heat step hands its field to the divergence and the Poisson step this way.
+// columns interleaved per row, rows in the factorization's elimination order (vertex v's entries
+// at iperm[v] * kk): the layout the refined solves work in, so the seeds, the heat step, the
+// divergence and the Poisson step hand fields on with no permutation pass between them.
 enum class FieldLayout : int { kColumns = 0, kRows = 1 };
 constexpr int64_t kFieldChunk = 64;
+
+// The work item a block of a one-dimensional launch takes, numbered so that each XCD's share of the
+// grid is one contiguous range: the hardware deals consecutive blocks to the eight XCDs in turn,
+// and neighbouring work items (a front's tiles, neighbouring rows) then meet in one XCD's L2.
+// A bijection of [0, gridDim.x).
+__device__ __forceinline__ int32_t xcd_block() {
+  const int32_t blocks = static_cast<int32_t>(gridDim.x), b = static_cast<int32_t>(blockIdx.x);
+  const int32_t per = blocks / 8, extra = blocks % 8, xcd = b % 8;
+  return xcd * per + min(xcd, extra) + b / 8;
+}
 
 // Device time of the phases of a solve, from HIP events on the stream the work is issued on.
 enum Phase : int {
@@ -106,17 +117,21 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
                           const double* mass, const double* scalars, const int32_t* row_segment,
                           int64_t n, double* lap, double* heat, double* poisson, float* heat32,
                           double* heat_inv_diag, double* poisson_inv_diag, hipStream_t stream);
-// rows_k > 0: u0 stored by rows (FieldLayout::kRows) over rows_k columns; 0: by columns.
+// rows_k > 0: u0 stored by rows (FieldLayout::kRows) over rows_k columns, vertex v at row
+// row_of[v]; 0: by columns.
 void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
-                 int64_t rows_k, double* u0, int32_t* invalid, hipStream_t stream);
+                 int64_t rows_k, const int32_t* row_of, double* u0, int32_t* invalid,
+                 hipStream_t stream);
 // Per face, 18 doubles: the gradient basis a_k = N x e_k and the corners' integration vectors.
 void launch_face_geometry(const float* xyz, const int32_t* faces, const float* normals,
                           const float* areas, const float* cot, int64_t n_faces, double* geom,
                           hipStream_t stream);
 // -b per vertex and column from the heat (both in `layout`), gathered over each vertex's corners.
-void launch_divergence(const int32_t* faces, const double* geom, const int32_t* vertex_ptr,
-                       const int32_t* vertex_corners, const double* heat, int64_t n,
-                       int64_t n_columns, double* out, FieldLayout layout, hipStream_t stream);
+// kRows: vertex v's row is row_of[v], and `faces` name rows (row_of applied to every corner).
+void launch_divergence(const int32_t* faces, const int32_t* row_of, const double* geom,
+                       const int32_t* vertex_ptr, const int32_t* vertex_corners,
+                       const double* heat, int64_t n, int64_t n_columns, double* out,
+                       FieldLayout layout, hipStream_t stream);
 int32_t shift_partial_blocks(int64_t n);
 // Columns start at col_off[q] with col_len[q] entries (device; both null: q * n and n), n the
 // longest.

@@ -2,8 +2,9 @@
 //
 // The whole refinement (ordito kernels/cholesky.py:1357-1463, cholesky.py:1606-1902) runs in the
 // elimination order with the k right-hand sides of a chunk interleaved ([row][k]): the operator is
-// held permuted (P A P^T), so the right-hand side is permuted in once, the solution out once, and
-// every round is forward sweep, backward sweep, residual -- no per-round permutation. Each level of
+// held permuted (P A P^T), so a right-hand side in the caller's layout is permuted in once and the
+// solution out once (fields stored by rows are already in that order), and every round is forward
+// sweep, backward sweep, residual -- no per-round permutation. Each level of
 // a sweep is a batch of dense tile products over every front of the level:
 //   forward : y(cols) = Z11 f(cols) and the front's update vector u = f(rows) + Z21 f(cols), where
 //             f = [r(cols); 0] + the children's update vectors, gathered as the tile loads it
@@ -88,27 +89,14 @@ __global__ __launch_bounds__(256) void perm_out_kernel(const double* __restrict_
   for (int q = qq; q < kk; q += 4) out[q * n + dst] = tile[ii][q];
 }
 
-// The same two permutations for fields stored by rows (FieldLayout::kRows: a chunk's kk columns
-// interleaved per vertex), where a vertex's kk entries are one contiguous run on both sides.
-__global__ __launch_bounds__(256) void perm_in_rows_kernel(const double* __restrict__ b,
-                                                           const int32_t* __restrict__ perm,
-                                                           const int32_t* __restrict__ pinned,
-                                                           int64_t n, int32_t kk,
-                                                           double* __restrict__ bp) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (e >= n * kk) return;
-  const int64_t i = e / kk, q = e % kk;
-  bp[e] = (pinned && pinned[i]) ? 0.0 : b[static_cast<int64_t>(perm[i]) * kk + q];
-}
-
-__global__ __launch_bounds__(256) void perm_out_rows_kernel(const double* __restrict__ x,
-                                                            const int32_t* __restrict__ perm,
-                                                            int64_t n, int32_t kk,
-                                                            double* __restrict__ out) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (e >= n * kk) return;
-  const int64_t i = e / kk, q = e % kk;
-  out[static_cast<int64_t>(perm[i]) * kk + q] = x[e];
+// Pinned rows of a right-hand side stored by rows: zero, the pins' values (a component's last
+// row, an unreferenced vertex) held there by the identity rows of the factor.
+__global__ __launch_bounds__(256) void zero_pinned_rows_kernel(const int32_t* __restrict__ pinned,
+                                                               int64_t n, int32_t kk,
+                                                               double* __restrict__ b) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n || !pinned[i]) return;
+  for (int32_t q = 0; q < kk; ++q) b[i * kk + q] = 0.0;
 }
 
 struct SweepArgs {
@@ -170,10 +158,11 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
   __shared__ double Ws[kInner][kTile + 1];  // [inner][row]
   __shared__ double Fs[kInner][kCols + 1];  // [inner][right-hand side]
-  const int32_t s = task_s[blockIdx.x];
-  const int32_t r0 = task_r0[blockIdx.x];
-  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
-  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t task = xcd_block();
+  const int32_t s = task_s[task];
+  const int32_t r0 = task_r0[task];
+  const int32_t k0 = task_k0[task], k1 = task_k1[task];
+  const int32_t slot = task_slot[task];
   const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
   const int32_t m = nc + a.nrow[s];
   const int32_t kk = a.kk;
@@ -284,8 +273,9 @@ __device__ __forceinline__ void add_partials(const double* __restrict__ partials
 __global__ __launch_bounds__(256) void fwd_reduce_kernel(
     const int32_t* __restrict__ grp_s, const int32_t* __restrict__ grp_r0,
     const int32_t* __restrict__ grp_first, const int32_t* __restrict__ grp_count, SweepArgs a) {
-  const int32_t s = grp_s[blockIdx.x], r0 = grp_r0[blockIdx.x];
-  const int32_t first = grp_first[blockIdx.x], count = grp_count[blockIdx.x];
+  const int32_t task = xcd_block();
+  const int32_t s = grp_s[task], r0 = grp_r0[task];
+  const int32_t first = grp_first[task], count = grp_count[task];
   const int32_t nc = a.ncol[s], m = nc + a.nrow[s], c0 = a.col_begin[s];
   const int32_t kk = a.kk;
   const int64_t vbase = a.vec_off[s];
@@ -332,10 +322,11 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
   __shared__ double Ws[kInner][kTile + 1];  // [inner][inverse-factor column]
   __shared__ double Gs[kInner][kCols + 1];  // [inner][right-hand side]
-  const int32_t s = task_s[blockIdx.x];
-  const int32_t j0 = task_j0[blockIdx.x];
-  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
-  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t task = xcd_block();
+  const int32_t s = task_s[task];
+  const int32_t j0 = task_j0[task];
+  const int32_t k0 = task_k0[task], k1 = task_k1[task];
+  const int32_t slot = task_slot[task];
   const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
   const int32_t m = nc + a.nrow[s];
   const int32_t kk = a.kk;
@@ -419,8 +410,9 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
 __global__ __launch_bounds__(256) void bwd_reduce_kernel(
     const int32_t* __restrict__ grp_s, const int32_t* __restrict__ grp_j0,
     const int32_t* __restrict__ grp_first, const int32_t* __restrict__ grp_count, SweepArgs a) {
-  const int32_t s = grp_s[blockIdx.x], j0 = grp_j0[blockIdx.x];
-  const int32_t first = grp_first[blockIdx.x], count = grp_count[blockIdx.x];
+  const int32_t task = xcd_block();
+  const int32_t s = grp_s[task], j0 = grp_j0[task];
+  const int32_t first = grp_first[task], count = grp_count[task];
   const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
   const int32_t kk = a.kk;
   constexpr int kRows = kTile / kReduceSplit;
@@ -458,10 +450,11 @@ __global__ __launch_bounds__(256) void fwd_tile1_kernel(
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
   __shared__ double Fs[256];
   __shared__ double part[4][kTile];
-  const int32_t s = task_s[blockIdx.x];
-  const int32_t r0 = task_r0[blockIdx.x];
-  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
-  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t task = xcd_block();
+  const int32_t s = task_s[task];
+  const int32_t r0 = task_r0[task];
+  const int32_t k0 = task_k0[task], k1 = task_k1[task];
+  const int32_t slot = task_slot[task];
   const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
   const int32_t m = nc + a.nrow[s];
   const TW* W = w + a.w_off[s];
@@ -504,10 +497,11 @@ __global__ __launch_bounds__(256) void bwd_tile1_kernel(
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
   __shared__ double part[kTile][5];
-  const int32_t s = task_s[blockIdx.x];
-  const int32_t j0 = task_j0[blockIdx.x];
-  const int32_t k0 = task_k0[blockIdx.x], k1 = task_k1[blockIdx.x];
-  const int32_t slot = task_slot[blockIdx.x];
+  const int32_t task = xcd_block();
+  const int32_t s = task_s[task];
+  const int32_t j0 = task_j0[task];
+  const int32_t k0 = task_k0[task], k1 = task_k1[task];
+  const int32_t slot = task_slot[task];
   const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
   const int32_t m = nc + a.nrow[s];
   const TW* W = w + a.w_off[s];
@@ -560,7 +554,7 @@ __global__ void residual_kernel(const int32_t* __restrict__ offsets,
                                 const double* __restrict__ bmax, int64_t n, int32_t kk,
                                 double tolerance, int32_t componentwise, double* __restrict__ r,
                                 int32_t* __restrict__ pending) {
-  int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t t = static_cast<int64_t>(xcd_block()) * blockDim.x + threadIdx.x;
   bool fail = false;
   if (t < n * kk) {
     const int64_t i =
@@ -595,7 +589,7 @@ __global__ void residual_kernel(const int32_t* __restrict__ offsets,
       scale += fabs(term);
     }
     const bool dropped = pinned && pinned[i];
-    r[t] = dropped ? 0.0 : total;
+    if (r) r[t] = dropped ? 0.0 : total;
     if (!componentwise) scale = fmax(scale, bmax[q]);
     fail = !dropped && !(fabs(total) <= tolerance * scale);
   }
@@ -782,7 +776,7 @@ void SolveWorkspace::reserve(const DevicePlan& plan, int64_t kk) {
 }
 
 int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp32,
-                      const double* b, double* x, int64_t k, double tolerance, bool componentwise,
+                      double* b, double* x, int64_t k, double tolerance, bool componentwise,
                       int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
                       hipStream_t stream, PhaseTimer* timer, int sweep_phase,
                       FieldLayout b_layout, FieldLayout x_layout) {
@@ -792,16 +786,15 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
   for (int64_t q0 = 0; q0 < k; q0 += kCols) {
     const int32_t kk = static_cast<int32_t>(std::min<int64_t>(kCols, k - q0));
     ws.reserve(plan, kk);
-    const double* bq = b + q0 * n;
+    double* bq = b + q0 * n;
     double* xq = x + q0 * n;
     const unsigned grid = blocks_for(n * kk);
     const unsigned row_tiles = static_cast<unsigned>((n + kTile - 1) / kTile);
-    if (b_layout == FieldLayout::kRows)
-      perm_in_rows_kernel<<<grid, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk,
-                                                    ws.bp.ptr);
-    else
-      perm_in_kernel<<<row_tiles, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk,
-                                                    ws.bp.ptr);
+    // The right-hand side in the elimination order with its pinned rows zero: b itself by rows
+    // (zeroed in place below, after the normwise scale has read it), else permuted into ws.bp.
+    double* bp = b_layout == FieldLayout::kRows ? bq : ws.bp.ptr;
+    if (b_layout == FieldLayout::kColumns)
+      perm_in_kernel<<<row_tiles, 256, 0, stream>>>(bq, plan.perm.ptr, pinned_new, n, kk, bp);
     double* bmax = ws.bmax.ptr + static_cast<int64_t>(blocks_for(n)) * kCols;
     if (!componentwise) {
       const unsigned blocks = blocks_for(n);
@@ -811,6 +804,8 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
         absmax_partial_kernel<<<dim3(blocks, kk), 256, 0, stream>>>(bq, n, ws.bmax.ptr);
       absmax_final_kernel<<<kk, 256, 0, stream>>>(ws.bmax.ptr, static_cast<int32_t>(blocks), bmax);
     }
+    if (b_layout == FieldLayout::kRows && pinned_new)
+      zero_pinned_rows_kernel<<<blocks_for(n), 256, 0, stream>>>(pinned_new, n, kk, bp);
     SweepArgs a{};
     a.col_begin = plan.col_begin.ptr;
     a.ncol = plan.ncol.ptr;
@@ -828,13 +823,14 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
     a.kk = kk;
     // The first round solves for the iterate itself from x = 0: its right-hand side is b and its
     // correction is written straight into the iterate. Later rounds correct it: d into ws.d and
-    // x + d into the other iterate buffer.
-    double* x_cur = ws.xb.ptr;
+    // x + d into the other iterate buffer. By rows the iterate lives in x itself (and ws.xa).
+    double* const x_home = x_layout == FieldLayout::kRows ? xq : ws.xb.ptr;
+    double* x_cur = x_home;
     int32_t rounds = 0;
     while (rounds < max_rounds) {
-      double* x_next = x_cur == ws.xb.ptr ? ws.xa.ptr : ws.xb.ptr;
+      double* x_next = x_cur == x_home ? ws.xa.ptr : x_home;
       if (rounds == 0) {
-        a.r = ws.bp.ptr;
+        a.r = bp;
         a.x = nullptr;
         a.x_next = nullptr;
         a.d = x_cur;
@@ -851,14 +847,17 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
       }
       if (rounds > 0) x_cur = x_next;
       ++rounds;
+      // Through FP64 factors the first round almost always passes, so its residual is formed for
+      // the test alone and written (a second pass) only when another round follows.
+      const bool test_only = !use_fp32 && rounds == 1;
+      auto residual = kk == kCols ? residual_kernel<true> : residual_kernel<false>;
       {
         PhaseScope scope(timer, kPhaseResidual, stream);
         HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
-        auto residual = kk == kCols ? residual_kernel<true> : residual_kernel<false>;
         residual<<<grid, 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
-                                           factor.pvalues.ptr, ws.bp.ptr, x_cur, pinned_new, bmax,
-                                           n, kk, tolerance, componentwise ? 1 : 0, ws.r.ptr,
-                                           ws.pending.ptr);
+                                           factor.pvalues.ptr, bp, x_cur, pinned_new, bmax,
+                                           n, kk, tolerance, componentwise ? 1 : 0,
+                                           test_only ? nullptr : ws.r.ptr, ws.pending.ptr);
       }
       HG_HIP(hipGetLastError());
       int32_t pending = 0;
@@ -867,11 +866,19 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
       HG_HIP(hipStreamSynchronize(stream));
       if (timer) timer->resolve();
       if (!pending) break;
+      if (test_only && rounds < max_rounds) {
+        PhaseScope scope(timer, kPhaseResidual, stream);
+        residual<<<grid, 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
+                                           factor.pvalues.ptr, bp, x_cur, pinned_new, bmax,
+                                           n, kk, tolerance, componentwise ? 1 : 0, ws.r.ptr,
+                                           ws.pending.ptr);
+        HG_HIP(hipGetLastError());
+      }
     }
-    if (x_layout == FieldLayout::kRows)
-      perm_out_rows_kernel<<<grid, 256, 0, stream>>>(x_cur, plan.perm.ptr, n, kk, xq);
-    else
+    if (x_layout == FieldLayout::kColumns)
       perm_out_kernel<<<row_tiles, 256, 0, stream>>>(x_cur, plan.perm.ptr, n, kk, xq);
+    else if (x_cur != xq)
+      HG_HIP(hipMemcpyAsync(xq, x_cur, sizeof(double) * n * kk, hipMemcpyDeviceToDevice, stream));
     HG_HIP(hipGetLastError());
     max_used = std::max(max_used, rounds);
   }
