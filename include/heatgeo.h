@@ -136,7 +136,7 @@ int hg_set_refinement(hg_mesh* mesh, int32_t precision, double heat_tolerance,
 
 /*
  * Tunables of the sparse Cholesky (0 keeps the default): the nested dissection's leaf size
- * (default 48 rows) and top merge (512 rows; ordito's leaf 32 / top 512, cholesky.py:538-660),
+ * (default 128 rows) and top merge (512 rows; ordito's leaf 32 / top 512, cholesky.py:538-660),
  * and the task count a sweep level's products are split towards (1024). A new leaf or top size
  * drops the plan and every factor (the next solve analyses and factors again); a new split target
  * only rebuilds the sweep tasks. The autotuner (ordito_amd/tune.py) times candidates through it.

@@ -140,8 +140,10 @@ struct Plan {
   int32_t max_front = 0;
 };
 
+// Leaf 128: on the 1024 x 512 torus with 64 right-hand sides, leaves of 96-192 rows serve fastest
+// (fewer, fuller levels), 128 by a small margin over 48 (3%) and 512 (16%).
 struct AnalysisOptions {
-  int32_t leaf_rows = 48;
+  int32_t leaf_rows = 128;
   int32_t top_rows = 512;
 };
 
