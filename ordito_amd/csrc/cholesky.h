@@ -12,6 +12,7 @@ struct DevicePlan {
   int32_t n_super = 0;
   int32_t n_levels = 0;
   std::vector<int32_t> level_ptr;  // host copies: the launches are per level
+  std::vector<int32_t> level_gathers;  // 1 where some front of the level has children
   std::vector<int32_t> host_col_begin, host_ncol, host_nrow;
   std::vector<int64_t> host_front_off;
   int64_t arena_size = 0, w_size = 0, vec_size = 0, n_entries = 0, nnz_w = 0;

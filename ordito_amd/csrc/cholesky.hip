@@ -618,6 +618,12 @@ void DevicePlan::upload(const Plan& plan, const HostPattern& pattern, int32_t ta
   n_super = plan.n_super;
   n_levels = plan.n_levels;
   level_ptr = plan.level_ptr;
+  level_gathers.assign(plan.n_levels, 0);
+  for (int32_t level = 0; level < plan.n_levels; ++level)
+    for (int32_t k = plan.level_ptr[level]; k < plan.level_ptr[level + 1]; ++k) {
+      const int32_t s = plan.level_nodes[k];
+      if (plan.child_ptr[s + 1] > plan.child_ptr[s]) level_gathers[level] = 1;
+    }
   arena_size = plan.arena_size;
   w_size = plan.w_size;
   vec_size = plan.vec_size;

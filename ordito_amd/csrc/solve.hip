@@ -30,26 +30,6 @@ __device__ __forceinline__ double widen(T v) {
   return static_cast<double>(v);
 }
 
-// Gathered front-vector entry: base plus the children's update rows adding into this position, in
-// ascending child order. The first two sources sit inline in gather_pair (-1: none; -2 in the
-// second: more than two, read on from the lists after the first).
-__device__ __forceinline__ double gathered(double base, int64_t row, int32_t q, int32_t kk,
-                                           const int32_t* __restrict__ gather_pair,
-                                           const int32_t* __restrict__ gather_ptr,
-                                           const int32_t* __restrict__ gather_src,
-                                           const double* __restrict__ fvec) {
-  const int2 p = reinterpret_cast<const int2*>(gather_pair)[row];
-  double v = base;
-  if (p.x >= 0) v += fvec[static_cast<int64_t>(p.x) * kk + q];
-  if (p.y >= 0) {
-    v += fvec[static_cast<int64_t>(p.y) * kk + q];
-  } else if (p.y == -2) {
-    for (int32_t g = gather_ptr[row] + 1; g < gather_ptr[row + 1]; ++g)
-      v += fvec[static_cast<int64_t>(gather_src[g]) * kk + q];
-  }
-  return v;
-}
-
 // b (user layout, column q at q*n) -> bp[i*kk + q] in elimination order through an LDS transpose
 // of 64 rows; a pinned row's equation is dropped (zero).
 __global__ __launch_bounds__(256) void perm_in_kernel(const double* __restrict__ b,
@@ -118,7 +98,51 @@ struct SweepArgs {
   double* x_next;    // x + d; null on the first round
   double* partials;
   int32_t kk;
+  int32_t gathers;   // the level's fronts have children (forward sweep): f gathers their updates
 };
+
+// One gathered entry: v plus the children's update rows adding into front-vector row `row`,
+// column q, in ascending child order (the sources as in gather4).
+__device__ __forceinline__ double gather1(const SweepArgs& a, int64_t row, int q, double v) {
+  const int2 p = reinterpret_cast<const int2*>(a.gather_pair)[row];
+  if (p.x >= 0) v += a.fvec[static_cast<int64_t>(p.x) * a.kk + q];
+  if (p.y >= 0) {
+    v += a.fvec[static_cast<int64_t>(p.y) * a.kk + q];
+  } else if (p.y == -2) {
+    for (int32_t g = a.gather_ptr[row] + 1; g < a.gather_ptr[row + 1]; ++g)
+      v += a.fvec[static_cast<int64_t>(a.gather_src[g]) * a.kk + q];
+  }
+  return v;
+}
+
+// Adds to v[u] the children's update rows that add into front-vector row row[u], column q[u] (u < 4;
+// live[u] false adds nothing), in ascending child order. The first two sources of a row sit inline
+// in gather_pair (-1: none; -2 in the second: more than two, the rest read from the lists after the
+// first). Every pair is loaded, then every update row, then the sums: two waits on memory for the
+// four entries, not two per entry. Absent sources read row 0 and are not added.
+__device__ __forceinline__ void gather4(const SweepArgs& a, const int64_t row[4], const int q[4],
+                                        const bool live[4], double v[4]) {
+  const int2* pairs = reinterpret_cast<const int2*>(a.gather_pair);
+  int2 p[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) p[u] = live[u] ? pairs[row[u]] : make_int2(-1, -1);
+  double first[4], second[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    first[u] = a.fvec[p[u].x >= 0 ? static_cast<int64_t>(p[u].x) * a.kk + q[u] : 0];
+    second[u] = a.fvec[p[u].y >= 0 ? static_cast<int64_t>(p[u].y) * a.kk + q[u] : 0];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (p[u].x >= 0) v[u] += first[u];
+    if (p[u].y >= 0) {
+      v[u] += second[u];
+    } else if (p[u].y == -2) {
+      for (int32_t g = a.gather_ptr[row[u]] + 1; g < a.gather_ptr[row[u] + 1]; ++g)
+        v[u] += a.fvec[static_cast<int64_t>(a.gather_src[g]) * a.kk + q[u]];
+    }
+  }
+}
 
 // The tile products run on the FP64 matrix cores: v_mfma_f64_16x16x4 takes a 16 x 4 block of A
 // (lane l holds A[l % 16][l / 16]) and a 4 x 16 block of B (lane l holds B[l / 16][l % 16]) and
@@ -173,7 +197,12 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
   const bool rows_live = r0 + 16 * wave < m;  // wave-uniform
   // Staging: entry e = t + 256u of a stage is W row e % 64 / f column e % 64 at inner e / 64.
   double wreg[4], freg[4];
+  // f = r(cols) plus the children's updates: every W and r load of a stage issues together, then
+  // the gathers (gather4).
   auto load = [&](int32_t j0) {
+    int64_t grow[4];
+    int gq[4];
+    bool glive[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
@@ -181,11 +210,12 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
       const int32_t i = r0 + ii, j = j0 + jj;
       wreg[u] = (i < m && j < k1) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
       const int q = e % kCols;
-      freg[u] = (q < kk && j < k1)
-                    ? gathered(a.r[static_cast<int64_t>(c0 + j) * kk + q], vbase + j, q, kk,
-                               a.gather_pair, a.gather_ptr, a.gather_src, a.fvec)
-                    : 0.0;
+      glive[u] = q < kk && j < k1;
+      freg[u] = glive[u] ? a.r[static_cast<int64_t>(c0 + j) * kk + q] : 0.0;
+      grow[u] = vbase + j;
+      gq[u] = q;
     }
+    if (a.gathers) gather4(a, grow, gq, glive, freg);
   };
   f64x4 acc[4];
 #pragma unroll
@@ -209,18 +239,20 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
   // A lane's outputs are rows lane / 16 + 4v of its wave's block; the sums keep the gather's order
   // (0, then each child in ascending order, then the product).
   double base[4][4] = {};
-  if (slot < 0) {
+  if (slot < 0 && a.gathers) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int32_t i = r0 + 16 * wave + lane / 16 + 4 * v;
-      if (i < nc || i >= m) continue;
+    for (int n = 0; n < 4; ++n) {
+      int64_t grow[4];
+      int gq[4];
+      bool glive[4];
 #pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        const int q = 16 * n + lane % 16;
-        if (q < kk)
-          base[n][v] = gathered(0.0, vbase + i, q, kk, a.gather_pair, a.gather_ptr, a.gather_src,
-                                a.fvec);
+      for (int v = 0; v < 4; ++v) {
+        const int32_t i = r0 + 16 * wave + lane / 16 + 4 * v;
+        grow[v] = vbase + i;
+        gq[v] = 16 * n + lane % 16;
+        glive[v] = i >= nc && i < m && gq[v] < kk;
       }
+      gather4(a, grow, gq, glive, base[n]);
     }
   }
 #pragma unroll
@@ -285,22 +317,21 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
   // At most kRows * kCols = 4 x 256 entries: each thread's four, every load before any store.
   // Entries past the group's rows read slot offset 0 and store nothing.
   double sum[4] = {}, base[4] = {};
-  int64_t off[4];
+  int64_t off[4], grow[4];
+  int gq[4];
+  bool glive[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int32_t e = threadIdx.x + 256 * u;
-    off[u] = e < rows * kk ? static_cast<int64_t>(lo + e / kk) * kk + e % kk : 0;
+    const bool live = e < rows * kk;
+    const int32_t ii = live ? lo + e / kk : 0;
+    off[u] = live ? static_cast<int64_t>(ii) * kk + e % kk : 0;
+    grow[u] = vbase + r0 + ii;
+    gq[u] = live ? e % kk : 0;
+    glive[u] = live && r0 + ii >= nc;
   }
+  if (a.gathers) gather4(a, grow, gq, glive, base);
   add_partials(a.partials, static_cast<int64_t>(kTile) * kk, first, count, off, sum);
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int32_t e = threadIdx.x + 256 * u;
-    if (e >= rows * kk) continue;
-    const int32_t ii = lo + e / kk, q = e % kk;
-    if (r0 + ii >= nc)
-      base[u] = gathered(0.0, vbase + r0 + ii, q, kk, a.gather_pair, a.gather_ptr, a.gather_src,
-                         a.fvec);
-  }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int32_t e = threadIdx.x + 256 * u;
@@ -338,21 +369,26 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
   // Staging: entry e = t + 256u is inverse-factor column e / 16 at inner e % 16 (contiguous in
   // memory), and g's right-hand side e % 64 at inner e / 64.
   double wreg[4], greg[4];
+  // Every W load and structure-row index of a stage issues together, then every g load.
   auto load = [&](int32_t i0) {
+    int32_t srow[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
       const int ii = e % kInner, jj = e / kInner;
       const int32_t i = i0 + ii, j = j0 + jj;
       wreg[u] = (i < k1 && j < nc) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
+      const int32_t ig = i0 + e / kCols;
+      srow[u] = (ig >= nc && ig < k1) ? srows[ig - nc] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
       const int q = e % kCols;
       const int32_t ig = i0 + e / kCols;
-      double g = 0.0;
-      if (q < kk && ig < k1) {
-        g = ig < nc ? a.y[static_cast<int64_t>(c0 + ig) * kk + q]
-                    : a.d[static_cast<int64_t>(srows[ig - nc]) * kk + q];
-      }
-      greg[u] = g;
+      const double* src = ig < nc ? a.y + static_cast<int64_t>(c0 + ig) * kk
+                                  : a.d + static_cast<int64_t>(srow[u]) * kk;
+      greg[u] = (q < kk && ig < k1) ? src[q] : 0.0;
     }
   };
   f64x4 acc[4];
@@ -465,9 +501,7 @@ __global__ __launch_bounds__(256) void fwd_tile1_kernel(
   for (int32_t j0 = k0; j0 < k1; j0 += 256) {
     const int32_t j = j0 + threadIdx.x;
     Fs[threadIdx.x] =
-        j < k1 ? gathered(a.r[c0 + j], vbase + j, 0, 1, a.gather_pair, a.gather_ptr, a.gather_src,
-                          a.fvec)
-               : 0.0;
+        j < k1 ? (a.gathers ? gather1(a, vbase + j, 0, a.r[c0 + j]) : a.r[c0 + j]) : 0.0;
     __syncthreads();
     const int32_t jend = min(k1 - j0, 256);
     if (i < m)
@@ -486,7 +520,7 @@ __global__ __launch_bounds__(256) void fwd_tile1_kernel(
     } else {
       const int64_t row = vbase + i;
       a.fvec[row] =
-          gathered(0.0, row, 0, 1, a.gather_pair, a.gather_ptr, a.gather_src, a.fvec) + sum;
+          (a.gathers ? gather1(a, row, 0, 0.0) : 0.0) + sum;
     }
   }
 }
@@ -709,6 +743,7 @@ void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream)
     const int32_t t0 = F.level_ptr[level];
     const int32_t tasks = F.level_ptr[level + 1] - t0;
     if (tasks == 0) continue;
+    a.gathers = plan.level_gathers[level];
     if (a.kk == 1) {
       fwd_tile1_kernel<TW><<<tasks, 256, 0, stream>>>(F.s.ptr + t0, F.tile.ptr + t0,
                                                       F.k0.ptr + t0, F.k1.ptr + t0,
@@ -725,6 +760,7 @@ void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream)
           F.rs.ptr + r0, F.rtile.ptr + r0, F.rfirst.ptr + r0, F.rcount.ptr + r0, a);
     }
   }
+  a.gathers = 0;
   for (int32_t level = plan.n_levels - 1; level >= 0; --level) {
     const int32_t t0 = B.level_ptr[level];
     const int32_t tasks = B.level_ptr[level + 1] - t0;
