@@ -102,7 +102,7 @@ struct SweepArgs {
 };
 
 // One gathered entry: v plus the children's update rows adding into front-vector row `row`,
-// column q, in ascending child order (the sources as in gather4).
+// column q, in ascending child order (the sources as in gather_pairs).
 __device__ __forceinline__ double gather1(const SweepArgs& a, int64_t row, int q, double v) {
   const int2 p = reinterpret_cast<const int2*>(a.gather_pair)[row];
   if (p.x >= 0) v += a.fvec[static_cast<int64_t>(p.x) * a.kk + q];
@@ -115,17 +115,21 @@ __device__ __forceinline__ double gather1(const SweepArgs& a, int64_t row, int q
   return v;
 }
 
-// Adds to v[u] the children's update rows that add into front-vector row row[u], column q[u] (u < 4;
-// live[u] false adds nothing), in ascending child order. The first two sources of a row sit inline
-// in gather_pair (-1: none; -2 in the second: more than two, the rest read from the lists after the
-// first). Every pair is loaded, then every update row, then the sums: two waits on memory for the
-// four entries, not two per entry. Absent sources read row 0 and are not added.
-__device__ __forceinline__ void gather4(const SweepArgs& a, const int64_t row[4], const int q[4],
-                                        const bool live[4], double v[4]) {
+// The children's update rows that add into front-vector rows row[u] (u < 4), in two steps so a
+// caller can issue other loads between them: gather_pairs loads each row's first two sources
+// (inline in gather_pair; -1: none; -2 in the second: more than two, the rest in the lists after
+// the first; live[u] false: none), gather_add loads every update row of column q[u] and then adds
+// them to v[u] in ascending child order. Two waits on memory for four entries, not two per entry.
+// Absent sources read row 0 and are not added.
+__device__ __forceinline__ void gather_pairs(const SweepArgs& a, const int64_t row[4],
+                                             const bool live[4], int2 p[4]) {
   const int2* pairs = reinterpret_cast<const int2*>(a.gather_pair);
-  int2 p[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) p[u] = live[u] ? pairs[row[u]] : make_int2(-1, -1);
+}
+
+__device__ __forceinline__ void gather_add(const SweepArgs& a, const int64_t row[4],
+                                           const int q[4], const int2 p[4], double v[4]) {
   double first[4], second[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -195,27 +199,46 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
   const int t = threadIdx.x, wave = t / 64, lane = t % 64;
   const int blocks = (kk + 15) / 16;
   const bool rows_live = r0 + 16 * wave < m;  // wave-uniform
+  // The epilogue's gather pairs (this lane's update rows lane / 16 + 4v), loaded before the stages.
+  int2 epair[4];
+  {
+    int64_t erow[4];
+    bool elive[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int32_t i = r0 + 16 * wave + lane / 16 + 4 * v;
+      erow[v] = vbase + i;
+      elive[v] = a.gathers && slot < 0 && i >= nc && i < m;
+    }
+    gather_pairs(a, erow, elive, epair);
+  }
   // Staging: entry e = t + 256u of a stage is W row e % 64 / f column e % 64 at inner e / 64.
   double wreg[4], freg[4];
-  // f = r(cols) plus the children's updates: every W and r load of a stage issues together, then
-  // the gathers (gather4).
+  // f = r(cols) plus the children's updates: a stage's gather pairs issue first, then its W and r
+  // loads, then (once the pairs are in, the rest still in flight) the update rows.
   auto load = [&](int32_t j0) {
     int64_t grow[4];
     int gq[4];
     bool glive[4];
+    int2 pair[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = t + 256 * u;
+      const int32_t j = j0 + e / kTile;
+      gq[u] = e % kCols;
+      glive[u] = gq[u] < kk && j < k1;
+      grow[u] = vbase + j;
+    }
+    if (a.gathers) gather_pairs(a, grow, glive, pair);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
       const int ii = e % kTile, jj = e / kTile;
       const int32_t i = r0 + ii, j = j0 + jj;
       wreg[u] = (i < m && j < k1) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
-      const int q = e % kCols;
-      glive[u] = q < kk && j < k1;
-      freg[u] = glive[u] ? a.r[static_cast<int64_t>(c0 + j) * kk + q] : 0.0;
-      grow[u] = vbase + j;
-      gq[u] = q;
+      freg[u] = glive[u] ? a.r[static_cast<int64_t>(c0 + j) * kk + gq[u]] : 0.0;
     }
-    if (a.gathers) gather4(a, grow, gq, glive, freg);
+    if (a.gathers) gather_add(a, grow, gq, pair, freg);
   };
   f64x4 acc[4];
 #pragma unroll
@@ -242,17 +265,15 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
   if (slot < 0 && a.gathers) {
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      int64_t grow[4];
+      if (16 * n + lane % 16 >= kk) continue;
+      int64_t erow[4];
       int gq[4];
-      bool glive[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int32_t i = r0 + 16 * wave + lane / 16 + 4 * v;
-        grow[v] = vbase + i;
+        erow[v] = vbase + r0 + 16 * wave + lane / 16 + 4 * v;
         gq[v] = 16 * n + lane % 16;
-        glive[v] = i >= nc && i < m && gq[v] < kk;
       }
-      gather4(a, grow, gq, glive, base[n]);
+      gather_add(a, erow, gq, epair, base[n]);
     }
   }
 #pragma unroll
@@ -330,8 +351,10 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
     gq[u] = live ? e % kk : 0;
     glive[u] = live && r0 + ii >= nc;
   }
-  if (a.gathers) gather4(a, grow, gq, glive, base);
+  int2 pair[4];
+  if (a.gathers) gather_pairs(a, grow, glive, pair);
   add_partials(a.partials, static_cast<int64_t>(kTile) * kk, first, count, off, sum);
+  if (a.gathers) gather_add(a, grow, gq, pair, base);
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int32_t e = threadIdx.x + 256 * u;
@@ -369,17 +392,21 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
   // Staging: entry e = t + 256u is inverse-factor column e / 16 at inner e % 16 (contiguous in
   // memory), and g's right-hand side e % 64 at inner e / 64.
   double wreg[4], greg[4];
-  // Every W load and structure-row index of a stage issues together, then every g load.
+  // A stage's structure-row indices issue first, then its W loads, then (once the indices are
+  // in, W still in flight) its g loads.
   auto load = [&](int32_t i0) {
     int32_t srow[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int32_t ig = i0 + (t + 256 * u) / kCols;
+      srow[u] = (ig >= nc && ig < k1) ? srows[ig - nc] : 0;
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
       const int ii = e % kInner, jj = e / kInner;
       const int32_t i = i0 + ii, j = j0 + jj;
       wreg[u] = (i < k1 && j < nc) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
-      const int32_t ig = i0 + e / kCols;
-      srow[u] = (ig >= nc && ig < k1) ? srows[ig - nc] : 0;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
