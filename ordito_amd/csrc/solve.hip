@@ -179,8 +179,11 @@ __device__ __forceinline__ void stage_products(const double (*As)[kTile + 1],
 // Forward tile product for up to 64 right-hand sides: rows [r0, r0+64) of the front's inverse
 // factor times f over the inner range [k0, k1), f gathered as it is staged. slot < 0 applies the
 // product; otherwise it is stored as a partial for the reduction.
-template <typename TW>
-__global__ __launch_bounds__(256) void fwd_tile_kernel(
+// kGathers: the level's fronts have children whose update vectors f gathers (a.gathers); the
+// leaves' instance carries none of that code, and fits 80 registers: six waves a SIMD instead of
+// four.
+template <typename TW, bool kGathers>
+__global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
     const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_r0,
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
@@ -208,7 +211,7 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
     for (int v = 0; v < 4; ++v) {
       const int32_t i = r0 + 16 * wave + lane / 16 + 4 * v;
       erow[v] = vbase + i;
-      elive[v] = a.gathers && slot < 0 && i >= nc && i < m;
+      elive[v] = kGathers && slot < 0 && i >= nc && i < m;
     }
     gather_pairs(a, erow, elive, epair);
   }
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
       glive[u] = gq[u] < kk && j < k1;
       grow[u] = vbase + j;
     }
-    if (a.gathers) gather_pairs(a, grow, glive, pair);
+    if (kGathers) gather_pairs(a, grow, glive, pair);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
       wreg[u] = (i < m && j < k1) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
       freg[u] = glive[u] ? a.r[static_cast<int64_t>(c0 + j) * kk + gq[u]] : 0.0;
     }
-    if (a.gathers) gather_add(a, grow, gq, pair, freg);
+    if (kGathers) gather_add(a, grow, gq, pair, freg);
   };
   f64x4 acc[4];
 #pragma unroll
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(256) void fwd_tile_kernel(
   // A lane's outputs are rows lane / 16 + 4v of its wave's block; the sums keep the gather's order
   // (0, then each child in ascending order, then the product).
   double base[4][4] = {};
-  if (slot < 0 && a.gathers) {
+  if (kGathers && slot < 0) {
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       if (16 * n + lane % 16 >= kk) continue;
@@ -776,9 +779,9 @@ void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream)
                                                       F.k0.ptr + t0, F.k1.ptr + t0,
                                                       F.slot.ptr + t0, w, a);
     } else {
-      fwd_tile_kernel<TW><<<tasks, 256, 0, stream>>>(F.s.ptr + t0, F.tile.ptr + t0,
-                                                     F.k0.ptr + t0, F.k1.ptr + t0,
-                                                     F.slot.ptr + t0, w, a);
+      auto tile = a.gathers ? fwd_tile_kernel<TW, true> : fwd_tile_kernel<TW, false>;
+      tile<<<tasks, 256, 0, stream>>>(F.s.ptr + t0, F.tile.ptr + t0, F.k0.ptr + t0, F.k1.ptr + t0,
+                                      F.slot.ptr + t0, w, a);
     }
     const int32_t r0 = F.reduce_ptr[level];
     const int32_t groups = F.reduce_ptr[level + 1] - r0;
