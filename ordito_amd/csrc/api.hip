@@ -66,6 +66,9 @@ struct hg_mesh {
   // Components: the Poisson system pins each component's last-eliminated row.
   int32_t n_comp = 0;
   std::vector<int32_t> labels;
+  std::vector<int32_t> comp_ptr_host;  // components' member ranges in label order
+  DeviceArray<int32_t> d_labels;
+  bool rows_by_component = false;      // every component's rows contiguous in the elimination order
   DeviceArray<int32_t> comp_ptr, comp_members;
   DeviceArray<int32_t> poisson_pinned_new, poisson_dropped_old, heat_pinned_new, heat_dropped_old;
   std::vector<int64_t> poisson_pin_slots, heat_pin_slots;
@@ -268,6 +271,12 @@ void ensure_plan(hg_mesh& m) {
     for (size_t e = 0; e < rows.size(); ++e) rows[e] = plan.iperm[m.h_faces[e]];
     m.faces_rows.upload(rows, m.stream);
   }
+  // The distance's finishing pass reads components as row ranges of the elimination order.
+  m.rows_by_component = true;
+  for (int64_t v = 0; v < m.n && m.rows_by_component; ++v) {
+    const int32_t c = m.labels[v], row = plan.iperm[v];
+    m.rows_by_component = row >= m.comp_ptr_host[c] && row < m.comp_ptr_host[c + 1];
+  }
   // Pins: the Poisson system's one row per component, its last-eliminated vertex
   // (kernels/cholesky.py:195-214); the heat system's empty rows (unreferenced vertices).
   const int64_t n = m.n;
@@ -415,9 +424,11 @@ double poisson_bound(hg_mesh& m) {
   return m.poisson_bound;
 }
 
-// The solution is stored by columns; the right-hand side may be stored by rows (factored only).
+// The right-hand side and the solution may be stored by rows (factored only). A solution stored by
+// columns has each component's mean removed here; one stored by rows is left for finish_rows.
 void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
-                   int32_t precision, FieldLayout rhs_layout = FieldLayout::kColumns) {
+                   int32_t precision, FieldLayout rhs_layout = FieldLayout::kColumns,
+                   FieldLayout x_layout = FieldLayout::kColumns) {
   ensure_operators(m);
   const bool factored = mode == HG_MODE_FACTORED || m.poisson_factor.held() || m.heat_factor.held();
   if (factored) {
@@ -426,11 +437,11 @@ void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
     m.stats.poisson_refinements = refined_solve(
         *m.plan, m.poisson_factor, mixed, rhs, x, k, m.refine_tolerance[mixed ? 1 : 0][1], false,
         mixed ? kMixedRefineRounds : kRefineRounds, m.poisson_pinned_new.ptr, m.solve, m.stream,
-        &m.timer, kPhasePoissonSweep, rhs_layout, FieldLayout::kColumns);
+        &m.timer, kPhasePoissonSweep, rhs_layout, x_layout);
     m.stats.poisson_rounds = 0;
   } else {
-    HG_REQUIRE(rhs_layout == FieldLayout::kColumns, HG_ERR_RUNTIME,
-               "an iterated solve takes columns");
+    HG_REQUIRE(rhs_layout == FieldLayout::kColumns && x_layout == FieldLayout::kColumns,
+               HG_ERR_RUNTIME, "an iterated solve takes columns");
     // The Jacobi-Chebyshev interval (linalg.py:6074-6090): [max(min(80/n, 0.5), 1 - r), 1 + r],
     // r the largest Gershgorin ratio of D^-1 A, at least 1e-3.
     double r = std::max(poisson_bound(m), 1e-3);
@@ -459,8 +470,9 @@ void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
     m.stats.poisson_rounds = rounds;
     m.stats.poisson_refinements = 0;
   }
-  remove_null_space(m.null_space, m.comp_ptr.ptr, m.comp_members.ptr, m.n, k, x, m.null_partials,
-                    m.stream);
+  if (x_layout == FieldLayout::kColumns)
+    remove_null_space(m.null_space, m.comp_ptr.ptr, m.comp_members.ptr, m.n, k, x, m.null_partials,
+                      m.stream);
 }
 
 void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets, int64_t k) {
@@ -521,9 +533,12 @@ void shift(hg_mesh& m, double* phi, int64_t k) {
 }
 
 // Seed, heat step, divergence and Poisson step over k columns of the whole (possibly packed)
-// system: the unshifted distance into phi. The sources must be uploaded.
-void solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, int32_t mode,
-                   int32_t precision) {
+// system into phi (the caller's layout). The sources must be uploaded. With `finish` and the
+// fields stored by rows, phi is the finished distance (null space removed, shifted, oriented) and
+// the call returns true; otherwise phi is the unshifted distance with each component's mean
+// removed.
+bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, int32_t mode,
+                   int32_t precision, bool finish = false) {
   const int64_t n = m.n;
   double* fields = reserve_fields(m, k);
   double* u0 = fields;
@@ -543,7 +558,17 @@ void solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   int32_t flags = diffuse(m, u0, heat, k, mode, precision, layout);
   m.stats.backward_error_failed = flags & 1;
   divergence(m, heat, negdiv, k, layout);
+  if (finish && layout == FieldLayout::kRows && m.rows_by_component) {
+    // The heat field is spent once the divergence has read it: its buffer takes the Poisson
+    // solution, which one pass turns into the distance in the caller's layout.
+    solve_poisson(m, negdiv, heat, k, mode, precision, layout, FieldLayout::kRows);
+    PhaseScope scope(&m.timer, kPhaseShift, m.stream);
+    finish_rows(*m.plan, m.null_space, m.comp_ptr.ptr, m.d_labels.ptr, heat, k, m.d_sources.ptr,
+                m.d_offsets.ptr, phi, m.null_partials, m.stream);
+    return true;
+  }
   solve_poisson(m, negdiv, phi, k, mode, precision, layout);
+  return false;
 }
 
 // The host and device state of a mesh from its host positions and faces (global indices).
@@ -568,6 +593,8 @@ void bind(hg_mesh& m) {
   m.vertex_ptr.upload(m.pattern.vertex_ptr, st);
   m.vertex_corners.upload(m.pattern.vertex_corners, st);
   m.comp_ptr.upload(comp_ptr, st);
+  m.comp_ptr_host = comp_ptr;
+  m.d_labels.upload(m.labels, st);
   m.null_space.build(comp_ptr, st);
   m.comp_members.upload(members, st);
   if (m.segments() > 1) {
@@ -852,8 +879,7 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     }
     ensure_operators(m);
     upload_sources(m, d_sources, source_offsets, k);
-    solve_columns(m, source_offsets, k, d_out, mode, precision);
-    shift(m, d_out, k);
+    if (!solve_columns(m, source_offsets, k, d_out, mode, precision, true)) shift(m, d_out, k);
     HG_HIP(hipStreamSynchronize(m.stream));
     m.timer.resolve();
     if (out_stats) *out_stats = m.stats;

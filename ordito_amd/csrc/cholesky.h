@@ -105,4 +105,14 @@ void remove_null_space(const NullSpace& ns, const int32_t* comp_ptr, const int32
                        int64_t n, int64_t k, double* x, DeviceArray<double>& scratch,
                        hipStream_t stream);
 
+// The distance from Poisson solutions stored by rows (k columns, FieldLayout::kRows): each
+// component's mean removed, then each column shifted to the mean over its sources (sources
+// [source_offsets[q], source_offsets[q + 1]) in the caller's numbering) and oriented, written in
+// the caller's layout. Every component's rows must be contiguous in the elimination order, in
+// component order (comp_ptr), which the analysis guarantees; labels: each vertex's component.
+void finish_rows(const DevicePlan& plan, const NullSpace& ns, const int32_t* comp_ptr,
+                 const int32_t* labels, const double* x, int64_t k, const int32_t* sources,
+                 const int64_t* source_offsets, double* out, DeviceArray<double>& scratch,
+                 hipStream_t stream);
+
 }  // namespace hg

@@ -371,8 +371,9 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
 
 // Backward tile product: inverse-factor columns [j0, j0+64) transposed, times
 // g = [y(cols); d(rows)] over the inner range [k0, k1); each wave owns 16 of the columns. Applied
-// results also give x_next = x + d.
-template <typename TW>
+// results also give x_next = x + d when kIterate (a later round; the first round's d is the
+// iterate itself, and its instance holds no iterate registers).
+template <typename TW, bool kIterate>
 __global__ __launch_bounds__(256) void bwd_tile_kernel(
     const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_j0,
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
@@ -440,8 +441,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
   if (!cols_live) return;
   // Epilogue: the iterate's entries are all loaded before any store (see the forward tile).
   double xv[4][4] = {};
-  const bool iterate = a.x != nullptr;  // first round: x = 0 and d is the next iterate itself
-  if (slot < 0 && iterate) {
+  if (kIterate && slot < 0) {
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int q = 16 * n + lane % 16;
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
       } else {
         const int64_t at = static_cast<int64_t>(c0 + j) * kk + q;
         a.d[at] = value;
-        if (iterate) a.x_next[at] = xv[n][v] + value;
+        if (kIterate) a.x_next[at] = xv[n][v] + value;
       }
     }
   }
@@ -763,6 +763,118 @@ __global__ void component_shift_kernel(const int32_t* __restrict__ chunk_comp,
   if (e < comp_ptr[c + 1]) x[q * n + members[e]] -= means[q * n_comp + c];
 }
 
+// ---- the distance from a Poisson solution stored by rows ------------------------------------
+// Every component's rows are contiguous in the elimination order (each component is dissected on
+// its own and numbered in component order), so the null-space chunks (256 members of one
+// component) are row ranges too: chunk k covers rows [chunk_first[k], min(+256, comp end)).
+
+// Column sums over each chunk: thread (q, part) adds rows part, part + 4, ... (four loads issued
+// before their adds), then the four parts add in order. partials[q * n_chunks + k], the layout
+// component_mean_kernel reads.
+__global__ __launch_bounds__(256) void rows_partial_kernel(const int32_t* __restrict__ chunk_comp,
+                                                           const int32_t* __restrict__ chunk_first,
+                                                           const int32_t* __restrict__ comp_ptr,
+                                                           const double* __restrict__ x,
+                                                           int32_t kk,
+                                                           double* __restrict__ partials) {
+  __shared__ double part_sum[4][kCols];
+  const int32_t chunk = blockIdx.x;
+  const int32_t first = chunk_first[chunk];
+  const int32_t end = min(first + 256, comp_ptr[chunk_comp[chunk] + 1]);
+  const int q = threadIdx.x % kCols, part = threadIdx.x / kCols;
+  double sum = 0.0;
+  if (q < kk) {
+    int32_t r = first + part;
+    for (; r + 12 < end; r += 16) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = x[static_cast<int64_t>(r + 4 * u) * kk + q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) sum += v[u];
+    }
+    for (; r < end; r += 4) sum += x[static_cast<int64_t>(r) * kk + q];
+  }
+  part_sum[part][q] = sum;
+  __syncthreads();
+  if (part == 0)
+    partials[static_cast<int64_t>(q) * gridDim.x + chunk] =
+        ((part_sum[0][q] + part_sum[1][q]) + part_sum[2][q]) + part_sum[3][q];
+}
+
+// One block per column q, after each component's mean (component_mean_kernel,
+// kernels/cholesky.py:1467-1660): the shift of the mean-removed field (kernels/heat.py:3383-3424),
+// offset = its mean over the column's sources (vertex v at row iperm[v]), negated when its mean
+// over every vertex (the chunks' sums less their share of the means) is below the offset.
+// shift[2q] = offset, shift[2q + 1] = 1 to negate.
+__global__ __launch_bounds__(256) void rows_decide_kernel(
+    const int32_t* __restrict__ comp_ptr, const int32_t* __restrict__ chunk_comp,
+    const int32_t* __restrict__ chunk_first, int32_t n_comp, int32_t n_chunks,
+    const double* __restrict__ partials, const double* __restrict__ means,
+    const double* __restrict__ x, int32_t kk, int64_t n, const int32_t* __restrict__ iperm,
+    const int32_t* __restrict__ labels, const int32_t* __restrict__ sources,
+    const int64_t* __restrict__ source_offsets, double* __restrict__ shift) {
+  __shared__ double s_total[256];
+  __shared__ double s_src[256];
+  const int q = blockIdx.x, t = threadIdx.x;
+  const double* mean = means + static_cast<int64_t>(q) * n_comp;
+  double total = 0.0;
+  for (int32_t k = t; k < n_chunks; k += 256) {
+    const int32_t c = chunk_comp[k];
+    const int32_t len = min(256, comp_ptr[c + 1] - chunk_first[k]);
+    total += partials[static_cast<int64_t>(q) * n_chunks + k] - static_cast<double>(len) * mean[c];
+  }
+  const int64_t first = source_offsets[q], last = source_offsets[q + 1];
+  double src = 0.0;
+  for (int64_t e = first + t; e < last; e += 256) {
+    const int32_t v = sources[e];
+    src += x[static_cast<int64_t>(iperm[v]) * kk + q] - mean[labels[v]];
+  }
+  s_total[t] = total;
+  s_src[t] = src;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (t < stride) {
+      s_total[t] += s_total[t + stride];
+      s_src[t] += s_src[t + stride];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int64_t count = last - first;
+    const double offset = count > 0 ? s_src[0] / static_cast<double>(count) : 0.0;
+    shift[2 * q] = offset;
+    shift[2 * q + 1] =
+        (count > 0 && s_total[0] / static_cast<double>(n) - offset < 0.0) ? 1.0 : 0.0;
+  }
+}
+
+// The distance in the caller's layout (column q at q * n): vertex v's entry is the mean-removed
+// value at row iperm[v], shifted and oriented. 64 vertices a block, transposed through LDS so the
+// row reads and the column writes are both contiguous.
+__global__ __launch_bounds__(256) void rows_out_kernel(const double* __restrict__ x, int32_t kk,
+                                                       int64_t n, const int32_t* __restrict__ iperm,
+                                                       const int32_t* __restrict__ labels,
+                                                       int32_t n_comp,
+                                                       const double* __restrict__ means,
+                                                       const double* __restrict__ shift,
+                                                       double* __restrict__ out) {
+  __shared__ double tile[kTile][kCols + 1];
+  const int64_t v0 = static_cast<int64_t>(blockIdx.x) * kTile;
+  const int32_t rows = static_cast<int32_t>(min<int64_t>(kTile, n - v0));
+  for (int e = threadIdx.x; e < rows * kk; e += 256) {
+    const int lv = e / kk, q = e % kk;
+    const int64_t v = v0 + lv;
+    const double value = x[static_cast<int64_t>(iperm[v]) * kk + q] -
+                         means[static_cast<int64_t>(q) * n_comp + labels[v]];
+    const double offset = shift[2 * q];
+    tile[lv][q] = shift[2 * q + 1] != 0.0 ? offset - value : value - offset;
+  }
+  __syncthreads();
+  const int lv = threadIdx.x % kTile, qq = threadIdx.x / kTile;
+  if (lv >= rows) return;
+  for (int q = qq; q < kk; q += 4) out[static_cast<int64_t>(q) * n + v0 + lv] = tile[lv][q];
+}
+
 inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 template <typename TW>
@@ -800,9 +912,9 @@ void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream)
                                                       B.k0.ptr + t0, B.k1.ptr + t0,
                                                       B.slot.ptr + t0, w, a);
     } else {
-      bwd_tile_kernel<TW><<<tasks, 256, 0, stream>>>(B.s.ptr + t0, B.tile.ptr + t0,
-                                                     B.k0.ptr + t0, B.k1.ptr + t0,
-                                                     B.slot.ptr + t0, w, a);
+      auto tile = a.x ? bwd_tile_kernel<TW, true> : bwd_tile_kernel<TW, false>;
+      tile<<<tasks, 256, 0, stream>>>(B.s.ptr + t0, B.tile.ptr + t0, B.k0.ptr + t0, B.k1.ptr + t0,
+                                      B.slot.ptr + t0, w, a);
     }
     const int32_t r0 = B.reduce_ptr[level];
     const int32_t groups = B.reduce_ptr[level + 1] - r0;
@@ -982,6 +1094,35 @@ void remove_null_space(const NullSpace& ns, const int32_t* comp_ptr, const int32
                           stream>>>(ns.comp_chunk_ptr.ptr, comp_ptr, ns.n_chunks, partials, means);
   component_shift_kernel<<<grid, 256, 0, stream>>>(ns.chunk_comp.ptr, ns.chunk_first.ptr,
                                                    comp_ptr, members, ns.n_comp, n, means, x);
+  HG_HIP(hipGetLastError());
+}
+
+void finish_rows(const DevicePlan& plan, const NullSpace& ns, const int32_t* comp_ptr,
+                 const int32_t* labels, const double* x, int64_t k, const int32_t* sources,
+                 const int64_t* source_offsets, double* out, DeviceArray<double>& scratch,
+                 hipStream_t stream) {
+  const int64_t n = plan.n;
+  if (n == 0 || k == 0 || ns.n_chunks == 0) return;
+  const int64_t need = static_cast<int64_t>(ns.n_chunks) * kCols +
+                       static_cast<int64_t>(ns.n_comp) * kCols + 2 * kCols;
+  if (scratch.size < need) scratch.allocate(need);
+  double* partials = scratch.ptr;
+  double* means = partials + static_cast<int64_t>(ns.n_chunks) * kCols;
+  double* shift = means + static_cast<int64_t>(ns.n_comp) * kCols;
+  for (int64_t q0 = 0; q0 < k; q0 += kCols) {
+    const int32_t kk = static_cast<int32_t>(std::min<int64_t>(kCols, k - q0));
+    const double* xq = x + q0 * n;
+    rows_partial_kernel<<<static_cast<unsigned>(ns.n_chunks), 256, 0, stream>>>(
+        ns.chunk_comp.ptr, ns.chunk_first.ptr, comp_ptr, xq, kk, partials);
+    component_mean_kernel<<<dim3(static_cast<unsigned>(ns.n_comp), static_cast<unsigned>(kk)), 256,
+                            0, stream>>>(ns.comp_chunk_ptr.ptr, comp_ptr, ns.n_chunks, partials,
+                                         means);
+    rows_decide_kernel<<<static_cast<unsigned>(kk), 256, 0, stream>>>(
+        comp_ptr, ns.chunk_comp.ptr, ns.chunk_first.ptr, ns.n_comp, ns.n_chunks, partials, means,
+        xq, kk, n, plan.iperm.ptr, labels, sources, source_offsets + q0, shift);
+    rows_out_kernel<<<static_cast<unsigned>((n + kTile - 1) / kTile), 256, 0, stream>>>(
+        xq, kk, n, plan.iperm.ptr, labels, ns.n_comp, means, shift, out + q0 * n);
+  }
   HG_HIP(hipGetLastError());
 }
 

@@ -33,7 +33,19 @@ MESHES = {
     "grid": lambda: __import__("ordito_amd.creation", fromlist=["x"]).planar_grid(30, 24, 1.0, 0.2, 4),
     "bumpy": lambda: __import__("ordito_amd.creation", fromlist=["x"]).bumpy_sphere(),
     "saddle_graded": lambda: __import__("ordito_amd.creation", fromlist=["x"]).saddle_graded(),
+    "two_bodies": lambda: _two_bodies(),
 }
+
+
+def _two_bodies() -> tuple[np.ndarray, np.ndarray]:
+    """One mesh of two connected components: a jittered icosphere and a torus beside it."""
+    from ordito_amd import creation
+
+    v0, f0 = creation.jittered_icosphere(3, 0.01, 2)
+    v1, f1 = creation.torus(24, 12, 1.0, 0.3, 0.1, 5)
+    v1 = v1 + np.array([3.0, 0.0, 0.0], dtype=v1.dtype)
+    faces = np.concatenate([np.asarray(f0).reshape(-1, 3), np.asarray(f1).reshape(-1, 3) + len(v0)])
+    return np.concatenate([v0, v1]).astype(np.float32), faces.astype(np.int32)
 
 
 def mesh(name: str) -> tuple[np.ndarray, np.ndarray]:

@@ -50,7 +50,7 @@ def test_operators_match_oracle(cuda, name):
     assert np.allclose(got, ref.heat, rtol=1e-14, atol=0.0)
 
 
-@pytest.mark.parametrize("name", ["icosphere4", "torus", "grid", "bumpy"])
+@pytest.mark.parametrize("name", ["icosphere4", "torus", "grid", "bumpy", "two_bodies"])
 def test_factored_fp64_matches_oracle(cuda, name):
     from ordito_amd import heat
 
@@ -65,7 +65,7 @@ def test_factored_fp64_matches_oracle(cuda, name):
     assert worst <= 1e-9, worst
 
 
-@pytest.mark.parametrize("name", ["icosphere4", "torus", "grid", "bumpy"])
+@pytest.mark.parametrize("name", ["icosphere4", "torus", "grid", "bumpy", "two_bodies"])
 def test_iterated_matches_oracle(cuda, name):
     from ordito_amd import heat
 
