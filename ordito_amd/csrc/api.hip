@@ -61,7 +61,7 @@ struct hg_mesh {
   int32_t time_from_edges = 1;
   DeviceArray<float> cot, normals, areas, heat32;
   DeviceArray<double> mass, lap, heat, poisson, heat_inv, poisson_inv, scalars, edge_partials;
-  DeviceArray<double> face_geom;  // 18 doubles a face: the divergence's per-face quantities
+  DeviceArray<double> face_geom;  // kFaceGeom doubles a face: the divergence's per-face data
   double poisson_bound = -1.0;
   // Components: the Poisson system pins each component's last-eliminated row.
   int32_t n_comp = 0;
@@ -214,7 +214,7 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   m.cot.allocate(3 * m.f);
   m.normals.allocate(3 * m.f);
   m.areas.allocate(m.f);
-  m.face_geom.allocate(18 * m.f);
+  m.face_geom.allocate(kFaceGeom * m.f);
   m.mass.allocate(n);
   m.lap.allocate(nnz);
   m.heat.allocate(nnz);

@@ -243,10 +243,12 @@ __global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* 
 }
 
 // Per face, the float64 quantities every query's divergence reuses (kernels/triangles.py:834-894,
-// kernels/heat.py:626-660), from the float32 positions, unit normal N, area and half-cotangents:
-//   geom[18f + 3k .. +3), k < 3: a_k = N x e_k, e_0 = v2 - v1, e_1 = v0 - v2, e_2 = v1 - v0, so
-//       that sum_k u_k a_k is the heat gradient times 2A (all zero where A = 0: no gradient);
-//   geom[18f + 9 + 3c .. +3), c < 3: corner c's integration vector, corner 0
+// kernels/heat.py:626-660), from the float32 positions, unit normal N, area and half-cotangents,
+// kFaceGeom doubles a face:
+//   [0, 6): a_0 = N x (v2 - v1) and a_1 = N x (v0 - v2). With a_2 = N x (v1 - v0) = -a_0 - a_1,
+//       sum_k u_k a_k = (u0 - u2) a_0 + (u1 - u2) a_1 is the heat gradient times 2A (a zero-area
+//       face stores zeros: no gradient);
+//   [6 + 3c, 9 + 3c), c < 3: corner c's integration vector, corner 0
 //       g_0 = c_2 (v1 - v0) + c_1 (v2 - v0), corner 1 g_1 = c_0 (v2 - v1) + c_2 (v0 - v1),
 //       corner 2 g_2 = c_1 (v0 - v2) + c_0 (v1 - v2): the corner's term is g_c . x for the unit
 //       gradient direction x.
@@ -263,42 +265,45 @@ __global__ void face_geometry_kernel(const float* __restrict__ xyz,
   const D3 nrm = widen(load3(normals, f));
   const bool live = static_cast<double>(areas[f]) > 0.0;
   const D3 zero{0.0, 0.0, 0.0};
-  const D3 a[3] = {live ? cross(nrm, sub(v2, v1)) : zero, live ? cross(nrm, sub(v0, v2)) : zero,
-                   live ? cross(nrm, sub(v1, v0)) : zero};
+  const D3 a[2] = {live ? cross(nrm, sub(v2, v1)) : zero, live ? cross(nrm, sub(v0, v2)) : zero};
   const double c0 = cot[3 * f], c1 = cot[3 * f + 1], c2 = cot[3 * f + 2];
   const D3 g[3] = {add(scale(c2, sub(v1, v0)), scale(c1, sub(v2, v0))),
                    add(scale(c0, sub(v2, v1)), scale(c2, sub(v0, v1))),
                    add(scale(c1, sub(v0, v2)), scale(c0, sub(v1, v2)))};
-  double* out = geom + 18 * f;
+  double* out = geom + kFaceGeom * f;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 2; ++k) {
     out[3 * k] = a[k].x;
     out[3 * k + 1] = a[k].y;
     out[3 * k + 2] = a[k].z;
-    out[9 + 3 * k] = g[k].x;
-    out[9 + 3 * k + 1] = g[k].y;
-    out[9 + 3 * k + 2] = g[k].z;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    out[6 + 3 * k] = g[k].x;
+    out[6 + 3 * k + 1] = g[k].y;
+    out[6 + 3 * k + 2] = g[k].z;
   }
 }
 
 // One corner's divergence term for heat values u0, u1, u2 at its face's vertices: the gradient
-// direction s = sum_k u_k a_k is scaled by the power of two that brings its largest component into
-// [0.5, 1) (exact, so tiny heat values neither underflow nor lose bits; the reference divides by
-// the largest component, kernels/predicates.py:718-724), normalized, and dotted with g. A zero
-// gradient contributes zero.
+// direction s = (u0 - u2) a_0 + (u1 - u2) a_1 is scaled by the power of two that brings its largest
+// component into [0.5, 1) (exact, so tiny heat values neither underflow nor lose bits; the
+// reference divides by the largest component, kernels/predicates.py:718-724), normalized, and
+// dotted with g_c. A zero gradient contributes zero.
 __device__ __forceinline__ double corner_term(const double* __restrict__ geom, int32_t corner,
                                               double u0, double u1, double u2) {
 #pragma clang fp contract(fast)
   const double* a = geom;
-  const double sx = u0 * a[0] + u1 * a[3] + u2 * a[6];
-  const double sy = u0 * a[1] + u1 * a[4] + u2 * a[7];
-  const double sz = u0 * a[2] + u1 * a[5] + u2 * a[8];
+  const double d0 = u0 - u2, d1 = u1 - u2;
+  const double sx = d0 * a[0] + d1 * a[3];
+  const double sy = d0 * a[1] + d1 * a[4];
+  const double sz = d0 * a[2] + d1 * a[5];
   const double largest = fmax(fmax(fabs(sx), fabs(sy)), fabs(sz));
   if (largest == 0.0) return 0.0;
   int e;
   (void)frexp(largest, &e);
   const double wx = ldexp(sx, -e), wy = ldexp(sy, -e), wz = ldexp(sz, -e);
-  const double* g = geom + 9 + 3 * corner;
+  const double* g = geom + 6 + 3 * corner;
   return (g[0] * wx + g[1] * wy + g[2] * wz) * rsqrt(wx * wx + wy * wy + wz * wz);
 }
 
@@ -306,7 +311,8 @@ __device__ __forceinline__ double corner_term(const double* __restrict__ geom, i
 // order (kernels/heat.py:626-687, heat.py:3218-3232), each term recomputed from the face's heat
 // values (no per-corner intermediate in memory). heat and out share a layout: entry (row r,
 // column q) at r * vstride + q * qstride, vertex i at row row_of[i] (row i when row_of is null),
-// and `faces` name rows.
+// and `faces` name rows. Corners go four at a time: their corner ids, then their faces' vertices,
+// then the heat values and face data, each step's loads issued together (a vertex has about six).
 //
 // kWaveVertex: one wave a vertex, its lanes the columns (kk <= 64 columns stored by rows, so the
 // heat rows it reads and the row it writes are contiguous); the vertex, its corners and their face
@@ -338,18 +344,32 @@ __global__ __launch_bounds__(256) void divergence_kernel(
   }
   const double* u = heat + q * qstride;
   const int64_t row = row_of ? row_of[i] : i;
-  const double ui = u[row * vstride];
   double sum = 0.0;
-  const int32_t k1 = vertex_ptr[i + 1];
-  for (int32_t k = vertex_ptr[i]; k < k1; ++k) {
-    const int32_t c = vertex_corners[k];
-    const int32_t f = c / 3, corner = c - 3 * f;
-    const int32_t* tri = faces + 3 * static_cast<int64_t>(f);
-    // The corner's own vertex is i; the other two are read.
-    const double u0 = corner == 0 ? ui : u[tri[0] * vstride];
-    const double u1 = corner == 1 ? ui : u[tri[1] * vstride];
-    const double u2 = corner == 2 ? ui : u[tri[2] * vstride];
-    sum += corner_term(geom + 18 * static_cast<int64_t>(f), corner, u0, u1, u2);
+  const int32_t kbeg = vertex_ptr[i], kend = vertex_ptr[i + 1];
+  for (int32_t kb = kbeg; kb < kend; kb += 4) {
+    int32_t f[4], corner[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int32_t c = vertex_corners[min(kb + w, kend - 1)];  // past the end: repeats, unused
+      f[w] = c / 3;
+      corner[w] = c - 3 * f[w];
+    }
+    int32_t v[4][3];
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[w][k] = faces[3 * static_cast<int64_t>(f[w]) + k];
+    double h[4][3];
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) h[w][k] = u[v[w][k] * vstride];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const double term = corner_term(geom + kFaceGeom * static_cast<int64_t>(f[w]), corner[w],
+                                      h[w][0], h[w][1], h[w][2]);
+      if (kb + w < kend) sum += term;
+    }
   }
   out[row * vstride + q * qstride] = sum;
 }

@@ -122,7 +122,9 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
 void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
                  int64_t rows_k, const int32_t* row_of, double* u0, int32_t* invalid,
                  hipStream_t stream);
-// Per face, 18 doubles: the gradient basis a_k = N x e_k and the corners' integration vectors.
+// Per face, kFaceGeom doubles: two of the gradient basis vectors a_k = N x e_k (the third is minus
+// their sum) and the three corners' integration vectors.
+constexpr int64_t kFaceGeom = 15;
 void launch_face_geometry(const float* xyz, const int32_t* faces, const float* normals,
                           const float* areas, const float* cot, int64_t n_faces, double* geom,
                           hipStream_t stream);

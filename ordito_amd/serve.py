@@ -72,7 +72,7 @@ def footprint_estimate(n_vertices: int, n_faces: int, precision: str = "fp64") -
     factor_entries = 108.0 * n_vertices * scale
     arena_entries = 420.0 * n_vertices * scale
     factor_bytes = factor_entries * (8 + (4 if precision == "mixed" else 0)) * 2  # two systems
-    operators = 3.67 / 0.524 * n_vertices * 8 * 5 + n_faces * (40 + 144)  # + the face geometry
+    operators = 3.67 / 0.524 * n_vertices * 8 * 5 + n_faces * (40 + 120)  # + the face geometry
     return int(factor_bytes + arena_entries * 8 + operators)
 
 
