@@ -312,7 +312,8 @@ __device__ __forceinline__ double corner_term(const double* __restrict__ geom, i
 // values (no per-corner intermediate in memory). heat and out share a layout: entry (row r,
 // column q) at r * vstride + q * qstride, vertex i at row row_of[i] (row i when row_of is null),
 // and `faces` name rows. Corners go four at a time: their corner ids, then their faces' vertices,
-// then the heat values and face data, each step's loads issued together (a vertex has about six).
+// then the other two heat values and the face data, each step's loads issued together (a vertex
+// has about six corners).
 //
 // kWaveVertex: one wave a vertex, its lanes the columns (kk <= 64 columns stored by rows, so the
 // heat rows it reads and the row it writes are contiguous); the vertex, its corners and their face
@@ -344,6 +345,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(
   }
   const double* u = heat + q * qstride;
   const int64_t row = row_of ? row_of[i] : i;
+  const double ui = u[row * vstride];
   double sum = 0.0;
   const int32_t kbeg = vertex_ptr[i], kend = vertex_ptr[i + 1];
   for (int32_t kb = kbeg; kb < kend; kb += 4) {
@@ -359,11 +361,12 @@ __global__ __launch_bounds__(256) void divergence_kernel(
     for (int w = 0; w < 4; ++w)
 #pragma unroll
       for (int k = 0; k < 3; ++k) v[w][k] = faces[3 * static_cast<int64_t>(f[w]) + k];
+    // The corner's own vertex is i: two heat values a corner are read.
     double h[4][3];
 #pragma unroll
     for (int w = 0; w < 4; ++w)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) h[w][k] = u[v[w][k] * vstride];
+      for (int k = 0; k < 3; ++k) h[w][k] = k == corner[w] ? ui : u[v[w][k] * vstride];
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
       const double term = corner_term(geom + kFaceGeom * static_cast<int64_t>(f[w]), corner[w],
