@@ -597,6 +597,14 @@ __global__ __launch_bounds__(256) void bwd_tile1_kernel(
 // Raises *flag once per block that fails, and not at all once the flag reads set: with every row
 // failing in a solve's first rounds, an atomic a wave on one word serializes the whole launch.
 // Block-collective.
+// Lane `lane`'s double, as a wave-uniform value.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(bits), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(bits >> 32), lane);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 __device__ __forceinline__ void raise_flag(bool fail, int32_t* flag) {
   const int any = __syncthreads_or(fail ? 1 : 0);
   if (any && threadIdx.x == 0 &&
@@ -625,36 +633,45 @@ __global__ void residual_kernel(const int32_t* __restrict__ offsets,
         kWaveRow ? __builtin_amdgcn_readfirstlane(static_cast<int32_t>(t / kCols)) : t / kk;
     const int64_t q = t % kk;
     const double bi = b[t];
+    const bool dropped = pinned && pinned[i];
+    const double bscale = componentwise ? 0.0 : bmax[q];
     double total = bi, scale = fabs(bi);
-    // Four entries at a time: their column and value loads, then their iterate loads, issue
-    // together; the sums stay in entry order.
-    int32_t e = offsets[i];
-    const int32_t e1 = offsets[i + 1];
-    for (; e + 4 <= e1; e += 4) {
-      int32_t c[4];
-      double v[4], xv[4];
+    // Eight entries at a time (a row of a triangle mesh has about seven): their column and value
+    // loads, then their iterate loads, issue together; entries past the row's end repeat its last
+    // one and are not added, and the sums stay in entry order. A wave on one row loads the eight
+    // columns and values in its first lanes at once and reads them out lane by lane.
+    const int32_t e0 = offsets[i], e1 = offsets[i + 1];
+    for (int32_t e = e0; e < e1; e += 8) {
+      int32_t c[8];
+      double v[8], xv[8];
+      if (kWaveRow) {
+        const int32_t at = min(e + static_cast<int32_t>(threadIdx.x % 8), e1 - 1);
+        const int32_t lane_c = columns[at];
+        const double lane_v = values[at];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        c[u] = columns[e + u];
-        v[u] = values[e + u];
+        for (int u = 0; u < 8; ++u) {
+          c[u] = __builtin_amdgcn_readlane(lane_c, u);
+          v[u] = readlane_f64(lane_v, u);
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int32_t at = min(e + u, e1 - 1);
+          c[u] = columns[at];
+          v[u] = values[at];
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xv[u] = x[static_cast<int64_t>(c[u]) * kk + q];
+      for (int u = 0; u < 8; ++u) xv[u] = x[static_cast<int64_t>(c[u]) * kk + q];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const double term = v[u] * xv[u];
+      for (int u = 0; u < 8; ++u) {
+        const double term = e + u < e1 ? v[u] * xv[u] : 0.0;  // subtracting 0 changes nothing
         total -= term;
         scale += fabs(term);
       }
     }
-    for (; e < e1; ++e) {
-      const double term = values[e] * x[static_cast<int64_t>(columns[e]) * kk + q];
-      total -= term;
-      scale += fabs(term);
-    }
-    const bool dropped = pinned && pinned[i];
     if (r) r[t] = dropped ? 0.0 : total;
-    if (!componentwise) scale = fmax(scale, bmax[q]);
+    if (!componentwise) scale = fmax(scale, bscale);
     fail = !dropped && !(fabs(total) <= tolerance * scale);
   }
   raise_flag(fail, pending);
