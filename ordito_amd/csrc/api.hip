@@ -79,6 +79,8 @@ struct hg_mesh {
   std::unique_ptr<Plan> host_plan;
   std::unique_ptr<DevicePlan> plan;
   DeviceArray<int32_t> faces_rows;  // the faces' corners as rows of the elimination order
+  DeviceArray<int4> row_corners;    // each row's corners {corner id, its face's three rows}
+  DeviceArray<int32_t> row_corner_ptr;
   Factor heat_factor, poisson_factor;
   // Refinement targets of the factored solves, [precision][0 heat (componentwise), 1 Poisson
   // (normwise)].
@@ -270,6 +272,20 @@ void ensure_plan(hg_mesh& m) {
     std::vector<int32_t> rows(m.h_faces.size());
     for (size_t e = 0; e < rows.size(); ++e) rows[e] = plan.iperm[m.h_faces[e]];
     m.faces_rows.upload(rows, m.stream);
+    // Row r's corners are vertex perm[r]'s, in its ascending corner order.
+    std::vector<int32_t> ptr(static_cast<size_t>(m.n) + 1, 0);
+    std::vector<int4> corners(m.pattern.vertex_corners.size());
+    for (int64_t r = 0; r < m.n; ++r) {
+      const int32_t v = plan.perm[r];
+      int32_t at = ptr[r];
+      for (int32_t k = m.pattern.vertex_ptr[v]; k < m.pattern.vertex_ptr[v + 1]; ++k) {
+        const int32_t c = m.pattern.vertex_corners[k], f = c / 3;
+        corners[at++] = make_int4(c, rows[3 * f], rows[3 * f + 1], rows[3 * f + 2]);
+      }
+      ptr[r + 1] = at;
+    }
+    m.row_corners.upload(corners, m.stream);
+    m.row_corner_ptr.upload(ptr, m.stream);
   }
   // The distance's finishing pass reads components as row ranges of the elimination order.
   m.rows_by_component = true;
@@ -511,8 +527,9 @@ void divergence(hg_mesh& m, const double* heat, double* out, int64_t k,
   PhaseScope scope(&m.timer, kPhaseDivergence, m.stream);
   const bool rows = layout == FieldLayout::kRows;
   launch_divergence(rows ? m.faces_rows.ptr : m.faces.ptr, rows ? m.plan->iperm.ptr : nullptr,
-                    m.face_geom.ptr, m.vertex_ptr.ptr, m.vertex_corners.ptr, heat, m.n, k, out,
-                    layout, m.stream);
+                    m.face_geom.ptr, m.vertex_ptr.ptr, m.vertex_corners.ptr,
+                    rows ? m.row_corners.ptr : nullptr, rows ? m.row_corner_ptr.ptr : nullptr,
+                    heat, m.n, k, out, layout, m.stream);
 }
 
 double* reserve_fields(hg_mesh& m, int64_t k) {

@@ -377,6 +377,46 @@ __global__ __launch_bounds__(256) void divergence_kernel(
   out[row * vstride + q * qstride] = sum;
 }
 
+// The same -b for a chunk of kk columns stored by rows, one wave a row r of the elimination order
+// (its lanes the columns): row r's corners are corners[corner_ptr[r] ..], each {corner id 3f + c,
+// rows of face f's vertices 0, 1, 2}, in the vertex's ascending corner order. A wave reads eight
+// corner records, then every heat value and face datum they name, then adds the terms of the
+// corners that exist (a uniform branch: a vertex's last batch computes no padding); the result row
+// is written contiguously. Same terms, same order as divergence_kernel.
+__global__ __launch_bounds__(256) void divergence_rows_kernel(
+    const int4* __restrict__ corners, const int32_t* __restrict__ corner_ptr,
+    const double* __restrict__ geom, const double* __restrict__ heat, int64_t n, int64_t kk,
+    double* __restrict__ out) {
+  const int64_t r =
+      __builtin_amdgcn_readfirstlane(static_cast<int32_t>(xcd_block() * 4 + threadIdx.x / 64));
+  const int64_t q = threadIdx.x % 64;
+  if (r >= n || q >= kk) return;
+  const int32_t kbeg = corner_ptr[r], kend = corner_ptr[r + 1];
+  double sum = 0.0;
+  for (int32_t kb = kbeg; kb < kend; kb += 8) {
+    int4 c[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) c[w] = corners[min(kb + w, kend - 1)];  // past the end: unused
+    // A row's base is wave-uniform and the column the lane's offset.
+    const uint32_t lane = static_cast<uint32_t>(q);
+    double h[8][3];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      h[w][0] = (heat + static_cast<int64_t>(c[w].y) * kk)[lane];
+      h[w][1] = (heat + static_cast<int64_t>(c[w].z) * kk)[lane];
+      h[w][2] = (heat + static_cast<int64_t>(c[w].w) * kk)[lane];
+    }
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      if (kb + w >= kend) break;
+      const int32_t f = c[w].x / 3;
+      sum += corner_term(geom + kFaceGeom * static_cast<int64_t>(f), c[w].x - 3 * f, h[w][0],
+                         h[w][1], h[w][2]);
+    }
+  }
+  out[r * kk + q] = sum;
+}
+
 // Columns of the shift: column q starts at col_off[q] and holds col_len[q] entries (packed meshes'
 // fields, each its own length), or at q * n with n entries when col_off is null.
 struct Columns {
@@ -534,17 +574,17 @@ void launch_face_geometry(const float* xyz, const int32_t* faces, const float* n
 
 void launch_divergence(const int32_t* faces, const int32_t* row_of, const double* geom,
                        const int32_t* vertex_ptr, const int32_t* vertex_corners,
+                       const int4* row_corners, const int32_t* row_corner_ptr,
                        const double* heat, int64_t n, int64_t n_columns, double* out,
                        FieldLayout layout, hipStream_t stream) {
   if (n == 0 || n_columns == 0) return;
   if (layout == FieldLayout::kRows) {
     for (int64_t c0 = 0; c0 < n_columns; c0 += kFieldChunk) {
       const int64_t kk = std::min<int64_t>(kFieldChunk, n_columns - c0);
-      // A wave a vertex while at least a quarter of its lanes hold columns.
+      // A wave a row while at least a quarter of its lanes hold columns.
       if (kk >= kFieldChunk / 4)
-        divergence_kernel<true><<<static_cast<unsigned>((n + 3) / 4), kBlock, 0, stream>>>(
-            faces, geom, vertex_ptr, vertex_corners, row_of, heat + c0 * n, n, kk, kk, 1,
-            out + c0 * n);
+        divergence_rows_kernel<<<static_cast<unsigned>((n + 3) / 4), kBlock, 0, stream>>>(
+            row_corners, row_corner_ptr, geom, heat + c0 * n, n, kk, out + c0 * n);
       else
         divergence_kernel<false><<<blocks_for(n * kk), kBlock, 0, stream>>>(
             faces, geom, vertex_ptr, vertex_corners, row_of, heat + c0 * n, n, kk, kk, 1,

@@ -129,9 +129,12 @@ void launch_face_geometry(const float* xyz, const int32_t* faces, const float* n
                           const float* areas, const float* cot, int64_t n_faces, double* geom,
                           hipStream_t stream);
 // -b per vertex and column from the heat (both in `layout`), gathered over each vertex's corners.
-// kRows: vertex v's row is row_of[v], and `faces` name rows (row_of applied to every corner).
+// kRows: vertex v's row is row_of[v], `faces` name rows (row_of applied to every corner), and
+// row_corners[row_corner_ptr[r] ..] are row r's corners in ascending corner order, each
+// {corner id 3f + c, rows of face f's vertices 0, 1, 2}.
 void launch_divergence(const int32_t* faces, const int32_t* row_of, const double* geom,
                        const int32_t* vertex_ptr, const int32_t* vertex_corners,
+                       const int4* row_corners, const int32_t* row_corner_ptr,
                        const double* heat, int64_t n, int64_t n_columns, double* out,
                        FieldLayout layout, hipStream_t stream);
 int32_t shift_partial_blocks(int64_t n);
