@@ -92,6 +92,7 @@ struct hg_mesh {
   DeviceArray<double> fields, shift_partials, shift_values, null_partials;
   NullSpace null_space;
   DeviceArray<int32_t> d_sources, invalid;
+  PinnedArray<int32_t> h_invalid;  // the seeds' range flag, read back with the call's last sync
   DeviceArray<int64_t> d_owner, d_offsets;
   int64_t field_capacity = 0;
   DeviceArray<double> packed_phi;  // packed meshes: the solved slot columns before extraction
@@ -515,10 +516,17 @@ void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0,
   const bool rows = layout == FieldLayout::kRows;
   launch_seed(m.d_sources.ptr, m.d_owner.ptr, total, m.n, rows ? k : 0,
               rows ? m.plan->iperm.ptr : nullptr, u0, m.invalid.ptr, m.stream);
-  int32_t flag = 0;
-  HG_HIP(hipMemcpyAsync(&flag, m.invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
-  HG_HIP(hipStreamSynchronize(m.stream));
-  HG_REQUIRE(flag == 0, HG_ERR_VALUE,
+  // The range flag comes back with the call's next synchronization (check_seeds); until then the
+  // solve runs on, every kernel that reads the sources skipping the ones out of range.
+  if (!m.h_invalid.ptr) m.h_invalid.allocate(1);
+  HG_HIP(hipMemcpyAsync(m.h_invalid.ptr, m.invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost,
+                        m.stream));
+}
+
+// After a synchronization of the mesh's stream: fail the call if a seed was out of range
+// (kernels/heat.py:599-623, heat.py:5488-5503).
+void check_seeds(const hg_mesh& m) {
+  HG_REQUIRE(!m.h_invalid.ptr || *m.h_invalid.ptr == 0, HG_ERR_VALUE,
              "every entry of sources must be a vertex index in [0, " + std::to_string(m.n) + ")");
 }
 
@@ -861,6 +869,8 @@ int hg_seed_sources(hg_mesh* mesh, const int32_t* d_sources, const int64_t* sour
     require_mesh(mesh);
     upload_sources(*mesh, d_sources, source_offsets, n_columns);
     seed(*mesh, source_offsets, n_columns, d_u0);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+    check_seeds(*mesh);
   });
 }
 
@@ -898,6 +908,7 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     upload_sources(m, d_sources, source_offsets, k);
     if (!solve_columns(m, source_offsets, k, d_out, mode, precision, true)) shift(m, d_out, k);
     HG_HIP(hipStreamSynchronize(m.stream));
+    check_seeds(m);
     m.timer.resolve();
     if (out_stats) *out_stats = m.stats;
   });
@@ -992,6 +1003,7 @@ int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* s
                    m.q_len.ptr, m.shift_partials.ptr, m.shift_values.ptr, m.stream);
     }
     HG_HIP(hipStreamSynchronize(m.stream));
+    check_seeds(m);
     m.timer.resolve();
     if (out_stats) *out_stats = m.stats;
   });

@@ -461,7 +461,11 @@ __global__ void shift_decide_kernel(const double* __restrict__ phi,
   for (int32_t b = threadIdx.x; b < blocks; b += kBlock) total += partials[q * blocks + b];
   int64_t first = source_offsets[q], last = source_offsets[q + 1];
   double src = 0.0;
-  for (int64_t e = first + threadIdx.x; e < last; e += kBlock) src += phi[base + sources[e]];
+  // A source out of range (the call then fails) is skipped.
+  for (int64_t e = first + threadIdx.x; e < last; e += kBlock) {
+    const int32_t v = sources[e];
+    if (v >= 0 && v < n) src += phi[base + v];
+  }
   s[threadIdx.x] = total;
   s_src[threadIdx.x] = src;
   __syncthreads();

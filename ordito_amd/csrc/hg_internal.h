@@ -80,6 +80,28 @@ struct DeviceArray {
   }
 };
 
+// A pinned host allocation (hipHostMalloc): device-to-host copies into it stay asynchronous.
+template <typename T>
+struct PinnedArray {
+  T* ptr = nullptr;
+  int64_t size = 0;
+  PinnedArray() = default;
+  PinnedArray(const PinnedArray&) = delete;
+  PinnedArray& operator=(const PinnedArray&) = delete;
+  ~PinnedArray() { reset(); }
+  void allocate(int64_t n) {
+    reset();
+    HG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr), sizeof(T) * static_cast<size_t>(n),
+                         hipHostMallocDefault));
+    size = n;
+  }
+  void reset() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    size = 0;
+  }
+};
+
 // ------------------------------------------------------------------------------------------------
 // Host analysis (pattern.cpp, analysis.cpp)
 // ------------------------------------------------------------------------------------------------

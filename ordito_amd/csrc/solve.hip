@@ -844,6 +844,7 @@ __global__ __launch_bounds__(256) void rows_decide_kernel(
   double src = 0.0;
   for (int64_t e = first + t; e < last; e += 256) {
     const int32_t v = sources[e];
+    if (v < 0 || v >= n) continue;  // out of range: the call fails once the stream is synced
     src += x[static_cast<int64_t>(iperm[v]) * kk + q] - mean[labels[v]];
   }
   s_total[t] = total;

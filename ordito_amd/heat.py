@@ -328,8 +328,12 @@ class HeatSolver:
         return int(out.value)
 
     def _factored(self) -> int:
+        return self._factored_on(self._native())
+
+    @staticmethod
+    def _factored_on(native: _Handle) -> int:
         out = ctypes.c_int32()
-        _lib.call("hg_factored", self._native().ptr, ctypes.byref(out))
+        _lib.call("hg_factored", native.ptr, ctypes.byref(out))
         return int(out.value)
 
     def release(self) -> None:
@@ -467,23 +471,37 @@ class HeatSolver:
         torch = _arrays.torch_module()
         mesh = self._bound_mesh()
         n = mesh.n_vertices
-        tensors = [_arrays.sources_tensor(s) for s in source_sets]
+        # Device int32 tensors pass as they are; anything else is converted.
+        tensors = [
+            s.reshape(-1)
+            if _arrays.is_torch(s) and s.is_cuda and s.dtype == torch.int32 and s.is_contiguous()
+            else _arrays.sources_tensor(s)
+            for s in source_sets
+        ]
         k = len(tensors)
-        out = torch.zeros((k, n), dtype=torch.float64, device="cuda")
         if k == 0 or n == 0 or mesh.n_faces == 0:
-            return out
+            return torch.zeros((k, n), dtype=torch.float64, device="cuda")
+        out = torch.empty((k, n), dtype=torch.float64, device="cuda")  # every entry is written
         offsets = [0]
         for s in tensors:
-            offsets.append(offsets[-1] + int(s.numel()))
+            offsets.append(offsets[-1] + s.numel())
         flat = torch.cat(tensors) if offsets[-1] > 0 else torch.zeros(1, dtype=torch.int32, device="cuda")
-        # Rent or buy, per system (heat.py:4394-4404): the second solve factors both.
-        self._count("heat")
-        self._count("poisson")
-        mode = _lib.HG_MODE_FACTORED if self._factored() & 1 else _lib.HG_MODE_ITERATE
+        # Rent or buy, per system (heat.py:4394-4404): the second solve factors both. One handle
+        # and one factored-state query for the whole call.
+        native = self._native()
+        factored = self._factored_on(native)
+        for role, system in (("heat", _lib.HG_SYSTEM_HEAT), ("poisson", _lib.HG_SYSTEM_POISSON)):
+            self._solves[role] += 1
+            if self._solves[role] >= 2 and not factored & system:
+                done = ctypes.c_int32()
+                _lib.call("hg_factor", native.ptr, system, self._precision_code, ctypes.byref(done))
+                factored |= system
+                _LOGGER.debug("HeatSolver: factored the %s system on its second solve", role)
+        mode = _lib.HG_MODE_FACTORED if factored & _lib.HG_SYSTEM_HEAT else _lib.HG_MODE_ITERATE
         stats = _lib.SolveStats()
         _lib.call(
             "hg_geodesic",
-            self._native().ptr,
+            native.ptr,
             _arrays.pointer(flat),
             _lib.offsets_array(offsets),
             k,
