@@ -471,21 +471,29 @@ class HeatSolver:
         torch = _arrays.torch_module()
         mesh = self._bound_mesh()
         n = mesh.n_vertices
-        # Device int32 tensors pass as they are; anything else is converted.
-        tensors = [
-            s.reshape(-1)
-            if _arrays.is_torch(s) and s.is_cuda and s.dtype == torch.int32 and s.is_contiguous()
-            else _arrays.sources_tensor(s)
-            for s in source_sets
-        ]
-        k = len(tensors)
+        k = len(source_sets)
         if k == 0 or n == 0 or mesh.n_faces == 0:
             return torch.zeros((k, n), dtype=torch.float64, device="cuda")
-        out = torch.empty((k, n), dtype=torch.float64, device="cuda")  # every entry is written
+        # One-dimensional tensors concatenate as they are (one conversion of the result); anything
+        # else is converted set by set.
+        same_kind = all(type(s) is torch.Tensor and s.dim() == 1 for s in source_sets) and len(
+            {(s.device, s.dtype) for s in source_sets}
+        ) == 1
+        if same_kind:
+            counts = [s.numel() for s in source_sets]
+            flat = torch.cat(source_sets) if sum(counts) > 0 else None
+            if flat is not None:
+                flat = flat.to(device="cuda", dtype=torch.int32).contiguous()
+        else:
+            tensors = [_arrays.sources_tensor(s) for s in source_sets]
+            counts = [s.numel() for s in tensors]
+            flat = torch.cat(tensors) if sum(counts) > 0 else None
+        if flat is None:
+            flat = torch.zeros(1, dtype=torch.int32, device="cuda")
         offsets = [0]
-        for s in tensors:
-            offsets.append(offsets[-1] + s.numel())
-        flat = torch.cat(tensors) if offsets[-1] > 0 else torch.zeros(1, dtype=torch.int32, device="cuda")
+        for c in counts:
+            offsets.append(offsets[-1] + c)
+        out = torch.empty((k, n), dtype=torch.float64, device="cuda")  # every entry is written
         # Rent or buy, per system (heat.py:4394-4404): the second solve factors both. One handle
         # and one factored-state query for the whole call.
         native = self._native()
