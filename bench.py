@@ -27,7 +27,8 @@ path: seed, heat solve, unit-gradient divergence, Poisson solve, shift -- all on
 
 Three further workloads print their own JSON line (the default run above is the headline):
 ``--workload sphere`` (icosphere(9) displaced by spherical harmonics, 2.6 M vertices, 16 queries,
-FP64 and mixed refined to 1e-10, errors against the oracle), ``--workload batch`` (256 meshes of
+FP64 and mixed refined to 1e-10, mixed against FP64 at full size and both against the oracle on the
+same surface at icosphere(7)), ``--workload batch`` (256 meshes of
 1e3-2e5 vertices packed into one system, one query each) and ``--workload serve`` (64 meshes of
 1e4-1e6 vertices behind the continuous-batching server, Poisson arrivals at 2,000 queries/s for
 60 s, 200 GB budget).
@@ -101,20 +102,51 @@ def sphere_workload(args) -> int:
     The large-mesh configuration: an icosphere subdivided 9 times (2,621,442 vertices) displaced by
     real spherical harmonics up to degree 16 with N(0, 0.05 / l) coefficients, 16 single-source
     queries as one batch, through FP64 factors and through FP32 factors refined to 1e-10
-    (componentwise heat, normwise Poisson); both fields' max and mean error against the FP64
-    oracle (SciPy SuperLU) on a sample of the queries.
+    (componentwise heat, normwise Poisson). Errors: the mixed fields against the FP64 fields at full
+    size, and both against the FP64 oracle (SciPy SuperLU) on the same surface subdivided 7 times
+    (163,842 vertices), since SuperLU's factorization of the full size exhausts its memory.
     """
     import torch
 
     from ordito_amd import creation, heat
     from oracle import heat_oracle as ho
 
+    def solve_both(vertices, faces, sources, steps, warmup):
+        sets = [torch.tensor([s], dtype=torch.int32, device="cuda") for s in sources]
+        v_gpu, f_gpu = torch.from_numpy(vertices).cuda(), torch.from_numpy(faces).cuda()
+        fields, lines = {}, {}
+        for precision in ("fp64", "mixed"):
+            mesh = heat.Trimesh(v_gpu, f_gpu)
+            solver = mesh.heat_solver(method="meshlab", precision=precision)
+            if precision == "mixed":
+                solver.set_refinement(heat=1e-10, poisson=1e-10)
+            t0 = time.perf_counter()
+            assert solver.factor()
+            torch.cuda.synchronize()
+            setup = time.perf_counter() - t0
+            elapsed, latencies, timing, out, stats = timed_steps(solver, sets, steps, warmup)
+            fields[precision] = out.cpu().numpy()
+            lines[precision] = {
+                "value": round(len(sets) * steps / elapsed, 2),
+                "ms_per_step": round(1e3 * elapsed / steps, 3),
+                "p50_ms": ms(latencies, 50),
+                "p99_ms": ms(latencies, 99),
+                "factor_s": round(setup, 3),
+                "factor_gb": round(solver.nbytes / 1e9, 3),
+                "refinements": stats,
+                "phase_ms_per_step": {k: round(v[0] / steps, 3) for k, v in timing.items() if v[1]},
+                "t": solver.time,
+            }
+            log(f"   {precision}: {lines[precision]['value']} queries/s, {stats}")
+            del solver, mesh
+            torch.cuda.empty_cache()
+        return fields, lines
+
     log("== generating the displaced sphere (icosphere(9), SH degree 16)")
     vertices, faces = creation.displaced_sphere(9, lmax=16, amplitude=0.05, seed=args.seed)
     n = len(vertices)
     sources = creation.random_sources(n, 16, seed=args.seed + 1)
-    sets = [torch.tensor([s], dtype=torch.int32, device="cuda") for s in sources]
-    v_gpu, f_gpu = torch.from_numpy(vertices).cuda(), torch.from_numpy(faces).cuda()
+    fields, lines = solve_both(vertices, faces, sources, args.steps, args.warmup)
     result = {
         "metric": METRIC,
         "unit": "queries/s",
@@ -127,45 +159,31 @@ def sphere_workload(args) -> int:
             "n_vertices": int(n),
             "n_faces": int(faces.size // 3),
         },
+        "value": lines["fp64"]["value"],
+        "dtype": "f64",
+        "fp64": lines["fp64"],
+        "mixed": lines["mixed"],
     }
-    fields = {}
-    for precision in ("fp64", "mixed"):
-        mesh = heat.Trimesh(v_gpu, f_gpu)
-        solver = mesh.heat_solver(method="meshlab", precision=precision)
-        if precision == "mixed":
-            solver.set_refinement(heat=1e-10, poisson=1e-10)
-        t0 = time.perf_counter()
-        assert solver.factor()
-        torch.cuda.synchronize()
-        setup = time.perf_counter() - t0
-        elapsed, latencies, timing, out, stats = timed_steps(solver, sets, args.steps, args.warmup)
-        fields[precision] = out.cpu().numpy()
-        result[precision] = {
-            "value": round(len(sets) * args.steps / elapsed, 2),
-            "ms_per_step": round(1e3 * elapsed / args.steps, 3),
-            "p50_ms": ms(latencies, 50),
-            "p99_ms": ms(latencies, 99),
-            "factor_s": round(setup, 3),
-            "factor_gb": round(solver.nbytes / 1e9, 3),
-            "refinements": stats,
-            "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timing.items() if v[1]},
-            "t": solver.time,
-        }
-        log(f"   {precision}: {result[precision]['value']} queries/s, {stats}")
-        del solver, mesh
-        torch.cuda.empty_cache()
-    result["value"] = result["fp64"]["value"]
-    result["dtype"] = "f64"
-    log("== oracle (SciPy SuperLU, 1 thread)")
+    diffs = [ho.relative_error(fields["mixed"][q], fields["fp64"][q]) for q in range(len(sources))]
+    result["mixed"]["vs_fp64_rel_err_max"] = max(d[0] for d in diffs)
+    result["mixed"]["vs_fp64_rel_err_mean"] = float(np.mean([d[1] for d in diffs]))
+    del fields
+    log("== the oracle on the same surface at icosphere(7) (SciPy SuperLU, 1 thread)")
+    v7, f7 = creation.displaced_sphere(7, lmax=16, amplitude=0.05, seed=args.seed)
+    sample = creation.random_sources(len(v7), max(1, args.cpu_sample), seed=args.seed + 1)
+    fields7, _ = solve_both(v7, f7, sample, 1, 1)
     t0 = time.perf_counter()
-    oracle = ho.FactoredOracle(vertices, faces)
-    result["oracle_factor_s"] = round(time.perf_counter() - t0, 1)
-    sample = list(range(min(args.cpu_sample, len(sets))))
+    oracle = ho.FactoredOracle(v7, f7)
+    log(f"   oracle factored in {time.perf_counter() - t0:.1f}s")
+    check = {"n_vertices": int(len(v7)), "queries": len(sample),
+             "oracle_factor_s": round(time.perf_counter() - t0, 1)}
     for precision in ("fp64", "mixed"):
-        errs = [ho.relative_error(fields[precision][q], oracle.distance(np.array([sources[q]]))) for q in sample]
-        result[precision]["rel_err_max"] = max(e[0] for e in errs)
-        result[precision]["rel_err_mean"] = float(np.mean([e[1] for e in errs]))
-    result["queries_checked"] = len(sample)
+        errs = [ho.relative_error(fields7[precision][q], oracle.distance(np.array([s])))
+                for q, s in enumerate(sample)]
+        check[precision] = {"rel_err_max": max(e[0] for e in errs),
+                            "rel_err_mean": float(np.mean([e[1] for e in errs]))}
+        log(f"   {precision}: max {check[precision]['rel_err_max']:.3g}")
+    result["oracle_check_icosphere7"] = check
     print(json.dumps(result))
     return 0
 
