@@ -426,9 +426,12 @@ def components(pattern: Pattern) -> tuple[int, np.ndarray]:
     return connected_components(graph, directed=False)
 
 
-def factorize(matrix: sp.csr_matrix):
-    """FP64 sparse LU (``scipy.sparse.linalg.splu``) under a symmetric minimum-degree ordering."""
-    return spla.splu(matrix.tocsc(), permc_spec="MMD_AT_PLUS_A")
+def factorize(matrix: sp.csr_matrix, ordering: str = "MMD_AT_PLUS_A"):
+    """FP64 sparse LU (``scipy.sparse.linalg.splu``) under a fill-reducing column ordering:
+    SuperLU's symmetric minimum degree by default; ``"COLAMD"`` where minimum degree is slow (it
+    takes minutes on subdivided icospheres of 10^5 vertices, where COLAMD takes seconds). The
+    ordering changes only the rounding of the direct solve."""
+    return spla.splu(matrix.tocsc(), permc_spec=ordering)
 
 
 def solve_direct(matrix: sp.csr_matrix, rhs: np.ndarray) -> np.ndarray:
@@ -442,12 +445,18 @@ class FactoredOracle:
     (``Trimesh``, factored) form (``heat.py:4394-4423``), solved directly in FP64.
     """
 
-    def __init__(self, vertices: np.ndarray, faces: np.ndarray, t: float | None = None) -> None:
+    def __init__(
+        self,
+        vertices: np.ndarray,
+        faces: np.ndarray,
+        t: float | None = None,
+        ordering: str = "MMD_AT_PLUS_A",
+    ) -> None:
         self.vertices = _f32(vertices)
         self.faces = np.asarray(faces, dtype=np.int64).reshape(-1)
         t_resolved, _ = resolve_time(self.vertices, self.faces, t)
         self.ops = assemble(self.vertices, self.faces, t_resolved)
-        self.heat_lu = factorize(self.ops.matrix(self.ops.heat))
+        self.heat_lu = factorize(self.ops.matrix(self.ops.heat), ordering)
         n = self.ops.pattern.n
         n_comp, labels = components(self.ops.pattern)
         self.members = [np.flatnonzero(labels == c) for c in range(n_comp)]
@@ -456,7 +465,7 @@ class FactoredOracle:
             pinned[members[-1]] = True
         self.keep = ~pinned
         reduced = self.ops.matrix(self.ops.poisson)[self.keep][:, self.keep]
-        self.poisson_lu = factorize(reduced)
+        self.poisson_lu = factorize(reduced, ordering)
 
     def distance(self, sources: np.ndarray) -> np.ndarray:
         n = self.ops.pattern.n
