@@ -5,8 +5,8 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``b
 import this module, and only as the checker: no product path calls it, and the product path fails
 loudly when the HIP library is missing instead of falling back to anything here.
 
-A restatement in numpy/SciPy of ordito's ``heat_geodesic(method="meshlab")`` (ordito 0.5.2), each
-function citing the reference lines it follows. Precision follows the reference's split: the
+A restatement in numpy/SciPy of ordito's ``heat_geodesic(method="meshlab")`` and
+``method="igl"`` (ordito 0.5.2), each function citing the reference lines it follows. Precision follows the reference's split: the
 per-face half-cotangents, unit normals and areas are formed in float32 from float32 positions,
 every assembled operator, vector and solve is float64. Direct solves use SciPy's ``splu``, which
 the reference's own Cholesky tests use as their oracle (``tests/test_cholesky.py:38-98``,
@@ -27,6 +27,14 @@ Pinning (what checks each function; see ``tests/test_oracle.py``):
 - ``geodesic``: the reference's docstring known answers (icosphere(3) pole to pole 3.09, maximum
   3.1, source exactly 0), great-circle error under 7 %, dense ``numpy.linalg`` solves on tiny meshes.
 - ``settled_cg``: converges to the direct solve within the settle rule's accuracy.
+- ``voronoi_corner_areas`` / ``voronoi_mass``: Meyer et al.'s cotangent formula
+  ``(|e_ij|^2 cot C_k + |e_ik|^2 cot C_j) / 8`` on acute faces, the right and obtuse corners' half
+  and quarters, even thirds for a degenerate face, ``sum(m) = sum(area)``.
+- ``rim_vertices`` / ``diffuse_pinned``: the rim against the edges counted on one face; the pinned
+  solve against a direct solve of the interior system padded with zeros; no rim on a closed mesh.
+- ``geodesic(method="igl")``: a source on the rim leaves only the (halved) Neumann heat, so the
+  distance is the Voronoi Neumann one bit for bit; the great-circle bound on a sphere; ``"igl"``
+  and ``"meshlab"`` differ where the choices matter.
 """
 
 from __future__ import annotations
@@ -215,6 +223,82 @@ def lumped_mass(n_vertices: int, faces: np.ndarray, areas: np.ndarray) -> np.nda
     return sums / 3.0
 
 
+def voronoi_corner_areas(vertices: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    """
+    ``(F, 3)`` float32: each corner's share of its face under Meyer et al.'s mixed Voronoi
+    lumping, as ``igl::massmatrix`` forms it at ``MASSMATRIX_TYPE_VORONOI``.
+
+    Follows ``kernels/triangles.py:276-330`` (``face_voronoi_areas``), in float64 from the float32
+    corners: edge lengths ``l_e = |p_{e+1} - p_{e+2}|``, area ``|cross(p1 - p0, p2 - p0)| / 2``,
+    cosines by the law of cosines over the rounded lengths, circumcentric partials
+    ``(cos_e l_e / sum) area``, each corner the mean of the two partials beside it; an obtuse
+    corner (cosine < 0) takes half the area and the others a quarter; a zero-length edge or a zero
+    cosine sum splits the area evenly. Rounded once to float32.
+    """
+    p = _f32(vertices).astype(np.float64)
+    tri = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    p0, p1, p2 = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
+    d0, d1, d2 = p1 - p2, p2 - p0, p0 - p1
+    l0 = np.sqrt(_dot3(d0, d0))
+    l1 = np.sqrt(_dot3(d1, d1))
+    l2 = np.sqrt(_dot3(d2, d2))
+    c = _cross3(p1 - p0, p2 - p0)
+    area = 0.5 * np.sqrt(_dot3(c, c))
+    even = area / 3.0
+    q = np.stack([even, even, even], axis=1)
+    sides = (l0 > 0.0) & (l1 > 0.0) & (l2 > 0.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c0 = (l2 * l2 + l1 * l1 - l0 * l0) / (2.0 * l1 * l2)
+        c1 = (l0 * l0 + l2 * l2 - l1 * l1) / (2.0 * l2 * l0)
+        c2 = (l1 * l1 + l0 * l0 - l2 * l2) / (2.0 * l0 * l1)
+        b0, b1, b2 = c0 * l0, c1 * l1, c2 * l2
+        weight = b0 + b1 + b2
+        circum = sides & (weight != 0.0)
+        part0 = b0 / weight * area
+        part1 = b1 / weight * area
+        part2 = b2 / weight * area
+    q[circum, 0] = (0.5 * (part1 + part2))[circum]
+    q[circum, 1] = (0.5 * (part2 + part0))[circum]
+    q[circum, 2] = (0.5 * (part0 + part1))[circum]
+    half, quarter = 0.5 * area, 0.25 * area
+    obtuse0 = sides & (c0 < 0.0)
+    obtuse1 = sides & ~obtuse0 & (c1 < 0.0)
+    obtuse2 = sides & ~obtuse0 & ~obtuse1 & (c2 < 0.0)
+    for corner, mask in ((0, obtuse0), (1, obtuse1), (2, obtuse2)):
+        q[mask] = quarter[mask, None]
+        q[mask, corner] = half[mask]
+    return q.astype(np.float32)
+
+
+def voronoi_mass(n_vertices: int, faces: np.ndarray, shares: np.ndarray) -> np.ndarray:
+    """
+    Voronoi lumped mass ``m_i = sum of the float64-widened corner shares at i`` (no division).
+
+    Follows ``kernels/heat.py:126-128`` (the shares' corner scatter) and ``heat.py:2169`` (divisor
+    1 for the Voronoi lumping).
+    """
+    tri = np.asarray(faces, dtype=np.int64).reshape(-1)
+    if not tri.size:
+        return np.zeros(n_vertices)
+    return np.bincount(tri, weights=shares.reshape(-1).astype(np.float64), minlength=n_vertices)
+
+
+def unique_edge_sums(vertices: np.ndarray, pattern: Pattern) -> tuple[float, float]:
+    """
+    ``(sum of |v_j - v_i|, count)`` over the strict upper triangle: every unique edge once.
+
+    Follows ``kernels/heat.py:143-235`` (``upper_edge_sums`` without ``face_edges``): float32
+    lengths widened to float64.
+    """
+    p = _f32(vertices)
+    rows = np.repeat(np.arange(pattern.n), np.diff(pattern.indptr))
+    cols = pattern.indices
+    upper = cols > rows
+    d = p[cols[upper]] - p[rows[upper]]
+    lengths = np.sqrt(_dot3(d, d)).astype(np.float64)
+    return float(np.sum(lengths)), float(lengths.size)
+
+
 def face_edge_sums(vertices: np.ndarray, pattern: Pattern) -> tuple[float, float]:
     """
     ``(sum of w * |v_j - v_i|, sum of w)`` over the strict upper triangle, ``w`` the faces holding
@@ -255,14 +339,34 @@ def diffusion_floor(vertices: np.ndarray) -> float:
     return (math.dist(low, high) / DIFFUSION_DECAY) ** 2
 
 
-def resolve_time(vertices: np.ndarray, faces: np.ndarray, t: float | None) -> tuple[float | None, int]:
+def mean_unique_edge_length(vertices: np.ndarray, faces: np.ndarray) -> float:
+    """Mean float32 length of the unique edges (``heat.py:2143-2146``), each edge once."""
+    pattern = operator_pattern(_f32(vertices).shape[0], faces)
+    total, count = unique_edge_sums(vertices, pattern)
+    return total / count if count > 0.0 else 0.0
+
+
+# heat.py:2001-2012: each method's (mass lumping, edges of the mean length h).
+METHODS = {
+    "meshlab": ("barycentric", "face_edges"),
+    "igl": ("voronoi", "unique_edges"),
+    "potpourri3d": ("barycentric", "unique_edges"),
+}
+
+
+def resolve_time(
+    vertices: np.ndarray, faces: np.ndarray, t: float | None, method: str = "meshlab"
+) -> tuple[float | None, int]:
     """
     The time the systems are built at (``None``: the device-formed ``h^2``) and the settle window.
 
     Follows ``HeatSolver._vertex_time`` (``heat.py:3714-3736``) and ``_settle_window``
-    (``heat.py:5234-5243``) at ``method="meshlab"``.
+    (``heat.py:5234-5243``), ``h`` over ``method``'s edges.
     """
-    mean = mean_face_edge_length(vertices, faces)
+    if METHODS[method][1] == "face_edges":
+        mean = mean_face_edge_length(vertices, faces)
+    else:
+        mean = mean_unique_edge_length(vertices, faces)
     default = mean * mean
     if t is None:
         floor = diffusion_floor(vertices)
@@ -274,9 +378,12 @@ def resolve_time(vertices: np.ndarray, faces: np.ndarray, t: float | None) -> tu
     return t, window
 
 
-def assemble(vertices: np.ndarray, faces: np.ndarray, t: float | None = None) -> Operators:
+def assemble(
+    vertices: np.ndarray, faces: np.ndarray, t: float | None = None, method: str = "meshlab"
+) -> Operators:
     """
-    ``L`` (igl sign: negative diagonal), ``M - tL`` and ``-L`` over one pattern, and ``m``.
+    ``L`` (igl sign: negative diagonal), ``M - tL`` and ``-L`` over one pattern, and ``m``
+    (barycentric, or Voronoi at ``method="igl"``).
 
     Follows ``heat.py:2015-2206`` (``_heat_operators``) and ``kernels/heat.py:298-465``
     (``heat_system_rows``): ``L_ij = sum of the run's float64-widened half-cotangents`` in
@@ -290,7 +397,11 @@ def assemble(vertices: np.ndarray, faces: np.ndarray, t: float | None = None) ->
     n = vertices.shape[0]
     terms = face_terms(vertices, faces)
     pattern = operator_pattern(n, faces)
-    mass = lumped_mass(n, faces, terms.areas)
+    lumping, edges = METHODS[method]
+    if lumping == "voronoi":
+        mass = voronoi_mass(n, faces, voronoi_corner_areas(vertices, faces))
+    else:
+        mass = lumped_mass(n, faces, terms.areas)
     cot_flat = terms.cot.reshape(-1).astype(np.float64)
     nnz = pattern.indices.size
     # Off-diagonal values: sequential sum over each entry's corner run.
@@ -304,7 +415,10 @@ def assemble(vertices: np.ndarray, faces: np.ndarray, t: float | None = None) ->
     rows = np.repeat(np.arange(n), np.diff(pattern.indptr))
     diagonal = pattern.indices == rows
     if t is None:
-        total, count = face_edge_sums(vertices, pattern)
+        if edges == "face_edges":
+            total, count = face_edge_sums(vertices, pattern)
+        else:
+            total, count = unique_edge_sums(vertices, pattern)
         h = total / count if count > 0.0 else 0.0
         factor = -1.0 * (h * h)
         t_value = h * h
@@ -451,12 +565,18 @@ class FactoredOracle:
         faces: np.ndarray,
         t: float | None = None,
         ordering: str = "MMD_AT_PLUS_A",
+        method: str = "meshlab",
     ) -> None:
         self.vertices = _f32(vertices)
         self.faces = np.asarray(faces, dtype=np.int64).reshape(-1)
-        t_resolved, _ = resolve_time(self.vertices, self.faces, t)
-        self.ops = assemble(self.vertices, self.faces, t_resolved)
+        t_resolved, _ = resolve_time(self.vertices, self.faces, t, method)
+        self.ops = assemble(self.vertices, self.faces, t_resolved, method)
         self.heat_lu = factorize(self.ops.matrix(self.ops.heat), ordering)
+        self.rim = rim_vertices(self.ops.pattern)
+        self.pinned_lu = None
+        if method == "igl" and self.rim.any():
+            pinned = self.ops.matrix(pinned_heat_values(self.ops, self.rim))
+            self.pinned_lu = factorize(pinned, ordering)
         n = self.ops.pattern.n
         n_comp, labels = components(self.ops.pattern)
         self.members = [np.flatnonzero(labels == c) for c in range(n_comp)]
@@ -471,6 +591,8 @@ class FactoredOracle:
         n = self.ops.pattern.n
         u0 = seed_indicator(n, sources)
         heat = self.heat_lu.solve(u0)
+        if self.pinned_lu is not None:
+            heat = average_heat(heat, self.pinned_lu.solve(np.where(self.rim, 0.0, u0)))
         neg_div = face_unit_divergence(self.vertices, self.faces, self.ops.terms, heat)
         phi = np.zeros(n)
         phi[self.keep] = self.poisson_lu.solve(neg_div[self.keep])
@@ -510,6 +632,52 @@ def solve_poisson(ops: Operators, rhs: np.ndarray) -> np.ndarray:
     return phi
 
 
+def rim_vertices(pattern: Pattern) -> np.ndarray:
+    """
+    ``(V,)`` bool: the vertices with an edge on a single face.
+
+    Follows ``kernels/laplacian.py:1286-1321`` (``pattern_row_on_rim``): some off-diagonal entry of
+    the row has a run of one corner slot.
+    """
+    rows = np.repeat(np.arange(pattern.n), np.diff(pattern.indptr))
+    single = (np.diff(pattern.corner_ptr) == 1) & (pattern.indices != rows)
+    rim = np.zeros(pattern.n, dtype=bool)
+    rim[rows[single]] = True
+    return rim
+
+
+def pinned_heat_values(ops: Operators, rim: np.ndarray) -> np.ndarray:
+    """
+    The heat system's values with the solution pinned to zero on the rim: a rim row becomes the
+    identity's, an interior row drops its rim columns (``kernels/heat.py:487-508``,
+    ``pinned_heat_rows``).
+    """
+    pattern = ops.pattern
+    rows = np.repeat(np.arange(pattern.n), np.diff(pattern.indptr))
+    values = ops.heat.copy()
+    values[rim[pattern.indices]] = 0.0
+    on_rim = rim[rows]
+    values[on_rim] = np.where(pattern.indices[on_rim] == rows[on_rim], 1.0, 0.0)
+    return values
+
+
+def diffuse_pinned(ops: Operators, rhs: np.ndarray) -> np.ndarray | None:
+    """
+    ``HeatSolver.diffuse_pinned`` (``heat.py:4153-4200``) solved directly: the pinned system
+    against the right-hand side zeroed on the rim (``kernels/heat.py:511-514``, ``off_rim``);
+    ``None`` for a mesh without a rim, where nothing is solved.
+    """
+    rim = rim_vertices(ops.pattern)
+    if not rim.any():
+        return None
+    return solve_direct(ops.matrix(pinned_heat_values(ops, rim)), np.where(rim, 0.0, rhs))
+
+
+def average_heat(neumann: np.ndarray, pinned: np.ndarray) -> np.ndarray:
+    """``0.5 * (neumann + pinned)``: igl's boundary rule (``kernels/heat.py:517-520``)."""
+    return 0.5 * (neumann + pinned)
+
+
 @dataclass
 class GeodesicResult:
     distance: np.ndarray
@@ -519,10 +687,16 @@ class GeodesicResult:
 
 
 def geodesic(
-    vertices: np.ndarray, faces: np.ndarray, sources: np.ndarray, t: float | None = None
+    vertices: np.ndarray,
+    faces: np.ndarray,
+    sources: np.ndarray,
+    t: float | None = None,
+    method: str = "meshlab",
 ) -> GeodesicResult:
     """
-    ``heat_geodesic(vertices, faces, sources, t, method="meshlab")`` solved directly in FP64.
+    ``heat_geodesic(vertices, faces, sources, t, method=method)`` solved directly in FP64
+    (``method`` "meshlab" or "igl"; at "igl" the Voronoi mass, the unique-edge time and the
+    heat averaged with the rim-pinned heat, ``heat.py:594-597``).
 
     Follows ``heat.py:552-609`` (``_heat_geodesic``): empty input gives zeros; the time rule of
     ``heat.py:3714-3736``; heat step ``(M - tL) u = u0`` (``heat.py:3890-3914``); divergence
@@ -538,9 +712,13 @@ def geodesic(
         z = np.zeros(n)
         return GeodesicResult(z, z, z, None)  # type: ignore[arg-type]
     u0 = seed_indicator(n, sources)
-    t_resolved, _ = resolve_time(vertices, faces, t)
-    ops = assemble(vertices, faces, t_resolved)
+    t_resolved, _ = resolve_time(vertices, faces, t, method)
+    ops = assemble(vertices, faces, t_resolved, method)
     heat = solve_direct(ops.matrix(ops.heat), u0)
+    if method == "igl":
+        pinned = diffuse_pinned(ops, u0)
+        if pinned is not None:
+            heat = average_heat(heat, pinned)
     neg_div = face_unit_divergence(vertices, faces, ops.terms, heat)
     phi = solve_poisson(ops, neg_div)
     distance = shift_and_orient(phi, sources)
@@ -548,10 +726,14 @@ def geodesic(
 
 
 def geodesic_columns(
-    vertices: np.ndarray, faces: np.ndarray, source_sets: list[np.ndarray], t: float | None = None
+    vertices: np.ndarray,
+    faces: np.ndarray,
+    source_sets: list[np.ndarray],
+    t: float | None = None,
+    method: str = "meshlab",
 ) -> np.ndarray:
     """One distance field per source set, sharing one factorization of each system: (k, V)."""
-    oracle = FactoredOracle(vertices, faces, t)
+    oracle = FactoredOracle(vertices, faces, t, method=method)
     return np.stack([oracle.distance(s) for s in source_sets]) if source_sets else np.zeros(
         (0, _f32(vertices).shape[0])
     )

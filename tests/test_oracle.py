@@ -289,3 +289,140 @@ def test_time_rule_raises_the_floor_on_fine_meshes():
     assert window == math.ceil(192 * math.sqrt(t / (h * h)))
     v, f = creation.icosphere(3)
     assert ho.resolve_time(v, f, None) == (None, 192)
+
+
+# --- method="igl": Voronoi mass, rim-pinned heat -----------------------------------------------
+
+
+def _meyer_shares(p: np.ndarray) -> np.ndarray:
+    """Meyer et al.'s Voronoi area of each corner of an acute triangle, by their cotangent formula
+    ``(|e_ij|^2 cot C_k + |e_ik|^2 cot C_j) / 8`` (a different route from the oracle's cosines
+    and circumcentric partials)."""
+
+    def cot(at, a, b):
+        u, v = a - at, b - at
+        return np.dot(u, v) / np.linalg.norm(np.cross(u, v))
+
+    out = np.zeros(3)
+    for i in range(3):
+        j, k = (i + 1) % 3, (i + 2) % 3
+        e_ij, e_ik = p[j] - p[i], p[k] - p[i]
+        out[i] = (np.dot(e_ij, e_ij) * cot(p[k], p[i], p[j]) + np.dot(e_ik, e_ik) * cot(p[j], p[i], p[k])) / 8
+    return out
+
+
+def test_voronoi_shares_follow_meyers_cotangent_formula_on_acute_faces():
+    rng = np.random.default_rng(11)
+    checked = 0
+    for _ in range(200):
+        p = rng.normal(size=(3, 3)).astype(np.float32)
+        q = p.astype(np.float64)
+        cosines = []
+        for i in range(3):
+            u, v = q[(i + 1) % 3] - q[i], q[(i + 2) % 3] - q[i]
+            cosines.append(np.dot(u, v) / (np.linalg.norm(u) * np.linalg.norm(v)))
+        if min(cosines) < 0.05:
+            continue  # the formula holds on acute faces only
+        shares = ho.voronoi_corner_areas(p, np.array([0, 1, 2]))[0]
+        assert np.allclose(shares, _meyer_shares(q), rtol=1e-6)
+        checked += 1
+    assert checked > 20
+
+
+def test_voronoi_right_and_obtuse_corners_take_half_and_degenerate_faces_split_evenly():
+    right = ho.voronoi_corner_areas(np.array([[0, 0, 0], [2, 0, 0], [0, 1, 0]], np.float32), np.arange(3))[0]
+    assert np.allclose(right, [0.5, 0.25, 0.25], rtol=1e-7)  # circumcentre on the hypotenuse
+    obtuse = ho.voronoi_corner_areas(np.array([[3, 0, 0], [0, 0, 0], [1, 0.2, 0]], np.float32), np.arange(3))[0]
+    area = 0.5 * 3 * 0.2
+    assert np.allclose(obtuse, [area / 4, area / 4, area / 2], rtol=1e-6)  # obtuse at corner 2
+    flat = ho.voronoi_corner_areas(np.array([[0, 0, 0], [1, 0, 0], [1, 0, 0]], np.float32), np.arange(3))[0]
+    assert not flat.any()
+    equilateral = ho.voronoi_corner_areas(
+        np.array([[0, 0, 0], [1, 0, 0], [0.5, math.sqrt(3) / 2, 0]], np.float32), np.arange(3)
+    )[0]
+    assert np.allclose(equilateral, math.sqrt(3) / 12, rtol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["bumpy", "grid", "saddle_graded"])
+def test_voronoi_mass_sums_to_the_area_and_differs_from_the_barycentric(name):
+    v, f = mesh(name)
+    igl = ho.assemble(v, f, None, method="igl")
+    meshlab = ho.assemble(v, f, None, method="meshlab")
+    p = v.astype(np.float64)
+    tri = f.reshape(-1, 3)
+    area = 0.5 * np.linalg.norm(np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]]), axis=1).sum()
+    assert np.isclose(igl.mass.sum(), area, rtol=1e-6)
+    assert np.abs(igl.mass - meshlab.mass).max() > 1e-3 * meshlab.mass.max()
+    # Same Laplacian; the time is the unique-edge mean squared.
+    assert np.array_equal(igl.laplacian, meshlab.laplacian)
+    edges = np.unique(np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), axis=1), axis=0)
+    h = np.linalg.norm(p[edges[:, 1]] - p[edges[:, 0]], axis=1).mean()
+    assert np.isclose(igl.t, h * h, rtol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["grid", "saddle_graded", "icosphere3"])
+def test_rim_is_the_vertices_of_edges_on_one_face(name):
+    v, f = mesh(name)
+    tri = f.reshape(-1, 3)
+    edges = np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), axis=1)
+    unique, counts = np.unique(edges, axis=0, return_counts=True)
+    expected = np.zeros(v.shape[0], dtype=bool)
+    expected[unique[counts == 1].reshape(-1)] = True
+    assert np.array_equal(ho.rim_vertices(ho.operator_pattern(v.shape[0], f)), expected)
+
+
+@pytest.mark.parametrize("name", ["grid", "saddle_graded"])
+def test_pinned_solve_is_the_interior_system_padded_with_zeros(name):
+    """The reference's own check (tests/test_heat.py:295-329): zero on the rim, the interior
+    system's direct solve elsewhere."""
+    v, f = mesh(name)
+    n = v.shape[0]
+    ops = ho.assemble(v, f, None, method="igl")
+    rhs = np.zeros(n)
+    rhs[n // 2 + 17] = 1.0
+    got = ho.diffuse_pinned(ops, rhs)
+    rim = ho.rim_vertices(ops.pattern)
+    interior = np.flatnonzero(~rim)
+    A = ops.matrix(ops.heat)
+    expected = np.zeros(n)
+    expected[interior] = np.linalg.solve(A[interior][:, interior].toarray(), rhs[interior])
+    assert np.array_equal(got[rim], np.zeros(rim.sum()))
+    assert np.allclose(got, expected, rtol=1e-10, atol=1e-14 * expected.max())
+
+
+def test_a_closed_mesh_has_no_rim_and_no_pinned_solve():
+    v, f = mesh("icosphere3")
+    ops = ho.assemble(v, f, None, method="igl")
+    assert ho.diffuse_pinned(ops, np.ones(v.shape[0])) is None
+
+
+def test_igl_from_a_rim_source_is_the_voronoi_neumann_distance():
+    """A source on the rim zeroes the pinned right-hand side, so the heat is the Neumann heat
+    halved; the unit gradient does not see the factor 2, so the distance is the same bits."""
+    v, f = mesh("grid")
+    ops = ho.assemble(v, f, None, method="igl")
+    rim = ho.rim_vertices(ops.pattern)
+    source = int(np.flatnonzero(rim)[0])
+    got = ho.geodesic(v, f, [source], method="igl").distance
+    u0 = ho.seed_indicator(v.shape[0], [source])
+    heat = ho.solve_direct(ops.matrix(ops.heat), u0)
+    phi = ho.solve_poisson(ops, ho.face_unit_divergence(v, f, ops.terms, heat))
+    assert np.array_equal(got, ho.shift_and_orient(phi, np.array([source])))
+
+
+def test_igl_choices_matter_and_stay_within_the_great_circle_bound():
+    v, f = mesh("icosphere4")
+    d = ho.geodesic(v, f, [0], method="igl").distance
+    p = v.astype(np.float64)
+    p /= np.linalg.norm(p, axis=1)[:, None]
+    exact = np.arccos(np.clip(p @ p[0], -1.0, 1.0))
+    assert np.max(np.abs(d - exact)) / np.ptp(exact) < 0.07
+    g, _ = mesh("grid")
+    _, gf = mesh("grid")
+    middle = g.shape[0] // 2 + 17
+    igl = ho.geodesic(g, gf, [middle], method="igl").distance
+    meshlab = ho.geodesic(g, gf, [middle]).distance
+    assert np.abs(igl - meshlab).max() > 1e-3 * np.ptp(meshlab)
+    factored = ho.FactoredOracle(g, gf, method="igl").distance(np.array([middle]))
+    assert np.allclose(factored, igl, atol=1e-12 * np.ptp(igl))
+
