@@ -35,6 +35,8 @@ extern "C" {
 
 /* Discretizations (heat.py:2002-2012): the lumped mass and the mean edge length of t = h^2. */
 #define HG_METHOD_MESHLAB 0     /* barycentric mass, h over every face's three edges */
+#define HG_METHOD_IGL 1         /* Voronoi mass, h over the unique edges; a geodesic averages the
+                                   Neumann heat with the heat pinned to zero on the rim */
 #define HG_METHOD_POTPOURRI3D 2 /* barycentric mass, h over the unique edges */
 
 /* How a solve runs (HeatSolver policy, heat.py:4348-4423). */
@@ -49,6 +51,7 @@ extern "C" {
 /* Which systems hg_factor builds. */
 #define HG_SYSTEM_HEAT 1
 #define HG_SYSTEM_POISSON 2
+#define HG_SYSTEM_PINNED 4 /* the heat system pinned to zero on the rim (nothing on a closed mesh) */
 
 typedef struct hg_mesh hg_mesh;
 
@@ -62,13 +65,14 @@ typedef struct hg_operator_view {
   const double* laplacian;  /* (nnz) L, igl sign convention (negative diagonal) */
   const double* heat;       /* (nnz) M - t L */
   const double* poisson;    /* (nnz) -L */
-  const double* mass;       /* (V) barycentric lumped mass */
+  const double* mass;       /* (V) lumped mass: barycentric, or Voronoi at HG_METHOD_IGL */
   const float* cot;         /* (F,3) half-cotangents */
   const float* normals;     /* (F,3) unit face normals */
   const float* areas;       /* (F) face areas */
   double t;                 /* the diffusion time the heat system was built at */
   int32_t settle_rounds;    /* the settle window of the heat solve (heat.py:5234-5243) */
-  int32_t time_from_edges;  /* 1: t is the device-formed squared mean face-edge length */
+  int32_t time_from_edges;  /* 1: t is the device-formed squared mean edge length (the method's
+                               edges) */
 } hg_operator_view;
 
 /* Statistics of the last solve on a mesh. */
@@ -77,8 +81,8 @@ typedef struct hg_solve_stats {
   int32_t poisson_rounds;
   int32_t heat_refinements;  /* refinement solves of the factored heat step */
   int32_t poisson_refinements;
-  int32_t backward_error_failed; /* 1 when the settled iterate failed its backward-error check */
-  int32_t reserved;
+  int32_t backward_error_failed; /* 1 when a settled iterate failed its backward-error check */
+  int32_t pinned_solves;     /* the rim-pinned heat step's CG rounds or refinement solves */
 } hg_solve_stats;
 
 /* Thread-local message of the last failing call ("" if none). */
@@ -121,7 +125,7 @@ int hg_operator_views(hg_mesh* mesh, hg_operator_view* out_view);
  */
 int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_factored);
 
-/* Whether each system holds a factor: bit 0 heat, bit 1 Poisson. */
+/* Whether each system holds a factor: bit 0 heat, bit 1 Poisson, bit 2 rim-pinned heat. */
 int hg_factored(hg_mesh* mesh, int32_t* out_systems);
 
 /*
@@ -156,6 +160,19 @@ int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n
                int32_t mode, int32_t precision, int32_t* out_flags);
 
 /*
+ * Diffuse with the solution pinned to zero on the mesh's rim (HeatSolver.diffuse_pinned,
+ * heat.py:4153-4200): (M - tL) u = rhs on the interior vertices, u = 0 on every vertex with an edge
+ * on a single face, the system built on first use (heat.py:4202-4233) and settled, verified and
+ * factored as hg_diffuse's; the rim entries of d_rhs are ignored. *out_has_rim is 0 for a mesh
+ * without a rim, whose d_solution is then left as it was.
+ */
+int hg_diffuse_pinned(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
+                      int32_t mode, int32_t precision, int32_t* out_flags, int32_t* out_has_rim);
+
+/* The number of rim vertices (an edge on a single face; kernels/laplacian.py:1286-1321). */
+int hg_rim_count(hg_mesh* mesh, int64_t* out_count);
+
+/*
  * Solve the Poisson system -L phi = rhs (HeatSolver.solve_poisson, heat.py:4062-4151): iterated
  * by conjugate gradient under the Jacobi-Chebyshev polynomial to relative residual 1e-8, or
  * through the held factor with normwise refinement to 1e-8 (heat.py:4138-4140). One row per
@@ -188,8 +205,9 @@ int hg_shift_to_sources(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
                         const int64_t* source_offsets, int64_t n_columns);
 
 /*
- * The whole path for n_queries source sets (heat_geodesic, heat.py:370-609): seed, diffuse,
- * divergence, Poisson, shift. Query q's sources are d_sources[off[q]..off[q+1]) with
+ * The whole path for n_queries source sets (heat_geodesic, heat.py:370-609): seed, diffuse (at
+ * HG_METHOD_IGL averaged with the rim-pinned diffusion on a mesh with a rim), divergence, Poisson,
+ * shift. Query q's sources are d_sources[off[q]..off[q+1]) with
  * source_offsets a host array; its (V,) float64 distance goes to d_out + q*V. A query with no
  * sources returns zeros. Statistics of the call go to *out_stats when not NULL.
  */

@@ -22,6 +22,7 @@ HG_ERR_NOT_PD = 3
 HG_ERR_BUDGET = 4
 
 HG_METHOD_MESHLAB = 0
+HG_METHOD_IGL = 1
 HG_METHOD_POTPOURRI3D = 2
 HG_MODE_ITERATE = 0
 HG_MODE_FACTORED = 1
@@ -29,6 +30,7 @@ HG_PRECISION_FP64 = 0
 HG_PRECISION_MIXED = 1
 HG_SYSTEM_HEAT = 1
 HG_SYSTEM_POISSON = 2
+HG_SYSTEM_PINNED = 4
 
 # The C ABI's exported symbols, in header order; the load test checks every one resolves.
 EXPORTED = (
@@ -44,6 +46,8 @@ EXPORTED = (
     "hg_set_refinement",
     "hg_set_plan_options",
     "hg_diffuse",
+    "hg_diffuse_pinned",
+    "hg_rim_count",
     "hg_solve_poisson",
     "hg_unit_gradient_divergence",
     "hg_seed_sources",
@@ -97,11 +101,11 @@ class SolveStats(ctypes.Structure):
         ("heat_refinements", ctypes.c_int32),
         ("poisson_refinements", ctypes.c_int32),
         ("backward_error_failed", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("pinned_solves", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict[str, int]:
-        return {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved"}
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
 _lib: ctypes.CDLL | None = None
@@ -126,6 +130,8 @@ _SIGNATURES = {
     "hg_set_refinement": ([_P, _I32, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "hg_set_plan_options": ([_P, _I32, _I32, _I32], ctypes.c_int),
     "hg_diffuse": ([_P, _P, _P, _I64, _I32, _I32, _PI32], ctypes.c_int),
+    "hg_diffuse_pinned": ([_P, _P, _P, _I64, _I32, _I32, _PI32, _PI32], ctypes.c_int),
+    "hg_rim_count": ([_P, _PI64], ctypes.c_int),
     "hg_solve_poisson": ([_P, _P, _P, _I64, _I32, _I32], ctypes.c_int),
     "hg_unit_gradient_divergence": ([_P, _P, _P, _I64], ctypes.c_int),
     "hg_seed_sources": ([_P, _P, _PI64, _I64, _P], ctypes.c_int),

@@ -56,12 +56,23 @@ struct hg_mesh {
       vertex_corners;
   // Operators.
   bool assembled = false;
+  int32_t method = HG_METHOD_MESHLAB;
   double t = 0.0;
   int32_t settle_rounds = kHeatSettleRounds;
   int32_t time_from_edges = 1;
   DeviceArray<float> cot, normals, areas, heat32;
   DeviceArray<double> mass, lap, heat, poisson, heat_inv, poisson_inv, scalars, edge_partials;
   DeviceArray<double> face_geom;  // kFaceGeom doubles a face: the divergence's per-face data
+  DeviceArray<float> voronoi;     // HG_METHOD_IGL: each corner's Voronoi share (F,3)
+  // The rim (vertices with an edge on one face) and the heat system pinned to zero there, built on
+  // its first use (HeatSolver._pinned_system, heat.py:4202-4233).
+  int64_t n_rim = 0;
+  std::vector<int32_t> rim;
+  DeviceArray<int32_t> d_rim, d_rim_rows;  // by vertex; by row of the elimination order
+  bool pinned_built = false;
+  DeviceArray<double> pinned, pinned_inv;
+  DeviceArray<float> pinned32;
+  DeviceArray<double> pinned_rhs;
   double poisson_bound = -1.0;
   // Components: the Poisson system pins each component's last-eliminated row.
   int32_t n_comp = 0;
@@ -81,7 +92,7 @@ struct hg_mesh {
   DeviceArray<int32_t> faces_rows;  // the faces' corners as rows of the elimination order
   DeviceArray<int4> row_corners;    // each row's corners {corner id, its face's three rows}
   DeviceArray<int32_t> row_corner_ptr;
-  Factor heat_factor, poisson_factor;
+  Factor heat_factor, poisson_factor, pinned_factor;
   // Refinement targets of the factored solves, [precision][0 heat (componentwise), 1 Poisson
   // (normwise)].
   double refine_tolerance[2][2] = {{kSettleDirectTolerance, kPoissonTolerance},
@@ -214,6 +225,7 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
     }
   }
 
+  m.method = method;
   m.cot.allocate(3 * m.f);
   m.normals.allocate(3 * m.f);
   m.areas.allocate(m.f);
@@ -232,7 +244,12 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   launch_face_terms(m.xyz.ptr, m.faces.ptr, m.f, m.cot.ptr, m.normals.ptr, m.areas.ptr, st);
   launch_face_geometry(m.xyz.ptr, m.faces.ptr, m.normals.ptr, m.areas.ptr, m.cot.ptr, m.f,
                        m.face_geom.ptr, st);
-  launch_mass(m.vertex_ptr.ptr, m.vertex_corners.ptr, m.areas.ptr, n, m.mass.ptr, st);
+  if (method == HG_METHOD_IGL) {
+    m.voronoi.allocate(3 * m.f);
+    launch_voronoi_shares(m.xyz.ptr, m.faces.ptr, m.f, m.voronoi.ptr, st);
+  }
+  launch_mass(m.vertex_ptr.ptr, m.vertex_corners.ptr, m.areas.ptr,
+              method == HG_METHOD_IGL ? m.voronoi.ptr : nullptr, n, m.mass.ptr, st);
   // One edge-sum reduction per segment, its blocks counted from the segment's first row: the sums
   // (and so the times) are those the segment's mesh forms alone. The launches share the partials
   // buffer in stream order.
@@ -255,6 +272,8 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   m.poisson_bound = -1.0;
   m.heat_factor.reset();
   m.poisson_factor.reset();
+  m.pinned_factor.reset();
+  m.pinned_built = false;
 }
 
 void ensure_operators(hg_mesh& m) {
@@ -287,6 +306,11 @@ void ensure_plan(hg_mesh& m) {
     }
     m.row_corners.upload(corners, m.stream);
     m.row_corner_ptr.upload(ptr, m.stream);
+  }
+  if (m.n_rim > 0) {
+    std::vector<int32_t> rim_rows(static_cast<size_t>(m.n), 0);
+    for (int64_t v = 0; v < m.n; ++v) rim_rows[plan.iperm[v]] = m.rim[v];
+    m.d_rim_rows.upload(rim_rows, m.stream);
   }
   // The distance's finishing pass reads components as row ranges of the elimination order.
   m.rows_by_component = true;
@@ -336,9 +360,31 @@ void ensure_plan(hg_mesh& m) {
   m.plan = std::move(device);
 }
 
+// The rim-pinned heat system over the heat system's pattern (meshes with a rim only).
+void ensure_pinned_system(hg_mesh& m) {
+  ensure_operators(m);
+  if (m.pinned_built) return;
+  const int64_t nnz = static_cast<int64_t>(m.pattern.columns.size());
+  m.pinned.allocate(nnz);
+  m.pinned32.allocate(nnz);
+  m.pinned_inv.allocate(m.n);
+  launch_pinned_rows(m.offsets.ptr, m.columns.ptr, m.d_rim.ptr, m.heat.ptr, m.n, m.pinned.ptr,
+                     m.pinned32.ptr, m.pinned_inv.ptr, m.stream);
+  m.pinned_built = true;
+}
+
 void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
   ensure_operators(m);
   ensure_plan(m);
+  if (system == HG_SYSTEM_PINNED) {
+    if (m.n_rim == 0) return;
+    ensure_pinned_system(m);
+    PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
+    if (!m.pinned_factor.held() || (keep_fp32 && !m.pinned_factor.w32.ptr))
+      factor_numeric(*m.plan, m.pinned.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
+                     m.heat_pin_slots, m.pinned_factor, keep_fp32, m.stream);
+    return;
+  }
   PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
   if (system == HG_SYSTEM_HEAT) {
     if (!m.heat_factor.held())
@@ -356,7 +402,25 @@ void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
 
 // ---- solves -----------------------------------------------------------------------------------
 
-int32_t diffuse_iterated(hg_mesh& m, const double* rhs, double* x, int64_t k) {
+// The heat system (Neumann on a rim) or the one pinned to zero on the rim.
+struct HeatSystem {
+  const double* values;
+  const float* values32;
+  const double* inv_diag;
+  Factor* factor;
+  int32_t code;  // HG_SYSTEM_HEAT or HG_SYSTEM_PINNED
+};
+
+HeatSystem heat_system(hg_mesh& m, bool pinned) {
+  if (pinned) {
+    ensure_pinned_system(m);
+    return {m.pinned.ptr, m.pinned32.ptr, m.pinned_inv.ptr, &m.pinned_factor, HG_SYSTEM_PINNED};
+  }
+  return {m.heat.ptr, m.heat32.ptr, m.heat_inv.ptr, &m.heat_factor, HG_SYSTEM_HEAT};
+}
+
+int32_t diffuse_iterated(hg_mesh& m, const HeatSystem& sys, const double* rhs, double* x,
+                         int64_t k) {
   PhaseScope scope(&m.timer, kPhaseCg, m.stream);
   int32_t total = 0;
   for (int64_t q0 = 0; q0 < k; q0 += 64) {
@@ -366,9 +430,9 @@ int32_t diffuse_iterated(hg_mesh& m, const double* rhs, double* x, int64_t k) {
     p.n_columns = kk;
     p.offsets = m.offsets.ptr;
     p.columns = m.columns.ptr;
-    p.values32 = m.heat32.ptr;
-    p.values64 = m.heat.ptr;
-    p.inv_diag = m.heat_inv.ptr;
+    p.values32 = sys.values32;
+    p.values64 = sys.values;
+    p.inv_diag = sys.inv_diag;
     p.rhs = rhs + q0 * m.n;
     p.x = x + q0 * m.n;
     p.settle = 1;
@@ -380,22 +444,23 @@ int32_t diffuse_iterated(hg_mesh& m, const double* rhs, double* x, int64_t k) {
   return total;
 }
 
-bool backward_error_fails(hg_mesh& m, const double* x, const double* b, int64_t k) {
+bool backward_error_fails(hg_mesh& m, const double* values, const double* x, const double* b,
+                          int64_t k) {
   if (m.invalid.size < 1) m.invalid.allocate(1);
   HG_HIP(hipMemsetAsync(m.invalid.ptr, 0, sizeof(int32_t), m.stream));
-  launch_backward_error(m.offsets.ptr, m.columns.ptr, m.heat.ptr, x, b, m.n, k,
-                        kSettleBackwardError, m.invalid.ptr, m.stream);
+  launch_backward_error(m.offsets.ptr, m.columns.ptr, values, x, b, m.n, k, kSettleBackwardError,
+                        m.invalid.ptr, m.stream);
   int32_t flag = 0;
   HG_HIP(hipMemcpyAsync(&flag, m.invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, m.stream));
   HG_HIP(hipStreamSynchronize(m.stream));
   return flag != 0;
 }
 
-int32_t diffuse_factored(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t precision,
-                         FieldLayout layout = FieldLayout::kColumns) {
+int32_t diffuse_factored(hg_mesh& m, const HeatSystem& sys, double* rhs, double* x, int64_t k,
+                         int32_t precision, FieldLayout layout = FieldLayout::kColumns) {
   const bool mixed = precision == HG_PRECISION_MIXED;
-  factor_system(m, HG_SYSTEM_HEAT, mixed);
-  return refined_solve(*m.plan, m.heat_factor, mixed, rhs, x, k,
+  factor_system(m, sys.code, mixed);
+  return refined_solve(*m.plan, *sys.factor, mixed, rhs, x, k,
                        m.refine_tolerance[mixed ? 1 : 0][0], true,
                        mixed ? kMixedRefineRounds : kRefineRounds,
                        m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr, m.solve, m.stream,
@@ -403,26 +468,34 @@ int32_t diffuse_factored(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t 
 }
 
 // The settled iterate, verified, falling back to the factorization (solve_spd_settled,
-// linalg.py:2659-2727). Returns the flags of hg_diffuse.
-// Fields stored by rows (FieldLayout::kRows) go through the factors only.
+// linalg.py:2659-2727), of the heat system or (pinned) the rim-pinned one. Returns the flags of
+// hg_diffuse; the solve's rounds or refinement solves go to the call's statistics. Fields stored
+// by rows (FieldLayout::kRows) go through the factors only.
 int32_t diffuse(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
-                int32_t precision, FieldLayout layout = FieldLayout::kColumns) {
+                int32_t precision, FieldLayout layout = FieldLayout::kColumns,
+                bool pinned = false) {
   ensure_operators(m);
+  const HeatSystem sys = heat_system(m, pinned);
   int32_t flags = 0;
-  if (mode == HG_MODE_FACTORED || m.heat_factor.held()) {
-    m.stats.heat_refinements = diffuse_factored(m, rhs, x, k, precision, layout);
-    m.stats.heat_rounds = 0;
-    return flags;
+  int32_t rounds = 0, refinements = 0;
+  if (mode == HG_MODE_FACTORED || sys.factor->held()) {
+    refinements = diffuse_factored(m, sys, rhs, x, k, precision, layout);
+  } else {
+    HG_REQUIRE(layout == FieldLayout::kColumns, HG_ERR_RUNTIME, "an iterated solve takes columns");
+    rounds = diffuse_iterated(m, sys, rhs, x, k);
+    if (backward_error_fails(m, sys.values, x, rhs, k)) {
+      flags |= 1;
+      m.stats.backward_error_failed = 1;
+      const bool was_held = sys.factor->held();
+      refinements = diffuse_factored(m, sys, rhs, x, k, HG_PRECISION_FP64);
+      if (!was_held) flags |= 2;
+    }
   }
-  HG_REQUIRE(layout == FieldLayout::kColumns, HG_ERR_RUNTIME, "an iterated solve takes columns");
-  m.stats.heat_rounds = diffuse_iterated(m, rhs, x, k);
-  m.stats.heat_refinements = 0;
-  if (backward_error_fails(m, x, rhs, k)) {
-    flags |= 1;
-    m.stats.backward_error_failed = 1;
-    bool was_held = m.heat_factor.held();
-    m.stats.heat_refinements = diffuse_factored(m, rhs, x, k, HG_PRECISION_FP64);
-    if (!was_held) flags |= 2;
+  if (pinned) {
+    m.stats.pinned_solves = refinements > 0 ? refinements : rounds;
+  } else {
+    m.stats.heat_rounds = rounds;
+    m.stats.heat_refinements = refinements;
   }
   return flags;
 }
@@ -581,6 +654,16 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   }
   seed(m, offsets, k, u0, layout);
   int32_t flags = diffuse(m, u0, heat, k, mode, precision, layout);
+  if (m.method == HG_METHOD_IGL && m.n_rim > 0) {
+    // igl::heat_geodesics' boundary rule (heat.py:594-597): the Neumann heat averaged with the
+    // heat pinned to zero on the rim, solved from the seeds zeroed there. The divergence's buffer
+    // holds the pinned heat until the average has read it.
+    launch_off_rim(u0, u0, layout == FieldLayout::kRows ? m.d_rim_rows.ptr : m.d_rim.ptr, n, k,
+                   layout, m.stream);
+    flags |= diffuse(m, u0, negdiv, k, layout == FieldLayout::kRows ? HG_MODE_FACTORED : mode,
+                     precision, layout, true);
+    launch_average(heat, negdiv, n * k, m.stream);
+  }
   m.stats.backward_error_failed = flags & 1;
   divergence(m, heat, negdiv, k, layout);
   if (finish && layout == FieldLayout::kRows && m.rows_by_component) {
@@ -617,6 +700,20 @@ void bind(hg_mesh& m) {
   m.corner_slots.upload(m.pattern.corner_slots, st);
   m.vertex_ptr.upload(m.pattern.vertex_ptr, st);
   m.vertex_corners.upload(m.pattern.vertex_corners, st);
+  // The rim: rows with an off-diagonal entry of a single corner, i.e. an edge on one face
+  // (kernels/laplacian.py:1286-1321).
+  m.rim.assign(static_cast<size_t>(n_vertices), 0);
+  m.n_rim = 0;
+  for (int64_t v = 0; v < n_vertices; ++v) {
+    for (int32_t e = m.pattern.offsets[v]; e < m.pattern.offsets[v + 1]; ++e) {
+      if (m.pattern.columns[e] != v && m.pattern.corner_ptr[e + 1] - m.pattern.corner_ptr[e] == 1) {
+        m.rim[v] = 1;
+        ++m.n_rim;
+        break;
+      }
+    }
+  }
+  if (m.n_rim > 0) m.d_rim.upload(m.rim, st);
   m.comp_ptr.upload(comp_ptr, st);
   m.comp_ptr_host = comp_ptr;
   m.d_labels.upload(m.labels, st);
@@ -725,8 +822,9 @@ int hg_mesh_set_stream(hg_mesh* mesh, void* stream) {
 int hg_operators(hg_mesh* mesh, double t, int32_t method, int32_t solver_default_time) {
   return guarded([&] {
     require_mesh(mesh);
-    HG_REQUIRE(method == HG_METHOD_MESHLAB || method == HG_METHOD_POTPOURRI3D, HG_ERR_VALUE,
-               "method must be 'meshlab' or 'potpourri3d'");
+    HG_REQUIRE(method == HG_METHOD_MESHLAB || method == HG_METHOD_IGL ||
+                   method == HG_METHOD_POTPOURRI3D,
+               HG_ERR_VALUE, "method must be 'meshlab', 'igl' or 'potpourri3d'");
     HG_REQUIRE(std::isnan(t) || t > 0.0, HG_ERR_VALUE, "t must be positive");
     assemble(*mesh, t, method, solver_default_time != 0);
   });
@@ -761,9 +859,11 @@ int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_fa
     const bool mixed = precision == HG_PRECISION_MIXED;
     if (systems & HG_SYSTEM_HEAT) factor_system(*mesh, HG_SYSTEM_HEAT, mixed);
     if (systems & HG_SYSTEM_POISSON) factor_system(*mesh, HG_SYSTEM_POISSON, mixed);
+    if (systems & HG_SYSTEM_PINNED) factor_system(*mesh, HG_SYSTEM_PINNED, mixed);
     if (out_factored) {
       bool ok = (!(systems & HG_SYSTEM_HEAT) || mesh->heat_factor.held()) &&
-                (!(systems & HG_SYSTEM_POISSON) || mesh->poisson_factor.held());
+                (!(systems & HG_SYSTEM_POISSON) || mesh->poisson_factor.held()) &&
+                (!(systems & HG_SYSTEM_PINNED) || mesh->n_rim == 0 || mesh->pinned_factor.held());
       *out_factored = ok ? 1 : 0;
     }
   });
@@ -772,7 +872,9 @@ int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_fa
 int hg_factored(hg_mesh* mesh, int32_t* out_systems) {
   return guarded([&] {
     require_mesh(mesh);
-    *out_systems = (mesh->heat_factor.held() ? 1 : 0) | (mesh->poisson_factor.held() ? 2 : 0);
+    *out_systems = (mesh->heat_factor.held() ? HG_SYSTEM_HEAT : 0) |
+                   (mesh->poisson_factor.held() ? HG_SYSTEM_POISSON : 0) |
+                   (mesh->pinned_factor.held() ? HG_SYSTEM_PINNED : 0);
   });
 }
 
@@ -792,6 +894,7 @@ int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows,
       HG_HIP(hipStreamSynchronize(m.stream));
       m.heat_factor.reset();
       m.poisson_factor.reset();
+      m.pinned_factor.reset();
       m.plan.reset();
       m.host_plan.reset();
       m.options.leaf_rows = leaf;
@@ -838,6 +941,40 @@ int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n
                             precision);
     HG_HIP(hipStreamSynchronize(mesh->stream));
     if (out_flags) *out_flags = flags;
+  });
+}
+
+int hg_diffuse_pinned(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
+                      int32_t mode, int32_t precision, int32_t* out_flags,
+                      int32_t* out_has_rim) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
+    require_single_or_factored(*mesh, mode);
+    hg_mesh& m = *mesh;
+    m.stats = hg_solve_stats{};
+    ensure_operators(m);
+    int32_t flags = 0;
+    if (m.n_rim > 0 && n_columns > 0) {
+      // The caller's right-hand side is only read: its copy with the rim rows zeroed is solved.
+      const int64_t count = m.n * n_columns;
+      if (m.pinned_rhs.size < count) m.pinned_rhs.allocate(count);
+      launch_off_rim(d_rhs, m.pinned_rhs.ptr, m.d_rim.ptr, m.n, n_columns, FieldLayout::kColumns,
+                     m.stream);
+      flags = diffuse(m, m.pinned_rhs.ptr, d_solution, n_columns, mode, precision,
+                      FieldLayout::kColumns, true);
+    }
+    HG_HIP(hipStreamSynchronize(m.stream));
+    if (out_flags) *out_flags = flags;
+    if (out_has_rim) *out_has_rim = m.n_rim > 0 ? 1 : 0;
+  });
+}
+
+int hg_rim_count(hg_mesh* mesh, int64_t* out_count) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(out_count != nullptr, HG_ERR_VALUE, "out_count is null");
+    *out_count = mesh->n_rim;
   });
 }
 
@@ -1043,7 +1180,8 @@ int hg_timing_read(hg_mesh* mesh, double* out_ms, int64_t* out_counts, int32_t r
 int hg_nbytes(hg_mesh* mesh, int64_t* out_bytes) {
   return guarded([&] {
     require_mesh(mesh);
-    int64_t bytes = mesh->heat_factor.bytes() + mesh->poisson_factor.bytes();
+    int64_t bytes =
+        mesh->heat_factor.bytes() + mesh->poisson_factor.bytes() + mesh->pinned_factor.bytes();
     if (bytes > 0 && mesh->plan) bytes += mesh->plan->bytes();
     *out_bytes = bytes;
   });
@@ -1055,6 +1193,7 @@ int hg_release(hg_mesh* mesh) {
     HG_HIP(hipStreamSynchronize(mesh->stream));
     mesh->heat_factor.reset();
     mesh->poisson_factor.reset();
+    mesh->pinned_factor.reset();
     mesh->plan.reset();
   });
 }

@@ -80,14 +80,76 @@ __global__ void face_terms_kernel(const float* __restrict__ xyz, const int32_t* 
   cot[3 * f + 2] = c2;
 }
 
-// m_i = (sum of the float64-widened areas of the faces at i, in ascending corner order) / 3:
-// kernels/heat.py:125-131 then 423-425.
+// Per face, each corner's share of the area under Meyer et al.'s mixed Voronoi lumping, as
+// igl::massmatrix forms it at MASSMATRIX_TYPE_VORONOI (kernels/triangles.py:276-330): in float64
+// from the float32 corners, edge lengths l_e = |p_{e+1} - p_{e+2}|, area |(p1 - p0) x (p2 - p0)|/2,
+// cosines by the law of cosines over the rounded lengths, circumcentric partials
+// (cos_e l_e / sum) area and each corner the mean of the two partials beside it; an obtuse corner
+// takes half the area and the others a quarter; a zero-length edge or a zero cosine sum splits
+// the area evenly. Rounded once to float32. One thread a face.
+__global__ void voronoi_shares_kernel(const float* __restrict__ xyz,
+                                      const int32_t* __restrict__ faces, int64_t n_faces,
+                                      float* __restrict__ shares) {
+  int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  const D3 p0 = widen(load3(xyz, faces[3 * f])), p1 = widen(load3(xyz, faces[3 * f + 1])),
+           p2 = widen(load3(xyz, faces[3 * f + 2]));
+  const D3 e0 = sub(p1, p2), e1 = sub(p2, p0), e2 = sub(p0, p1);
+  const double l0 = sqrt(dot(e0, e0)), l1 = sqrt(dot(e1, e1)), l2 = sqrt(dot(e2, e2));
+  const D3 c = cross(sub(p1, p0), sub(p2, p0));
+  const double area = 0.5 * sqrt(dot(c, c));
+  const double even = area / 3.0;
+  double q0 = even, q1 = even, q2 = even;
+  if (l0 > 0.0 && l1 > 0.0 && l2 > 0.0) {
+    const double c0 = (l2 * l2 + l1 * l1 - l0 * l0) / (2.0 * l1 * l2);
+    const double c1 = (l0 * l0 + l2 * l2 - l1 * l1) / (2.0 * l2 * l0);
+    const double c2 = (l1 * l1 + l0 * l0 - l2 * l2) / (2.0 * l0 * l1);
+    const double b0 = c0 * l0, b1 = c1 * l1, b2 = c2 * l2;
+    const double weight = b0 + b1 + b2;
+    if (weight != 0.0) {
+      const double part0 = b0 / weight * area, part1 = b1 / weight * area,
+                   part2 = b2 / weight * area;
+      q0 = 0.5 * (part1 + part2);
+      q1 = 0.5 * (part2 + part0);
+      q2 = 0.5 * (part0 + part1);
+    }
+    const double half = 0.5 * area, quarter = 0.25 * area;
+    if (c0 < 0.0) {
+      q0 = half;
+      q1 = quarter;
+      q2 = quarter;
+    } else if (c1 < 0.0) {
+      q0 = quarter;
+      q1 = half;
+      q2 = quarter;
+    } else if (c2 < 0.0) {
+      q0 = quarter;
+      q1 = quarter;
+      q2 = half;
+    }
+  }
+  shares[3 * f] = static_cast<float>(q0);
+  shares[3 * f + 1] = static_cast<float>(q1);
+  shares[3 * f + 2] = static_cast<float>(q2);
+}
+
+// The lumped mass, summed over each vertex's corners in ascending corner order: with `shares`
+// null, m_i = (sum of the float64-widened areas of the faces at i) / 3 (barycentric,
+// kernels/heat.py:125-131 then 423-425); otherwise m_i = the sum of the float64-widened Voronoi
+// shares of i's corners, with no division (kernels/heat.py:126-128, heat.py:2169).
 __global__ void mass_kernel(const int32_t* __restrict__ vertex_ptr,
                             const int32_t* __restrict__ vertex_corners,
-                            const float* __restrict__ areas, int64_t n, double* __restrict__ mass) {
+                            const float* __restrict__ areas, const float* __restrict__ shares,
+                            int64_t n, double* __restrict__ mass) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double sum = 0.0;
+  if (shares) {
+    for (int32_t k = vertex_ptr[i]; k < vertex_ptr[i + 1]; ++k)
+      sum += static_cast<double>(shares[vertex_corners[k]]);
+    mass[i] = sum;
+    return;
+  }
   for (int32_t k = vertex_ptr[i]; k < vertex_ptr[i + 1]; ++k)
     sum += static_cast<double>(areas[vertex_corners[k] / 3]);
   mass[i] = sum / 3.0;
@@ -215,6 +277,57 @@ __global__ void operator_rows_kernel(
   }
   heat_inv_diag[i] = hd != 0.0 ? 1.0 / hd : 0.0;
   poisson_inv_diag[i] = pd != 0.0 ? 1.0 / pd : 0.0;
+}
+
+// The heat system with its solution pinned to zero on the rim (kernels/heat.py:487-508): a rim row
+// becomes the identity's and an interior row drops its rim columns, over the heat system's own
+// pattern. Symmetric positive definite; against a right-hand side zero on the rim its solution is
+// the interior system's padded with zeros. Also the values narrowed to float32 and the inverse
+// diagonal, which the settled CG reads. One thread a row.
+__global__ void pinned_rows_kernel(const int32_t* __restrict__ offsets,
+                                   const int32_t* __restrict__ columns,
+                                   const int32_t* __restrict__ rim, const double* __restrict__ heat,
+                                   int64_t n, double* __restrict__ out, float* __restrict__ out32,
+                                   double* __restrict__ inv_diag) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool on_rim = rim[i] != 0;
+  double diagonal = 0.0;
+  for (int32_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+    const int32_t c = columns[k];
+    double value = heat[k];
+    if (on_rim) value = c == i ? 1.0 : 0.0;
+    else if (rim[c] != 0) value = 0.0;
+    out[k] = value;
+    out32[k] = static_cast<float>(value);
+    if (c == i) diagonal = value;
+  }
+  inv_diag[i] = diagonal != 0.0 ? 1.0 / diagonal : 0.0;
+}
+
+// out = in with row r zeroed wherever rim_of[r] is set (kernels/heat.py:511-514, the pinned
+// solve's right-hand side), over n_columns columns in `layout` (rim_of indexed by vertex for
+// columns, by row of the elimination order for rows). in == out is allowed.
+__global__ void off_rim_kernel(const double* in, double* out, const int32_t* __restrict__ rim_of,
+                               int64_t n, int64_t n_columns, int32_t rows_layout) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n * n_columns) return;
+  int64_t r;
+  if (rows_layout) {
+    const int64_t chunk = e / (kFieldChunk * n);
+    const int64_t kk = min(kFieldChunk, n_columns - chunk * kFieldChunk);
+    r = (e - chunk * kFieldChunk * n) / kk;
+  } else {
+    r = e % n;
+  }
+  out[e] = rim_of[r] != 0 ? 0.0 : in[e];
+}
+
+// a = (a + b) / 2: the Neumann heat averaged with the rim-pinned one (kernels/heat.py:517-520).
+__global__ void average_kernel(double* __restrict__ a, const double* __restrict__ b, int64_t count) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  a[e] = 0.5 * (a[e] + b[e]);
 }
 
 // u0[q, s] = 1 at each source of column q (duplicates allowed); an index outside [0, n) writes
@@ -523,10 +636,41 @@ void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, 
   HG_HIP(hipGetLastError());
 }
 
+void launch_voronoi_shares(const float* xyz, const int32_t* faces, int64_t n_faces, float* shares,
+                           hipStream_t stream) {
+  if (n_faces == 0) return;
+  voronoi_shares_kernel<<<blocks_for(n_faces), kBlock, 0, stream>>>(xyz, faces, n_faces, shares);
+  HG_HIP(hipGetLastError());
+}
+
 void launch_mass(const int32_t* vertex_ptr, const int32_t* vertex_corners, const float* areas,
-                 int64_t n, double* mass, hipStream_t stream) {
+                 const float* shares, int64_t n, double* mass, hipStream_t stream) {
   if (n == 0) return;
-  mass_kernel<<<blocks_for(n), kBlock, 0, stream>>>(vertex_ptr, vertex_corners, areas, n, mass);
+  mass_kernel<<<blocks_for(n), kBlock, 0, stream>>>(vertex_ptr, vertex_corners, areas, shares, n,
+                                                    mass);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_pinned_rows(const int32_t* offsets, const int32_t* columns, const int32_t* rim,
+                        const double* heat, int64_t n, double* out, float* out32, double* inv_diag,
+                        hipStream_t stream) {
+  if (n == 0) return;
+  pinned_rows_kernel<<<blocks_for(n), kBlock, 0, stream>>>(offsets, columns, rim, heat, n, out,
+                                                           out32, inv_diag);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_off_rim(const double* in, double* out, const int32_t* rim_of, int64_t n,
+                    int64_t n_columns, FieldLayout layout, hipStream_t stream) {
+  if (n * n_columns == 0) return;
+  off_rim_kernel<<<blocks_for(n * n_columns), kBlock, 0, stream>>>(
+      in, out, rim_of, n, n_columns, layout == FieldLayout::kRows ? 1 : 0);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_average(double* a, const double* b, int64_t count, hipStream_t stream) {
+  if (count == 0) return;
+  average_kernel<<<blocks_for(count), kBlock, 0, stream>>>(a, b, count);
   HG_HIP(hipGetLastError());
 }
 

@@ -103,8 +103,22 @@ struct PhaseScope {
 // assembly.hip
 void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, float* cot,
                        float* normals, float* areas, hipStream_t stream);
+// Each corner's Voronoi share of its face's area (F,3) float32.
+void launch_voronoi_shares(const float* xyz, const int32_t* faces, int64_t n_faces, float* shares,
+                           hipStream_t stream);
+// The lumped mass: barycentric (shares null) or the sum of the corners' Voronoi shares.
 void launch_mass(const int32_t* vertex_ptr, const int32_t* vertex_corners, const float* areas,
-                 int64_t n, double* mass, hipStream_t stream);
+                 const float* shares, int64_t n, double* mass, hipStream_t stream);
+// The rim-pinned heat system's values (rim flags by vertex), float32 copy and inverse diagonal.
+void launch_pinned_rows(const int32_t* offsets, const int32_t* columns, const int32_t* rim,
+                        const double* heat, int64_t n, double* out, float* out32, double* inv_diag,
+                        hipStream_t stream);
+// out = in with the flagged rows zeroed (in == out allowed); rim_of by vertex for kColumns, by
+// row of the elimination order for kRows.
+void launch_off_rim(const double* in, double* out, const int32_t* rim_of, int64_t n,
+                    int64_t n_columns, FieldLayout layout, hipStream_t stream);
+// a = (a + b) / 2 elementwise.
+void launch_average(double* a, const double* b, int64_t count, hipStream_t stream);
 int32_t edge_sum_blocks(int64_t n);
 // Edge-length sums over rows [row0, row0 + n) into scalars[0..4).
 void launch_edge_sums(const int32_t* offsets, const int32_t* columns, const int32_t* corner_ptr,

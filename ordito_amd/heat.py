@@ -28,14 +28,18 @@ from .mesh import Trimesh
 _LOGGER = logging.getLogger(__name__)
 
 _METHODS = ("meshlab", "igl", "potpourri3d")
-_METHOD_CODES = {"meshlab": _lib.HG_METHOD_MESHLAB, "potpourri3d": _lib.HG_METHOD_POTPOURRI3D}
+_METHOD_CODES = {
+    "meshlab": _lib.HG_METHOD_MESHLAB,
+    "igl": _lib.HG_METHOD_IGL,
+    "potpourri3d": _lib.HG_METHOD_POTPOURRI3D,
+}
 _PRECISIONS = {"fp64": _lib.HG_PRECISION_FP64, "mixed": _lib.HG_PRECISION_MIXED}
 
 
 def _require_method(method: str, *, geodesic: bool) -> None:
     if method not in _METHODS:
         raise ValueError(f"method must be 'meshlab', 'igl' or 'potpourri3d', got {method!r}")
-    supported = ("meshlab",) if geodesic else ("meshlab", "potpourri3d")
+    supported = ("meshlab", "igl") if geodesic else _METHODS
     if method not in supported:
         raise NotImplementedError(
             f"method={method!r} is not implemented on the MI355X path (implemented: {supported})"
@@ -270,7 +274,8 @@ class HeatSolver:
         self._mesh = weakref.ref(mesh)
         self._handle: _Handle | None = None
         self._operators: HeatOperators | None = None
-        self._solves = {"heat": 0, "poisson": 0}
+        self._solves = {"heat": 0, "poisson": 0, "pinned": 0}
+        self._rim: int | None = None
         self._ill_conditioned = False
         self.last_stats: dict[str, int] = {}
 
@@ -340,7 +345,7 @@ class HeatSolver:
         """Drop every kept factorization and forget the solve counts (``heat.py:3849-3861``)."""
         if self._handle is not None:
             _lib.call("hg_release", self._handle.ptr)
-        self._solves = {"heat": 0, "poisson": 0}
+        self._solves = {"heat": 0, "poisson": 0, "pinned": 0}
 
     def factor(self) -> bool:
         """Factor the heat and Poisson systems now (``heat.py:3863-3888``)."""
@@ -407,11 +412,26 @@ class HeatSolver:
             )
         return 1 if rhs.ndim == 1 else int(rhs.shape[0])
 
+    @property
+    def rim_size(self) -> int:
+        """The number of vertices with an edge on a single face (0 for a closed mesh)."""
+        if self._rim is None:
+            out = ctypes.c_int64()
+            _lib.call("hg_rim_count", self._native().ptr, ctypes.byref(out))
+            self._rim = int(out.value)
+        return self._rim
+
+    _SYSTEMS = {
+        "heat": _lib.HG_SYSTEM_HEAT,
+        "poisson": _lib.HG_SYSTEM_POISSON,
+        "pinned": _lib.HG_SYSTEM_PINNED,
+    }
+
     def _count(self, role: str) -> None:
         """Count a solve of ``role``'s system; factor it on its second (rent or buy)."""
         self._solves[role] += 1
         if self._solves[role] >= 2:
-            system = _lib.HG_SYSTEM_HEAT if role == "heat" else _lib.HG_SYSTEM_POISSON
+            system = self._SYSTEMS[role]
             if not self._factored() & system:
                 out = ctypes.c_int32()
                 _lib.call("hg_factor", self._native().ptr, system, self._precision_code, ctypes.byref(out))
@@ -440,6 +460,39 @@ class HeatSolver:
                 "HeatSolver: the heat system's settled iterate failed its backward-error check; "
                 "factored it"
             )
+
+    def diffuse_pinned(self, rhs, solution) -> bool:
+        """
+        Solve ``(M - tL) u = rhs`` on the interior vertices with ``u = 0`` on every vertex with an
+        edge on a single face, settled as ``diffuse`` is (``heat.py:4153-4200``); ``rhs`` (V,) or
+        (k, V) float64 on the GPU, its rim entries ignored. Returns whether the mesh has a rim: a
+        closed mesh has nothing to pin, and ``solution`` is then left as it was.
+        """
+        columns = self._require_fields("HeatSolver.diffuse_pinned", rhs, solution, 1)
+        if self.rim_size == 0:
+            return False
+        self._count("pinned")
+        mode = _lib.HG_MODE_FACTORED if self._factored() & _lib.HG_SYSTEM_PINNED else _lib.HG_MODE_ITERATE
+        flags = ctypes.c_int32()
+        has_rim = ctypes.c_int32()
+        _lib.call(
+            "hg_diffuse_pinned",
+            self._native().ptr,
+            _arrays.pointer(rhs),
+            _arrays.pointer(solution),
+            columns,
+            mode,
+            self._precision_code,
+            ctypes.byref(flags),
+            ctypes.byref(has_rim),
+        )
+        if flags.value & 1:
+            self._ill_conditioned = True
+            _LOGGER.debug(
+                "HeatSolver: the pinned system's settled iterate failed its backward-error check; "
+                "factored it"
+            )
+        return bool(has_rim.value)
 
     def solve_poisson(self, rhs, solution) -> None:
         """Solve ``-L phi = rhs`` to the heat method's tolerance (``heat.py:4062-4151``)."""
@@ -494,11 +547,16 @@ class HeatSolver:
         for c in counts:
             offsets.append(offsets[-1] + c)
         out = torch.empty((k, n), dtype=torch.float64, device="cuda")  # every entry is written
-        # Rent or buy, per system (heat.py:4394-4404): the second solve factors both. One handle
-        # and one factored-state query for the whole call.
+        # Rent or buy, per system (heat.py:4394-4404): the second solve factors each. One handle
+        # and one factored-state query for the whole call. At "igl" a mesh with a rim also solves
+        # the rim-pinned system (heat.py:594-597), which the library solves through its factor
+        # whenever the heat system's is held.
         native = self._native()
         factored = self._factored_on(native)
-        for role, system in (("heat", _lib.HG_SYSTEM_HEAT), ("poisson", _lib.HG_SYSTEM_POISSON)):
+        roles = [("heat", _lib.HG_SYSTEM_HEAT), ("poisson", _lib.HG_SYSTEM_POISSON)]
+        if self.method == "igl" and self.rim_size > 0:
+            roles.append(("pinned", _lib.HG_SYSTEM_PINNED))
+        for role, system in roles:
             self._solves[role] += 1
             if self._solves[role] >= 2 and not factored & system:
                 done = ctypes.c_int32()
@@ -554,9 +612,12 @@ def heat_geodesic(
 ):
     """
     Approximate geodesic distance to the nearest source vertex by the heat method (ordito
-    ``heat.py:370-609``), MeshLab's discretization. Takes a ``Trimesh`` (which keeps the operators
-    and factorizations for later calls) or ``vertices, faces``; returns ``(V,)`` float64 -- a torch
-    tensor on the GPU when the vertices were one, a numpy array otherwise.
+    ``heat.py:370-609``): MeshLab's discretization (``method="meshlab"``, the default) or
+    ``igl::heat_geodesics``' (``"igl"``: the Voronoi mass, ``h`` over the unique edges, and the
+    Neumann heat averaged with the heat pinned to zero on the rim). Takes a ``Trimesh`` (which
+    keeps the operators and factorizations for later calls) or ``vertices, faces``; returns
+    ``(V,)`` float64 -- a torch tensor on the GPU when the vertices were one, a numpy array
+    otherwise.
 
     Raises ``ValueError`` for an unknown ``method`` or a source outside ``[0, n_vertices)``.
     """

@@ -198,3 +198,89 @@ def test_reference_known_answers(cuda):
     assert f"{d[south]:.2f}" == "3.09"
     d0 = heat.heat_geodesic(vertices, faces, np.array([0], np.int32))
     assert f"{d0.max():.1f}" == "3.1"
+
+
+# --- method="igl" --------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("name", ["grid", "saddle_graded", "icosphere3", "bumpy"])
+def test_igl_operators_match_oracle(cuda, name):
+    """The Voronoi mass: each corner's share is formed in float64 from the float32 corners with
+    the oracle's operations in its order and rounded once to float32, so the two agree to one
+    float32 ulp of a share (2^-23 of a vertex's mass) whatever the float64 library rounds; the
+    Laplacian is bit-exact and the heat system's off-diagonal within the time's rounding."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh(name)
+    ops = heat.heat_operators(_tensor(cuda, vertices), _tensor(cuda, faces), method="igl")
+    heat_sys, _, laplacian, _, _, _, _, _ = ops
+    ref = ho.assemble(vertices, faces, None, method="igl")
+    assert np.array_equal(laplacian.values.cpu().numpy(), ref.laplacian)
+    n = len(vertices)
+    got = heat_sys.values.cpu().numpy()
+    rows = np.repeat(np.arange(n), np.diff(ref.pattern.indptr))
+    diagonal = ref.pattern.indices == rows
+    assert np.allclose(got[~diagonal], ref.heat[~diagonal], rtol=1e-14, atol=0.0)
+    assert np.allclose(got[diagonal], ref.heat[diagonal], rtol=2.0**-23, atol=0.0)
+
+
+@pytest.mark.parametrize("name", ["grid", "saddle_graded", "icosphere4", "two_bodies"])
+def test_igl_geodesic_matches_oracle(cuda, name):
+    """heat_geodesic(method="igl"), from the middle and from a rim vertex, iterated on the first
+    call (1e-5 of the range) and through the factors on the later ones (1e-9)."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh(name)
+    n = len(vertices)
+    rim = ho.rim_vertices(ho.operator_pattern(n, faces))
+    sources = [n // 2 + 17] + ([int(np.flatnonzero(rim)[0])] if rim.any() else [])
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method="igl")
+    assert solver.rim_size == int(rim.sum())
+    for s in sources:
+        expected = ho.geodesic(vertices, faces, np.array([s]), method="igl").distance
+        once = heat.heat_geodesic(vertices, faces, np.array([s], np.int32), method="igl")
+        worst, _ = ho.relative_error(once, expected)
+        assert worst <= 1e-5, (s, worst)
+        for call in range(2):
+            got = heat.heat_geodesic(trimesh, _tensor(cuda, np.array([s], np.int32)), method="igl")
+            worst, _ = ho.relative_error(got.cpu().numpy(), expected)
+            assert worst <= (1e-5 if call == 0 and s == sources[0] else 1e-9), (s, call, worst)
+    if rim.any():
+        assert solver.last_stats["pinned_solves"] >= 1
+        assert solver._factored() & 4
+        meshlab = ho.geodesic(vertices, faces, np.array([sources[0]])).distance
+        assert np.abs(got.cpu().numpy() - meshlab).max() > 1e-3 * np.ptp(meshlab)
+
+
+def test_diffuse_pinned_holds_the_rim_at_zero(cuda):
+    """HeatSolver.diffuse_pinned (the reference's check, tests/test_heat.py:295-329): zero on the
+    rim and the interior system's direct solve elsewhere, iterated on the first call (rtol 1e-6,
+    the reference's bar) and factored on the second (1e-10); a closed mesh reports no rim and
+    leaves the solution alone."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh("saddle_graded")
+    n = len(vertices)
+    t, _ = ho.resolve_time(vertices, faces, None)  # the solver's time rule
+    ref = ho.assemble(vertices, faces, t, method="meshlab")
+    rim = ho.rim_vertices(ref.pattern)
+    rhs = np.zeros(n)
+    rhs[n // 2 + 17] = 1.0
+    expected = ho.diffuse_pinned(ref, rhs)
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method="meshlab")
+    for call, rtol in enumerate((1e-6, 1e-10)):
+        solution = cuda.zeros(n, dtype=cuda.float64, device="cuda")
+        assert solver.diffuse_pinned(_tensor(cuda, rhs), solution)
+        got = solution.cpu().numpy()
+        assert np.array_equal(got[rim], np.zeros(int(rim.sum()))), call
+        assert np.allclose(got, expected, rtol=rtol, atol=1e-12 * expected.max()), call
+    assert solver._factored() & 4
+
+    closed_v, closed_f = mesh("icosphere3")
+    closed_mesh = heat.Trimesh(_tensor(cuda, closed_v), _tensor(cuda, closed_f))
+    closed = closed_mesh.heat_solver()
+    untouched = cuda.full((len(closed_v),), 7.0, dtype=cuda.float64, device="cuda")
+    assert not closed.diffuse_pinned(cuda.zeros_like(untouched), untouched)
+    assert bool((untouched == 7.0).all())
