@@ -163,12 +163,13 @@ __device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
 
 // Four 4-deep steps over one stage: A from As[k][16 * wave + l % 16] (the wave's 16 output rows),
 // B from Bs[k][16 n + l % 16] for each 16-column block n holding right-hand sides.
+template <int kDepth = kInner>
 __device__ __forceinline__ void stage_products(const double (*As)[kTile + 1],
                                                const double (*Bs)[kCols + 1], int wave, int lane,
                                                int blocks, f64x4 acc[4]) {
   const int li = lane % 16, lk = lane / 16;
 #pragma unroll 1
-  for (int kb = 0; kb < kInner; kb += 4) {
+  for (int kb = 0; kb < kDepth; kb += 4) {
     const double av = As[kb + lk][16 * wave + li];
 #pragma unroll
     for (int n = 0; n < 4; ++n)
@@ -373,13 +374,18 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
 // g = [y(cols); d(rows)] over the inner range [k0, k1); each wave owns 16 of the columns. Applied
 // results also give x_next = x + d when kIterate (a later round; the first round's d is the
 // iterate itself, and its instance holds no iterate registers).
+// The backward stage is twice the forward's depth: an inverse-factor column's stage is then 32
+// contiguous doubles (256 bytes) instead of 16.
+constexpr int kBwdInner = 16;
+
 template <typename TW, bool kIterate>
-__global__ __launch_bounds__(256) void bwd_tile_kernel(
+__global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
     const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_j0,
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
-  __shared__ double Ws[kInner][kTile + 1];  // [inner][inverse-factor column]
-  __shared__ double Gs[kInner][kCols + 1];  // [inner][right-hand side]
+  constexpr int kD = kBwdInner, kPer = kD * kTile / 256;  // entries a thread stages per operand
+  __shared__ double Ws[kD][kTile + 1];  // [inner][inverse-factor column]
+  __shared__ double Gs[kD][kCols + 1];  // [inner][right-hand side]
   const int32_t task = xcd_block();
   const int32_t s = task_s[task];
   const int32_t j0 = task_j0[task];
@@ -393,49 +399,56 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(
   const int t = threadIdx.x, wave = t / 64, lane = t % 64;
   const int blocks = (kk + 15) / 16;
   const bool cols_live = j0 + 16 * wave < nc;  // wave-uniform
-  // Staging: entry e = t + 256u is inverse-factor column e / 16 at inner e % 16 (contiguous in
+  // Staging: entry e = t + 256u is inverse-factor column e / kD at inner e % kD (contiguous in
   // memory), and g's right-hand side e % 64 at inner e / 64.
-  double wreg[4], greg[4];
+  double wreg[kPer], greg[kPer];
   // A stage's structure-row indices issue first, then its W loads, then (once the indices are
   // in, W still in flight) its g loads.
   auto load = [&](int32_t i0) {
-    int32_t srow[4];
+    int32_t srow[kPer];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kPer; ++u) {
       const int32_t ig = i0 + (t + 256 * u) / kCols;
       srow[u] = (ig >= nc && ig < k1) ? srows[ig - nc] : 0;
     }
+    // 32-bit element offsets from wave-uniform bases (a front's block and a field both stay
+    // under 2^31 entries), one register an address.
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kPer; ++u) {
       const int e = t + 256 * u;
-      const int ii = e % kInner, jj = e / kInner;
+      const int ii = e % kD, jj = e / kD;
       const int32_t i = i0 + ii, j = j0 + jj;
-      wreg[u] = (i < k1 && j < nc) ? widen(W[static_cast<int64_t>(j) * m + i]) : 0.0;
+      wreg[u] = (i < k1 && j < nc)
+                    ? widen(W[static_cast<uint32_t>(j) * static_cast<uint32_t>(m) + i])
+                    : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kPer; ++u) {
       const int e = t + 256 * u;
       const int q = e % kCols;
       const int32_t ig = i0 + e / kCols;
-      const double* src = ig < nc ? a.y + static_cast<int64_t>(c0 + ig) * kk
-                                  : a.d + static_cast<int64_t>(srow[u]) * kk;
-      greg[u] = (q < kk && ig < k1) ? src[q] : 0.0;
+      double g = 0.0;
+      if (q < kk && ig < k1) {
+        if (ig < nc) g = a.y[static_cast<uint32_t>(c0 + ig) * static_cast<uint32_t>(kk) + q];
+        else g = a.d[static_cast<uint32_t>(srow[u]) * static_cast<uint32_t>(kk) + q];
+      }
+      greg[u] = g;
     }
   };
   f64x4 acc[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
   load(k0);
-  for (int32_t i0 = k0; i0 < k1; i0 += kInner) {
+  for (int32_t i0 = k0; i0 < k1; i0 += kD) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kPer; ++u) {
       const int e = t + 256 * u;
-      Ws[e % kInner][e / kInner] = wreg[u];
+      Ws[e % kD][e / kD] = wreg[u];
       Gs[e / kCols][e % kCols] = greg[u];
     }
     __syncthreads();
-    if (i0 + kInner < k1) load(i0 + kInner);
-    if (cols_live) stage_products(Ws, Gs, wave, lane, blocks, acc);
+    if (i0 + kD < k1) load(i0 + kD);
+    if (cols_live) stage_products<kD>(Ws, Gs, wave, lane, blocks, acc);
     __syncthreads();
   }
   if (!cols_live) return;
@@ -981,6 +994,10 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
   int32_t max_used = 0;
   for (int64_t q0 = 0; q0 < k; q0 += kCols) {
     const int32_t kk = static_cast<int32_t>(std::min<int64_t>(kCols, k - q0));
+    // The backward tiles address a chunk's fields with 32-bit element offsets.
+    HG_REQUIRE(n * kk < (int64_t{1} << 31), HG_ERR_VALUE,
+               "a solve chunk of " + std::to_string(n) + " rows x " + std::to_string(kk) +
+                   " columns exceeds 2^31 entries; solve fewer columns per call");
     ws.reserve(plan, kk);
     double* bq = b + q0 * n;
     double* xq = x + q0 * n;
