@@ -374,8 +374,8 @@ __global__ __launch_bounds__(256) void fwd_reduce_kernel(
 // g = [y(cols); d(rows)] over the inner range [k0, k1); each wave owns 16 of the columns. Applied
 // results also give x_next = x + d when kIterate (a later round; the first round's d is the
 // iterate itself, and its instance holds no iterate registers).
-// The backward stage is twice the forward's depth: an inverse-factor column's stage is then 32
-// contiguous doubles (256 bytes) instead of 16.
+// The backward stage's depth. 32 (256 contiguous bytes of an inverse-factor column a stage) needs
+// 120 registers, four waves a SIMD, and measured slower than 16 at six waves.
 constexpr int kBwdInner = 16;
 
 template <typename TW, bool kIterate>
@@ -686,6 +686,82 @@ __global__ void residual_kernel(const int32_t* __restrict__ offsets,
     if (r) r[t] = dropped ? 0.0 : total;
     if (!componentwise) scale = fmax(scale, bscale);
     fail = !dropped && !(fabs(total) <= tolerance * scale);
+  }
+  raise_flag(fail, pending);
+}
+
+// The residual of residual_kernel for 64 right-hand sides, kRowsPerWave rows a wave: a lane is a
+// right-hand side of each of its wave's rows. Every round of loads covers all the wave's rows
+// (their offsets; then eight entries of each, columns and values loaded lane-parallel and read out
+// lane by lane; then every iterate entry they name), so a wave keeps several rows' loads in flight
+// where one row would wait on three dependent rounds alone. Sums keep entry order.
+constexpr int kRowsPerWave = 4;
+
+__global__ __launch_bounds__(256) void residual_rows_kernel(
+    const int32_t* __restrict__ offsets, const int32_t* __restrict__ columns,
+    const double* __restrict__ values, const double* __restrict__ b,
+    const double* __restrict__ x, const int32_t* __restrict__ pinned,
+    const double* __restrict__ bmax, int64_t n, double tolerance, int32_t componentwise,
+    double* __restrict__ r, int32_t* __restrict__ pending) {
+  constexpr int R = kRowsPerWave;
+  const int32_t lane = static_cast<int32_t>(threadIdx.x % 64);
+  const int64_t first = static_cast<int64_t>(__builtin_amdgcn_readfirstlane(
+                            static_cast<int32_t>(xcd_block() * 4 + threadIdx.x / 64))) * R;
+  const double bscale = componentwise ? 0.0 : bmax[lane];
+  int32_t e0[R], e1[R];
+  double bi[R], total[R], scale[R];
+  bool dropped[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = first + u;
+    const bool live = i < n;
+    e0[u] = live ? offsets[i] : 0;
+    e1[u] = live ? offsets[i + 1] : 0;
+    bi[u] = live ? b[i * kCols + lane] : 0.0;
+    dropped[u] = !live || (pinned && pinned[i]);
+    total[u] = bi[u];
+    scale[u] = fabs(bi[u]);
+  }
+  for (int32_t base = 0;; base += 8) {
+    bool more = false;
+#pragma unroll
+    for (int u = 0; u < R; ++u) more |= e0[u] + base < e1[u];
+    if (!more) break;  // wave-uniform
+    // Lane 8u + j loads entry base + j of row u (an entry past the row's end, or of an empty
+    // row, reads one in range and is not added).
+    const int u_lane = min(lane / 8, R - 1);
+    int32_t at = e0[0];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+      if (u == u_lane) at = e1[u] > e0[u] ? min(e0[u] + base + lane % 8, e1[u] - 1) : 0;
+    const int32_t lane_c = columns[at];
+    const double lane_v = values[at];
+    double xv[R][8];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int32_t c = __builtin_amdgcn_readlane(lane_c, 8 * u + j);
+        xv[u][j] = x[static_cast<int64_t>(c) * kCols + lane];
+      }
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const double v = readlane_f64(lane_v, 8 * u + j);
+        const double term = e0[u] + base + j < e1[u] ? v * xv[u][j] : 0.0;
+        total[u] -= term;
+        scale[u] += fabs(term);
+      }
+  }
+  bool fail = false;
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = first + u;
+    if (i >= n) continue;
+    if (r) r[i * kCols + lane] = dropped[u] ? 0.0 : total[u];
+    const double sc = componentwise ? scale[u] : fmax(scale[u], bscale);
+    fail |= !dropped[u] && !(fabs(total[u]) <= tolerance * sc);
   }
   raise_flag(fail, pending);
 }
@@ -1063,14 +1139,23 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
       // Through FP64 factors the first round almost always passes, so its residual is formed for
       // the test alone and written (a second pass) only when another round follows.
       const bool test_only = !use_fp32 && rounds == 1;
-      auto residual = kk == kCols ? residual_kernel<true> : residual_kernel<false>;
+      auto residual = [&](double* r_out) {
+        if (kk == kCols)
+          residual_rows_kernel<<<static_cast<unsigned>((n + 4 * kRowsPerWave - 1) /
+                                                       (4 * kRowsPerWave)),
+                                 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
+                                                   factor.pvalues.ptr, bp, x_cur, pinned_new,
+                                                   bmax, n, tolerance, componentwise ? 1 : 0,
+                                                   r_out, ws.pending.ptr);
+        else
+          residual_kernel<false><<<grid, 256, 0, stream>>>(
+              plan.p_offsets.ptr, plan.p_columns.ptr, factor.pvalues.ptr, bp, x_cur, pinned_new,
+              bmax, n, kk, tolerance, componentwise ? 1 : 0, r_out, ws.pending.ptr);
+      };
       {
         PhaseScope scope(timer, kPhaseResidual, stream);
         HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
-        residual<<<grid, 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
-                                           factor.pvalues.ptr, bp, x_cur, pinned_new, bmax,
-                                           n, kk, tolerance, componentwise ? 1 : 0,
-                                           test_only ? nullptr : ws.r.ptr, ws.pending.ptr);
+        residual(test_only ? nullptr : ws.r.ptr);
       }
       HG_HIP(hipGetLastError());
       int32_t pending = 0;
@@ -1081,10 +1166,7 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
       if (!pending) break;
       if (test_only && rounds < max_rounds) {
         PhaseScope scope(timer, kPhaseResidual, stream);
-        residual<<<grid, 256, 0, stream>>>(plan.p_offsets.ptr, plan.p_columns.ptr,
-                                           factor.pvalues.ptr, bp, x_cur, pinned_new, bmax,
-                                           n, kk, tolerance, componentwise ? 1 : 0, ws.r.ptr,
-                                           ws.pending.ptr);
+        residual(ws.r.ptr);
         HG_HIP(hipGetLastError());
       }
     }
