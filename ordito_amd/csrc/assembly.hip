@@ -530,6 +530,46 @@ __global__ __launch_bounds__(256) void divergence_rows_kernel(
   out[r * kk + q] = sum;
 }
 
+// divergence_rows_kernel for kk <= 32 columns: 64 / kk rows a wave, lane l holding column l % kk
+// of row l / kk (lanes past the last whole row idle), so narrow batches keep their lanes busy.
+// Each lane walks its own row's corners four at a time (loads first, then the terms in ascending
+// corner order): the same terms in the same order as divergence_rows_kernel.
+__global__ __launch_bounds__(256) void divergence_rows_packed_kernel(
+    const int4* __restrict__ corners, const int32_t* __restrict__ corner_ptr,
+    const double* __restrict__ geom, const double* __restrict__ heat, int64_t n, int32_t kk,
+    double* __restrict__ out) {
+  const int32_t per_wave = 64 / kk;
+  const int32_t lane = static_cast<int32_t>(threadIdx.x % 64);
+  const int32_t sub = lane / kk;
+  const int64_t r =
+      static_cast<int64_t>(xcd_block() * 4 + static_cast<int32_t>(threadIdx.x / 64)) * per_wave +
+      sub;
+  const int32_t q = lane % kk;
+  if (sub >= per_wave || r >= n) return;
+  const int32_t kbeg = corner_ptr[r], kend = corner_ptr[r + 1];
+  double sum = 0.0;
+  for (int32_t kb = kbeg; kb < kend; kb += 4) {
+    int4 c[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) c[w] = corners[min(kb + w, kend - 1)];  // past the end: unused
+    double h[4][3];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      h[w][0] = heat[static_cast<int64_t>(c[w].y) * kk + q];
+      h[w][1] = heat[static_cast<int64_t>(c[w].z) * kk + q];
+      h[w][2] = heat[static_cast<int64_t>(c[w].w) * kk + q];
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      if (kb + w >= kend) break;
+      const int32_t f = c[w].x / 3;
+      sum += corner_term(geom + kFaceGeom * static_cast<int64_t>(f), c[w].x - 3 * f, h[w][0],
+                         h[w][1], h[w][2]);
+    }
+  }
+  out[r * kk + q] = sum;
+}
+
 // Columns of the shift: column q starts at col_off[q] and holds col_len[q] entries (packed meshes'
 // fields, each its own length), or at q * n with n entries when col_off is null.
 struct Columns {
@@ -729,14 +769,19 @@ void launch_divergence(const int32_t* faces, const int32_t* row_of, const double
   if (layout == FieldLayout::kRows) {
     for (int64_t c0 = 0; c0 < n_columns; c0 += kFieldChunk) {
       const int64_t kk = std::min<int64_t>(kFieldChunk, n_columns - c0);
-      // A wave a row while at least a quarter of its lanes hold columns.
-      if (kk >= kFieldChunk / 4)
+      // A wave a row while at least half its lanes hold columns; 64 / kk rows a wave below.
+      if (kk > kFieldChunk / 2) {
         divergence_rows_kernel<<<static_cast<unsigned>((n + 3) / 4), kBlock, 0, stream>>>(
             row_corners, row_corner_ptr, geom, heat + c0 * n, n, kk, out + c0 * n);
-      else
-        divergence_kernel<false><<<blocks_for(n * kk), kBlock, 0, stream>>>(
-            faces, geom, vertex_ptr, vertex_corners, row_of, heat + c0 * n, n, kk, kk, 1,
-            out + c0 * n);
+      } else {
+        const int64_t rows_per_block = 4 * (kFieldChunk / kk);
+        divergence_rows_packed_kernel<<<static_cast<unsigned>((n + rows_per_block - 1) /
+                                                              rows_per_block),
+                                        kBlock, 0, stream>>>(row_corners, row_corner_ptr, geom,
+                                                             heat + c0 * n, n,
+                                                             static_cast<int32_t>(kk),
+                                                             out + c0 * n);
+      }
     }
   } else {
     divergence_kernel<false><<<blocks_for(n * n_columns), kBlock, 0, stream>>>(

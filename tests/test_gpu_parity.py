@@ -284,3 +284,27 @@ def test_diffuse_pinned_holds_the_rim_at_zero(cuda):
     untouched = cuda.full((len(closed_v),), 7.0, dtype=cuda.float64, device="cuda")
     assert not closed.diffuse_pinned(cuda.zeros_like(untouched), untouched)
     assert bool((untouched == 7.0).all())
+
+
+@pytest.mark.parametrize("width", [1, 7, 16, 33, 64, 70])
+def test_batch_widths_give_each_query_its_own_answer(cuda, width):
+    """One call of `width` source sets through the factors: every column equals the same source
+    set solved alone to 1e-12 of the range (the columns never mix: per-column sums in fixed
+    order, whatever the kernels' lane packing at that width), and the oracle to 1e-9."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh("bumpy")
+    n = len(vertices)
+    rng = np.random.default_rng(width)
+    sets = [rng.integers(0, n, size=1 + q % 3).astype(np.int32) for q in range(width)]
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method="meshlab")
+    assert solver.factor()
+    together = solver.geodesic([_tensor(cuda, s) for s in sets]).cpu().numpy()
+    oracle = ho.FactoredOracle(vertices, faces)
+    for q in (0, width // 2, width - 1):
+        alone = solver.geodesic([_tensor(cuda, sets[q])])[0].cpu().numpy()
+        span = np.ptp(alone)
+        assert np.abs(together[q] - alone).max() <= 1e-12 * span, q
+        worst, _ = ho.relative_error(together[q], oracle.distance(sets[q]))
+        assert worst <= 1e-9, (q, worst)
