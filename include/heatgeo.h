@@ -122,11 +122,22 @@ int hg_operator_views(hg_mesh* mesh, hg_operator_view* out_view);
  * Factor the systems named by `systems` (HeatSolver.factor, heat.py:3863-3888; the Poisson
  * factorization shares the heat analysis, cholesky.py:313-362). Sets *out_factored to 1 when
  * every named system holds a factor. `precision` HG_PRECISION_MIXED also keeps FP32 solve blocks.
+ * A factorization whose peak (front arena plus kept blocks) exceeds the memory budget is refused,
+ * not an error: the system stays unfactored, its solves iterate, and *out_factored is 0.
  */
 int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_factored);
 
-/* Whether each system holds a factor: bit 0 heat, bit 1 Poisson, bit 2 rim-pinned heat. */
+/* Whether each system holds a factor: bit 0 heat, bit 1 Poisson, bit 2 rim-pinned heat; bits 4-6
+ * the same systems' refusals under the memory budget. */
 int hg_factored(hg_mesh* mesh, int32_t* out_systems);
+
+/*
+ * The factorizations' memory budget in bytes (sparse_cholesky's CHOLESKY_MEMORY_BUDGET,
+ * cholesky.py:72, there a fixed 4 GiB): a factorization whose peak would exceed it is refused and
+ * its system solved by iteration. 0 (the default) takes 90 % of the device memory free when each
+ * factorization starts. Setting it forgets earlier refusals.
+ */
+int hg_set_memory_budget(hg_mesh* mesh, int64_t bytes);
 
 /*
  * Refinement targets of the factored solves at one precision: the heat step's componentwise
@@ -152,7 +163,8 @@ int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows, int3
  * heat.py:3890-3914). HG_MODE_ITERATE runs the settled Jacobi-CG (solve_spd_settled,
  * linalg.py:2490-2727) and its componentwise backward-error check; on failure it factors the
  * heat system and solves again refined (the reference's fallback), reporting it in *out_flags
- * (bit 0: the iterate failed its check, bit 1: a factorization was built). HG_MODE_FACTORED solves
+ * (bit 0: the iterate failed its check, bit 1: a factorization was built, bit 2: the factorization
+ * was refused under the memory budget and the iterate stands). HG_MODE_FACTORED solves
  * through the held factor, refined componentwise to 1e-12 (linalg.py:2659-2665), factoring first
  * if none is held. d_solution is overwritten (the iterate starts from zero).
  */

@@ -93,6 +93,10 @@ struct hg_mesh {
   DeviceArray<int4> row_corners;    // each row's corners {corner id, its face's three rows}
   DeviceArray<int32_t> row_corner_ptr;
   Factor heat_factor, poisson_factor, pinned_factor;
+  // The factorizations' memory budget in bytes (0: 90 % of the device memory free when a
+  // factorization starts) and the systems refused under it (HG_SYSTEM_* bits).
+  int64_t budget = 0;
+  int32_t refused = 0;
   // Refinement targets of the factored solves, [precision][0 heat (componentwise), 1 Poisson
   // (normwise)].
   double refine_tolerance[2][2] = {{kSettleDirectTolerance, kPoissonTolerance},
@@ -373,6 +377,30 @@ void ensure_pinned_system(hg_mesh& m) {
   m.pinned_built = true;
 }
 
+// A factorization's peak device memory: the transient front arena plus the blocks it keeps.
+int64_t factor_peak_bytes(const DevicePlan& plan, bool keep_fp32) {
+  return 8 * plan.arena_size + (keep_fp32 ? 12 : 8) * plan.w_size + 8 * plan.nnz_w;
+}
+
+// Refuse (HG_ERR_BUDGET) a factorization whose peak exceeds the budget, as sparse_cholesky refuses
+// one past CHOLESKY_MEMORY_BUDGET (cholesky.py:72, linalg.py:2393-2420); the system is then solved
+// by iteration. With no budget set the limit is 90 % of the device memory free now.
+void require_budget(hg_mesh& m, int32_t system, bool keep_fp32) {
+  const int64_t need = factor_peak_bytes(*m.plan, keep_fp32);
+  int64_t limit = m.budget;
+  if (limit <= 0) {
+    size_t free_bytes = 0, total_bytes = 0;
+    HG_HIP(hipMemGetInfo(&free_bytes, &total_bytes));
+    limit = static_cast<int64_t>(0.9 * static_cast<double>(free_bytes));
+  }
+  if (need > limit) m.refused |= system;
+  HG_REQUIRE(need <= limit, HG_ERR_BUDGET,
+             "the factorization needs " + std::to_string(need) + " bytes at its peak, over the " +
+                 std::to_string(limit) + "-byte budget");
+}
+
+bool is_budget_refusal(const Error& e) { return e.status == HG_ERR_BUDGET; }
+
 void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
   ensure_operators(m);
   ensure_plan(m);
@@ -380,11 +408,15 @@ void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
     if (m.n_rim == 0) return;
     ensure_pinned_system(m);
     PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
-    if (!m.pinned_factor.held() || (keep_fp32 && !m.pinned_factor.w32.ptr))
+    if (!m.pinned_factor.held() || (keep_fp32 && !m.pinned_factor.w32.ptr)) {
+      require_budget(m, system, keep_fp32);
       factor_numeric(*m.plan, m.pinned.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
                      m.heat_pin_slots, m.pinned_factor, keep_fp32, m.stream);
+    }
     return;
   }
+  Factor& target = system == HG_SYSTEM_HEAT ? m.heat_factor : m.poisson_factor;
+  if (!target.held() || (keep_fp32 && !target.w32.ptr)) require_budget(m, system, keep_fp32);
   PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
   if (system == HG_SYSTEM_HEAT) {
     if (!m.heat_factor.held())
@@ -487,8 +519,14 @@ int32_t diffuse(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
       flags |= 1;
       m.stats.backward_error_failed = 1;
       const bool was_held = sys.factor->held();
-      refinements = diffuse_factored(m, sys, rhs, x, k, HG_PRECISION_FP64);
-      if (!was_held) flags |= 2;
+      try {
+        refinements = diffuse_factored(m, sys, rhs, x, k, HG_PRECISION_FP64);
+        if (!was_held) flags |= 2;
+      } catch (const Error& e) {
+        // Refused under the budget before anything was solved: the iterate stands.
+        if (!is_budget_refusal(e)) throw;
+        flags |= 4;
+      }
     }
   }
   if (pinned) {
@@ -520,9 +558,18 @@ void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
                    int32_t precision, FieldLayout rhs_layout = FieldLayout::kColumns,
                    FieldLayout x_layout = FieldLayout::kColumns) {
   ensure_operators(m);
-  const bool factored = mode == HG_MODE_FACTORED || m.poisson_factor.held() || m.heat_factor.held();
+  const bool mixed = precision == HG_PRECISION_MIXED;
+  bool factored = mode == HG_MODE_FACTORED || m.poisson_factor.held() || m.heat_factor.held();
+  if (factored && rhs_layout == FieldLayout::kColumns && x_layout == FieldLayout::kColumns) {
+    // By columns a refusal under the budget leaves the solve to the iteration.
+    try {
+      factor_system(m, HG_SYSTEM_POISSON, mixed);
+    } catch (const Error& e) {
+      if (!is_budget_refusal(e)) throw;
+      factored = false;
+    }
+  }
   if (factored) {
-    const bool mixed = precision == HG_PRECISION_MIXED;
     factor_system(m, HG_SYSTEM_POISSON, mixed);
     m.stats.poisson_refinements = refined_solve(
         *m.plan, m.poisson_factor, mixed, rhs, x, k, m.refine_tolerance[mixed ? 1 : 0][1], false,
@@ -644,14 +691,24 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   double* negdiv = fields + 2 * n * k;
   // Through the factors the intermediate fields stay in the solves' row layout, in the elimination
   // order: the seeds land there, the heat step's solution stays there for the divergence, which
-  // writes the Poisson step's right-hand side there; only the distance is permuted out.
-  const FieldLayout layout = (mode == HG_MODE_FACTORED || m.heat_factor.held())
-                                 ? FieldLayout::kRows
-                                 : FieldLayout::kColumns;
-  if (layout == FieldLayout::kRows) {
+  // writes the Poisson step's right-hand side there; only the distance is permuted out. Every
+  // system the path solves is factored first; one refused under the memory budget sends the whole
+  // call down the iterated path (packed meshes have none: the refusal stands).
+  bool factored = mode == HG_MODE_FACTORED || m.heat_factor.held();
+  if (factored) {
     ensure_operators(m);
-    ensure_plan(m);
+    const bool mixed = precision == HG_PRECISION_MIXED;
+    try {
+      factor_system(m, HG_SYSTEM_HEAT, mixed);
+      factor_system(m, HG_SYSTEM_POISSON, mixed);
+      if (m.method == HG_METHOD_IGL && m.n_rim > 0) factor_system(m, HG_SYSTEM_PINNED, mixed);
+    } catch (const Error& e) {
+      if (!is_budget_refusal(e) || m.segments() > 1) throw;
+      factored = false;
+    }
   }
+  mode = factored ? HG_MODE_FACTORED : HG_MODE_ITERATE;
+  const FieldLayout layout = factored ? FieldLayout::kRows : FieldLayout::kColumns;
   seed(m, offsets, k, u0, layout);
   int32_t flags = diffuse(m, u0, heat, k, mode, precision, layout);
   if (m.method == HG_METHOD_IGL && m.n_rim > 0) {
@@ -857,9 +914,16 @@ int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_fa
   return guarded([&] {
     require_mesh(mesh);
     const bool mixed = precision == HG_PRECISION_MIXED;
-    if (systems & HG_SYSTEM_HEAT) factor_system(*mesh, HG_SYSTEM_HEAT, mixed);
-    if (systems & HG_SYSTEM_POISSON) factor_system(*mesh, HG_SYSTEM_POISSON, mixed);
-    if (systems & HG_SYSTEM_PINNED) factor_system(*mesh, HG_SYSTEM_PINNED, mixed);
+    // A refusal under the memory budget is an answer, not an error (OperatorFactorization.factor
+    // returns False, linalg.py:2393-2420): *out_factored reports it.
+    for (int32_t system : {HG_SYSTEM_HEAT, HG_SYSTEM_POISSON, HG_SYSTEM_PINNED}) {
+      if (!(systems & system)) continue;
+      try {
+        factor_system(*mesh, system, mixed);
+      } catch (const Error& e) {
+        if (!is_budget_refusal(e)) throw;
+      }
+    }
     if (out_factored) {
       bool ok = (!(systems & HG_SYSTEM_HEAT) || mesh->heat_factor.held()) &&
                 (!(systems & HG_SYSTEM_POISSON) || mesh->poisson_factor.held()) &&
@@ -874,7 +938,7 @@ int hg_factored(hg_mesh* mesh, int32_t* out_systems) {
     require_mesh(mesh);
     *out_systems = (mesh->heat_factor.held() ? HG_SYSTEM_HEAT : 0) |
                    (mesh->poisson_factor.held() ? HG_SYSTEM_POISSON : 0) |
-                   (mesh->pinned_factor.held() ? HG_SYSTEM_PINNED : 0);
+                   (mesh->pinned_factor.held() ? HG_SYSTEM_PINNED : 0) | (mesh->refused << 4);
   });
 }
 
@@ -967,6 +1031,15 @@ int hg_diffuse_pinned(hg_mesh* mesh, const double* d_rhs, double* d_solution, in
     HG_HIP(hipStreamSynchronize(m.stream));
     if (out_flags) *out_flags = flags;
     if (out_has_rim) *out_has_rim = m.n_rim > 0 ? 1 : 0;
+  });
+}
+
+int hg_set_memory_budget(hg_mesh* mesh, int64_t bytes) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(bytes >= 0, HG_ERR_VALUE, "the memory budget must not be negative");
+    mesh->budget = bytes;
+    mesh->refused = 0;
   });
 }
 

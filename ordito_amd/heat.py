@@ -208,7 +208,8 @@ def heat_operators(
     preconditioner, the float64 cotangent Laplacian ``L`` (negative diagonal), ``-L``, its
     Jacobi-Chebyshev preconditioner, the ``(F, 3)`` float32 half-cotangents, unit normals and
     areas. ``t=None`` is the squared mean edge length of ``method`` (MeshLab: every face's three
-    edges; potpourri3d: the unique edges). A ``cot_entries`` table, when given, must have shape
+    edges; igl and potpourri3d: the unique edges); the lumped mass is barycentric, or Voronoi at
+    ``method="igl"``. A ``cot_entries`` table, when given, must have shape
     ``(n_faces, 3)``; the table is formed from the positions either way (what
     ``cotmatrix_entries`` forms). ``pattern`` is accepted for signature compatibility.
     """
@@ -347,8 +348,22 @@ class HeatSolver:
             _lib.call("hg_release", self._handle.ptr)
         self._solves = {"heat": 0, "poisson": 0, "pinned": 0}
 
+    @property
+    def unfactorable(self) -> bool:
+        """Whether a factorization was refused under the memory budget (its solves iterate)."""
+        return bool(self._factored() >> 4)
+
+    def set_memory_budget(self, nbytes: int) -> None:
+        """
+        The factorizations' memory budget in bytes (ordito's ``CHOLESKY_MEMORY_BUDGET``,
+        ``cholesky.py:72``): one whose peak would exceed it is refused and its system iterated. 0
+        (the default) takes 90 % of the device memory free when a factorization starts.
+        """
+        _lib.call("hg_set_memory_budget", self._native().ptr, int(nbytes))
+
     def factor(self) -> bool:
-        """Factor the heat and Poisson systems now (``heat.py:3863-3888``)."""
+        """Factor the heat and Poisson systems now (``heat.py:3863-3888``); False when one is
+        refused under the memory budget, whose solves then keep iterating."""
         out = ctypes.c_int32()
         _lib.call(
             "hg_factor",
@@ -432,10 +447,14 @@ class HeatSolver:
         self._solves[role] += 1
         if self._solves[role] >= 2:
             system = self._SYSTEMS[role]
-            if not self._factored() & system:
+            state = self._factored()
+            if not state & system and not state & (system << 4):
                 out = ctypes.c_int32()
                 _lib.call("hg_factor", self._native().ptr, system, self._precision_code, ctypes.byref(out))
-                _LOGGER.debug("HeatSolver: factored the %s system on its second solve", role)
+                if out.value:
+                    _LOGGER.debug("HeatSolver: factored the %s system on its second solve", role)
+                else:
+                    _LOGGER.debug("HeatSolver: the %s system is over the memory budget", role)
 
     def diffuse(self, rhs, solution) -> None:
         """Solve ``(M - tL) u = rhs`` to settled entries, ``rhs`` (V,) or (k, V) float64 on the GPU
@@ -558,11 +577,15 @@ class HeatSolver:
             roles.append(("pinned", _lib.HG_SYSTEM_PINNED))
         for role, system in roles:
             self._solves[role] += 1
-            if self._solves[role] >= 2 and not factored & system:
+            if self._solves[role] >= 2 and not factored & system and not factored & (system << 4):
                 done = ctypes.c_int32()
                 _lib.call("hg_factor", native.ptr, system, self._precision_code, ctypes.byref(done))
-                factored |= system
-                _LOGGER.debug("HeatSolver: factored the %s system on its second solve", role)
+                if done.value:
+                    factored |= system
+                    _LOGGER.debug("HeatSolver: factored the %s system on its second solve", role)
+                else:
+                    factored |= system << 4
+                    _LOGGER.debug("HeatSolver: the %s system is over the memory budget", role)
         mode = _lib.HG_MODE_FACTORED if factored & _lib.HG_SYSTEM_HEAT else _lib.HG_MODE_ITERATE
         stats = _lib.SolveStats()
         _lib.call(

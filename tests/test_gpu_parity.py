@@ -308,3 +308,39 @@ def test_batch_widths_give_each_query_its_own_answer(cuda, width):
         assert np.abs(together[q] - alone).max() <= 1e-12 * span, q
         worst, _ = ho.relative_error(together[q], oracle.distance(sets[q]))
         assert worst <= 1e-9, (q, worst)
+
+
+def test_memory_budget_refuses_and_the_solves_iterate(cuda):
+    """A factorization whose peak exceeds the memory budget is refused, not an error
+    (OperatorFactorization.factor returns False, linalg.py:2393-2420): factor() is False, nothing
+    is held, and every later call iterates to the iterated bar; lifting the budget factors again.
+    On the graded saddle, whose iterate fails its backward-error check, the refused fallback leaves
+    the iterate standing and marks the mesh ill-conditioned."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh("torus")
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method="meshlab")
+    solver.set_memory_budget(1)
+    assert not solver.factor()
+    assert solver.nbytes == 0 and solver.unfactorable
+    expected = ho.geodesic(vertices, faces, np.array([3])).distance
+    source = _tensor(cuda, np.array([3], np.int32))
+    for _ in range(3):
+        got = heat.heat_geodesic(trimesh, source, method="meshlab").cpu().numpy()
+        worst, _ = ho.relative_error(got, expected)
+        assert worst <= 1e-5, worst
+        assert solver.last_stats["heat_rounds"] > 0 and solver.nbytes == 0
+    solver.set_memory_budget(0)
+    assert solver.factor() and not solver.unfactorable
+    got = heat.heat_geodesic(trimesh, source, method="meshlab").cpu().numpy()
+    worst, _ = ho.relative_error(got, expected)
+    assert worst <= 1e-9, worst
+
+    sv, sf = mesh("saddle_graded")
+    saddle = heat.Trimesh(_tensor(cuda, sv), _tensor(cuda, sf))
+    graded = saddle.heat_solver(method="meshlab")
+    graded.set_memory_budget(1)
+    d = heat.heat_geodesic(saddle, _tensor(cuda, np.array([0], np.int32)), method="meshlab")
+    assert bool(cuda.isfinite(d).all())
+    assert graded.ill_conditioned and graded.nbytes == 0
