@@ -97,6 +97,11 @@ struct hg_mesh {
   // factorization starts) and the systems refused under it (HG_SYSTEM_* bits).
   int64_t budget = 0;
   int32_t refused = 0;
+  // Speculative refinement (FP64 factors): the verdicts of the first rounds of the heat, pinned and
+  // Poisson solves of a call, [system][chunk], read at the call's last synchronization.
+  PinnedArray<int32_t> h_speculate;
+  bool speculating = false;
+  int64_t spec_chunks = 0;
   // Refinement targets of the factored solves, [precision][0 heat (componentwise), 1 Poisson
   // (normwise)].
   double refine_tolerance[2][2] = {{kSettleDirectTolerance, kPoissonTolerance},
@@ -492,11 +497,14 @@ int32_t diffuse_factored(hg_mesh& m, const HeatSystem& sys, double* rhs, double*
                          int32_t precision, FieldLayout layout = FieldLayout::kColumns) {
   const bool mixed = precision == HG_PRECISION_MIXED;
   factor_system(m, sys.code, mixed);
+  int32_t* speculate =
+      m.speculating ? m.h_speculate.ptr + (sys.code == HG_SYSTEM_PINNED ? m.spec_chunks : 0)
+                    : nullptr;
   return refined_solve(*m.plan, *sys.factor, mixed, rhs, x, k,
                        m.refine_tolerance[mixed ? 1 : 0][0], true,
                        mixed ? kMixedRefineRounds : kRefineRounds,
                        m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr, m.solve, m.stream,
-                       &m.timer, kPhaseHeatSweep, layout, layout);
+                       &m.timer, kPhaseHeatSweep, layout, layout, speculate);
 }
 
 // The settled iterate, verified, falling back to the factorization (solve_spd_settled,
@@ -574,7 +582,8 @@ void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
     m.stats.poisson_refinements = refined_solve(
         *m.plan, m.poisson_factor, mixed, rhs, x, k, m.refine_tolerance[mixed ? 1 : 0][1], false,
         mixed ? kMixedRefineRounds : kRefineRounds, m.poisson_pinned_new.ptr, m.solve, m.stream,
-        &m.timer, kPhasePoissonSweep, rhs_layout, x_layout);
+        &m.timer, kPhasePoissonSweep, rhs_layout, x_layout,
+        m.speculating ? m.h_speculate.ptr + 2 * m.spec_chunks : nullptr);
     m.stats.poisson_rounds = 0;
   } else {
     HG_REQUIRE(rhs_layout == FieldLayout::kColumns && x_layout == FieldLayout::kColumns,
@@ -675,6 +684,27 @@ void shift(hg_mesh& m, double* phi, int64_t k) {
   PhaseScope scope(&m.timer, kPhaseShift, m.stream);
   launch_shift(phi, m.d_sources.ptr, m.d_offsets.ptr, m.n, k, nullptr, nullptr,
                m.shift_partials.ptr, m.shift_values.ptr, m.stream);
+}
+
+// Arm (or disarm) speculative refinement for a call of k columns: with FP64 factors the first
+// refinement round almost always passes its test, so the call runs on without waiting for the
+// verdicts and checks them at its last synchronization (speculation_failed), solving again without
+// speculating in the rare case one failed. Mixed precision always refines, and never speculates.
+void arm_speculation(hg_mesh& m, int64_t k, bool on) {
+  m.speculating = on;
+  if (!on) return;
+  m.spec_chunks = (k + kFieldChunk - 1) / kFieldChunk;
+  if (m.h_speculate.size < 3 * m.spec_chunks) m.h_speculate.allocate(3 * m.spec_chunks);
+  std::fill(m.h_speculate.ptr, m.h_speculate.ptr + 3 * m.spec_chunks, 0);
+}
+
+// After the stream's synchronization: whether a speculated first round failed its test.
+bool speculation_failed(hg_mesh& m) {
+  const bool was = m.speculating;
+  m.speculating = false;
+  if (!was) return false;
+  return std::any_of(m.h_speculate.ptr, m.h_speculate.ptr + 3 * m.spec_chunks,
+                     [](int32_t v) { return v != 0; });
 }
 
 // Seed, heat step, divergence and Poisson step over k columns of the whole (possibly packed)
@@ -1116,8 +1146,13 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     }
     ensure_operators(m);
     upload_sources(m, d_sources, source_offsets, k);
-    if (!solve_columns(m, source_offsets, k, d_out, mode, precision, true)) shift(m, d_out, k);
-    HG_HIP(hipStreamSynchronize(m.stream));
+    for (bool speculate : {precision == HG_PRECISION_FP64, false}) {
+      m.stats = hg_solve_stats{};
+      arm_speculation(m, k, speculate);
+      if (!solve_columns(m, source_offsets, k, d_out, mode, precision, true)) shift(m, d_out, k);
+      HG_HIP(hipStreamSynchronize(m.stream));
+      if (!speculation_failed(m)) break;
+    }
     check_seeds(m);
     m.timer.resolve();
     if (out_stats) *out_stats = m.stats;
@@ -1195,7 +1230,14 @@ int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* s
       m.packed_phi.allocate(m.n * width);
       m.packed_capacity = m.n * width;
     }
+    arm_speculation(m, width, precision == HG_PRECISION_FP64);
     solve_columns(m, seed_offsets.data(), width, m.packed_phi.ptr, HG_MODE_FACTORED, precision);
+    // A speculated first round that failed its test: solve again without speculating.
+    HG_HIP(hipStreamSynchronize(m.stream));
+    if (speculation_failed(m)) {
+      m.stats = hg_solve_stats{};
+      solve_columns(m, seed_offsets.data(), width, m.packed_phi.ptr, HG_MODE_FACTORED, precision);
+    }
     // Each query's segment of its slot's column, then its shift against its own sources.
     m.q_src_off.assign(src_off, m.stream);
     m.q_dst_off.assign(dst_off, m.stream);

@@ -86,12 +86,16 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
 // max_rounds sweeps ran; returns the most sweeps any 64-column chunk took. Pinned rows are
 // dropped. b stored by columns is only read; b stored by rows is the solve's own right-hand side,
 // its pinned rows zeroed in place.
+// speculate (host memory from hipHostMalloc, one int a 64-column chunk; FP64 solve blocks only):
+// run the first round alone and return without waiting for its test, whose verdict (nonzero: a
+// row still above tolerance) lands in speculate[chunk] with the stream's next synchronization. The
+// caller checks it there and solves again without speculating when any is set.
 int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp32,
                       double* b, double* x, int64_t k, double tolerance, bool componentwise,
                       int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
                       hipStream_t stream, PhaseTimer* timer = nullptr,
                       int sweep_phase = kPhaseHeatSweep, FieldLayout b_layout = FieldLayout::kColumns,
-                      FieldLayout x_layout = FieldLayout::kColumns);
+                      FieldLayout x_layout = FieldLayout::kColumns, int32_t* speculate = nullptr);
 
 // The components' members cut into 256-member chunks, for the null-space projection's reductions.
 struct NullSpace {

@@ -1064,10 +1064,11 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
                       double* b, double* x, int64_t k, double tolerance, bool componentwise,
                       int32_t max_rounds, const int32_t* pinned_new, SolveWorkspace& ws,
                       hipStream_t stream, PhaseTimer* timer, int sweep_phase,
-                      FieldLayout b_layout, FieldLayout x_layout) {
+                      FieldLayout b_layout, FieldLayout x_layout, int32_t* speculate) {
   const int64_t n = plan.n;
   if (n == 0 || k == 0) return 0;
   int32_t max_used = 0;
+  if (use_fp32) speculate = nullptr;
   for (int64_t q0 = 0; q0 < k; q0 += kCols) {
     const int32_t kk = static_cast<int32_t>(std::min<int64_t>(kCols, k - q0));
     // The backward tiles address a chunk's fields with 32-bit element offsets.
@@ -1158,6 +1159,11 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
         residual(test_only ? nullptr : ws.r.ptr);
       }
       HG_HIP(hipGetLastError());
+      if (speculate) {
+        HG_HIP(hipMemcpyAsync(speculate + q0 / kCols, ws.pending.ptr, sizeof(int32_t),
+                              hipMemcpyDeviceToHost, stream));
+        break;
+      }
       int32_t pending = 0;
       HG_HIP(hipMemcpyAsync(&pending, ws.pending.ptr, sizeof(int32_t), hipMemcpyDeviceToHost,
                             stream));

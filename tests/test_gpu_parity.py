@@ -344,3 +344,28 @@ def test_memory_budget_refuses_and_the_solves_iterate(cuda):
     d = heat.heat_geodesic(saddle, _tensor(cuda, np.array([0], np.int32)), method="meshlab")
     assert bool(cuda.isfinite(d).all())
     assert graded.ill_conditioned and graded.nbytes == 0
+
+
+def test_a_failed_speculated_round_solves_again(cuda):
+    """Through FP64 factors the first refinement round runs on without waiting for its test, whose
+    verdict is read at the call's end. A heat tolerance no first round meets (1e-15 componentwise)
+    makes the call solve again, refining round by round: the answer still matches the oracle and
+    the statistics count the extra rounds."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh("bumpy")
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method="meshlab")
+    assert solver.factor()
+    sets = [np.array([s], dtype=np.int32) for s in (0, 17, 300)]
+    expected = ho.geodesic_columns(vertices, faces, sets)
+    solver.set_refinement(heat=1e-15)
+    got = solver.geodesic([_tensor(cuda, s) for s in sets]).cpu().numpy()
+    assert solver.last_stats["heat_refinements"] > 1
+    for q in range(len(sets)):
+        worst, _ = ho.relative_error(got[q], expected[q])
+        assert worst <= 1e-9, (q, worst)
+    solver.set_refinement(heat=1e-12)
+    again = solver.geodesic([_tensor(cuda, s) for s in sets]).cpu().numpy()
+    assert solver.last_stats["heat_refinements"] == 1
+    assert np.abs(again - got).max() <= 1e-9 * np.ptp(expected)
