@@ -55,8 +55,8 @@ HBM_PEAK_GBS = 8000.0
 def parse() -> argparse.Namespace:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=5)
-    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=30)
+    parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--queries", type=int, default=64)
     parser.add_argument("--plain-queries", type=int, default=8)
     parser.add_argument("--cpu-sample", type=int, default=4)

@@ -218,13 +218,10 @@ __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
   }
   // Staging: entry e = t + 256u of a stage is W row e % 64 / f column e % 64 at inner e / 64.
   double wreg[4], freg[4];
-  // f = r(cols) plus the children's updates: a stage's gather pairs issue first, then its W and r
-  // loads, then (once the pairs are in, the rest still in flight) the update rows.
-  auto load = [&](int32_t j0) {
-    int64_t grow[4];
-    int gq[4];
-    bool glive[4];
-    int2 pair[4];
+  // f = r(cols) plus the children's updates: a stage's gather pairs are loaded a stage ahead
+  // (load_pairs), so its W, r and update-row loads issue together.
+  int2 pair[4];
+  auto stage_rows = [&](int32_t j0, int64_t grow[4], int gq[4], bool glive[4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
@@ -233,7 +230,20 @@ __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
       glive[u] = gq[u] < kk && j < k1;
       grow[u] = vbase + j;
     }
-    if (kGathers) gather_pairs(a, grow, glive, pair);
+  };
+  auto load_pairs = [&](int32_t j0) {
+    if (!kGathers) return;
+    int64_t grow[4];
+    int gq[4];
+    bool glive[4];
+    stage_rows(j0, grow, gq, glive);
+    gather_pairs(a, grow, glive, pair);
+  };
+  auto load = [&](int32_t j0) {
+    int64_t grow[4];
+    int gq[4];
+    bool glive[4];
+    stage_rows(j0, grow, gq, glive);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = t + 256 * u;
@@ -247,7 +257,9 @@ __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
   f64x4 acc[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
+  load_pairs(k0);
   load(k0);
+  if (k0 + kInner < k1) load_pairs(k0 + kInner);
   for (int32_t j0 = k0; j0 < k1; j0 += kInner) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -256,7 +268,10 @@ __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
       Fs[e / kCols][e % kCols] = freg[u];
     }
     __syncthreads();
-    if (j0 + kInner < k1) load(j0 + kInner);
+    if (j0 + kInner < k1) {
+      load(j0 + kInner);
+      if (j0 + 2 * kInner < k1) load_pairs(j0 + 2 * kInner);
+    }
     if (rows_live) stage_products(Ws, Fs, wave, lane, blocks, acc);
     __syncthreads();
   }
@@ -402,15 +417,17 @@ __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
   // Staging: entry e = t + 256u is inverse-factor column e / kD at inner e % kD (contiguous in
   // memory), and g's right-hand side e % 64 at inner e / 64.
   double wreg[kPer], greg[kPer];
-  // A stage's structure-row indices issue first, then its W loads, then (once the indices are
-  // in, W still in flight) its g loads.
-  auto load = [&](int32_t i0) {
-    int32_t srow[kPer];
+  // A stage's structure-row indices are loaded a stage ahead (load_rows), so its W and g loads
+  // issue together.
+  int32_t srow[kPer];
+  auto load_rows = [&](int32_t i0) {
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int32_t ig = i0 + (t + 256 * u) / kCols;
       srow[u] = (ig >= nc && ig < k1) ? srows[ig - nc] : 0;
     }
+  };
+  auto load = [&](int32_t i0) {
     // 32-bit element offsets from wave-uniform bases (a front's block and a field both stay
     // under 2^31 entries), one register an address.
 #pragma unroll
@@ -438,7 +455,9 @@ __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
   f64x4 acc[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
+  load_rows(k0);
   load(k0);
+  if (k0 + kD < k1) load_rows(k0 + kD);
   for (int32_t i0 = k0; i0 < k1; i0 += kD) {
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -447,7 +466,10 @@ __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
       Gs[e / kCols][e % kCols] = greg[u];
     }
     __syncthreads();
-    if (i0 + kD < k1) load(i0 + kD);
+    if (i0 + kD < k1) {
+      load(i0 + kD);
+      if (i0 + 2 * kD < k1) load_rows(i0 + 2 * kD);
+    }
     if (cols_live) stage_products<kD>(Ws, Gs, wave, lane, blocks, acc);
     __syncthreads();
   }
