@@ -990,13 +990,34 @@ __global__ __launch_bounds__(256) void rows_out_kernel(const double* __restrict_
   __shared__ double tile[kTile][kCols + 1];
   const int64_t v0 = static_cast<int64_t>(blockIdx.x) * kTile;
   const int32_t rows = static_cast<int32_t>(min<int64_t>(kTile, n - v0));
-  for (int e = threadIdx.x; e < rows * kk; e += 256) {
-    const int lv = e / kk, q = e % kk;
-    const int64_t v = v0 + lv;
-    const double value = x[static_cast<int64_t>(iperm[v]) * kk + q] -
-                         means[static_cast<int64_t>(q) * n_comp + labels[v]];
+  if (kk == kCols) {
+    // A lane is a column q; wave w reads rows w, w + 4, ... (each a contiguous row of x), all
+    // sixteen rows' loads issued before any is used.
+    const int q = threadIdx.x % kCols, w = threadIdx.x / kCols;
     const double offset = shift[2 * q];
-    tile[lv][q] = shift[2 * q + 1] != 0.0 ? offset - value : value - offset;
+    const bool flip = shift[2 * q + 1] != 0.0;
+    double xv[kTile / 4], mv[kTile / 4];
+#pragma unroll
+    for (int j = 0; j < kTile / 4; ++j) {
+      const int lv = w + 4 * j;
+      const int64_t v = v0 + min(lv, rows - 1);
+      xv[j] = x[static_cast<int64_t>(iperm[v]) * kCols + q];
+      mv[j] = means[static_cast<int64_t>(q) * n_comp + labels[v]];
+    }
+#pragma unroll
+    for (int j = 0; j < kTile / 4; ++j) {
+      const double value = xv[j] - mv[j];
+      tile[w + 4 * j][q] = flip ? offset - value : value - offset;
+    }
+  } else {
+    for (int e = threadIdx.x; e < rows * kk; e += 256) {
+      const int lv = e / kk, q = e % kk;
+      const int64_t v = v0 + lv;
+      const double value = x[static_cast<int64_t>(iperm[v]) * kk + q] -
+                           means[static_cast<int64_t>(q) * n_comp + labels[v]];
+      const double offset = shift[2 * q];
+      tile[lv][q] = shift[2 * q + 1] != 0.0 ? offset - value : value - offset;
+    }
   }
   __syncthreads();
   const int lv = threadIdx.x % kTile, qq = threadIdx.x / kTile;
