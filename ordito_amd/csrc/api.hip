@@ -114,6 +114,7 @@ struct hg_mesh {
   DeviceArray<int32_t> d_sources, invalid;
   PinnedArray<int32_t> h_invalid;  // the seeds' range flag, read back with the call's last sync
   DeviceArray<int64_t> d_owner, d_offsets;
+  PinnedArray<int64_t> h_stage;  // the owners and offsets of a call, staged for an async upload
   int64_t field_capacity = 0;
   DeviceArray<double> packed_phi;  // packed meshes: the solved slot columns before extraction
   int64_t packed_capacity = 0;
@@ -621,15 +622,22 @@ void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
                       m.stream);
 }
 
+// Each source entry's query and the queries' offsets, staged in pinned memory so both uploads stay
+// asynchronous (the stage is rewritten only by the next call, after this one's synchronization).
 void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets, int64_t k) {
   const int64_t total = offsets[k] - offsets[0];
-  std::vector<int64_t> owner(static_cast<size_t>(std::max<int64_t>(total, 1)));
+  const int64_t owners = std::max<int64_t>(total, 1);
+  if (m.h_stage.size < owners + k + 1) m.h_stage.allocate(owners + k + 1);
+  int64_t* owner = m.h_stage.ptr;
+  int64_t* rel_offsets = m.h_stage.ptr + owners;
+  owner[0] = 0;
   for (int64_t q = 0; q < k; ++q)
     for (int64_t e = offsets[q]; e < offsets[q + 1]; ++e) owner[e - offsets[0]] = q;
-  std::vector<int64_t> rel_offsets(k + 1);
   for (int64_t q = 0; q <= k; ++q) rel_offsets[q] = offsets[q] - offsets[0];
-  m.d_owner.assign(owner, m.stream);
-  m.d_offsets.assign(rel_offsets, m.stream);
+  if (m.d_owner.size < owners) m.d_owner.allocate(owners);
+  if (m.d_offsets.size < k + 1) m.d_offsets.allocate(k + 1);
+  m.d_owner.upload(owner, owners, m.stream);
+  m.d_offsets.upload(rel_offsets, k + 1, m.stream);
   if (m.d_sources.size < std::max<int64_t>(total, 1)) m.d_sources.allocate(std::max<int64_t>(total, 1));
   if (total > 0)
     HG_HIP(hipMemcpyAsync(m.d_sources.ptr, d_sources + offsets[0], sizeof(int32_t) * total,
@@ -1135,6 +1143,9 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     HG_REQUIRE(source_offsets != nullptr, HG_ERR_VALUE, "source_offsets is null");
     HG_REQUIRE(mesh->segments() == 1, HG_ERR_VALUE,
                "a packed mesh answers queries through hg_geodesic_packed");
+    for (int64_t q = 0; q < n_queries; ++q)
+      HG_REQUIRE(source_offsets[q] <= source_offsets[q + 1], HG_ERR_VALUE,
+                 "source_offsets must not decrease");
     hg_mesh& m = *mesh;
     m.stats = hg_solve_stats{};
     const int64_t n = m.n, k = n_queries;

@@ -565,6 +565,36 @@ class HeatSolver:
         offsets = [0]
         for c in counts:
             offsets.append(offsets[-1] + c)
+        return self.geodesic_flat(flat, offsets)
+
+    def geodesic_flat(self, sources, offsets):
+        """
+        ``geodesic`` for source sets given as one flat ``(S,)`` int32 tensor on the GPU and
+        ``k + 1`` host offsets (a sequence of ints or a ``ctypes`` array of ``c_int64``): set ``q``
+        is ``sources[offsets[q]:offsets[q + 1]]``. The serving form: a batch assembled once is
+        solved without converting or concatenating its sets on every call.
+        """
+        torch = _arrays.torch_module()
+        mesh = self._bound_mesh()
+        n = mesh.n_vertices
+        k = len(offsets) - 1
+        if k < 0:
+            raise ValueError("geodesic_flat: offsets must hold k + 1 >= 1 entries")
+        if (
+            type(sources) is not torch.Tensor
+            or sources.dtype != torch.int32
+            or sources.device.type != "cuda"
+            or sources.dim() != 1
+            or not sources.is_contiguous()
+        ):
+            raise ValueError("geodesic_flat: sources must be a contiguous (S,) int32 tensor on the GPU")
+        if offsets[0] != 0 or offsets[k] > sources.numel():
+            # (that they do not decrease the library checks)
+            raise ValueError("geodesic_flat: offsets must start at 0 and end within sources")
+        if k == 0 or n == 0 or mesh.n_faces == 0:
+            return torch.zeros((k, n), dtype=torch.float64, device="cuda")
+        if not isinstance(offsets, ctypes.Array):
+            offsets = _lib.offsets_array(offsets)
         out = torch.empty((k, n), dtype=torch.float64, device="cuda")  # every entry is written
         # Rent or buy, per system (heat.py:4394-4404): the second solve factors each. One handle
         # and one factored-state query for the whole call. At "igl" a mesh with a rim also solves
@@ -591,8 +621,8 @@ class HeatSolver:
         _lib.call(
             "hg_geodesic",
             native.ptr,
-            _arrays.pointer(flat),
-            _lib.offsets_array(offsets),
+            _arrays.pointer(sources),
+            offsets,
             k,
             _arrays.pointer(out),
             mode,

@@ -369,3 +369,23 @@ def test_a_failed_speculated_round_solves_again(cuda):
     again = solver.geodesic([_tensor(cuda, s) for s in sets]).cpu().numpy()
     assert solver.last_stats["heat_refinements"] == 1
     assert np.abs(again - got).max() <= 1e-9 * np.ptp(expected)
+
+
+def test_geodesic_flat_equals_geodesic_and_checks_its_offsets(cuda):
+    from ordito_amd import heat
+
+    vertices, faces = mesh("icosphere3")
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method="meshlab")
+    assert solver.factor()  # both calls through the factors
+    sets = [np.array([0, 5], np.int32), np.array([], np.int32), np.array([17], np.int32)]
+    listed = solver.geodesic([_tensor(cuda, s) for s in sets]).cpu().numpy()
+    flat = _tensor(cuda, np.concatenate(sets))
+    got = solver.geodesic_flat(flat, [0, 2, 2, 3]).cpu().numpy()
+    assert np.array_equal(got, listed)
+    with pytest.raises(ValueError):
+        solver.geodesic_flat(flat, [0, 2, 4])  # past the end of the sources
+    with pytest.raises(ValueError):
+        solver.geodesic_flat(flat, [0, 3, 2, 3])  # decreasing
+    with pytest.raises(ValueError):
+        solver.geodesic_flat(flat.long(), [0, 3])  # not int32
