@@ -595,7 +595,8 @@ __global__ __launch_bounds__(256) void bwd_tile1_kernel(
     const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_j0,
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
-  __shared__ double part[kTile][5];
+  constexpr int kStage = 1024;  // entries of g staged at a time
+  __shared__ double Gs[kStage];
   const int32_t task = xcd_block();
   const int32_t s = task_s[task];
   const int32_t j0 = task_j0[task];
@@ -605,26 +606,42 @@ __global__ __launch_bounds__(256) void bwd_tile1_kernel(
   const int32_t m = nc + a.nrow[s];
   const TW* W = w + a.w_off[s];
   const int32_t* srows = a.rows + a.row_ptr[s];
-  // Four lanes a solve-block column, each striding the column's rows (contiguous in memory).
-  const int lane = threadIdx.x % 4, jj = threadIdx.x / 4;
-  const int32_t j = j0 + jj;
-  double acc = 0.0;
-  if (j < nc) {
-    const TW* wj = W + static_cast<int64_t>(j) * m;
-    for (int32_t i = max(k0, j) + lane; i < k1; i += 4) {
-      const double g = i < nc ? a.y[c0 + i] : a.d[srows[i - nc]];
-      acc += widen(wj[i]) * g;
+  // g = [y(cols); d(rows)] over the inner range is staged in LDS once (its structure-row gathers
+  // off the products' path); then sixteen lanes a solve-block column, each group of sixteen
+  // reading 16 consecutive entries of its column (contiguous in memory) a step. The 64 columns go
+  // in four passes of sixteen; each column's sixteen lane sums fold in a fixed butterfly.
+  const int sub = threadIdx.x % 16, grp = threadIdx.x / 16;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int32_t s0 = k0; s0 < k1; s0 += kStage) {
+    const int32_t s1 = min(k1, s0 + kStage);
+    for (int32_t i = s0 + static_cast<int32_t>(threadIdx.x); i < s1; i += 256)
+      Gs[i - s0] = i < nc ? a.y[c0 + i] : a.d[srows[i - nc]];
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int32_t j = j0 + grp + 16 * p;
+      if (j >= nc) continue;
+      const TW* wj = W + static_cast<int64_t>(j) * m;
+      // Rows above the diagonal of the inverse's leading block are zero: start at j's group.
+      const int32_t first = s0 + max(0, (j - s0) / 16 * 16);
+      for (int32_t i = first + sub; i < s1; i += 16) acc[p] += widen(wj[i]) * Gs[i - s0];
     }
+    __syncthreads();
   }
-  part[jj][lane] = acc;
-  __syncthreads();
-  if (lane == 0 && j < nc) {
-    const double sum = ((part[jj][0] + part[jj][1]) + part[jj][2]) + part[jj][3];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) acc[p] += __shfl_xor(acc[p], off, 16);
+  if (sub != 0) return;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int32_t jj = grp + 16 * p, j = j0 + jj;
+    if (j >= nc) continue;
     if (slot >= 0) {
-      a.partials[static_cast<int64_t>(slot) * kTile + jj] = sum;
+      a.partials[static_cast<int64_t>(slot) * kTile + jj] = acc[p];
     } else {
-      a.d[c0 + j] = sum;
-      if (a.x) a.x_next[c0 + j] = a.x[c0 + j] + sum;
+      a.d[c0 + j] = acc[p];
+      if (a.x) a.x_next[c0 + j] = a.x[c0 + j] + acc[p];
     }
   }
 }
@@ -809,8 +826,26 @@ __global__ void absmax_partial_kernel(const double* __restrict__ b, int64_t n,
 __global__ void absmax_rows_partial_kernel(const double* __restrict__ b, int64_t n, int32_t kk,
                                            double* __restrict__ partials) {
   __shared__ double scratch[4][64];
+  __shared__ unsigned long long best[kCols];
   const int q = threadIdx.x % 64, part = threadIdx.x / 64;
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * 256;
+  if (kk < kCols / 4) {
+    // Few columns: the block's rows are read as one contiguous run, every lane busy, and each
+    // column's maximum kept by an LDS atomic on the bits (for non-negative doubles their order is
+    // the values' order, and a maximum does not depend on the order it is taken in).
+    if (threadIdx.x < kCols) best[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int64_t count = min<int64_t>(n - i0, 256) * kk;
+    const double* run = b + i0 * kk;
+    for (int64_t e = threadIdx.x; e < count; e += 256)
+      atomicMax(&best[e % kk], static_cast<unsigned long long>(
+                                   __builtin_bit_cast(uint64_t, fabs(run[e]))));
+    __syncthreads();
+    if (threadIdx.x < kk)
+      partials[static_cast<int64_t>(threadIdx.x) * gridDim.x + blockIdx.x] =
+          __builtin_bit_cast(double, static_cast<uint64_t>(best[threadIdx.x]));
+    return;
+  }
   double v = 0.0;
   if (q < kk)
     for (int64_t i = i0 + part; i < min<int64_t>(n, i0 + 256); i += 4) v = fmax(v, fabs(b[i * kk + q]));
