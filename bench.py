@@ -314,8 +314,7 @@ def serve_workload(args) -> int:
     server = GeodesicServer(meshes, budget_bytes=args.budget_gb * 1e9, workers=args.workers)
     log(f"== warming {args.meshes} meshes ({sum(sizes.values())} vertices)")
     t0 = time.perf_counter()
-    for future in [server.submit(i, [0]) for i in meshes]:
-        future.result()
+    server.warm()
     setup = time.perf_counter() - t0
     log(f"   warm in {setup:.1f}s; {args.duration:.0f}s of arrivals at {args.rate:.0f}/s")
     start = time.perf_counter()

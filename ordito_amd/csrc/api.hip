@@ -622,23 +622,30 @@ void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
                       m.stream);
 }
 
+// Per-call buffers grow to at least twice their size and never below 1024 entries: a caller
+// whose batches vary reallocates (a free waits on every stream's work) a few times, not per call.
+template <typename Array>
+void reserve(Array& a, int64_t need) {
+  if (a.size < need) a.allocate(std::max<int64_t>({need, 2 * a.size, 1024}));
+}
+
 // Each source entry's query and the queries' offsets, staged in pinned memory so both uploads stay
 // asynchronous (the stage is rewritten only by the next call, after this one's synchronization).
 void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets, int64_t k) {
   const int64_t total = offsets[k] - offsets[0];
   const int64_t owners = std::max<int64_t>(total, 1);
-  if (m.h_stage.size < owners + k + 1) m.h_stage.allocate(owners + k + 1);
+  reserve(m.h_stage, owners + k + 1);
   int64_t* owner = m.h_stage.ptr;
   int64_t* rel_offsets = m.h_stage.ptr + owners;
   owner[0] = 0;
   for (int64_t q = 0; q < k; ++q)
     for (int64_t e = offsets[q]; e < offsets[q + 1]; ++e) owner[e - offsets[0]] = q;
   for (int64_t q = 0; q <= k; ++q) rel_offsets[q] = offsets[q] - offsets[0];
-  if (m.d_owner.size < owners) m.d_owner.allocate(owners);
-  if (m.d_offsets.size < k + 1) m.d_offsets.allocate(k + 1);
+  reserve(m.d_owner, owners);
+  reserve(m.d_offsets, k + 1);
   m.d_owner.upload(owner, owners, m.stream);
   m.d_offsets.upload(rel_offsets, k + 1, m.stream);
-  if (m.d_sources.size < std::max<int64_t>(total, 1)) m.d_sources.allocate(std::max<int64_t>(total, 1));
+  reserve(m.d_sources, owners);
   if (total > 0)
     HG_HIP(hipMemcpyAsync(m.d_sources.ptr, d_sources + offsets[0], sizeof(int32_t) * total,
                           hipMemcpyDefault, m.stream));
@@ -677,18 +684,21 @@ void divergence(hg_mesh& m, const double* heat, double* out, int64_t k,
                     heat, m.n, k, out, layout, m.stream);
 }
 
+// The call's three fields, sized for the next power of two of columns (see SolveWorkspace).
 double* reserve_fields(hg_mesh& m, int64_t k) {
   if (m.field_capacity < 3 * m.n * k) {
-    m.fields.allocate(std::max<int64_t>(3 * m.n * k, 1));
-    m.field_capacity = 3 * m.n * k;
+    int64_t columns = 1;
+    while (columns < k) columns *= 2;
+    m.fields.allocate(std::max<int64_t>(3 * m.n * columns, 1));
+    m.field_capacity = 3 * m.n * columns;
   }
   return m.fields.ptr;
 }
 
 void shift(hg_mesh& m, double* phi, int64_t k) {
   int32_t blocks = shift_partial_blocks(m.n);
-  if (m.shift_partials.size < blocks * k) m.shift_partials.allocate(blocks * k);
-  if (m.shift_values.size < 2 * k) m.shift_values.allocate(2 * k);
+  reserve(m.shift_partials, blocks * k);
+  reserve(m.shift_values, 2 * k);
   PhaseScope scope(&m.timer, kPhaseShift, m.stream);
   launch_shift(phi, m.d_sources.ptr, m.d_offsets.ptr, m.n, k, nullptr, nullptr,
                m.shift_partials.ptr, m.shift_values.ptr, m.stream);

@@ -1113,7 +1113,12 @@ void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream)
 
 }  // namespace
 
-void SolveWorkspace::reserve(const DevicePlan& plan, int64_t kk) {
+void SolveWorkspace::reserve(const DevicePlan& plan, int64_t kk_needed) {
+  // Sized for the next power of two of columns: a caller whose batches vary in width reallocates
+  // (a hipFree waits on every stream's work) a handful of times, not once per new width.
+  int64_t kk = 1;
+  while (kk < kk_needed) kk *= 2;
+  kk = std::min<int64_t>(kk, kCols);
   const int64_t entries = plan.n * kk;
   if (capacity < entries) {
     bp.allocate(entries);

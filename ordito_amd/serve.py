@@ -61,8 +61,10 @@ class _Resident:
     pending: collections.deque = field(default_factory=collections.deque)
 
 
-def footprint_estimate(n_vertices: int, n_faces: int, precision: str = "fp64") -> int:
-    """Bytes a mesh holds once factored, plus the transient arena of a factorization, from its size.
+def footprint_estimate(n_vertices: int, n_faces: int, precision: str = "fp64",
+                       columns: int = 16) -> int:
+    """Bytes a mesh holds once factored, plus the transient arena of a factorization and the solve
+    buffers of a ``columns``-wide batch, from its size.
 
     Measured on the benchmark torus (524,288 vertices): 56.8 M inverse-factor entries (108 per
     vertex) and 220 M front-arena entries (420 per vertex); surfaces of other sizes scale close to
@@ -73,7 +75,9 @@ def footprint_estimate(n_vertices: int, n_faces: int, precision: str = "fp64") -
     arena_entries = 420.0 * n_vertices * scale
     factor_bytes = factor_entries * (8 + (4 if precision == "mixed" else 0)) * 2  # two systems
     operators = 3.67 / 0.524 * n_vertices * 8 * 5 + n_faces * (40 + 120)  # + the face geometry
-    return int(factor_bytes + arena_entries * 8 + operators)
+    # Three fields and six refinement vectors of n x columns, the update vectors (about 2.5 n).
+    buffers = (3 + 6 + 2.5 * scale) * n_vertices * columns * 8
+    return int(factor_bytes + arena_entries * 8 + operators + buffers)
 
 
 class GeodesicServer:
@@ -83,12 +87,13 @@ class GeodesicServer:
     """
 
     def __init__(self, meshes: dict, *, budget_bytes: float = 200e9, max_batch: int = 64,
-                 workers: int = 4, precision: str = "fp64") -> None:
+                 workers: int = 4, precision: str = "fp64", warm_columns: int = 16) -> None:
         _arrays.require_gpu()
         self._torch = _arrays.torch_module()
         self._meshes = dict(meshes)
         self.budget = int(budget_bytes)
         self.max_batch = int(max_batch)
+        self.warm_columns = int(warm_columns)
         self.precision = precision
         self._resident: dict[Any, _Resident] = {}
         self._lock = threading.Condition()
@@ -103,6 +108,26 @@ class GeodesicServer:
         self._scheduler.start()
 
     # -- client side -----------------------------------------------------------------------------
+
+    def warm(self, columns: int | None = None) -> None:
+        """
+        Factor every mesh and size its solve buffers for batches of up to ``columns`` queries
+        (default ``warm_columns``, rounded up to a power of two), so that serving such batches
+        allocates nothing: a device free waits on every stream's work, and would stall every worker.
+        """
+        columns = self.warm_columns if columns is None else int(columns)
+        futures = []
+        for mesh_id in self._meshes:
+            with self._lock:
+                entry = self._resident.get(mesh_id) or self._new_entry(mesh_id)
+                for _ in range(int(columns)):
+                    request = _Request(mesh_id, np.zeros(1, dtype=np.int32),
+                                       concurrent.futures.Future(), time.perf_counter())
+                    entry.pending.append(request)
+                    futures.append(request.future)
+                self._lock.notify_all()
+        for future in futures:
+            future.result()
 
     def submit(self, mesh_id, sources) -> concurrent.futures.Future:
         """Queue one query; the future resolves to the (V,) float64 distance on the GPU."""
@@ -156,7 +181,8 @@ class GeodesicServer:
         n_vertices = int(np.prod(vertices.shape)) // 3
         n_faces = int(np.prod(faces.shape)) // 3
         entry = _Resident(n_vertices, n_faces,
-                          estimate=footprint_estimate(n_vertices, n_faces, self.precision))
+                          estimate=footprint_estimate(n_vertices, n_faces, self.precision,
+                                                      self.warm_columns))
         self._resident[mesh_id] = entry
         return entry
 
@@ -226,8 +252,12 @@ class GeodesicServer:
                     entry.solver.factor()
                     entry.factored = True
                     entry.nbytes = entry.solver.nbytes
-                sources = [torch.from_numpy(r.sources).to("cuda", non_blocking=False) for r in batch]
-                fields = entry.solver.geodesic(sources)
+                # The batch's sources in one flat upload, its sets delimited by offsets.
+                offsets = [0]
+                for r in batch:
+                    offsets.append(offsets[-1] + int(r.sources.size))
+                flat = torch.from_numpy(np.concatenate([r.sources for r in batch])).to("cuda")
+                fields = entry.solver.geodesic_flat(flat, offsets)
                 self._streams[worker].synchronize()
             done = time.perf_counter()
             for q, r in enumerate(batch):
