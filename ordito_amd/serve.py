@@ -47,6 +47,17 @@ class _Request:
     done: float = 0.0
 
 
+@dataclass(frozen=True)
+class Completion:
+    """When a finished request was submitted, started and done (perf_counter seconds): the server
+    keeps these, not the requests, whose results belong to the callers."""
+
+    mesh_id: Any
+    submitted: float
+    started: float
+    done: float
+
+
 @dataclass
 class _Resident:
     n_vertices: int
@@ -98,7 +109,7 @@ class GeodesicServer:
         self._resident: dict[Any, _Resident] = {}
         self._lock = threading.Condition()
         self._closing = False
-        self._completed: list[_Request] = []
+        self._completed: list[Completion] = []
         self._evictions = 0
         self._batches = 0
         self._streams = [self._torch.cuda.Stream() for _ in range(workers)]
@@ -271,7 +282,8 @@ class GeodesicServer:
             with self._lock:
                 entry.busy = False
                 entry.last_used = time.perf_counter()
-                self._completed.extend(r for r in batch if r.done)
+                self._completed.extend(Completion(r.mesh_id, r.submitted, r.started, r.done)
+                                       for r in batch if r.done)
                 self._batches += 1
                 self._idle.append(worker)
                 self._lock.notify_all()
