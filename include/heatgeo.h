@@ -132,6 +132,17 @@ int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_fa
 int hg_factored(hg_mesh* mesh, int32_t* out_systems);
 
 /*
+ * Graph replay of hg_geodesic (no counterpart in the reference, whose kernels are launched one by
+ * one): with every system factored in FP64 and timing off, the whole path (seeds, refined sweeps,
+ * divergence, Poisson, finish) is captured once per column count as a HIP graph on a stream of the
+ * library's own and replayed, the call's sources, offsets and output reaching it through staged
+ * buffers. On by default; enable = 0 solves eagerly. A capture that fails leaves the mesh eager.
+ * hg_graph_state reports the graphs held and whether a capture failed.
+ */
+int hg_set_graphs(hg_mesh* mesh, int32_t enable);
+int hg_graph_state(hg_mesh* mesh, int64_t* out_cached, int32_t* out_broken);
+
+/*
  * The factorizations' memory budget in bytes (sparse_cholesky's CHOLESKY_MEMORY_BUDGET,
  * cholesky.py:72, there a fixed 4 GiB): a factorization whose peak would exceed it is refused and
  * its system solved by iteration. 0 (the default) takes 90 % of the device memory free when each

@@ -43,6 +43,8 @@ EXPORTED = (
     "hg_operator_views",
     "hg_factor",
     "hg_factored",
+    "hg_set_graphs",
+    "hg_graph_state",
     "hg_set_memory_budget",
     "hg_set_refinement",
     "hg_set_plan_options",
@@ -134,6 +136,8 @@ _SIGNATURES = {
     "hg_diffuse_pinned": ([_P, _P, _P, _I64, _I32, _I32, _PI32, _PI32], ctypes.c_int),
     "hg_rim_count": ([_P, _PI64], ctypes.c_int),
     "hg_set_memory_budget": ([_P, _I64], ctypes.c_int),
+    "hg_set_graphs": ([_P, _I32], ctypes.c_int),
+    "hg_graph_state": ([_P, _PI64, _PI32], ctypes.c_int),
     "hg_solve_poisson": ([_P, _P, _P, _I64, _I32, _I32], ctypes.c_int),
     "hg_unit_gradient_divergence": ([_P, _P, _P, _I64], ctypes.c_int),
     "hg_seed_sources": ([_P, _P, _PI64, _I64, _P], ctypes.c_int),

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 
 #include "cholesky.h"
 
@@ -115,6 +116,27 @@ struct hg_mesh {
   PinnedArray<int32_t> h_invalid;  // the seeds' range flag, read back with the call's last sync
   DeviceArray<int64_t> d_owner, d_offsets;
   PinnedArray<int64_t> h_stage;  // the owners and offsets of a call, staged for an async upload
+  bool sources_on_device = false;  // d_offsets[k] holds this call's source count
+  // The distance's destination per 64-column chunk, read by the finishing kernel (so a replayed
+  // graph writes to each call's buffer), staged in pinned memory.
+  DeviceArray<double*> d_slots;
+  PinnedArray<double*> h_slots;
+  // Replay of the factored FP64 path: one instantiated graph per column count, valid while every
+  // buffer it names is the one it was captured with (signature) and no state change intervened.
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> signature;
+    hg_solve_stats stats{};
+    Graph() = default;
+    Graph(const Graph&) = delete;
+    Graph& operator=(const Graph&) = delete;
+    ~Graph() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+    }
+  };
+  std::map<int64_t, std::unique_ptr<Graph>> graphs;
+  bool use_graphs = true;
+  bool graphs_broken = false;  // a capture failed: this mesh runs eagerly from then on
   int64_t field_capacity = 0;
   DeviceArray<double> packed_phi;  // packed meshes: the solved slot columns before extraction
   int64_t packed_capacity = 0;
@@ -153,6 +175,9 @@ void require_single_or_factored(const hg_mesh& m, int32_t mode) {
   HG_REQUIRE(m.segments() == 1 || mode == HG_MODE_FACTORED, HG_ERR_VALUE,
              "packed meshes solve in HG_MODE_FACTORED only");
 }
+
+// Drops every captured graph: the state they were captured against is changing.
+void invalidate_graphs(hg_mesh& m) { m.graphs.clear(); }
 
 // ---- operators --------------------------------------------------------------------------------
 
@@ -284,6 +309,7 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   m.poisson_factor.reset();
   m.pinned_factor.reset();
   m.pinned_built = false;
+  invalidate_graphs(m);
 }
 
 void ensure_operators(hg_mesh& m) {
@@ -381,6 +407,7 @@ void ensure_pinned_system(hg_mesh& m) {
   launch_pinned_rows(m.offsets.ptr, m.columns.ptr, m.d_rim.ptr, m.heat.ptr, m.n, m.pinned.ptr,
                      m.pinned32.ptr, m.pinned_inv.ptr, m.stream);
   m.pinned_built = true;
+  invalidate_graphs(m);
 }
 
 // A factorization's peak device memory: the transient front arena plus the blocks it keeps.
@@ -416,13 +443,17 @@ void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
     PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
     if (!m.pinned_factor.held() || (keep_fp32 && !m.pinned_factor.w32.ptr)) {
       require_budget(m, system, keep_fp32);
+      invalidate_graphs(m);
       factor_numeric(*m.plan, m.pinned.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
                      m.heat_pin_slots, m.pinned_factor, keep_fp32, m.stream);
     }
     return;
   }
   Factor& target = system == HG_SYSTEM_HEAT ? m.heat_factor : m.poisson_factor;
-  if (!target.held() || (keep_fp32 && !target.w32.ptr)) require_budget(m, system, keep_fp32);
+  if (!target.held() || (keep_fp32 && !target.w32.ptr)) {
+    require_budget(m, system, keep_fp32);
+    invalidate_graphs(m);
+  }
   PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
   if (system == HG_SYSTEM_HEAT) {
     if (!m.heat_factor.held())
@@ -645,6 +676,7 @@ void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets
   reserve(m.d_offsets, k + 1);
   m.d_owner.upload(owner, owners, m.stream);
   m.d_offsets.upload(rel_offsets, k + 1, m.stream);
+  m.sources_on_device = true;
   reserve(m.d_sources, owners);
   if (total > 0)
     HG_HIP(hipMemcpyAsync(m.d_sources.ptr, d_sources + offsets[0], sizeof(int32_t) * total,
@@ -654,11 +686,14 @@ void upload_sources(hg_mesh& m, const int32_t* d_sources, const int64_t* offsets
 void seed(hg_mesh& m, const int64_t* offsets, int64_t k, double* u0,
           FieldLayout layout = FieldLayout::kColumns) {
   const int64_t total = offsets[k] - offsets[0];
-  HG_HIP(hipMemsetAsync(u0, 0, sizeof(double) * m.n * k, m.stream));
+  launch_zero(u0, static_cast<int64_t>(sizeof(double)) * m.n * k, m.stream);
   if (m.invalid.size < 1) m.invalid.allocate(1);
-  HG_HIP(hipMemsetAsync(m.invalid.ptr, 0, sizeof(int32_t), m.stream));
+  launch_zero(m.invalid.ptr, sizeof(int32_t), m.stream);
   const bool rows = layout == FieldLayout::kRows;
-  launch_seed(m.d_sources.ptr, m.d_owner.ptr, total, m.n, rows ? k : 0,
+  // With the offsets on the device the kernel reads the count there (a replay's own count).
+  const bool on_device = m.sources_on_device;
+  launch_seed(m.d_sources.ptr, m.d_owner.ptr, on_device ? m.d_owner.size : total,
+              on_device ? m.d_offsets.ptr + k : nullptr, m.n, rows ? k : 0,
               rows ? m.plan->iperm.ptr : nullptr, u0, m.invalid.ptr, m.stream);
   // The range flag comes back with the call's next synchronization (check_seeds); until then the
   // solve runs on, every kernel that reads the sources skipping the ones out of range.
@@ -725,6 +760,102 @@ bool speculation_failed(hg_mesh& m) {
                      [](int32_t v) { return v != 0; });
 }
 
+// Every buffer and setting a captured geodesic graph names: a replay is valid while all are the
+// same (buffers only grow, so one at the same address is at least as large).
+std::vector<uintptr_t> graph_signature(const hg_mesh& m, int64_t k) {
+  auto p = [](const void* q) { return reinterpret_cast<uintptr_t>(q); };
+  const SolveWorkspace& w = m.solve;
+  uint64_t tol[4];
+  std::memcpy(tol, m.refine_tolerance, sizeof(tol));
+  return {static_cast<uintptr_t>(k), p(m.fields.ptr), p(w.bp.ptr), p(w.xa.ptr), p(w.xb.ptr),
+          p(w.r.ptr), p(w.y.ptr), p(w.d.ptr), p(w.fvec.ptr), p(w.bmax.ptr), p(w.partials.ptr),
+          p(w.pending.ptr), static_cast<uintptr_t>(w.capacity), p(m.d_sources.ptr),
+          p(m.d_owner.ptr), static_cast<uintptr_t>(m.d_owner.size), p(m.d_offsets.ptr),
+          p(m.invalid.ptr), p(m.h_invalid.ptr), p(m.h_speculate.ptr), p(m.d_slots.ptr),
+          p(m.h_slots.ptr), p(m.null_partials.ptr), p(m.heat_factor.w64.ptr),
+          p(m.heat_factor.pvalues.ptr), p(m.poisson_factor.w64.ptr),
+          p(m.poisson_factor.pvalues.ptr), p(m.pinned_factor.w64.ptr),
+          p(m.pinned_factor.pvalues.ptr), p(m.plan.get()), static_cast<uintptr_t>(m.method),
+          static_cast<uintptr_t>(m.n_rim), tol[0], tol[1], tol[2], tol[3]};
+}
+
+// Whether this call can replay a graph: FP64 factors held for every system the path solves,
+// the fused finish, no phase timing, a single mesh.
+bool graphable(const hg_mesh& m, int32_t mode, int32_t precision) {
+  return m.use_graphs && !m.graphs_broken && precision == HG_PRECISION_FP64 &&
+         !m.timer.enabled && m.segments() == 1 && m.plan && m.rows_by_component &&
+         (mode == HG_MODE_FACTORED || m.heat_factor.held()) && m.heat_factor.held() &&
+         m.poisson_factor.held() &&
+         (m.method != HG_METHOD_IGL || m.n_rim == 0 || m.pinned_factor.held());
+}
+
+bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, int32_t mode,
+                   int32_t precision, bool finish);
+
+// The factored FP64 path of hg_geodesic replayed as one graph: captured on the first such call at
+// this column count (capturing runs nothing), then launched; each call's inputs reach it through
+// the staged sources, offsets and output slots. Returns false when the caller must solve eagerly
+// (a capture failed, or the speculated first round failed its test). The sources must be uploaded.
+// Graphs are captured and launched on a stream of the library's own, one per host thread (a
+// caller's stream may be the null stream, which cannot capture; threads serving different meshes
+// keep theirs apart), ordered after the caller's stream by an event. Both live until the process
+// ends.
+struct GraphStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t ready = nullptr;
+};
+thread_local GraphStream t_graph_stream;
+
+bool run_graph(hg_mesh& m, const int64_t* offsets, int64_t k, double* d_out, int32_t precision) {
+  GraphStream& gs = t_graph_stream;
+  if (!gs.stream) {
+    HG_HIP(hipStreamCreateWithFlags(&gs.stream, hipStreamNonBlocking));
+    HG_HIP(hipEventCreateWithFlags(&gs.ready, hipEventDisableTiming));
+  }
+  // The uploads on the caller's stream come first.
+  HG_HIP(hipEventRecord(gs.ready, m.stream));
+  HG_HIP(hipStreamWaitEvent(gs.stream, gs.ready, 0));
+  std::unique_ptr<hg_mesh::Graph>& g = m.graphs[k];
+  if (!g || g->signature != graph_signature(m, k)) {
+    g.reset();
+    auto graph = std::make_unique<hg_mesh::Graph>();
+    arm_speculation(m, k, true);
+    m.stats = hg_solve_stats{};
+    const hipStream_t caller = m.stream;
+    m.stream = gs.stream;
+    hipGraph_t captured = nullptr;
+    bool ok = hipStreamBeginCapture(gs.stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+    if (ok) {
+      try {
+        solve_columns(m, offsets, k, d_out, HG_MODE_FACTORED, precision, true);
+      } catch (const Error&) {
+        ok = false;
+      }
+      ok = hipStreamEndCapture(gs.stream, &captured) == hipSuccess && ok;
+    }
+    m.stream = caller;
+    if (ok) ok = hipGraphInstantiate(&graph->exec, captured, nullptr, nullptr, 0) == hipSuccess;
+    if (captured) (void)hipGraphDestroy(captured);
+    m.speculating = false;
+    if (!ok) {
+      graph->exec = nullptr;
+      m.graphs_broken = true;  // from now on this mesh solves eagerly
+      (void)hipGetLastError();
+      return false;
+    }
+    graph->stats = m.stats;
+    graph->signature = graph_signature(m, k);
+    g = std::move(graph);
+  }
+  arm_speculation(m, k, true);
+  const int64_t chunks = (k + kFieldChunk - 1) / kFieldChunk;
+  for (int64_t c = 0; c < chunks; ++c) m.h_slots.ptr[c] = d_out + c * kFieldChunk * m.n;
+  m.stats = g->stats;
+  HG_HIP(hipGraphLaunch(g->exec, gs.stream));
+  HG_HIP(hipStreamSynchronize(gs.stream));
+  return !speculation_failed(m);
+}
+
 // Seed, heat step, divergence and Poisson step over k columns of the whole (possibly packed)
 // system into phi (the caller's layout). The sources must be uploaded. With `finish` and the
 // fields stored by rows, phi is the finished distance (null space removed, shifted, oriented) and
@@ -776,8 +907,13 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
     // solution, which one pass turns into the distance in the caller's layout.
     solve_poisson(m, negdiv, heat, k, mode, precision, layout, FieldLayout::kRows);
     PhaseScope scope(&m.timer, kPhaseShift, m.stream);
+    const int64_t chunks = (k + kFieldChunk - 1) / kFieldChunk;
+    reserve(m.d_slots, chunks);
+    reserve(m.h_slots, chunks);
+    for (int64_t c = 0; c < chunks; ++c) m.h_slots.ptr[c] = phi + c * kFieldChunk * n;
+    m.d_slots.upload(m.h_slots.ptr, chunks, m.stream);
     finish_rows(*m.plan, m.null_space, m.comp_ptr.ptr, m.d_labels.ptr, heat, k, m.d_sources.ptr,
-                m.d_offsets.ptr, phi, m.null_partials, m.stream);
+                m.d_offsets.ptr, m.d_slots.ptr, m.null_partials, m.stream);
     return true;
   }
   solve_poisson(m, negdiv, phi, k, mode, precision, layout);
@@ -1004,6 +1140,7 @@ int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows,
     if (leaf != m.options.leaf_rows || top != m.options.top_rows) {
       // A new ordering: the plan and every factor built on it go.
       HG_HIP(hipStreamSynchronize(m.stream));
+      invalidate_graphs(m);
       m.heat_factor.reset();
       m.poisson_factor.reset();
       m.pinned_factor.reset();
@@ -1016,6 +1153,7 @@ int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows,
     }
     if (split != m.split_target) {
       m.split_target = split;
+      invalidate_graphs(m);
       if (m.plan) {
         HG_HIP(hipStreamSynchronize(m.stream));
         m.plan->upload_sweeps(*m.host_plan, split, m.stream);
@@ -1038,6 +1176,7 @@ int hg_set_refinement(hg_mesh* mesh, int32_t precision, double heat_tolerance,
                "the Poisson refinement tolerance must lie in (0, 1)");
     if (!keep_heat) mesh->refine_tolerance[precision][0] = heat_tolerance;
     if (!keep_poisson) mesh->refine_tolerance[precision][1] = poisson_tolerance;
+    invalidate_graphs(*mesh);
   });
 }
 
@@ -1082,12 +1221,29 @@ int hg_diffuse_pinned(hg_mesh* mesh, const double* d_rhs, double* d_solution, in
   });
 }
 
+int hg_set_graphs(hg_mesh* mesh, int32_t enable) {
+  return guarded([&] {
+    require_mesh(mesh);
+    mesh->use_graphs = enable != 0;
+    if (!mesh->use_graphs) invalidate_graphs(*mesh);
+  });
+}
+
+int hg_graph_state(hg_mesh* mesh, int64_t* out_cached, int32_t* out_broken) {
+  return guarded([&] {
+    require_mesh(mesh);
+    if (out_cached) *out_cached = static_cast<int64_t>(mesh->graphs.size());
+    if (out_broken) *out_broken = mesh->graphs_broken ? 1 : 0;
+  });
+}
+
 int hg_set_memory_budget(hg_mesh* mesh, int64_t bytes) {
   return guarded([&] {
     require_mesh(mesh);
     HG_REQUIRE(bytes >= 0, HG_ERR_VALUE, "the memory budget must not be negative");
     mesh->budget = bytes;
     mesh->refused = 0;
+    invalidate_graphs(*mesh);
   });
 }
 
@@ -1167,12 +1323,16 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
     }
     ensure_operators(m);
     upload_sources(m, d_sources, source_offsets, k);
-    for (bool speculate : {precision == HG_PRECISION_FP64, false}) {
-      m.stats = hg_solve_stats{};
-      arm_speculation(m, k, speculate);
-      if (!solve_columns(m, source_offsets, k, d_out, mode, precision, true)) shift(m, d_out, k);
-      HG_HIP(hipStreamSynchronize(m.stream));
-      if (!speculation_failed(m)) break;
+    bool done = false;
+    if (graphable(m, mode, precision)) done = run_graph(m, source_offsets, k, d_out, precision);
+    if (!done) {
+      for (bool speculate : {precision == HG_PRECISION_FP64, false}) {
+        m.stats = hg_solve_stats{};
+        arm_speculation(m, k, speculate);
+        if (!solve_columns(m, source_offsets, k, d_out, mode, precision, true)) shift(m, d_out, k);
+        HG_HIP(hipStreamSynchronize(m.stream));
+        if (!speculation_failed(m)) break;
+      }
     }
     check_seeds(m);
     m.timer.resolve();
@@ -1252,6 +1412,7 @@ int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* s
       m.packed_capacity = m.n * width;
     }
     arm_speculation(m, width, precision == HG_PRECISION_FP64);
+    m.sources_on_device = false;  // the seeds' count is the host's
     solve_columns(m, seed_offsets.data(), width, m.packed_phi.ptr, HG_MODE_FACTORED, precision);
     // A speculated first round that failed its test: solve again without speculating.
     HG_HIP(hipStreamSynchronize(m.stream));
@@ -1327,6 +1488,7 @@ int hg_release(hg_mesh* mesh) {
   return guarded([&] {
     require_mesh(mesh);
     HG_HIP(hipStreamSynchronize(mesh->stream));
+    invalidate_graphs(*mesh);
     mesh->heat_factor.reset();
     mesh->poisson_factor.reset();
     mesh->pinned_factor.reset();

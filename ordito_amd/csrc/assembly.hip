@@ -334,25 +334,37 @@ __global__ void average_kernel(double* __restrict__ a, const double* __restrict_
 // nothing and raises the flag (kernels/heat.py:599-623). One thread a source entry.
 // rows_k > 0: u0 is stored by rows (FieldLayout::kRows) over rows_k columns, vertex v at row
 // row_of[v].
+// count_at (device, nullable): the entry count is min(*count_at, n_entries) (a replayed launch
+// reads each call's count); the grid strides over the entries.
 __global__ void seed_kernel(const int32_t* __restrict__ sources, const int64_t* __restrict__ owner,
-                            int64_t n_entries, int64_t n, int64_t rows_k,
-                            const int32_t* __restrict__ row_of, double* __restrict__ u0,
-                            int32_t* __restrict__ invalid) {
-  int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n_entries) return;
-  int32_t v = sources[e];
-  if (v < 0 || v >= n) {
-    atomicOr(invalid, 1);
-    return;
+                            int64_t n_entries, const int64_t* __restrict__ count_at, int64_t n,
+                            int64_t rows_k, const int32_t* __restrict__ row_of,
+                            double* __restrict__ u0, int32_t* __restrict__ invalid) {
+  const int64_t count = count_at ? min(*count_at, n_entries) : n_entries;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < count;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t v = sources[e];
+    if (v < 0 || v >= n) {
+      atomicOr(invalid, 1);
+      continue;
+    }
+    const int64_t q = owner[e];
+    if (rows_k > 0) {
+      const int64_t chunk = q / kFieldChunk;
+      const int64_t kk = min(kFieldChunk, rows_k - chunk * kFieldChunk);
+      u0[chunk * kFieldChunk * n + static_cast<int64_t>(row_of[v]) * kk + q % kFieldChunk] = 1.0;
+    } else {
+      u0[q * n + v] = 1.0;
+    }
   }
-  const int64_t q = owner[e];
-  if (rows_k > 0) {
-    const int64_t chunk = q / kFieldChunk;
-    const int64_t kk = min(kFieldChunk, rows_k - chunk * kFieldChunk);
-    u0[chunk * kFieldChunk * n + static_cast<int64_t>(row_of[v]) * kk + q % kFieldChunk] = 1.0;
-  } else {
-    u0[q * n + v] = 1.0;
-  }
+}
+
+// Zeroes `words` 32-bit words: the clears of a solve, as kernels so a captured graph holds no
+// memset node.
+__global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t words) {
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < words;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[e] = 0u;
 }
 
 // Per face, the float64 quantities every query's divergence reuses (kernels/triangles.py:834-894,
@@ -742,12 +754,22 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
   HG_HIP(hipGetLastError());
 }
 
-void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
-                 int64_t rows_k, const int32_t* row_of, double* u0, int32_t* invalid,
-                 hipStream_t stream) {
+void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries,
+                 const int64_t* count_at, int64_t n, int64_t rows_k, const int32_t* row_of,
+                 double* u0, int32_t* invalid, hipStream_t stream) {
   if (n_entries == 0) return;
-  seed_kernel<<<blocks_for(n_entries), kBlock, 0, stream>>>(sources, owner, n_entries, n, rows_k,
-                                                            row_of, u0, invalid);
+  const unsigned blocks = std::min<unsigned>(blocks_for(n_entries), 1024u);
+  seed_kernel<<<blocks, kBlock, 0, stream>>>(sources, owner, n_entries, count_at, n, rows_k,
+                                             row_of, u0, invalid);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_zero(void* p, int64_t bytes, hipStream_t stream) {
+  if (bytes <= 0) return;
+  HG_REQUIRE(bytes % 4 == 0, HG_ERR_RUNTIME, "a cleared range is whole 32-bit words");
+  const int64_t words = bytes / 4;
+  const unsigned blocks = std::min<unsigned>(blocks_for(words), 2048u);
+  zero_words_kernel<<<blocks, kBlock, 0, stream>>>(static_cast<uint32_t*>(p), words);
   HG_HIP(hipGetLastError());
 }
 

@@ -133,9 +133,12 @@ void launch_operator_rows(const int32_t* offsets, const int32_t* columns,
                           double* heat_inv_diag, double* poisson_inv_diag, hipStream_t stream);
 // rows_k > 0: u0 stored by rows (FieldLayout::kRows) over rows_k columns, vertex v at row
 // row_of[v]; 0: by columns.
-void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries, int64_t n,
-                 int64_t rows_k, const int32_t* row_of, double* u0, int32_t* invalid,
-                 hipStream_t stream);
+// count_at (device, nullable): read the entry count there, at most n_entries.
+void launch_seed(const int32_t* sources, const int64_t* owner, int64_t n_entries,
+                 const int64_t* count_at, int64_t n, int64_t rows_k, const int32_t* row_of,
+                 double* u0, int32_t* invalid, hipStream_t stream);
+// Zeroes `bytes` (whole 32-bit words) with a kernel (graph-capturable, unlike a memset node).
+void launch_zero(void* p, int64_t bytes, hipStream_t stream);
 // Per face, kFaceGeom doubles: two of the gradient basis vectors a_k = N x e_k (the third is minus
 // their sum) and the three corners' integration vectors.
 constexpr int64_t kFaceGeom = 15;

@@ -1021,8 +1021,9 @@ __global__ __launch_bounds__(256) void rows_out_kernel(const double* __restrict_
                                                        int32_t n_comp,
                                                        const double* __restrict__ means,
                                                        const double* __restrict__ shift,
-                                                       double* __restrict__ out) {
+                                                       double* const* __restrict__ out_slot) {
   __shared__ double tile[kTile][kCols + 1];
+  double* __restrict__ out = *out_slot;
   const int64_t v0 = static_cast<int64_t>(blockIdx.x) * kTile;
   const int32_t rows = static_cast<int32_t>(min<int64_t>(kTile, n - v0));
   if (kk == kCols) {
@@ -1238,7 +1239,7 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
       };
       {
         PhaseScope scope(timer, kPhaseResidual, stream);
-        HG_HIP(hipMemsetAsync(ws.pending.ptr, 0, sizeof(int32_t), stream));
+        launch_zero(ws.pending.ptr, sizeof(int32_t), stream);
         residual(test_only ? nullptr : ws.r.ptr);
       }
       HG_HIP(hipGetLastError());
@@ -1305,8 +1306,8 @@ void remove_null_space(const NullSpace& ns, const int32_t* comp_ptr, const int32
 
 void finish_rows(const DevicePlan& plan, const NullSpace& ns, const int32_t* comp_ptr,
                  const int32_t* labels, const double* x, int64_t k, const int32_t* sources,
-                 const int64_t* source_offsets, double* out, DeviceArray<double>& scratch,
-                 hipStream_t stream) {
+                 const int64_t* source_offsets, double* const* out_slots,
+                 DeviceArray<double>& scratch, hipStream_t stream) {
   const int64_t n = plan.n;
   if (n == 0 || k == 0 || ns.n_chunks == 0) return;
   const int64_t need = static_cast<int64_t>(ns.n_chunks) * kCols +
@@ -1327,7 +1328,7 @@ void finish_rows(const DevicePlan& plan, const NullSpace& ns, const int32_t* com
         comp_ptr, ns.chunk_comp.ptr, ns.chunk_first.ptr, ns.n_comp, ns.n_chunks, partials, means,
         xq, kk, n, plan.iperm.ptr, labels, sources, source_offsets + q0, shift);
     rows_out_kernel<<<static_cast<unsigned>((n + kTile - 1) / kTile), 256, 0, stream>>>(
-        xq, kk, n, plan.iperm.ptr, labels, ns.n_comp, means, shift, out + q0 * n);
+        xq, kk, n, plan.iperm.ptr, labels, ns.n_comp, means, shift, out_slots + q0 / kCols);
   }
   HG_HIP(hipGetLastError());
 }

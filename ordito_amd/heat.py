@@ -353,6 +353,17 @@ class HeatSolver:
         """Whether a factorization was refused under the memory budget (its solves iterate)."""
         return bool(self._factored() >> 4)
 
+    def set_graphs(self, enable: bool = True) -> None:
+        """Replay the factored FP64 geodesic path as a HIP graph per column count (the default),
+        or launch it kernel by kernel. (No counterpart in the reference.)"""
+        _lib.call("hg_set_graphs", self._native().ptr, int(bool(enable)))
+
+    def graph_state(self) -> tuple[int, bool]:
+        """(graphs held, whether a capture failed and the mesh now solves eagerly)."""
+        cached, broken = ctypes.c_int64(), ctypes.c_int32()
+        _lib.call("hg_graph_state", self._native().ptr, ctypes.byref(cached), ctypes.byref(broken))
+        return int(cached.value), bool(broken.value)
+
     def set_memory_budget(self, nbytes: int) -> None:
         """
         The factorizations' memory budget in bytes (ordito's ``CHOLESKY_MEMORY_BUDGET``,

@@ -389,3 +389,34 @@ def test_geodesic_flat_equals_geodesic_and_checks_its_offsets(cuda):
         solver.geodesic_flat(flat, [0, 3, 2, 3])  # decreasing
     with pytest.raises(ValueError):
         solver.geodesic_flat(flat.long(), [0, 3])  # not int32
+
+
+@pytest.mark.parametrize("name", ["torus", "grid"])
+def test_graph_replays_equal_the_eager_path(cuda, name):
+    """The factored FP64 path replayed as a HIP graph gives the eager path's bits, call after call:
+    new sources and a new output buffer each replay, several column counts (one graph each),
+    eager calls and synchronizations in between, and the igl path with its rim-pinned solve."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh(name)
+    n = len(vertices)
+    method = "igl" if name == "grid" else "meshlab"
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method=method)
+    assert solver.factor()
+    solver.geodesic([_tensor(cuda, np.array([1], np.int32))])  # factors the pinned system at igl
+    rng = np.random.default_rng(5)
+    for k in (1, 5, 64, 5, 1, 64):
+        sets = [rng.integers(0, n, size=1 + q % 4).astype(np.int32) for q in range(k)]
+        solver.set_graphs(False)
+        eager = solver.geodesic([_tensor(cuda, s) for s in sets]).cpu().numpy()
+        solver.set_graphs(True)
+        for _ in range(2):
+            replay = solver.geodesic([_tensor(cuda, s) for s in sets])
+            cuda.cuda.synchronize()
+            assert np.array_equal(replay.cpu().numpy(), eager), k
+        cached, broken = solver.graph_state()
+        assert cached >= 1 and not broken
+    expected = ho.geodesic(vertices, faces, sets[0], method=method).distance
+    worst, _ = ho.relative_error(eager[0], expected)
+    assert worst <= 1e-9, worst
