@@ -72,11 +72,12 @@ def parse() -> argparse.Namespace:
     )
     parser.add_argument(
         "--workload",
-        choices=("torus", "sphere", "batch", "serve"),
+        choices=("torus", "sphere", "batch", "serve", "igl"),
         default="torus",
         help="torus: the headline configuration (default); sphere: the 2.6 M-vertex displaced "
         "sphere, mixed and FP64 against the oracle; batch: many meshes packed into one system; "
-        "serve: the continuous-batching server under Poisson arrivals",
+        "serve: the continuous-batching server under Poisson arrivals; igl: method='igl' on an "
+        "open grid (its rim-pinned heat solve), beside method='meshlab'",
     )
     many = parser.add_argument_group("--workload batch / serve")
     many.add_argument("--meshes", type=int, help="default 256 (batch), 64 (serve)")
@@ -184,6 +185,74 @@ def sphere_workload(args) -> int:
                             "rel_err_mean": float(np.mean([e[1] for e in errs]))}
         log(f"   {precision}: max {check[precision]['rel_err_max']:.3g}")
     result["oracle_check_icosphere7"] = check
+    print(json.dumps(result))
+    return 0
+
+
+def igl_workload(args) -> int:
+    """
+    ``method="igl"`` on an open surface: a jittered 724 x 724 planar grid (524,176 vertices, about
+    the torus's size, 2,892 on the rim), 64 single-source queries a step through FP64 factors of all
+    three systems (the Neumann heat, the rim-pinned heat, the Poisson), beside ``method="meshlab"``
+    on the same grid; the error of ``"igl"`` against the FP64 oracle on a 64 x 64 grid of the same
+    kind. Prints one JSON line.
+    """
+    import torch
+
+    from ordito_amd import creation, heat
+    from oracle import heat_oracle as ho
+
+    vertices, faces = creation.planar_grid(724, 724, 1.0, 0.2, args.seed)
+    rng = np.random.default_rng(args.seed + 1)
+    sources = rng.integers(0, len(vertices), size=args.queries).astype(np.int32)
+    source_sets = [torch.tensor([s], dtype=torch.int32, device="cuda") for s in sources]
+    lines = {}
+    for method in ("meshlab", "igl"):
+        mesh = heat.Trimesh(torch.from_numpy(vertices).cuda(), torch.from_numpy(faces).cuda())
+        solver = mesh.heat_solver(method=method, precision="fp64")
+        assert solver.factor()
+        solver._solves = {"heat": 2, "poisson": 2, "pinned": 2}  # past rent-or-buy
+        log(f"== {method}: {args.steps} steps of {len(source_sets)} queries")
+        elapsed, latencies, timing, _, stats = timed_steps(solver, source_sets, args.steps, args.warmup)
+        lines[method] = {
+            "value": round(len(source_sets) * args.steps / elapsed, 2),
+            "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+            "p50_ms": ms(latencies, 50),
+            "p99_ms": ms(latencies, 99),
+            "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in timing.items() if v[1]},
+            "refinements": stats,
+            "factor_gb": round(solver.nbytes / 1e9, 3),
+        }
+        log(f"   {lines[method]['value']} queries/s, {stats}")
+        del solver, mesh
+        torch.cuda.empty_cache()
+    small_v, small_f = creation.planar_grid(64, 64, 1.0, 0.2, args.seed)
+    picks = [0, len(small_v) // 2 + 17]
+    errors = [
+        ho.relative_error(
+            heat.heat_geodesic(small_v, small_f, np.array([p], np.int32), method="igl"),
+            ho.geodesic(small_v, small_f, np.array([p]), method="igl").distance,
+        )[0]
+        for p in picks
+    ]
+    result = {
+        "metric": METRIC,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "data": "synthetic",
+        "dtype": "f64",
+        "config": {
+            "workload": "planar grid 724x724 jittered 0.2h, 64 single-source queries a step as one "
+            "batch, factors resident",
+            "n_vertices": int(len(vertices)),
+        },
+        "value": lines["igl"]["value"],
+        "igl": lines["igl"],
+        "meshlab": lines["meshlab"],
+        # One-shot calls (a new mesh each: the settled CG, the iterated bar 1e-5).
+        "igl_one_shot_oracle_rel_err_max_64x64": max(errors),
+    }
     print(json.dumps(result))
     return 0
 
@@ -492,6 +561,8 @@ def main() -> int:
         return batch_workload(args)
     if args.workload == "sphere":
         return sphere_workload(args)
+    if args.workload == "igl":
+        return igl_workload(args)
     log(f"== generating torus {args.major}x{args.minor}")
     vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)
     n = vertices.shape[0]
