@@ -110,6 +110,8 @@ struct hg_mesh {
   // Workspaces.
   CgWorkspace cg;
   SolveWorkspace solve;
+  SolveWorkspace solve_side;  // the rim-pinned heat solve's, when it runs beside the Neumann one
+  bool on_side = false;       // the solves being enqueued use solve_side
   DeviceArray<double> fields, shift_partials, shift_values, null_partials;
   NullSpace null_space;
   DeviceArray<int32_t> d_sources, invalid;
@@ -535,7 +537,8 @@ int32_t diffuse_factored(hg_mesh& m, const HeatSystem& sys, double* rhs, double*
   return refined_solve(*m.plan, *sys.factor, mixed, rhs, x, k,
                        m.refine_tolerance[mixed ? 1 : 0][0], true,
                        mixed ? kMixedRefineRounds : kRefineRounds,
-                       m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr, m.solve, m.stream,
+                       m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
+                       m.on_side ? m.solve_side : m.solve, m.stream,
                        &m.timer, kPhaseHeatSweep, layout, layout, speculate);
 }
 
@@ -776,7 +779,12 @@ std::vector<uintptr_t> graph_signature(const hg_mesh& m, int64_t k) {
           p(m.heat_factor.pvalues.ptr), p(m.poisson_factor.w64.ptr),
           p(m.poisson_factor.pvalues.ptr), p(m.pinned_factor.w64.ptr),
           p(m.pinned_factor.pvalues.ptr), p(m.plan.get()), static_cast<uintptr_t>(m.method),
-          static_cast<uintptr_t>(m.n_rim), tol[0], tol[1], tol[2], tol[3]};
+          static_cast<uintptr_t>(m.n_rim), tol[0], tol[1], tol[2], tol[3],
+          p(m.solve_side.bp.ptr), p(m.solve_side.xa.ptr), p(m.solve_side.xb.ptr),
+          p(m.solve_side.r.ptr), p(m.solve_side.y.ptr), p(m.solve_side.d.ptr),
+          p(m.solve_side.fvec.ptr), p(m.solve_side.bmax.ptr), p(m.solve_side.partials.ptr),
+          p(m.solve_side.pending.ptr), static_cast<uintptr_t>(m.solve_side.capacity),
+          p(m.pinned_rhs.ptr)};
 }
 
 // Whether this call can replay a graph: FP64 factors held for every system the path solves,
@@ -805,6 +813,14 @@ struct GraphStream {
   hipEvent_t ready = nullptr;
 };
 thread_local GraphStream t_graph_stream;
+
+// The second stream of the igl path's side-by-side heat solves, per host thread, with its fork and
+// join events; both live until the process ends.
+struct SideStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+thread_local SideStream t_side_stream;
 
 bool run_graph(hg_mesh& m, const int64_t* offsets, int64_t k, double* d_out, int32_t precision) {
   GraphStream& gs = t_graph_stream;
@@ -889,8 +905,43 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   mode = factored ? HG_MODE_FACTORED : HG_MODE_ITERATE;
   const FieldLayout layout = factored ? FieldLayout::kRows : FieldLayout::kColumns;
   seed(m, offsets, k, u0, layout);
-  int32_t flags = diffuse(m, u0, heat, k, mode, precision, layout);
-  if (m.method == HG_METHOD_IGL && m.n_rim > 0) {
+  const bool igl = m.method == HG_METHOD_IGL && m.n_rim > 0;
+  int32_t flags = 0;
+  if (igl && layout == FieldLayout::kRows && m.speculating) {
+    // igl::heat_geodesics' boundary rule (heat.py:594-597) with the two heat solves side by side:
+    // the rim-pinned one, from the seeds zeroed on the rim, runs on a second stream with its own
+    // workspace and first-round verdict while the Neumann one runs here; neither waits on the host
+    // (speculation), and each one's latency-bound separator levels fill the other's idle time.
+    reserve(m.pinned_rhs, n * k);
+    launch_off_rim(u0, m.pinned_rhs.ptr, m.d_rim_rows.ptr, n, k, layout, m.stream);
+    SideStream& side = t_side_stream;
+    if (!side.stream) {
+      HG_HIP(hipStreamCreateWithFlags(&side.stream, hipStreamNonBlocking));
+      HG_HIP(hipEventCreateWithFlags(&side.fork, hipEventDisableTiming));
+      HG_HIP(hipEventCreateWithFlags(&side.join, hipEventDisableTiming));
+    }
+    HG_HIP(hipEventRecord(side.fork, m.stream));
+    HG_HIP(hipStreamWaitEvent(side.stream, side.fork, 0));
+    const hipStream_t main_stream = m.stream;
+    m.stream = side.stream;
+    m.on_side = true;
+    try {
+      flags |= diffuse(m, m.pinned_rhs.ptr, negdiv, k, HG_MODE_FACTORED, precision, layout, true);
+    } catch (...) {
+      m.stream = main_stream;
+      m.on_side = false;
+      throw;
+    }
+    m.stream = main_stream;
+    m.on_side = false;
+    flags |= diffuse(m, u0, heat, k, mode, precision, layout);
+    HG_HIP(hipEventRecord(side.join, side.stream));
+    HG_HIP(hipStreamWaitEvent(m.stream, side.join, 0));
+    launch_average(heat, negdiv, n * k, m.stream);
+  } else {
+    flags = diffuse(m, u0, heat, k, mode, precision, layout);
+  }
+  if (igl && !(layout == FieldLayout::kRows && m.speculating)) {
     // igl::heat_geodesics' boundary rule (heat.py:594-597): the Neumann heat averaged with the
     // heat pinned to zero on the rim, solved from the seeds zeroed there. The divergence's buffer
     // holds the pinned heat until the average has read it.
