@@ -85,3 +85,29 @@ def test_unknown_mesh_and_closed_server(cuda):
     server.close()
     with pytest.raises(RuntimeError):
         server.submit("a", [0])
+
+
+def test_warmed_server_factors_every_mesh_first_and_answers_the_same(cuda):
+    """``warm`` factors every mesh and sizes its buffers for batches of ``columns`` queries before
+    any request: afterwards every mesh is resident, the completion log holds only times (no
+    answers), and requests still match the same request solved alone."""
+    from ordito_amd.serve import Completion, GeodesicServer
+
+    meshes = {name: mesh(name) for name in NAMES}
+    server = GeodesicServer(meshes, workers=2, max_batch=8, warm_columns=4)
+    server.warm()
+    stats = server.stats()
+    assert stats["resident"] == len(NAMES) and stats["completed"] == 4 * len(NAMES)
+    assert all(isinstance(c, Completion) for c in server._completed)
+    rng = np.random.default_rng(11)
+    requests = []
+    for name in NAMES:
+        n = len(meshes[name][0])
+        sources = rng.choice(n, size=2, replace=False).astype(np.int32)
+        requests.append((name, sources, server.submit(name, sources)))
+    results = [(name, sources, future.result(timeout=120).cpu().numpy()) for name, sources, future in requests]
+    server.close()
+    for name, sources, got in results:
+        v, f = meshes[name]
+        worst, _ = ho.relative_error(got, _alone(cuda, v, f, sources))
+        assert worst <= 1e-10, (name, worst)
