@@ -226,62 +226,6 @@ def displaced_sphere(
     return (unit * radius[:, None]).astype(np.float32), faces
 
 
-def saddle_graded(resolution: int = 34, exponent: float = 3.4) -> tuple[np.ndarray, np.ndarray]:
-    """
-    A ``resolution x resolution`` grid on the saddle ``z = 0.35 (x^2 - 0.6 y^2)``, graded along x as
-    ``|s| ** exponent``: an open disk with triangles of aspect ratio up to ~4 900. The geometry of
-    ordito's ``saddle_graded`` test fixture (``tests/conftest.py:471-502``), the mesh on which a
-    settled conjugate-gradient iterate fails its backward-error check.
-    """
-    k = int(resolution)
-    step = np.linspace(-1.0, 1.0, k)
-    u, v = np.meshgrid(np.sign(step) * np.abs(step) ** exponent, step, indexing="ij")
-    vertices = np.column_stack((u.ravel(), v.ravel(), 0.35 * (u * u - 0.6 * v * v).ravel()))
-    i, j = np.meshgrid(np.arange(k - 1), np.arange(k - 1), indexing="ij")
-    corner = (i * k + j).ravel()
-    faces = np.vstack(
-        (
-            np.column_stack((corner, corner + k, corner + k + 1)),
-            np.column_stack((corner, corner + k + 1, corner + 1)),
-        )
-    )
-    return vertices.astype(np.float32), faces.astype(np.int32).reshape(-1)
-
-
-def bumpy_sphere(n_points: int = 500, squeeze: float = 0.97, seed: int = 5) -> tuple[np.ndarray, np.ndarray]:
-    """
-    Hull of ``n_points`` random directions pushed out to ``1 + 0.35 sin3t cos2p + 0.15 cos5t`` with
-    the longitude squeezed by ``p - squeeze sin p``: an irregular closed genus-0 mesh with needles and
-    mostly obtuse faces, in the spirit of ordito's ``sphere_irregular`` fixture
-    (``tests/conftest.py:595-639``). Vertex and face order are shuffled from the seed.
-    """
-    from scipy.spatial import ConvexHull
-
-    rng = np.random.default_rng(seed)
-    directions = rng.normal(size=(n_points, 3))
-    directions /= np.linalg.norm(directions, axis=1)[:, None]
-    faces = ConvexHull(directions).simplices
-    corners = directions[faces]
-    normals = np.cross(corners[:, 1] - corners[:, 0], corners[:, 2] - corners[:, 0])
-    inward = np.einsum("ij,ij->i", normals, corners.mean(axis=1)) < 0.0
-    faces[inward] = faces[inward][:, [0, 2, 1]]
-    theta = np.arccos(np.clip(directions[:, 2], -1.0, 1.0))
-    phi = np.arctan2(directions[:, 1], directions[:, 0])
-    phi = phi - squeeze * np.sin(phi)
-    radius = 1.0 + 0.35 * np.sin(3.0 * theta) * np.cos(2.0 * phi) + 0.15 * np.cos(5.0 * theta)
-    vertices = radius[:, None] * np.column_stack(
-        (np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta))
-    )
-    order = rng.permutation(vertices.shape[0])
-    rank = np.empty_like(order)
-    rank[order] = np.arange(order.size)
-    faces = rank[faces][rng.permutation(faces.shape[0])]
-    shift = rng.integers(0, 3, faces.shape[0])
-    faces = np.take_along_axis(faces, (np.arange(3)[None, :] + shift[:, None]) % 3, axis=1)
-    vertices = vertices[order]
-    return vertices.astype(np.float32), faces.astype(np.int32).reshape(-1)
-
-
 def mean_face_edge_length(vertices: np.ndarray, faces: np.ndarray) -> float:
     """Mean length over every face's three edges (an interior edge twice), in float64."""
     tri = np.asarray(faces).reshape(-1, 3)

@@ -31,8 +31,8 @@ MESHES = {
     "icosphere4": lambda: __import__("ordito_amd.creation", fromlist=["x"]).jittered_icosphere(4, 0.01, 0),
     "torus": lambda: __import__("ordito_amd.creation", fromlist=["x"]).torus(48, 24, 1.0, 0.3, 0.1, 3),
     "grid": lambda: __import__("ordito_amd.creation", fromlist=["x"]).planar_grid(30, 24, 1.0, 0.2, 4),
-    "bumpy": lambda: __import__("ordito_amd.creation", fromlist=["x"]).bumpy_sphere(),
-    "saddle_graded": lambda: __import__("ordito_amd.creation", fromlist=["x"]).saddle_graded(),
+    "bumpy": lambda: __import__("tests.meshes", fromlist=["x"]).bumpy_hull(),
+    "saddle_graded": lambda: __import__("tests.meshes", fromlist=["x"]).graded_saddle(),
     "two_bodies": lambda: _two_bodies(),
 }
 

@@ -19,12 +19,13 @@ sys.path.insert(0, str(HERE.parents[1]))
 
 from oracle import heat_oracle as ho  # noqa: E402
 from ordito_amd import creation  # noqa: E402
+from tests import meshes  # noqa: E402
 
 CASES = {
     "icosphere3_jitter": (lambda: creation.jittered_icosphere(3, 0.01, 1), [[0], [100, 200]]),
     "torus48x24": (lambda: creation.torus(48, 24, 1.0, 0.3, 0.1, 3), [[5], [0, 600]]),
     "grid30x24": (lambda: creation.planar_grid(30, 24, 1.0, 0.2, 4), [[0], [371]]),
-    "bumpy500": (lambda: creation.bumpy_sphere(), [[0], [3, 77, 250]]),
+    "bumpy500": (lambda: meshes.bumpy_hull(), [[0], [3, 77, 250]]),
 }
 
 

@@ -277,6 +277,126 @@ def test_settled_cg_reaches_the_direct_solve_within_the_settle_rule():
     assert ho.backward_error_exceeds(A, x * (1 + 1e-3), b, 1e-6)
 
 
+# --- the lumped masses' corner scatter ----------------------------------------------------------
+#
+# Sum(m) = Sum(area) and (M - tL) 1 = M 1 hold whichever vertex an area lands on, so the scatter is
+# pinned by brute force instead: a Python loop over every face's three corners, each vertex's list
+# of contributions summed with math.fsum (kernels/heat.py:125-131, 423-425; heat.py:2169).
+
+
+def _corner_loop(n_vertices, faces, value_of):
+    """Per vertex, the exactly rounded sum of value_of(face, corner) over the corners it holds."""
+    lists = [[] for _ in range(n_vertices)]
+    for f, corners in enumerate(np.asarray(faces).reshape(-1, 3)):
+        for c, v in enumerate(corners):
+            lists[int(v)].append(value_of(f, c))
+    return np.array([math.fsum(values) for values in lists])
+
+
+def _float64_areas(vertices, faces):
+    """Face areas from an independent float64 cross product of the float32 corners."""
+    p = np.asarray(vertices, dtype=np.float32).astype(np.float64)
+    tri = np.asarray(faces).reshape(-1, 3)
+    cross = np.cross(p[tri[:, 1]] - p[tri[:, 0]], p[tri[:, 2]] - p[tri[:, 0]])
+    return 0.5 * np.linalg.norm(cross, axis=1)
+
+
+def _barycentric_mass_error(vertices, faces, mass_fn):
+    """Largest relative gap between mass_fn's barycentric mass and the two brute-force sums: one
+    over the oracle's float32 areas (to 1e-14), one over float64 areas (to float32 rounding)."""
+    n = vertices.shape[0]
+    areas32 = ho.face_terms(vertices, faces).areas
+    got = mass_fn(n, faces, areas32)
+    exact32 = _corner_loop(n, faces, lambda f, c: float(areas32[f])) / 3.0
+    exact64 = _corner_loop(n, faces, lambda f, c, a=_float64_areas(vertices, faces): a[f]) / 3.0
+    return max(
+        np.max(np.abs(got - exact32) / exact32) / 1e-14,
+        np.max(np.abs(got - exact64) / exact64) / 1e-5,
+    )
+
+
+@pytest.mark.parametrize("name", ["saddle_graded", "bumpy", "icosphere3", "grid"])
+def test_barycentric_mass_equals_a_per_corner_loop(name):
+    v, f = mesh(name)
+    assert _barycentric_mass_error(v, f, ho.lumped_mass) <= 1.0
+
+
+def test_barycentric_mass_pin_catches_misassigned_areas():
+    """Two slips that keep Sum(m) = Sum(area): areas laid out face-major instead of corner-major
+    (np.tile for np.repeat), and every face given its neighbour's area. Each fails the pin."""
+
+    def tiled(n, faces, areas):
+        tri = np.asarray(faces, dtype=np.int64).reshape(-1)
+        return np.bincount(tri, weights=np.tile(areas.astype(np.float64), 3), minlength=n) / 3.0
+
+    def rolled(n, faces, areas):
+        return ho.lumped_mass(n, faces, np.roll(areas, 1))
+
+    for name in ("saddle_graded", "bumpy", "icosphere3"):
+        v, f = mesh(name)
+        assert _barycentric_mass_error(v, f, tiled) > 1.0, name
+        assert _barycentric_mass_error(v, f, rolled) > 1.0, name
+
+
+def _voronoi_mass_error(vertices, faces, mass_fn):
+    n = vertices.shape[0]
+    shares = ho.voronoi_corner_areas(vertices, faces)
+    got = mass_fn(n, faces, shares)
+    exact = _corner_loop(n, faces, lambda f, c: float(shares[f, c]))
+    return np.max(np.abs(got - exact) / exact) / 1e-14
+
+
+@pytest.mark.parametrize("name", ["saddle_graded", "bumpy", "icosphere3", "grid"])
+def test_voronoi_mass_equals_a_per_corner_loop(name):
+    v, f = mesh(name)
+    assert _voronoi_mass_error(v, f, ho.voronoi_mass) <= 1.0
+
+
+def test_voronoi_mass_pin_catches_misassigned_shares():
+    """Shares read face-major ((F, 3) transposed) or with the corners rotated keep Sum(m); each
+    fails the pin on an irregular mesh."""
+
+    def transposed(n, faces, shares):
+        return ho.voronoi_mass(n, faces, np.ascontiguousarray(shares.T).reshape(-1, 3))
+
+    def rotated(n, faces, shares):
+        return ho.voronoi_mass(n, faces, np.roll(shares, 1, axis=1))
+
+    for name in ("saddle_graded", "bumpy"):
+        v, f = mesh(name)
+        assert _voronoi_mass_error(v, f, transposed) > 1.0, name
+        assert _voronoi_mass_error(v, f, rotated) > 1.0, name
+
+
+# --- the diffusion-time floor --------------------------------------------------------------------
+
+
+def _floor_closed_form(extent):
+    """(|bounding-box diagonal| / 400)^2 for a box of the given edge lengths (heat.py:5228-5231)."""
+    return (math.sqrt(sum(e * e for e in extent)) / 400.0) ** 2
+
+
+def test_diffusion_floor_matches_closed_form_bounding_boxes():
+    """An unjittered 1024 x 512 torus (R = 1, r = 0.3) has grid angles at every quarter turn, so its
+    box is exactly 2.6 x 2.6 x 0.6; a 40 x 30 grid of step 0.25 spans 9.75 x 7.25 x 0; a box's
+    eight corners span it. A floor from the wrong diagonal (one axis dropped, a half box, the
+    decay applied once) misses each by far more than float32 rounding."""
+    from ordito_amd import creation
+
+    v, f = creation.torus(1024, 512, 1.0, 0.3)
+    assert math.isclose(ho.diffusion_floor(v), _floor_closed_form((2.6, 2.6, 0.6)), rel_tol=1e-6)
+    xs, ys = np.meshgrid(np.arange(40) * 0.25, np.arange(30) * 0.25, indexing="ij")
+    grid = np.column_stack([xs.ravel(), ys.ravel(), np.zeros(xs.size)]).astype(np.float32)
+    assert math.isclose(ho.diffusion_floor(grid), _floor_closed_form((9.75, 7.25, 0.0)), rel_tol=1e-7)
+    corners = np.array([[x, y, z] for x in (-1.5, 2.0) for y in (0.5, 3.0) for z in (-4.0, 4.0)],
+                       dtype=np.float32)
+    assert math.isclose(ho.diffusion_floor(corners), _floor_closed_form((3.5, 2.5, 8.0)), rel_tol=1e-7)
+    # The floor decides the time on the torus (h^2 ~ 3.2e-5 < 8.7e-5) and the window follows it.
+    t, window = ho.resolve_time(v, f, None)
+    assert t == ho.diffusion_floor(v)
+    assert math.isclose(t, 8.6755e-5, rel_tol=1e-3)
+
+
 def test_time_rule_raises_the_floor_on_fine_meshes():
     """(diag/400)^2 replaces h^2 when longer, and the settle window grows as sqrt(t/h^2)
     (heat.py:3714-3736, 5234-5243)."""
