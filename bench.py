@@ -257,29 +257,6 @@ def igl_workload(args) -> int:
     return 0
 
 
-def _generated_meshes(count: int, low: float, high: float, rng) -> dict:
-    """Seeded meshes with log-uniform vertex counts in [low, high]: jittered icospheres at the
-    nearest level, jittered tori and jittered planar grids (open, with a rim)."""
-    from ordito_amd import creation
-
-    meshes = {}
-    for i in range(count):
-        target = float(np.exp(rng.uniform(np.log(low), np.log(high))))
-        kind = int(rng.integers(3))
-        seed = int(rng.integers(1 << 30))
-        if kind == 0:
-            level = int(np.clip(round(np.log(max(target - 2, 10) / 10) / np.log(4)), 1, 9))
-            meshes[i] = creation.jittered_icosphere(level, 0.01, seed)
-        elif kind == 1:
-            minor = max(8, int(round(np.sqrt(target / 2))))
-            major = max(8, int(round(target / minor)))
-            meshes[i] = creation.torus(major, minor, 1.0, 0.3, jitter=0.1, seed=seed)
-        else:
-            side = max(8, int(round(np.sqrt(target))))
-            meshes[i] = creation.planar_grid(side, side, 1.0, jitter=0.2, seed=seed)
-    return meshes
-
-
 def batch_workload(args) -> int:
     """
     The batched configuration: ``--meshes`` (default 256) meshes with log-uniform vertex counts in
@@ -291,11 +268,12 @@ def batch_workload(args) -> int:
     """
     import torch
 
+    from ordito_amd import creation
     from ordito_amd.batch import MeshBatch
 
     rng = np.random.default_rng(args.seed)
     log(f"== generating {args.meshes} meshes")
-    meshes = _generated_meshes(args.meshes, args.min_vertices, args.max_vertices, rng)
+    meshes = creation.mesh_collection(args.meshes, args.min_vertices, args.max_vertices, rng)
     sizes = [len(meshes[i][0]) for i in range(args.meshes)]
     queries = []
     for i in range(args.meshes):
@@ -374,11 +352,12 @@ def serve_workload(args) -> int:
     apart. Prints one JSON line: throughput, p50/p99 latency, and the error of sampled answers
     against the FP64 oracle.
     """
+    from ordito_amd import creation
     from ordito_amd.serve import GeodesicServer
 
     rng = np.random.default_rng(args.seed)
     log(f"== generating {args.meshes} meshes")
-    meshes = _generated_meshes(args.meshes, args.min_vertices, args.max_vertices, rng)
+    meshes = creation.mesh_collection(args.meshes, args.min_vertices, args.max_vertices, rng)
     sizes = {i: len(v) for i, (v, _) in meshes.items()}
     server = GeodesicServer(meshes, budget_bytes=args.budget_gb * 1e9, workers=args.workers)
     log(f"== warming {args.meshes} meshes ({sum(sizes.values())} vertices)")

@@ -278,8 +278,16 @@ int hg_timing_read(hg_mesh* mesh, double* out_ms8, int64_t* out_counts8, int32_t
 /* Device bytes the held factorizations take (HeatSolver.nbytes, heat.py:3838-3847). */
 int hg_nbytes(hg_mesh* mesh, int64_t* out_bytes);
 
-/* Drop every held factorization; the operators stay (HeatSolver.release, heat.py:3849-3861). */
+/* Drop every held factorization and the per-call solve workspaces; the operators stay
+ * (HeatSolver.release, heat.py:3849-3861). */
 int hg_release(hg_mesh* mesh);
+
+/*
+ * Every device byte the mesh holds: its copy of the mesh, the pattern, the operators, the plan and
+ * factors, and the per-call workspaces (which grow to the widest batch served). No counterpart in
+ * the reference; a serving layer sums it over its resident meshes to admit new ones by memory.
+ */
+int hg_device_bytes(hg_mesh* mesh, int64_t* out_bytes);
 
 /* Analysis statistics of the mesh's plan (analysing it if needed): supernodes, levels, stored
  * solve-block entries, factorization flops, arena entries, largest front, front-vector entries

@@ -61,6 +61,7 @@ EXPORTED = (
     "hg_packed_times",
     "hg_nbytes",
     "hg_release",
+    "hg_device_bytes",
     "hg_plan_stats",
     "hg_plan_create",
     "hg_plan_sizes",
@@ -151,6 +152,7 @@ _SIGNATURES = {
     "hg_packed_times": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "hg_nbytes": ([_P, _PI64], ctypes.c_int),
     "hg_release": ([_P], ctypes.c_int),
+    "hg_device_bytes": ([_P, _PI64], ctypes.c_int),
     "hg_plan_stats": ([_P, _PI64], ctypes.c_int),
     "hg_plan_create": ([_P, _P, _I64, _P, _I32, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "hg_plan_sizes": ([_P, _PI64], ctypes.c_int),

@@ -226,6 +226,30 @@ def displaced_sphere(
     return (unit * radius[:, None]).astype(np.float32), faces
 
 
+def mesh_collection(count: int, low: float, high: float, rng) -> dict:
+    """
+    ``count`` seeded meshes keyed 0..count-1 with vertex counts log-uniform in ``[low, high]``, each
+    a jittered icosphere at the nearest subdivision level, a jittered torus or a jittered planar
+    grid (open, with a rim), chosen by ``rng`` (a ``numpy.random.Generator``).
+    """
+    meshes = {}
+    for i in range(count):
+        target = float(np.exp(rng.uniform(np.log(low), np.log(high))))
+        kind = int(rng.integers(3))
+        seed = int(rng.integers(1 << 30))
+        if kind == 0:
+            level = int(np.clip(round(np.log(max(target - 2, 10) / 10) / np.log(4)), 1, 9))
+            meshes[i] = jittered_icosphere(level, 0.01, seed)
+        elif kind == 1:
+            minor = max(8, int(round(np.sqrt(target / 2))))
+            major = max(8, int(round(target / minor)))
+            meshes[i] = torus(major, minor, 1.0, 0.3, jitter=0.1, seed=seed)
+        else:
+            side = max(8, int(round(np.sqrt(target))))
+            meshes[i] = planar_grid(side, side, 1.0, jitter=0.2, seed=seed)
+    return meshes
+
+
 def mean_face_edge_length(vertices: np.ndarray, faces: np.ndarray) -> float:
     """Mean length over every face's three edges (an interior edge twice), in float64."""
     tri = np.asarray(faces).reshape(-1, 3)

@@ -181,6 +181,52 @@ void require_single_or_factored(const hg_mesh& m, int32_t mode) {
 // Drops every captured graph: the state they were captured against is changing.
 void invalidate_graphs(hg_mesh& m) { m.graphs.clear(); }
 
+// Every device byte the mesh holds: its copy of the mesh and pattern, the operators, the plan and
+// factors, and the per-call workspaces.
+int64_t mesh_device_bytes(const hg_mesh& m) {
+  int64_t total = 0;
+  for (const DeviceArray<int32_t>* a :
+       {&m.row_segment, &m.faces, &m.offsets, &m.columns, &m.corner_ptr, &m.corner_slots,
+        &m.vertex_ptr, &m.vertex_corners, &m.d_rim, &m.d_rim_rows, &m.d_labels, &m.comp_ptr,
+        &m.comp_members, &m.poisson_pinned_new, &m.poisson_dropped_old, &m.heat_pinned_new,
+        &m.heat_dropped_old, &m.faces_rows, &m.row_corner_ptr, &m.d_sources, &m.invalid,
+        &m.q_local})
+    total += a->bytes();
+  for (const DeviceArray<float>* a : {&m.xyz, &m.cot, &m.normals, &m.areas, &m.heat32, &m.voronoi,
+                                      &m.pinned32})
+    total += a->bytes();
+  for (const DeviceArray<double>* a :
+       {&m.mass, &m.lap, &m.heat, &m.poisson, &m.heat_inv, &m.poisson_inv, &m.scalars,
+        &m.edge_partials, &m.face_geom, &m.pinned, &m.pinned_inv, &m.pinned_rhs, &m.fields,
+        &m.shift_partials, &m.shift_values, &m.null_partials, &m.packed_phi})
+    total += a->bytes();
+  for (const DeviceArray<int64_t>* a : {&m.d_owner, &m.d_offsets, &m.q_src_off, &m.q_dst_off,
+                                        &m.q_len})
+    total += a->bytes();
+  total += m.row_corners.bytes() + m.d_slots.bytes();
+  total += m.heat_factor.bytes() + m.poisson_factor.bytes() + m.pinned_factor.bytes();
+  if (m.plan) total += m.plan->bytes();
+  total += m.cg.bytes() + m.solve.bytes() + m.solve_side.bytes() + m.null_space.bytes();
+  return total;
+}
+
+// Frees what solving built beyond the operators: the per-call workspaces (they grow to the widest
+// batch a mesh has served and are rebuilt by the next solve).
+void release_workspaces(hg_mesh& m) {
+  invalidate_graphs(m);
+  m.cg.release();
+  m.solve.release();
+  m.solve_side.release();
+  m.fields.reset();
+  m.field_capacity = 0;
+  m.packed_phi.reset();
+  m.packed_capacity = 0;
+  m.pinned_rhs.reset();
+  m.shift_partials.reset();
+  m.shift_values.reset();
+  m.null_partials.reset();
+}
+
 // ---- operators --------------------------------------------------------------------------------
 
 // The mean unique-edge length of segment s: each edge once, over the pattern's strict upper
@@ -812,6 +858,35 @@ struct GraphStream {
   hipStream_t stream = nullptr;
   hipEvent_t ready = nullptr;
 };
+
+// Issues the mesh's work on another stream (optionally through the side workspace) for a scope,
+// restoring the mesh's own stream and workspace choice however the scope ends.
+struct StreamSwap {
+  hg_mesh& m;
+  hipStream_t stream;
+  bool on_side;
+  StreamSwap(hg_mesh& mesh, hipStream_t other, bool side = false)
+      : m(mesh), stream(mesh.stream), on_side(mesh.on_side) {
+    m.stream = other;
+    m.on_side = side;
+  }
+  ~StreamSwap() {
+    m.stream = stream;
+    m.on_side = on_side;
+  }
+  StreamSwap(const StreamSwap&) = delete;
+  StreamSwap& operator=(const StreamSwap&) = delete;
+};
+
+// Leaves speculation disarmed when a call ends, normally or by an exception: a later call must
+// never stop after a first refinement round whose verdict nobody reads.
+struct SpeculationDisarm {
+  hg_mesh& m;
+  explicit SpeculationDisarm(hg_mesh& mesh) : m(mesh) { m.speculating = false; }
+  ~SpeculationDisarm() { m.speculating = false; }
+  SpeculationDisarm(const SpeculationDisarm&) = delete;
+  SpeculationDisarm& operator=(const SpeculationDisarm&) = delete;
+};
 thread_local GraphStream t_graph_stream;
 
 // The second stream of the igl path's side-by-side heat solves, per host thread, with its fork and
@@ -837,19 +912,22 @@ bool run_graph(hg_mesh& m, const int64_t* offsets, int64_t k, double* d_out, int
     auto graph = std::make_unique<hg_mesh::Graph>();
     arm_speculation(m, k, true);
     m.stats = hg_solve_stats{};
-    const hipStream_t caller = m.stream;
-    m.stream = gs.stream;
     hipGraph_t captured = nullptr;
-    bool ok = hipStreamBeginCapture(gs.stream, hipStreamCaptureModeRelaxed) == hipSuccess;
-    if (ok) {
-      try {
-        solve_columns(m, offsets, k, d_out, HG_MODE_FACTORED, precision, true);
-      } catch (const Error&) {
-        ok = false;
+    bool ok = false;
+    {
+      // Whatever the capture throws, the stream leaves capture mode and the mesh gets its own
+      // stream back (and its main workspace) before the exception goes on.
+      StreamSwap swap(m, gs.stream);
+      ok = hipStreamBeginCapture(gs.stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+      if (ok) {
+        try {
+          solve_columns(m, offsets, k, d_out, HG_MODE_FACTORED, precision, true);
+        } catch (...) {
+          ok = false;
+        }
+        ok = hipStreamEndCapture(gs.stream, &captured) == hipSuccess && ok;
       }
-      ok = hipStreamEndCapture(gs.stream, &captured) == hipSuccess && ok;
     }
-    m.stream = caller;
     if (ok) ok = hipGraphInstantiate(&graph->exec, captured, nullptr, nullptr, 0) == hipSuccess;
     if (captured) (void)hipGraphDestroy(captured);
     m.speculating = false;
@@ -922,18 +1000,10 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
     }
     HG_HIP(hipEventRecord(side.fork, m.stream));
     HG_HIP(hipStreamWaitEvent(side.stream, side.fork, 0));
-    const hipStream_t main_stream = m.stream;
-    m.stream = side.stream;
-    m.on_side = true;
-    try {
+    {
+      StreamSwap swap(m, side.stream, true);
       flags |= diffuse(m, m.pinned_rhs.ptr, negdiv, k, HG_MODE_FACTORED, precision, layout, true);
-    } catch (...) {
-      m.stream = main_stream;
-      m.on_side = false;
-      throw;
     }
-    m.stream = main_stream;
-    m.on_side = false;
     flags |= diffuse(m, u0, heat, k, mode, precision, layout);
     HG_HIP(hipEventRecord(side.join, side.stream));
     HG_HIP(hipStreamWaitEvent(m.stream, side.join, 0));
@@ -1237,6 +1307,7 @@ int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n
     require_mesh(mesh);
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
     require_single_or_factored(*mesh, mode);
+    SpeculationDisarm disarm(*mesh);
     mesh->stats = hg_solve_stats{};
     // By columns (the caller's layout) the solve only reads its right-hand side.
     int32_t flags = diffuse(*mesh, const_cast<double*>(d_rhs), d_solution, n_columns, mode,
@@ -1254,6 +1325,7 @@ int hg_diffuse_pinned(hg_mesh* mesh, const double* d_rhs, double* d_solution, in
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
     require_single_or_factored(*mesh, mode);
     hg_mesh& m = *mesh;
+    SpeculationDisarm disarm(m);
     m.stats = hg_solve_stats{};
     ensure_operators(m);
     int32_t flags = 0;
@@ -1312,6 +1384,7 @@ int hg_solve_poisson(hg_mesh* mesh, const double* d_rhs, double* d_solution, int
     require_mesh(mesh);
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
     require_single_or_factored(*mesh, mode);
+    SpeculationDisarm disarm(*mesh);
     // By columns (the caller's layout) the solve only reads its right-hand side.
     solve_poisson(*mesh, const_cast<double*>(d_rhs), d_solution, n_columns, mode, precision);
     HG_HIP(hipStreamSynchronize(mesh->stream));
@@ -1364,6 +1437,7 @@ int hg_geodesic(hg_mesh* mesh, const int32_t* d_sources, const int64_t* source_o
       HG_REQUIRE(source_offsets[q] <= source_offsets[q + 1], HG_ERR_VALUE,
                  "source_offsets must not decrease");
     hg_mesh& m = *mesh;
+    SpeculationDisarm disarm(m);
     m.stats = hg_solve_stats{};
     const int64_t n = m.n, k = n_queries;
     if (k == 0) return;
@@ -1400,6 +1474,7 @@ int hg_geodesic_packed(hg_mesh* mesh, const int32_t* d_sources, const int64_t* s
     HG_REQUIRE(source_offsets != nullptr && query_mesh != nullptr, HG_ERR_VALUE,
                "source_offsets or query_mesh is null");
     hg_mesh& m = *mesh;
+    SpeculationDisarm disarm(m);
     m.stats = hg_solve_stats{};
     if (n_queries == 0) return;
     const int64_t n_seg = m.segments();
@@ -1539,11 +1614,19 @@ int hg_release(hg_mesh* mesh) {
   return guarded([&] {
     require_mesh(mesh);
     HG_HIP(hipStreamSynchronize(mesh->stream));
-    invalidate_graphs(*mesh);
+    release_workspaces(*mesh);
     mesh->heat_factor.reset();
     mesh->poisson_factor.reset();
     mesh->pinned_factor.reset();
     mesh->plan.reset();
+  });
+}
+
+int hg_device_bytes(hg_mesh* mesh, int64_t* out_bytes) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(out_bytes != nullptr, HG_ERR_VALUE, "out_bytes is null");
+    *out_bytes = mesh_device_bytes(*mesh);
   });
 }
 

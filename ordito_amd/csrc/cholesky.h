@@ -73,6 +73,15 @@ struct SolveWorkspace {
   DeviceArray<int32_t> pending;
   int64_t capacity = 0, fcap = 0, pcap = 0;
   void reserve(const DevicePlan& plan, int64_t kk);
+  int64_t bytes() const {
+    return bp.bytes() + xa.bytes() + xb.bytes() + r.bytes() + y.bytes() + d.bytes() +
+           fvec.bytes() + bmax.bytes() + partials.bytes() + pending.bytes();
+  }
+  void release() {
+    for (DeviceArray<double>* a : {&bp, &xa, &xb, &r, &y, &d, &fvec, &bmax, &partials}) a->reset();
+    pending.reset();
+    capacity = fcap = pcap = 0;
+  }
 };
 
 // Numeric factorization of `values` (the operator over the mesh pattern) with the rows flagged in
@@ -101,6 +110,7 @@ int32_t refined_solve(const DevicePlan& plan, const Factor& factor, bool use_fp3
 struct NullSpace {
   int32_t n_comp = 0, n_chunks = 0;
   DeviceArray<int32_t> chunk_comp, chunk_first, comp_chunk_ptr;
+  int64_t bytes() const { return chunk_comp.bytes() + chunk_first.bytes() + comp_chunk_ptr.bytes(); }
   void build(const std::vector<int32_t>& comp_ptr_host, hipStream_t stream);
 };
 

@@ -195,6 +195,15 @@ struct CgWorkspace {
   int64_t capacity_entries = 0;
   int32_t capacity_blocks = 0;
   int32_t capacity_columns = 0;
+  int64_t bytes() const { return vectors.bytes() + partials.bytes() + counters.bytes(); }
+  void release() {
+    vectors.reset();
+    partials.reset();
+    counters.reset();
+    capacity_entries = 0;
+    capacity_blocks = 0;
+    capacity_columns = 0;
+  }
 };
 
 // Runs the solve; returns the rounds that stepped.

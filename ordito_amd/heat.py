@@ -333,6 +333,16 @@ class HeatSolver:
         _lib.call("hg_nbytes", self._handle.ptr, ctypes.byref(out))
         return int(out.value)
 
+    @property
+    def device_bytes(self) -> int:
+        """Every device byte the solver's mesh state holds: operators, plan, factors and the
+        per-call workspaces (this build's addition, for admission by memory)."""
+        if self._handle is None:
+            return 0
+        out = ctypes.c_int64()
+        _lib.call("hg_device_bytes", self._handle.ptr, ctypes.byref(out))
+        return int(out.value)
+
     def _factored(self) -> int:
         return self._factored_on(self._native())
 

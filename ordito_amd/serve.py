@@ -10,9 +10,13 @@ needs around that, with no counterpart in the reference:
   solves them as one batch (one multi-column refined solve per system). Requests that arrive while
   a batch runs wait for the next batch of their mesh; batches of different meshes run at once on
   the workers' own HIP streams, so their work overlaps on the device.
-- **Admission by memory.** A mesh's factors are built on its first batch and stay resident. Before
-  building, the scheduler evicts the least recently used idle meshes until the new mesh's estimated
-  footprint (its factors and the transient front arena) fits the HBM budget.
+- **Admission by memory.** A mesh's operators and factors are built on its first batch and stay
+  resident. Every resident mesh counts every device byte it holds (operators, plan, factors and
+  its solve workspaces, measured by its worker after each batch); before building a new one, the
+  scheduler evicts the least recently used idle meshes -- freeing all of their device state -- until
+  the new mesh's estimated footprint (factors, the transient front arena and the workspaces of a
+  ``max_batch``-wide batch) fits the HBM budget. A mesh whose factorization the library refuses
+  under its own budget is served by iteration.
 - **Latency.** Every request records its submit and completion times; ``stats()`` reports
   throughput and p50/p99.
 
@@ -43,6 +47,7 @@ class _Request:
     sources: Any
     future: concurrent.futures.Future
     submitted: float
+    consumer: Any = None  # the submitting thread's current stream, which will read the result
     started: float = 0.0
     done: float = 0.0
 
@@ -65,7 +70,7 @@ class _Resident:
     trimesh: Trimesh | None = None  # made by the first batch's worker, on its stream
     solver: HeatSolver | None = None
     factored: bool = False
-    nbytes: int = 0
+    nbytes: int = 0  # every device byte the mesh's state holds, measured after each batch
     estimate: int = 0
     last_used: float = 0.0
     busy: bool = False
@@ -145,7 +150,8 @@ class GeodesicServer:
         if mesh_id not in self._meshes:
             raise ValueError(f"unknown mesh {mesh_id!r}")
         request = _Request(mesh_id, np.asarray(sources, dtype=np.int32).reshape(-1),
-                           concurrent.futures.Future(), time.perf_counter())
+                           concurrent.futures.Future(), time.perf_counter(),
+                           consumer=self._torch.cuda.current_stream())
         with self._lock:
             if self._closing:
                 raise RuntimeError("the server is closed")
@@ -182,7 +188,7 @@ class GeodesicServer:
             "batches": self._batches,
             "evictions": self._evictions,
             "resident": sum(1 for e in self._resident.values() if e.factored),
-            "resident_bytes": sum(e.nbytes for e in self._resident.values() if e.factored),
+            "resident_bytes": self._held_bytes(),
         }
 
     # -- scheduling ------------------------------------------------------------------------------
@@ -193,24 +199,31 @@ class GeodesicServer:
         n_faces = int(np.prod(faces.shape)) // 3
         entry = _Resident(n_vertices, n_faces,
                           estimate=footprint_estimate(n_vertices, n_faces, self.precision,
-                                                      self.warm_columns))
+                                                      self.max_batch))
         self._resident[mesh_id] = entry
         return entry
 
-    def _held_bytes(self) -> int:
-        return sum(e.nbytes for e in self._resident.values() if e.factored)
+    def _held_bytes(self, exclude=None) -> int:
+        """Device bytes of every mesh with state on the device but ``exclude`` (a busy mesh's as of
+        its last batch; a mesh being built for the first time counts its estimate)."""
+        return sum(max(e.nbytes, e.estimate if e.busy and not e.factored else 0)
+                   for k, e in self._resident.items()
+                   if k != exclude and (e.solver is not None or e.busy))
 
     def _admit(self, mesh_id, entry: _Resident) -> None:
-        """Evict least recently used idle factored meshes until ``entry`` fits the budget."""
-        need = entry.estimate
-        while self._held_bytes() + need > self.budget:
+        """Evict least recently used idle meshes, freeing all their device state, until ``entry``
+        fits the budget."""
+        need = max(entry.estimate, entry.nbytes)
+        while self._held_bytes(exclude=mesh_id) + need > self.budget:
             idle = [(e.last_used, k) for k, e in self._resident.items()
-                    if e.factored and not e.busy and k != mesh_id]
+                    if e.solver is not None and not e.busy and k != mesh_id]
             if not idle:
                 break
             _, victim = min(idle)
             e = self._resident[victim]
             e.solver.release()
+            e.solver = None
+            e.trimesh = None  # its hg_mesh (operators, pattern, workspaces) goes with the solver
             e.factored = False
             e.nbytes = 0
             self._evictions += 1
@@ -263,9 +276,8 @@ class GeodesicServer:
                     # several threads contend: measured p50 4.5 ms eager against 10.6 ms replayed.
                     entry.solver.set_graphs(False)
                 if not entry.factored:
-                    entry.solver.factor()
-                    entry.factored = True
-                    entry.nbytes = entry.solver.nbytes
+                    # A refusal under the library's own budget leaves the mesh iterating.
+                    entry.factored = entry.solver.factor()
                 # The batch's sources in one flat upload, its sets delimited by offsets.
                 offsets = [0]
                 for r in batch:
@@ -273,6 +285,13 @@ class GeodesicServer:
                 flat = torch.from_numpy(np.concatenate([r.sources for r in batch])).to("cuda")
                 fields = entry.solver.geodesic_flat(flat, offsets)
                 self._streams[worker].synchronize()
+                entry.nbytes = entry.solver.device_bytes
+            # The results' memory comes from this worker stream's pool: each consumer's stream is
+            # recorded on it, so a result freed while its reads are still queued there is not
+            # reused by this worker's next batch before they finish.
+            for consumer in {r.consumer for r in batch if r.consumer is not None}:
+                if consumer != self._streams[worker]:
+                    fields.record_stream(consumer)
             done = time.perf_counter()
             for q, r in enumerate(batch):
                 r.done = done

@@ -76,6 +76,29 @@ def test_admission_evicts_the_least_recently_used_mesh(cuda):
     assert np.array_equal(first, again)
 
 
+def test_admission_counts_every_device_byte_a_mesh_holds(cuda):
+    """Resident bytes are each mesh's whole device state (operators, plan, factors and the solve
+    workspaces of the widest batch served), more than its factors alone; an eviction frees all of
+    it; a factorization refused under the library's budget leaves the mesh served by iteration."""
+    from ordito_amd.serve import GeodesicServer
+
+    meshes = {name: mesh(name) for name in NAMES[:2]}
+    server = GeodesicServer(meshes, workers=1, max_batch=8)
+    for name in meshes:
+        server.submit(name, np.array([0, 5], dtype=np.int32)).result(timeout=120)
+    entries = [server._resident[name] for name in meshes]
+    held = server.stats()["resident_bytes"]
+    assert held == sum(e.solver.device_bytes for e in entries)
+    assert held > sum(e.solver.nbytes for e in entries) > 0
+    server._admit("torus", server._resident["torus"])  # fits: nothing goes
+    assert server.stats()["evictions"] == 0
+    server.budget = 1  # nothing fits: every other idle mesh goes, with all its device state
+    server._admit("torus", server._resident["torus"])
+    assert server._resident["icosphere3"].solver is None and server.stats()["evictions"] == 1
+    assert server._held_bytes() == server._resident["torus"].nbytes
+    server.close()
+
+
 def test_unknown_mesh_and_closed_server(cuda):
     from ordito_amd.serve import GeodesicServer
 
