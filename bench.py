@@ -60,8 +60,21 @@ def parse() -> argparse.Namespace:
     parser.add_argument("--queries", type=int, default=64)
     parser.add_argument("--plain-queries", type=int, default=8)
     parser.add_argument("--cpu-sample", type=int, default=4)
+    parser.add_argument(
+        "--cpu-sample-all", type=int, default=32, help="queries of the all-cores CPU baseline"
+    )
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-mixed", action="store_true", help="skip the served_mixed line")
+    parser.add_argument(
+        "--no-extra",
+        action="store_true",
+        help="skip the sub-objects of the other workloads (one-shot C1/C2, displaced sphere, "
+        "256 packed meshes, 64 served meshes)",
+    )
+    parser.add_argument("--extra-steps", type=int, default=10, help="timed steps of the extras")
+    parser.add_argument(
+        "--serve-probes", type=str, default="6000,12000", help="capacity probe rates (serve extra)"
+    )
     parser.add_argument("--major", type=int, default=1024)
     parser.add_argument("--minor", type=int, default=512)
     parser.add_argument(
@@ -98,7 +111,7 @@ def parse() -> argparse.Namespace:
     return args
 
 
-def sphere_workload(args) -> int:
+def sphere_result(args, oracle_check: bool = True) -> dict:
     """
     The large-mesh configuration: an icosphere subdivided 9 times (2,621,442 vertices) displaced by
     real spherical harmonics up to degree 16 with N(0, 0.05 / l) coefficients, 16 single-source
@@ -169,6 +182,8 @@ def sphere_workload(args) -> int:
     result["mixed"]["vs_fp64_rel_err_max"] = max(d[0] for d in diffs)
     result["mixed"]["vs_fp64_rel_err_mean"] = float(np.mean([d[1] for d in diffs]))
     del fields
+    if not oracle_check:
+        return result
     log("== the oracle on the same surface at icosphere(7) (SciPy SuperLU under COLAMD, 1 thread)")
     v7, f7 = creation.displaced_sphere(7, lmax=16, amplitude=0.05, seed=args.seed)
     sample = creation.random_sources(len(v7), max(1, args.cpu_sample), seed=args.seed + 1)
@@ -185,7 +200,11 @@ def sphere_workload(args) -> int:
                             "rel_err_mean": float(np.mean([e[1] for e in errs]))}
         log(f"   {precision}: max {check[precision]['rel_err_max']:.3g}")
     result["oracle_check_icosphere7"] = check
-    print(json.dumps(result))
+    return result
+
+
+def sphere_workload(args) -> int:
+    print(json.dumps(sphere_result(args)))
     return 0
 
 
@@ -257,7 +276,7 @@ def igl_workload(args) -> int:
     return 0
 
 
-def batch_workload(args) -> int:
+def batch_result(args) -> dict:
     """
     The batched configuration: ``--meshes`` (default 256) meshes with log-uniform vertex counts in
     [``--min-vertices``, ``--max-vertices``] (default 1e3..2e5), one query each with 1-16 random
@@ -338,11 +357,16 @@ def batch_workload(args) -> int:
         "sampled_rel_err_max": max(errors),
         "factor_gb": round(batch.nbytes / 1e9, 3),
     }
-    print(json.dumps(result))
+    batch.close()
+    return result
+
+
+def batch_workload(args) -> int:
+    print(json.dumps(batch_result(args)))
     return 0
 
 
-def serve_workload(args) -> int:
+def serve_result(args, probe_rates=()) -> dict:
     """
     The serving configuration: ``--meshes`` meshes with log-uniform vertex counts in
     [min, max] (seeded jittered icospheres at the nearest level, tori and jittered planar grids),
@@ -389,8 +413,33 @@ def serve_workload(args) -> int:
         if time.perf_counter() > next_report:
             log(f"   {submitted} submitted, {server.stats(since=start).get('completed', 0)} done")
             next_report += 5.0
-    server.close()  # every queued request finishes first
+    server.drain()
     stats = server.stats(since=start)
+    # Capacity: short streams at higher offered rates, each measured on its own.
+    probes = []
+    for rate in probe_rates:
+        log(f"   probe: 10s of arrivals at {rate:.0f}/s")
+        p_start = time.perf_counter()
+        p_arrival = p_start
+        offered = 0
+        while True:
+            p_arrival += rng.exponential(1.0 / rate)
+            if p_arrival - p_start > 10.0:
+                break
+            delay = p_arrival - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            m = int(rng.integers(args.meshes))
+            count = min(int(rng.integers(1, 33)), sizes[m])
+            server.submit(m, rng.choice(sizes[m], size=count, replace=False).astype(np.int32))
+            offered += 1
+        server.drain()
+        p = server.stats(since=p_start)
+        probes.append({"offered_per_s": round(offered / 10.0, 1),
+                       "completed_per_s": round(p["queries_per_s"], 1),
+                       "p50_ms": round(p["p50_ms"], 3), "p99_ms": round(p["p99_ms"], 3)})
+        log(f"   {probes[-1]}")
+    server.close()  # every queued request finishes first
     # Accuracy: sampled answers on the smallest meshes against the FP64 oracle.
     from oracle import heat_oracle as ho
 
@@ -425,8 +474,106 @@ def serve_workload(args) -> int:
         "setup_s": round(setup, 2),
         "sampled_rel_err_max": max(errors) if errors else None,
     }
-    print(json.dumps(result))
+    if probes:
+        result["capacity_probes"] = probes
+    return result
+
+
+def serve_workload(args) -> int:
+    print(json.dumps(serve_result(args)))
     return 0
+
+
+def one_shot_result(oracle_c2, c2_vertices, c2_faces, c2_source: int) -> dict:
+    """
+    The reference's one-shot form (``heat_geodesic(vertices, faces, sources)``: every call builds
+    the operators and iterates both solves, nothing kept): the jittered icosphere(4) from vertex 0
+    (BASELINE's first configuration; ordito's published 3.47 ms is an unjittered icosphere(4) on
+    an RTX 5090, ``docs/benchmarks/heat.md:52``), and one source on the torus 1024 x 512.
+    """
+    import torch
+
+    from ordito_amd import creation, heat
+    from oracle import heat_oracle as ho
+
+    out = {}
+    v, f = creation.jittered_icosphere(4, 0.01, 0)
+    vg, fg = torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda()
+    src = torch.tensor([0], dtype=torch.int32, device="cuda")
+    expected = ho.geodesic(v, f, np.array([0])).distance
+    for _ in range(2):
+        heat.heat_geodesic(vg, fg, src)
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        got = heat.heat_geodesic(vg, fg, src)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    out["icosphere4_jittered"] = {
+        "n_vertices": int(len(v)),
+        "mean_ms": round(1e3 * float(np.mean(lat)), 3),
+        "p50_ms": ms(lat, 50),
+        "p99_ms": ms(lat, 99),
+        "calls": len(lat),
+        "rel_err_max": ho.relative_error(got.cpu().numpy(), expected)[0],
+    }
+    log(f"   one-shot icosphere(4): {out['icosphere4_jittered']}")
+    vg, fg = torch.from_numpy(c2_vertices).cuda(), torch.from_numpy(c2_faces).cuda()
+    src = torch.tensor([c2_source], dtype=torch.int32, device="cuda")
+    lat = []
+    stats = {}
+    for _ in range(2):
+        t0 = time.perf_counter()
+        trimesh = heat.Trimesh(vg, fg)
+        got = heat.heat_geodesic(trimesh, src)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+        stats = dict(trimesh.heat_solver(method="meshlab").last_stats)
+        del trimesh
+    out["torus1024x512"] = {
+        "n_vertices": int(len(c2_vertices)),
+        "mean_ms": round(1e3 * float(np.mean(lat)), 3),
+        "calls": len(lat),
+        "rounds": stats,
+        "rel_err_max": ho.relative_error(got.cpu().numpy(), oracle_c2.distance(np.array([c2_source])))[0],
+    }
+    log(f"   one-shot torus: {out['torus1024x512']}")
+    return out
+
+
+def extra_workloads(args, oracle_c2, c2_vertices, c2_faces, c2_sources) -> dict:
+    """The other configurations' figures, each a sub-object of the default line."""
+    import copy
+
+    import torch
+
+    extras = {}
+    log("== one-shot (iterated) calls")
+    extras["one_shot"] = one_shot_result(oracle_c2, c2_vertices, c2_faces, int(c2_sources[0]))
+    torch.cuda.empty_cache()
+    log("== displaced icosphere(9), FP64 and mixed")
+    sub = copy.copy(args)
+    sub.steps, sub.warmup = args.extra_steps, 2
+    extras["displaced_sphere"] = sphere_result(sub, oracle_check=False)
+    torch.cuda.empty_cache()
+    log("== 256 packed meshes")
+    sub = copy.copy(args)
+    sub.steps, sub.warmup = args.extra_steps, 2
+    sub.meshes, sub.min_vertices, sub.max_vertices = 256, 1e3, 2e5
+    extras["packed_256"] = batch_result(sub)
+    torch.cuda.empty_cache()
+    log("== 64 served meshes")
+    sub = copy.copy(args)
+    sub.meshes, sub.min_vertices, sub.max_vertices = 64, 1e4, 1e6
+    sub.duration = min(args.duration, 20.0)
+    probes = [float(r) for r in args.serve_probes.split(",") if r.strip()]
+    extras["served_64"] = serve_result(sub, probe_rates=probes)
+    torch.cuda.empty_cache()
+    for value in extras.values():
+        for key in ("metric", "n_gpus", "higher_is_better", "data"):
+            value.pop(key, None)
+    return extras
 
 
 def log(message: str) -> None:
@@ -677,13 +824,16 @@ def main() -> int:
         }
         del mixed, mesh_mixed
 
-    if not args.no_cpu_baseline:
+    oracle = None
+    oracle_setup = 0.0
+    if not args.no_cpu_baseline or not args.no_extra:
         from oracle import heat_oracle as ho
 
-        log("== cpu baseline (oracle, SciPy SuperLU, 1 thread)")
         t0 = time.perf_counter()
         oracle = ho.FactoredOracle(vertices, faces)
         oracle_setup = time.perf_counter() - t0
+    if not args.no_cpu_baseline:
+        log("== cpu baseline (oracle, SciPy SuperLU, 1 thread)")
         sample = list(range(min(args.cpu_sample, k)))
         t0 = time.perf_counter()
         expected = [oracle.distance(np.array([sources[q]])) for q in sample]
@@ -694,6 +844,7 @@ def main() -> int:
             for i, q in enumerate(sample)
             if q < len(plain_out)
         ]
+        all_cores = cpu_all_cores(oracle, sources, args.cpu_sample_all)
         result["cpu_baseline"] = {
             "value": round(len(sample) / cpu_elapsed, 4),
             "unit": "queries/s",
@@ -703,6 +854,7 @@ def main() -> int:
             f"oracle with both systems pre-factored (factorization {oracle_setup:.1f}s excluded)",
             "cpu": _cpu_model(),
             "nproc": os.cpu_count(),
+            "all_cores": all_cores,
         }
         result["rel_err"] = {
             "served_max": max(e[0] for e in errs),
@@ -710,8 +862,49 @@ def main() -> int:
             "plain_max": max(e[0] for e in plain_errs) if plain_errs else None,
             "queries_checked": len(sample),
         }
+    if not args.no_extra:
+        del solver, mesh
+        torch.cuda.empty_cache()
+        result["workloads"] = extra_workloads(args, oracle, vertices, faces, sources)
     print(json.dumps(result))
     return 0
+
+
+def _oracle_query(q: int) -> float:
+    # Runs in a forked worker: the factored oracle is inherited, not pickled.
+    _FORK_STATE["oracle"].distance(np.array([_FORK_STATE["sources"][q]]))
+    return 0.0
+
+
+_FORK_STATE: dict = {}
+
+
+def cpu_all_cores(oracle, sources, count: int) -> dict | None:
+    """The same oracle queries on every host core: worker processes forked after the
+    factorization (each solve single-threaded), ``count`` queries spread over them."""
+    import multiprocessing
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(int(cores or 1), 16))
+    if count <= 0 or cores <= 1:
+        return None
+    _FORK_STATE["oracle"] = oracle
+    _FORK_STATE["sources"] = sources
+    context = multiprocessing.get_context("fork")
+    queries = [q % len(sources) for q in range(count)]
+    with context.Pool(cores) as pool:
+        pool.map(_oracle_query, queries[:cores])  # workers up
+        t0 = time.perf_counter()
+        pool.map(_oracle_query, queries, chunksize=1)
+        elapsed = time.perf_counter() - t0
+    _FORK_STATE.clear()
+    return {
+        "value": round(count / elapsed, 4),
+        "unit": "queries/s",
+        "cores": cores,
+        "sample": f"{count} single-source queries over {cores} forked worker processes sharing the "
+        "pre-factored oracle (SciPy SuperLU, one thread each)",
+    }
 
 
 def _cpu_model() -> str:

@@ -37,7 +37,8 @@ extern "C" {
 #define HG_METHOD_MESHLAB 0     /* barycentric mass, h over every face's three edges */
 #define HG_METHOD_IGL 1         /* Voronoi mass, h over the unique edges; a geodesic averages the
                                    Neumann heat with the heat pinned to zero on the rim */
-#define HG_METHOD_POTPOURRI3D 2 /* barycentric mass, h over the unique edges */
+#define HG_METHOD_POTPOURRI3D 2 /* barycentric mass, h over the unique edges; a geodesic's Poisson
+                                   step solves -L + 1e-6 I (geometry-central's regularization) */
 
 /* How a solve runs (HeatSolver policy, heat.py:4348-4423). */
 #define HG_MODE_ITERATE 0  /* settled Jacobi-CG heat solve; Jacobi-Chebyshev CG Poisson solve */
@@ -52,6 +53,8 @@ extern "C" {
 #define HG_SYSTEM_HEAT 1
 #define HG_SYSTEM_POISSON 2
 #define HG_SYSTEM_PINNED 4 /* the heat system pinned to zero on the rim (nothing on a closed mesh) */
+#define HG_SYSTEM_REGULARIZED 8 /* HG_METHOD_POTPOURRI3D's Poisson step, -L + 1e-6 I
+                                   (HeatSolver._solve_regularized_poisson, heat.py:4085-4110) */
 
 typedef struct hg_mesh hg_mesh;
 
@@ -83,6 +86,9 @@ typedef struct hg_solve_stats {
   int32_t poisson_refinements;
   int32_t backward_error_failed; /* 1 when a settled iterate failed its backward-error check */
   int32_t pinned_solves;     /* the rim-pinned heat step's CG rounds or refinement solves */
+  int32_t heat_marched;      /* 1 when the heat step was marched band by band (refused under the
+                                memory budget; heat_rounds then counts the bands' CG rounds) */
+  int32_t poisson_multigrid; /* 1 when the Poisson step ran under the multigrid V-cycle */
 } hg_solve_stats;
 
 /* Thread-local message of the last failing call ("" if none). */
@@ -127,8 +133,8 @@ int hg_operator_views(hg_mesh* mesh, hg_operator_view* out_view);
  */
 int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_factored);
 
-/* Whether each system holds a factor: bit 0 heat, bit 1 Poisson, bit 2 rim-pinned heat; bits 4-6
- * the same systems' refusals under the memory budget. */
+/* Whether each system holds a factor: bit 0 heat, bit 1 Poisson, bit 2 rim-pinned heat, bit 3 the
+ * regularized Poisson system; bits 4-7 the same systems' refusals under the memory budget. */
 int hg_factored(hg_mesh* mesh, int32_t* out_systems);
 
 /*
@@ -149,6 +155,25 @@ int hg_graph_state(hg_mesh* mesh, int64_t* out_cached, int32_t* out_broken);
  * factorization starts. Setting it forgets earlier refusals.
  */
 int hg_set_memory_budget(hg_mesh* mesh, int64_t bytes);
+
+/*
+ * Whether a system is asked of its analysis, before it iterates, if its factorization would exceed
+ * an explicitly set memory budget (default 1): a refused system is then marched at once. With 0
+ * only a refusal already met counts, and a refusal is found when a rejected settled iterate falls
+ * back to a factorization -- ordito's size test (_could_exceed_budget, linalg.py:2729-2745)
+ * skipping the analysis of an operator it deems small enough.
+ */
+int hg_set_budget_probe(hg_mesh* mesh, int32_t probe);
+
+/*
+ * The settle rule of the heat solves' iteration (solve_spd_settled's check_rounds,
+ * change_tolerance and settle_rounds, linalg.py:2490-2560): every check_rounds rounds the iterate
+ * stops once each entry changed by less than change_tolerance of itself since the last check, or
+ * once no new entry was reached for settle_rounds rounds. Defaults 16, 1e-6 and the window the
+ * operators' time gives (heat.py:5234-5243), which settle_rounds 0 restores.
+ */
+int hg_set_settle_rule(hg_mesh* mesh, int32_t check_rounds, double change_tolerance,
+                       int32_t settle_rounds);
 
 /*
  * Refinement targets of the factored solves at one precision: the heat step's componentwise
@@ -174,10 +199,13 @@ int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows, int3
  * heat.py:3890-3914). HG_MODE_ITERATE runs the settled Jacobi-CG (solve_spd_settled,
  * linalg.py:2490-2727) and its componentwise backward-error check; on failure it factors the
  * heat system and solves again refined (the reference's fallback), reporting it in *out_flags
- * (bit 0: the iterate failed its check, bit 1: a factorization was built, bit 2: the factorization
- * was refused under the memory budget and the iterate stands). HG_MODE_FACTORED solves
- * through the held factor, refined componentwise to 1e-12 (linalg.py:2659-2665), factoring first
- * if none is held. d_solution is overwritten (the iterate starts from zero).
+ * (bit 0: the iterate failed its check, bit 1: a factorization was built, bit 2: the system's
+ * factorization is refused under the memory budget and the system was marched band by band
+ * (_march_settled, linalg.py:2765-2939): at once when the refusal was known before iterating, as
+ * after a refusal or with a budget set that the analysis says the factorization exceeds, else in
+ * place of the rejected iterate). HG_MODE_FACTORED solves through the held factor, refined
+ * componentwise to 1e-12 (linalg.py:2659-2665), factoring first if none is held (marching when
+ * that is refused). d_solution is overwritten (the iterate starts from zero).
  */
 int hg_diffuse(hg_mesh* mesh, const double* d_rhs, double* d_solution, int64_t n_columns,
                int32_t mode, int32_t precision, int32_t* out_flags);
@@ -274,6 +302,54 @@ int hg_packed_times(hg_mesh* mesh, double* out_times);
  */
 int hg_timing_enable(hg_mesh* mesh, int32_t enable);
 int hg_timing_read(hg_mesh* mesh, double* out_ms8, int64_t* out_counts8, int32_t reset);
+
+/*
+ * The geodesic query server: continuous batching over resident per-mesh factors, with admission
+ * by memory. No counterpart in the reference, which answers one call at a time (heat.py:370-609)
+ * and keeps a mesh's factors in its HeatSolver (mesh.py:2187-2265). `workers` threads (1-16) each
+ * own a HIP stream; a free worker takes the mesh whose oldest waiting request is oldest, with up
+ * to `max_batch` of its requests, and solves them as one hg_geodesic batch through the mesh's
+ * factors (built on its first batch, at `precision`), so batches of different meshes overlap on
+ * the device and requests that arrive meanwhile join their mesh's next batch. Before a mesh is
+ * factored the least recently used idle meshes are released (hg_release) until its footprint --
+ * its analysis' factor and front-arena bytes and a max_batch-wide batch's workspaces -- fits
+ * `budget_bytes` beside every device byte the resident meshes hold.
+ *
+ * hg_server_add_mesh binds a mesh (copied, as hg_mesh_create; MeshLab's discretization at the
+ * solver's default time) and returns its id. hg_server_submit queues one query: `sources` is a
+ * host array copied at the call; the (V,) float64 distance is written to `d_out` (device memory
+ * the caller keeps until the ticket completes). hg_server_wait waits for a ticket (timeout_ms < 0:
+ * without limit) and sets *out_done; a completed ticket is consumed by the wait that sees it, and
+ * a query's failure is that wait's status. hg_server_stats reports the queries completed at or
+ * after `since_seconds` (seconds since the server's creation). hg_server_destroy answers every
+ * queued query first, then frees the meshes.
+ */
+typedef struct hg_server hg_server;
+typedef struct hg_server_report {
+  int64_t completed;      /* queries completed since `since_seconds` */
+  int64_t batches;        /* batches solved since creation */
+  int64_t evictions;      /* meshes released to admit others */
+  int32_t resident;       /* meshes holding factors now */
+  int32_t reserved;
+  int64_t resident_bytes; /* device bytes the meshes hold now */
+  double queries_per_s;   /* completed / (last completion - first submission) */
+  double p50_ms, p99_ms, mean_ms; /* latency from submission to completion */
+  double now_seconds;     /* seconds since the server's creation */
+} hg_server_report;
+
+int hg_server_create(int64_t budget_bytes, int32_t workers, int32_t max_batch, int32_t precision,
+                     hg_server** out_server);
+int hg_server_add_mesh(hg_server* server, const float* d_vertices, int64_t n_vertices,
+                       const int32_t* d_faces, int64_t n_faces, int64_t* out_mesh_id);
+int hg_server_submit(hg_server* server, int64_t mesh_id, const int32_t* sources,
+                     int64_t n_sources, double* d_out, int64_t* out_ticket);
+int hg_server_wait(hg_server* server, int64_t ticket, double timeout_ms, int32_t* out_done);
+int hg_server_stats(hg_server* server, double since_seconds, hg_server_report* out_stats);
+int hg_server_destroy(hg_server* server);
+
+/* Statistics of the mesh's last solve (hg_diffuse, hg_diffuse_pinned, hg_solve_poisson,
+ * hg_geodesic, hg_geodesic_packed). No counterpart in the reference, which logs them. */
+int hg_last_stats(hg_mesh* mesh, hg_solve_stats* out_stats);
 
 /* Device bytes the held factorizations take (HeatSolver.nbytes, heat.py:3838-3847). */
 int hg_nbytes(hg_mesh* mesh, int64_t* out_bytes);

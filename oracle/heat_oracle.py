@@ -35,6 +35,11 @@ Pinning (what checks each function; see ``tests/test_oracle.py``):
 - ``geodesic(method="igl")``: a source on the rim leaves only the (halved) Neumann heat, so the
   distance is the Voronoi Neumann one bit for bit; the great-circle bound on a sphere; ``"igl"``
   and ``"meshlab"`` differ where the choices matter.
+- ``solve_regularized_poisson`` / ``geodesic(method="potpourri3d")``: ``(-L + 1e-6 I) 1 = 1e-6``
+  (``L 1 = 0``), so the right-hand side ``1e-6`` gives ones exactly up to rounding and the
+  regularization's sign and size are fixed; an unreferenced vertex stays zero; the great-circle
+  bound on a sphere; the regularization moves the distance by well under the method's own error
+  on a coarse mesh.
 """
 
 from __future__ import annotations
@@ -57,6 +62,8 @@ SETTLE_BACKWARD_ERROR = 1e-6
 SETTLE_DIRECT_TOLERANCE = 1e-12
 # heat.py:108: the Poisson solve's tolerance.
 CG_TOLERANCE = 1e-8
+# heat.py:190-193: geometry-central's identity multiple on the Poisson system ("potpourri3d").
+POTPOURRI_POISSON_SHIFT = 1e-6
 
 
 def _f32(x):
@@ -584,8 +591,13 @@ class FactoredOracle:
         for members in self.members:
             pinned[members[-1]] = True
         self.keep = ~pinned
-        reduced = self.ops.matrix(self.ops.poisson)[self.keep][:, self.keep]
-        self.poisson_lu = factorize(reduced, ordering)
+        self.regularized = method == "potpourri3d"
+        if self.regularized:
+            self.keep = np.ones(n, dtype=bool)
+            self.poisson_lu = factorize(regularized_poisson_matrix(self.ops), ordering)
+        else:
+            reduced = self.ops.matrix(self.ops.poisson)[self.keep][:, self.keep]
+            self.poisson_lu = factorize(reduced, ordering)
 
     def distance(self, sources: np.ndarray) -> np.ndarray:
         n = self.ops.pattern.n
@@ -595,6 +607,10 @@ class FactoredOracle:
             heat = average_heat(heat, self.pinned_lu.solve(np.where(self.rim, 0.0, u0)))
         neg_div = face_unit_divergence(self.vertices, self.faces, self.ops.terms, heat)
         phi = np.zeros(n)
+        if self.regularized:
+            rhs = np.where(np.diff(self.ops.pattern.indptr) == 0, 0.0, neg_div)
+            phi = self.poisson_lu.solve(rhs)
+            return shift_and_orient(phi, sources)
         phi[self.keep] = self.poisson_lu.solve(neg_div[self.keep])
         for members in self.members:
             phi[members] -= phi[members].mean()
@@ -630,6 +646,32 @@ def solve_poisson(ops: Operators, rhs: np.ndarray) -> np.ndarray:
     for _, members in pins:
         phi[members] -= phi[members].mean()
     return phi
+
+
+def regularized_poisson_matrix(ops: Operators) -> sp.csr_matrix:
+    """
+    geometry-central's Poisson system ``-L + 1e-6 I`` over the Poisson pattern
+    (``HeatSolver._solve_regularized_poisson``, ``heat.py:4085-4110``: ``add_to_diagonal`` on the
+    stored diagonal entries); a row with no stored diagonal (an unreferenced vertex) holds the
+    identity's, so its unknown stays zero, as the factorization's pin keeps it.
+    """
+    pattern = ops.pattern
+    rows = np.repeat(np.arange(pattern.n), np.diff(pattern.indptr))
+    diagonal = pattern.indices == rows
+    values = ops.poisson.copy()
+    values[diagonal] = values[diagonal] + POTPOURRI_POISSON_SHIFT
+    matrix = ops.matrix(values).tolil()
+    for v in np.flatnonzero(np.diff(pattern.indptr) == 0):
+        matrix[v, v] = 1.0
+    return matrix.tocsr()
+
+
+def solve_regularized_poisson(ops: Operators, rhs: np.ndarray) -> np.ndarray:
+    """``(-L + 1e-6 I) phi = rhs``, nonsingular: no pin, no mean removed (``heat.py:3121-3128``);
+    an unreferenced vertex's right-hand side is taken as zero."""
+    rhs = np.array(rhs, dtype=np.float64)
+    rhs[np.diff(ops.pattern.indptr) == 0] = 0.0
+    return solve_direct(regularized_poisson_matrix(ops), rhs)
 
 
 def rim_vertices(pattern: Pattern) -> np.ndarray:
@@ -695,8 +737,9 @@ def geodesic(
 ) -> GeodesicResult:
     """
     ``heat_geodesic(vertices, faces, sources, t, method=method)`` solved directly in FP64
-    (``method`` "meshlab" or "igl"; at "igl" the Voronoi mass, the unique-edge time and the
-    heat averaged with the rim-pinned heat, ``heat.py:594-597``).
+    (``method`` "meshlab", "igl" or "potpourri3d"; at "igl" the Voronoi mass, the unique-edge time
+    and the heat averaged with the rim-pinned heat, ``heat.py:594-597``; at "potpourri3d" the
+    unique-edge time and the regularized Poisson system, ``heat.py:3121-3128``).
 
     Follows ``heat.py:552-609`` (``_heat_geodesic``): empty input gives zeros; the time rule of
     ``heat.py:3714-3736``; heat step ``(M - tL) u = u0`` (``heat.py:3890-3914``); divergence
@@ -720,7 +763,10 @@ def geodesic(
         if pinned is not None:
             heat = average_heat(heat, pinned)
     neg_div = face_unit_divergence(vertices, faces, ops.terms, heat)
-    phi = solve_poisson(ops, neg_div)
+    if method == "potpourri3d":
+        phi = solve_regularized_poisson(ops, neg_div)
+    else:
+        phi = solve_poisson(ops, neg_div)
     distance = shift_and_orient(phi, sources)
     return GeodesicResult(distance, heat, neg_div, ops)
 

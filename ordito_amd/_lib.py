@@ -31,6 +31,7 @@ HG_PRECISION_MIXED = 1
 HG_SYSTEM_HEAT = 1
 HG_SYSTEM_POISSON = 2
 HG_SYSTEM_PINNED = 4
+HG_SYSTEM_REGULARIZED = 8
 
 # The C ABI's exported symbols, in header order; the load test checks every one resolves.
 EXPORTED = (
@@ -46,6 +47,8 @@ EXPORTED = (
     "hg_set_graphs",
     "hg_graph_state",
     "hg_set_memory_budget",
+    "hg_set_budget_probe",
+    "hg_set_settle_rule",
     "hg_set_refinement",
     "hg_set_plan_options",
     "hg_diffuse",
@@ -59,6 +62,13 @@ EXPORTED = (
     "hg_mesh_create_packed",
     "hg_geodesic_packed",
     "hg_packed_times",
+    "hg_server_create",
+    "hg_server_add_mesh",
+    "hg_server_submit",
+    "hg_server_wait",
+    "hg_server_stats",
+    "hg_server_destroy",
+    "hg_last_stats",
     "hg_nbytes",
     "hg_release",
     "hg_device_bytes",
@@ -98,6 +108,25 @@ class OperatorView(ctypes.Structure):
     ]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("evictions", ctypes.c_int64),
+        ("resident", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("resident_bytes", ctypes.c_int64),
+        ("queries_per_s", ctypes.c_double),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("now_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [
         ("heat_rounds", ctypes.c_int32),
@@ -106,6 +135,8 @@ class SolveStats(ctypes.Structure):
         ("poisson_refinements", ctypes.c_int32),
         ("backward_error_failed", ctypes.c_int32),
         ("pinned_solves", ctypes.c_int32),
+        ("heat_marched", ctypes.c_int32),
+        ("poisson_multigrid", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict[str, int]:
@@ -137,6 +168,8 @@ _SIGNATURES = {
     "hg_diffuse_pinned": ([_P, _P, _P, _I64, _I32, _I32, _PI32, _PI32], ctypes.c_int),
     "hg_rim_count": ([_P, _PI64], ctypes.c_int),
     "hg_set_memory_budget": ([_P, _I64], ctypes.c_int),
+    "hg_set_budget_probe": ([_P, _I32], ctypes.c_int),
+    "hg_set_settle_rule": ([_P, _I32, ctypes.c_double, _I32], ctypes.c_int),
     "hg_set_graphs": ([_P, _I32], ctypes.c_int),
     "hg_graph_state": ([_P, _PI64, _PI32], ctypes.c_int),
     "hg_solve_poisson": ([_P, _P, _P, _I64, _I32, _I32], ctypes.c_int),
@@ -150,6 +183,13 @@ _SIGNATURES = {
         ctypes.c_int,
     ),
     "hg_packed_times": ([_P, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "hg_server_create": ([_I64, _I32, _I32, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "hg_server_add_mesh": ([_P, _P, _I64, _P, _I64, _PI64], ctypes.c_int),
+    "hg_server_submit": ([_P, _I64, _P, _I64, _P, _PI64], ctypes.c_int),
+    "hg_server_wait": ([_P, _I64, ctypes.c_double, _PI32], ctypes.c_int),
+    "hg_server_stats": ([_P, ctypes.c_double, ctypes.POINTER(ServerStats)], ctypes.c_int),
+    "hg_server_destroy": ([_P], ctypes.c_int),
+    "hg_last_stats": ([_P, ctypes.POINTER(SolveStats)], ctypes.c_int),
     "hg_nbytes": ([_P, _PI64], ctypes.c_int),
     "hg_release": ([_P], ctypes.c_int),
     "hg_device_bytes": ([_P, _PI64], ctypes.c_int),

@@ -183,46 +183,56 @@ def planar_grid(
     return vertices.astype(np.float32), faces.astype(np.int32).reshape(-1)
 
 
-def _real_spherical_harmonics(lmax: int, theta: np.ndarray, phi: np.ndarray):
-    """Orthonormal real spherical harmonics ``Y_lm`` for ``1 <= l <= lmax``, ``-l <= m <= l``, one
-    at a time in that order (a 2.6 M-vertex sphere's 288 of them would take 6 GB at once)."""
-    from scipy.special import lpmv
-
-    x = np.cos(theta)
-    for degree in range(1, lmax + 1):
-        for order in range(-degree, degree + 1):
-            k = abs(order)
-            norm = math.sqrt(
-                (2 * degree + 1) / (4.0 * math.pi) * math.factorial(degree - k)
-                / math.factorial(degree + k)
-            )
-            # scipy's lpmv includes the Condon-Shortley phase; it cancels in a random-sign sum.
-            legendre = lpmv(k, degree, x)
-            if order == 0:
-                yield norm * legendre
-            elif order > 0:
-                yield math.sqrt(2.0) * norm * legendre * np.cos(k * phi)
-            else:
-                yield math.sqrt(2.0) * norm * legendre * np.sin(k * phi)
-
-
 def displaced_sphere(
     subdivisions: int, lmax: int = 16, amplitude: float = 0.05, seed: int = 0
 ) -> tuple[np.ndarray, np.ndarray]:
     """
-    Icosphere with radius ``1 + sum_{l<=lmax, m} c_lm Y_lm``, ``c_lm ~ N(0, amplitude / l)``.
+    Icosphere with radius ``1 + sum_{l<=lmax, m} c_lm Y_lm``, ``c_lm ~ N(0, amplitude / l)``, the
+    orthonormal real spherical harmonics (Condon-Shortley phase) of ``1 <= l <= lmax``,
+    ``-l <= m <= l``, the coefficients drawn in that order.
+
+    The associated Legendre functions come from the standard three-term recurrence in the degree
+    at each order (``P_m^m = (-1)^m (2m-1)!! (1 - x^2)^(m/2)``, ``P_{m+1}^m = (2m+1) x P_m^m``,
+    ``(l - m) P_l^m = (2l - 1) x P_{l-1}^m - (l + m - 1) P_{l-2}^m``), one order at a time, so only
+    three functions of the vertices are held at once.
     """
     vertices, faces = icosphere(subdivisions)
     unit = vertices.astype(np.float64)
     unit /= np.linalg.norm(unit, axis=1)[:, None]
-    theta = np.arccos(np.clip(unit[:, 2], -1.0, 1.0))
+    x = np.clip(unit[:, 2], -1.0, 1.0)
     phi = np.arctan2(unit[:, 1], unit[:, 0])
     rng = np.random.default_rng(seed)
-    radius = np.ones(unit.shape[0])
-    basis = _real_spherical_harmonics(lmax, theta, phi)
+    coefficients = {}
     for degree in range(1, lmax + 1):
-        for _ in range(2 * degree + 1):
-            radius += rng.normal(0.0, amplitude / degree) * next(basis)
+        for order in range(-degree, degree + 1):
+            coefficients[(degree, order)] = rng.normal(0.0, amplitude / degree)
+    radius = np.ones(unit.shape[0])
+    sine = np.sqrt(np.maximum(1.0 - x * x, 0.0))
+    p_mm = np.ones_like(x)  # P_m^m, built up order by order
+    for k in range(0, lmax + 1):
+        if k > 0:
+            p_mm = -(2 * k - 1) * sine * p_mm
+        cos_k, sin_k = (np.cos(k * phi), np.sin(k * phi)) if k > 0 else (None, None)
+        previous, current = None, p_mm
+        for degree in range(k, lmax + 1):
+            if degree > k:
+                if degree == k + 1:
+                    nxt = (2 * k + 1) * x * current
+                else:
+                    nxt = ((2 * degree - 1) * x * current - (degree + k - 1) * previous) / (degree - k)
+                previous, current = current, nxt
+            if degree == 0:
+                continue
+            norm = math.sqrt(
+                (2 * degree + 1) / (4.0 * math.pi) * math.factorial(degree - k)
+                / math.factorial(degree + k)
+            )
+            if k == 0:
+                radius += coefficients[(degree, 0)] * (norm * current)
+            else:
+                scaled = math.sqrt(2.0) * norm * current
+                radius += coefficients[(degree, k)] * (scaled * cos_k)
+                radius += coefficients[(degree, -k)] * (scaled * sin_k)
     return (unit * radius[:, None]).astype(np.float32), faces
 
 

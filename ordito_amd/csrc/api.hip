@@ -7,6 +7,7 @@
 #include <map>
 
 #include "cholesky.h"
+#include "multigrid.h"
 
 namespace hg {
 
@@ -32,6 +33,7 @@ constexpr int32_t kRefineRounds = 12;             // cholesky.py:74
 constexpr double kMixedHeatTolerance = 1e-6;
 constexpr double kMixedPoissonTolerance = 1e-6;
 constexpr int32_t kMixedRefineRounds = 24;
+constexpr double kPotpourriPoissonShift = 1e-6;   // heat.py:193: geometry-central's -L + 1e-6 I
 constexpr int32_t kChebyshevSteps = 12;           // linalg.py CHEBYSHEV_DEGREE: 11 products
 constexpr double kChebyshevInterval = 80.0;       // linalg.py CHEBYSHEV_INTERVAL
 constexpr double kChebyshevCap = 0.5;
@@ -75,6 +77,12 @@ struct hg_mesh {
   DeviceArray<float> pinned32;
   DeviceArray<double> pinned_rhs;
   double poisson_bound = -1.0;
+  // HG_METHOD_POTPOURRI3D's Poisson step: -L + 1e-6 I over the same pattern, built on first use
+  // (HeatSolver._solve_regularized_poisson, heat.py:4085-4110); no null space, its empty rows
+  // (unreferenced vertices) pinned as the heat system's are.
+  bool reg_built = false;
+  DeviceArray<double> reg, reg_inv;
+  double reg_bound = -1.0;
   // Components: the Poisson system pins each component's last-eliminated row.
   int32_t n_comp = 0;
   std::vector<int32_t> labels;
@@ -93,11 +101,18 @@ struct hg_mesh {
   DeviceArray<int32_t> faces_rows;  // the faces' corners as rows of the elimination order
   DeviceArray<int4> row_corners;    // each row's corners {corner id, its face's three rows}
   DeviceArray<int32_t> row_corner_ptr;
-  Factor heat_factor, poisson_factor, pinned_factor;
+  Factor heat_factor, poisson_factor, pinned_factor, reg_factor;
   // The factorizations' memory budget in bytes (0: 90 % of the device memory free when a
   // factorization starts) and the systems refused under it (HG_SYSTEM_* bits).
   int64_t budget = 0;
   int32_t refused = 0;
+  // Whether a system is asked of its analysis before it iterates (hg_set_budget_probe).
+  bool budget_probe = true;
+  // The settle rule of the heat solves' iteration (hg_set_settle_rule); settle_window 0 keeps the
+  // window the operators' time gives (settle_rounds).
+  int32_t check_rounds = kHeatCheckRounds;
+  double change_tolerance = kHeatChangeTolerance;
+  int32_t settle_window = 0;
   // Speculative refinement (FP64 factors): the verdicts of the first rounds of the heat, pinned and
   // Poisson solves of a call, [system][chunk], read at the call's last synchronization.
   PinnedArray<int32_t> h_speculate;
@@ -109,6 +124,10 @@ struct hg_mesh {
                                    {kMixedHeatTolerance, kMixedPoissonTolerance}};
   // Workspaces.
   CgWorkspace cg;
+  MarchWorkspace march;  // the band march of a system refused under the budget
+  // The Poisson system's multigrid hierarchy, built on its first solve past the budget.
+  std::unique_ptr<Multigrid> poisson_mg;
+  bool poisson_mg_tried = false;
   SolveWorkspace solve;
   SolveWorkspace solve_side;  // the rim-pinned heat solve's, when it runs beside the Neumann one
   bool on_side = false;       // the solves being enqueued use solve_side
@@ -197,14 +216,17 @@ int64_t mesh_device_bytes(const hg_mesh& m) {
     total += a->bytes();
   for (const DeviceArray<double>* a :
        {&m.mass, &m.lap, &m.heat, &m.poisson, &m.heat_inv, &m.poisson_inv, &m.scalars,
-        &m.edge_partials, &m.face_geom, &m.pinned, &m.pinned_inv, &m.pinned_rhs, &m.fields,
+        &m.edge_partials, &m.face_geom, &m.pinned, &m.pinned_inv, &m.pinned_rhs, &m.reg,
+        &m.reg_inv, &m.fields,
         &m.shift_partials, &m.shift_values, &m.null_partials, &m.packed_phi})
     total += a->bytes();
   for (const DeviceArray<int64_t>* a : {&m.d_owner, &m.d_offsets, &m.q_src_off, &m.q_dst_off,
                                         &m.q_len})
     total += a->bytes();
-  total += m.row_corners.bytes() + m.d_slots.bytes();
-  total += m.heat_factor.bytes() + m.poisson_factor.bytes() + m.pinned_factor.bytes();
+  total += m.row_corners.bytes() + m.d_slots.bytes() + m.march.bytes();
+  if (m.poisson_mg) total += m.poisson_mg->bytes();
+  total += m.heat_factor.bytes() + m.poisson_factor.bytes() + m.pinned_factor.bytes() +
+           m.reg_factor.bytes();
   if (m.plan) total += m.plan->bytes();
   total += m.cg.bytes() + m.solve.bytes() + m.solve_side.bytes() + m.null_space.bytes();
   return total;
@@ -215,6 +237,9 @@ int64_t mesh_device_bytes(const hg_mesh& m) {
 void release_workspaces(hg_mesh& m) {
   invalidate_graphs(m);
   m.cg.release();
+  m.march.release();
+  m.poisson_mg.reset();
+  m.poisson_mg_tried = false;
   m.solve.release();
   m.solve_side.release();
   m.fields.reset();
@@ -357,6 +382,9 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   m.poisson_factor.reset();
   m.pinned_factor.reset();
   m.pinned_built = false;
+  m.reg_factor.reset();
+  m.reg_built = false;
+  m.reg_bound = -1.0;
   invalidate_graphs(m);
 }
 
@@ -444,6 +472,21 @@ void ensure_plan(hg_mesh& m) {
   m.plan = std::move(device);
 }
 
+// HG_METHOD_POTPOURRI3D's Poisson system -L + 1e-6 I over the Poisson pattern (built on first
+// use, so its factorization shares the heat analysis; heat.py:4097-4108).
+void ensure_regularized_system(hg_mesh& m) {
+  ensure_operators(m);
+  if (m.reg_built) return;
+  const int64_t nnz = static_cast<int64_t>(m.pattern.columns.size());
+  m.reg.allocate(std::max<int64_t>(nnz, 1));
+  m.reg_inv.allocate(std::max<int64_t>(m.n, 1));
+  launch_shifted_rows(m.offsets.ptr, m.columns.ptr, m.poisson.ptr, m.n, kPotpourriPoissonShift,
+                      m.reg.ptr, m.reg_inv.ptr, m.stream);
+  m.reg_built = true;
+  m.reg_bound = -1.0;
+  invalidate_graphs(m);
+}
+
 // The rim-pinned heat system over the heat system's pattern (meshes with a rim only).
 void ensure_pinned_system(hg_mesh& m) {
   ensure_operators(m);
@@ -482,6 +525,18 @@ void require_budget(hg_mesh& m, int32_t system, bool keep_fp32) {
 
 bool is_budget_refusal(const Error& e) { return e.status == HG_ERR_BUDGET; }
 
+// Whether `system`'s factorization would be refused, asked of the analysis before any iteration
+// (OperatorFactorization.exceeds_budget, linalg.py:2393-2420): with no budget set only a system
+// already refused is; with one set, the plan's peak decides and a refusal is remembered.
+bool exceeds_budget(hg_mesh& m, int32_t system) {
+  if (m.refused & system) return true;
+  if (m.budget <= 0 || !m.budget_probe) return false;
+  ensure_plan(m);
+  if (factor_peak_bytes(*m.plan, false) <= m.budget) return false;
+  m.refused |= system;
+  return true;
+}
+
 void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
   ensure_operators(m);
   ensure_plan(m);
@@ -494,6 +549,17 @@ void factor_system(hg_mesh& m, int32_t system, bool keep_fp32) {
       invalidate_graphs(m);
       factor_numeric(*m.plan, m.pinned.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
                      m.heat_pin_slots, m.pinned_factor, keep_fp32, m.stream);
+    }
+    return;
+  }
+  if (system == HG_SYSTEM_REGULARIZED) {
+    ensure_regularized_system(m);
+    PhaseScope scope(&m.timer, kPhaseFactor, m.stream);
+    if (!m.reg_factor.held() || (keep_fp32 && !m.reg_factor.w32.ptr)) {
+      require_budget(m, system, keep_fp32);
+      invalidate_graphs(m);
+      factor_numeric(*m.plan, m.reg.ptr, m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr,
+                     m.heat_pin_slots, m.reg_factor, keep_fp32, m.stream);
     }
     return;
   }
@@ -553,9 +619,9 @@ int32_t diffuse_iterated(hg_mesh& m, const HeatSystem& sys, const double* rhs, d
     p.rhs = rhs + q0 * m.n;
     p.x = x + q0 * m.n;
     p.settle = 1;
-    p.check_rounds = kHeatCheckRounds;
-    p.change_tolerance = kHeatChangeTolerance;
-    p.settle_rounds = m.settle_rounds;
+    p.check_rounds = m.check_rounds;
+    p.change_tolerance = m.change_tolerance;
+    p.settle_rounds = m.settle_window > 0 ? m.settle_window : m.settle_rounds;
     total = std::max(total, run_cg(p, m.cg, m.stream));
   }
   return total;
@@ -599,8 +665,26 @@ int32_t diffuse(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
   const HeatSystem sys = heat_system(m, pinned);
   int32_t flags = 0;
   int32_t rounds = 0, refinements = 0;
+  const MarchSystem marched{m.n, m.offsets.ptr, m.columns.ptr, sys.values};
+  auto march = [&] {
+    HG_REQUIRE(layout == FieldLayout::kColumns, HG_ERR_RUNTIME, "a marched solve takes columns");
+    PhaseScope scope(&m.timer, kPhaseCg, m.stream);
+    rounds = march_settled(marched, rhs, x, k, m.march, m.stream);
+    flags |= 4;
+    m.stats.heat_marched = 1;
+  };
   if (mode == HG_MODE_FACTORED || sys.factor->held()) {
-    refinements = diffuse_factored(m, sys, rhs, x, k, precision, layout);
+    try {
+      refinements = diffuse_factored(m, sys, rhs, x, k, precision, layout);
+    } catch (const Error& e) {
+      // Refused under the budget before anything was solved: marched instead.
+      if (!is_budget_refusal(e) || layout != FieldLayout::kColumns) throw;
+      march();
+    }
+  } else if (exceeds_budget(m, sys.code)) {
+    // Past the budget the march costs less than the settled iteration, whether its iterate would
+    // pass the check or not: marched at once (linalg.py:2646-2651).
+    march();
   } else {
     HG_REQUIRE(layout == FieldLayout::kColumns, HG_ERR_RUNTIME, "an iterated solve takes columns");
     rounds = diffuse_iterated(m, sys, rhs, x, k);
@@ -612,9 +696,13 @@ int32_t diffuse(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
         refinements = diffuse_factored(m, sys, rhs, x, k, HG_PRECISION_FP64);
         if (!was_held) flags |= 2;
       } catch (const Error& e) {
-        // Refused under the budget before anything was solved: the iterate stands.
+        // The fallback refused under the budget before anything was solved: marched, which
+        // reaches every entry with its own precision too (linalg.py:2710-2716). The call's rounds
+        // stay the iteration's, as the reference returns them.
         if (!is_budget_refusal(e)) throw;
-        flags |= 4;
+        const int32_t iterated = rounds;
+        march();
+        rounds = iterated;
       }
     }
   }
@@ -627,77 +715,116 @@ int32_t diffuse(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
   return flags;
 }
 
-double poisson_bound(hg_mesh& m) {
-  if (m.poisson_bound < 0.0) {
+// The largest Gershgorin ratio of D^-1 A over a Poisson-type system's rows, reduced once.
+double dominance_bound(hg_mesh& m, const double* values, double& cached) {
+  if (cached < 0.0) {
     int32_t blocks = static_cast<int32_t>((std::max<int64_t>(m.n, 1) + kBlock - 1) / kBlock);
     DeviceArray<double> partials(blocks), out(1);
-    launch_dominance_bound(m.offsets.ptr, m.columns.ptr, m.poisson.ptr, m.n, partials.ptr, out.ptr,
+    launch_dominance_bound(m.offsets.ptr, m.columns.ptr, values, m.n, partials.ptr, out.ptr,
                            m.stream);
     double bound = 0.0;
     HG_HIP(hipMemcpyAsync(&bound, out.ptr, sizeof(double), hipMemcpyDeviceToHost, m.stream));
     HG_HIP(hipStreamSynchronize(m.stream));
-    m.poisson_bound = bound;
+    cached = bound;
   }
-  return m.poisson_bound;
+  return cached;
 }
 
 // The right-hand side and the solution may be stored by rows (factored only). A solution stored by
 // columns has each component's mean removed here; one stored by rows is left for finish_rows.
+// `regularized` solves HG_METHOD_POTPOURRI3D's -L + 1e-6 I instead of -L: no null space, its
+// empty rows pinned as the heat system's are, the same normwise tolerance (heat.py:4127-4151).
 void solve_poisson(hg_mesh& m, double* rhs, double* x, int64_t k, int32_t mode,
                    int32_t precision, FieldLayout rhs_layout = FieldLayout::kColumns,
-                   FieldLayout x_layout = FieldLayout::kColumns) {
+                   FieldLayout x_layout = FieldLayout::kColumns, bool regularized = false) {
   ensure_operators(m);
+  if (regularized) ensure_regularized_system(m);
   const bool mixed = precision == HG_PRECISION_MIXED;
-  bool factored = mode == HG_MODE_FACTORED || m.poisson_factor.held() || m.heat_factor.held();
+  const int32_t system = regularized ? HG_SYSTEM_REGULARIZED : HG_SYSTEM_POISSON;
+  Factor& factor = regularized ? m.reg_factor : m.poisson_factor;
+  const double* values = regularized ? m.reg.ptr : m.poisson.ptr;
+  const double* inv_diag = regularized ? m.reg_inv.ptr : m.poisson_inv.ptr;
+  const int32_t* pins = regularized ? (m.has_heat_pins ? m.heat_pinned_new.ptr : nullptr)
+                                    : m.poisson_pinned_new.ptr;
+  bool factored = mode == HG_MODE_FACTORED || factor.held() || m.heat_factor.held();
   if (factored && rhs_layout == FieldLayout::kColumns && x_layout == FieldLayout::kColumns) {
     // By columns a refusal under the budget leaves the solve to the iteration.
     try {
-      factor_system(m, HG_SYSTEM_POISSON, mixed);
+      factor_system(m, system, mixed);
     } catch (const Error& e) {
       if (!is_budget_refusal(e)) throw;
       factored = false;
     }
   }
   if (factored) {
-    factor_system(m, HG_SYSTEM_POISSON, mixed);
+    factor_system(m, system, mixed);
     m.stats.poisson_refinements = refined_solve(
-        *m.plan, m.poisson_factor, mixed, rhs, x, k, m.refine_tolerance[mixed ? 1 : 0][1], false,
-        mixed ? kMixedRefineRounds : kRefineRounds, m.poisson_pinned_new.ptr, m.solve, m.stream,
-        &m.timer, kPhasePoissonSweep, rhs_layout, x_layout,
+        *m.plan, factor, mixed, rhs, x, k, m.refine_tolerance[mixed ? 1 : 0][1], false,
+        mixed ? kMixedRefineRounds : kRefineRounds, pins, m.solve, m.stream, &m.timer,
+        kPhasePoissonSweep, rhs_layout, x_layout,
         m.speculating ? m.h_speculate.ptr + 2 * m.spec_chunks : nullptr);
     m.stats.poisson_rounds = 0;
   } else {
     HG_REQUIRE(rhs_layout == FieldLayout::kColumns && x_layout == FieldLayout::kColumns,
                HG_ERR_RUNTIME, "an iterated solve takes columns");
-    // The Jacobi-Chebyshev interval (linalg.py:6074-6090): [max(min(80/n, 0.5), 1 - r), 1 + r],
-    // r the largest Gershgorin ratio of D^-1 A, at least 1e-3.
-    double r = std::max(poisson_bound(m), 1e-3);
-    double lo = std::max(std::min(kChebyshevInterval / static_cast<double>(m.n), kChebyshevCap), 1.0 - r);
-    double hi = 1.0 + r;
-    PhaseScope scope(&m.timer, kPhaseCg, m.stream);
-    int32_t rounds = 0;
-    for (int64_t q0 = 0; q0 < k; q0 += 64) {
-      int64_t kk = std::min<int64_t>(64, k - q0);
-      CgProblem p;
-      p.n = m.n;
-      p.n_columns = kk;
-      p.offsets = m.offsets.ptr;
-      p.columns = m.columns.ptr;
-      p.values64 = m.poisson.ptr;
-      p.inv_diag = m.poisson_inv.ptr;
-      p.rhs = rhs + q0 * m.n;
-      p.x = x + q0 * m.n;
-      p.settle = 0;
-      p.tolerance = kPoissonTolerance;
-      p.chebyshev_steps = kChebyshevSteps;
-      p.lambda_min = lo;
-      p.lambda_max = hi;
-      rounds = std::max(rounds, run_cg(p, m.cg, m.stream));
+    // No factorization past the budget (the Poisson system's, or the mesh's heat system refused):
+    // a multigrid hierarchy's rounds, unlike the polynomial's, do not grow with the mesh
+    // (heat.py:4141-4149). An operator that does not coarsen keeps the polynomial.
+    const bool unfactorable =
+        !regularized &&
+        (exceeds_budget(m, HG_SYSTEM_POISSON) || (m.refused & HG_SYSTEM_HEAT) != 0);
+    if (unfactorable && !m.poisson_mg && !m.poisson_mg_tried) {
+      m.poisson_mg_tried = true;
+      HostCsr fine;
+      fine.rows = fine.cols = m.n;
+      fine.ptr = m.pattern.offsets;
+      fine.idx = m.pattern.columns;
+      fine.val.resize(m.pattern.columns.size());
+      if (!fine.val.empty())
+        HG_HIP(hipMemcpyAsync(fine.val.data(), m.poisson.ptr, sizeof(double) * fine.val.size(),
+                              hipMemcpyDeviceToHost, m.stream));
+      HG_HIP(hipStreamSynchronize(m.stream));
+      m.poisson_mg = build_multigrid(fine, 0, m.stream);
     }
-    m.stats.poisson_rounds = rounds;
-    m.stats.poisson_refinements = 0;
+    if (unfactorable && m.poisson_mg) {
+      PhaseScope scope(&m.timer, kPhaseCg, m.stream);
+      m.stats.poisson_rounds =
+          multigrid_cg(*m.poisson_mg, rhs, x, k, kPoissonTolerance, 10 * m.n, m.stream);
+      m.stats.poisson_refinements = 0;
+      m.stats.poisson_multigrid = 1;
+    } else {
+      // The Jacobi-Chebyshev interval (linalg.py:6074-6090): [max(min(80/n, 0.5), 1 - r), 1 + r],
+      // r the largest Gershgorin ratio of D^-1 A, at least 1e-3.
+      double r = std::max(dominance_bound(m, values, regularized ? m.reg_bound : m.poisson_bound),
+                          1e-3);
+      double lo = std::max(std::min(kChebyshevInterval / static_cast<double>(m.n), kChebyshevCap),
+                           1.0 - r);
+      double hi = 1.0 + r;
+      PhaseScope scope(&m.timer, kPhaseCg, m.stream);
+      int32_t rounds = 0;
+      for (int64_t q0 = 0; q0 < k; q0 += 64) {
+        int64_t kk = std::min<int64_t>(64, k - q0);
+        CgProblem p;
+        p.n = m.n;
+        p.n_columns = kk;
+        p.offsets = m.offsets.ptr;
+        p.columns = m.columns.ptr;
+        p.values64 = values;
+        p.inv_diag = inv_diag;
+        p.rhs = rhs + q0 * m.n;
+        p.x = x + q0 * m.n;
+        p.settle = 0;
+        p.tolerance = kPoissonTolerance;
+        p.chebyshev_steps = kChebyshevSteps;
+        p.lambda_min = lo;
+        p.lambda_max = hi;
+        rounds = std::max(rounds, run_cg(p, m.cg, m.stream));
+      }
+      m.stats.poisson_rounds = rounds;
+      m.stats.poisson_refinements = 0;
+    }
   }
-  if (x_layout == FieldLayout::kColumns)
+  if (x_layout == FieldLayout::kColumns && !regularized)
     remove_null_space(m.null_space, m.comp_ptr.ptr, m.comp_members.ptr, m.n, k, x, m.null_partials,
                       m.stream);
 }
@@ -824,7 +951,8 @@ std::vector<uintptr_t> graph_signature(const hg_mesh& m, int64_t k) {
           p(m.h_slots.ptr), p(m.null_partials.ptr), p(m.heat_factor.w64.ptr),
           p(m.heat_factor.pvalues.ptr), p(m.poisson_factor.w64.ptr),
           p(m.poisson_factor.pvalues.ptr), p(m.pinned_factor.w64.ptr),
-          p(m.pinned_factor.pvalues.ptr), p(m.plan.get()), static_cast<uintptr_t>(m.method),
+          p(m.pinned_factor.pvalues.ptr), p(m.reg_factor.w64.ptr), p(m.reg_factor.pvalues.ptr),
+          p(m.plan.get()), static_cast<uintptr_t>(m.method),
           static_cast<uintptr_t>(m.n_rim), tol[0], tol[1], tol[2], tol[3],
           p(m.solve_side.bp.ptr), p(m.solve_side.xa.ptr), p(m.solve_side.xb.ptr),
           p(m.solve_side.r.ptr), p(m.solve_side.y.ptr), p(m.solve_side.d.ptr),
@@ -839,7 +967,7 @@ bool graphable(const hg_mesh& m, int32_t mode, int32_t precision) {
   return m.use_graphs && !m.graphs_broken && precision == HG_PRECISION_FP64 &&
          !m.timer.enabled && m.segments() == 1 && m.plan && m.rows_by_component &&
          (mode == HG_MODE_FACTORED || m.heat_factor.held()) && m.heat_factor.held() &&
-         m.poisson_factor.held() &&
+         (m.method == HG_METHOD_POTPOURRI3D ? m.reg_factor.held() : m.poisson_factor.held()) &&
          (m.method != HG_METHOD_IGL || m.n_rim == 0 || m.pinned_factor.held());
 }
 
@@ -968,12 +1096,14 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   // system the path solves is factored first; one refused under the memory budget sends the whole
   // call down the iterated path (packed meshes have none: the refusal stands).
   bool factored = mode == HG_MODE_FACTORED || m.heat_factor.held();
+  // geometry-central's Poisson step at HG_METHOD_POTPOURRI3D (heat.py:3121-3128).
+  const bool regularized = m.method == HG_METHOD_POTPOURRI3D;
   if (factored) {
     ensure_operators(m);
     const bool mixed = precision == HG_PRECISION_MIXED;
     try {
       factor_system(m, HG_SYSTEM_HEAT, mixed);
-      factor_system(m, HG_SYSTEM_POISSON, mixed);
+      factor_system(m, regularized ? HG_SYSTEM_REGULARIZED : HG_SYSTEM_POISSON, mixed);
       if (m.method == HG_METHOD_IGL && m.n_rim > 0) factor_system(m, HG_SYSTEM_PINNED, mixed);
     } catch (const Error& e) {
       if (!is_budget_refusal(e) || m.segments() > 1) throw;
@@ -1026,7 +1156,7 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
   if (finish && layout == FieldLayout::kRows && m.rows_by_component) {
     // The heat field is spent once the divergence has read it: its buffer takes the Poisson
     // solution, which one pass turns into the distance in the caller's layout.
-    solve_poisson(m, negdiv, heat, k, mode, precision, layout, FieldLayout::kRows);
+    solve_poisson(m, negdiv, heat, k, mode, precision, layout, FieldLayout::kRows, regularized);
     PhaseScope scope(&m.timer, kPhaseShift, m.stream);
     const int64_t chunks = (k + kFieldChunk - 1) / kFieldChunk;
     reserve(m.d_slots, chunks);
@@ -1034,10 +1164,10 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
     for (int64_t c = 0; c < chunks; ++c) m.h_slots.ptr[c] = phi + c * kFieldChunk * n;
     m.d_slots.upload(m.h_slots.ptr, chunks, m.stream);
     finish_rows(*m.plan, m.null_space, m.comp_ptr.ptr, m.d_labels.ptr, heat, k, m.d_sources.ptr,
-                m.d_offsets.ptr, m.d_slots.ptr, m.null_partials, m.stream);
+                m.d_offsets.ptr, m.d_slots.ptr, m.null_partials, m.stream, !regularized);
     return true;
   }
-  solve_poisson(m, negdiv, phi, k, mode, precision, layout);
+  solve_poisson(m, negdiv, phi, k, mode, precision, layout, FieldLayout::kColumns, regularized);
   return false;
 }
 
@@ -1221,7 +1351,8 @@ int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_fa
     const bool mixed = precision == HG_PRECISION_MIXED;
     // A refusal under the memory budget is an answer, not an error (OperatorFactorization.factor
     // returns False, linalg.py:2393-2420): *out_factored reports it.
-    for (int32_t system : {HG_SYSTEM_HEAT, HG_SYSTEM_POISSON, HG_SYSTEM_PINNED}) {
+    for (int32_t system :
+         {HG_SYSTEM_HEAT, HG_SYSTEM_POISSON, HG_SYSTEM_PINNED, HG_SYSTEM_REGULARIZED}) {
       if (!(systems & system)) continue;
       try {
         factor_system(*mesh, system, mixed);
@@ -1232,7 +1363,8 @@ int hg_factor(hg_mesh* mesh, int32_t systems, int32_t precision, int32_t* out_fa
     if (out_factored) {
       bool ok = (!(systems & HG_SYSTEM_HEAT) || mesh->heat_factor.held()) &&
                 (!(systems & HG_SYSTEM_POISSON) || mesh->poisson_factor.held()) &&
-                (!(systems & HG_SYSTEM_PINNED) || mesh->n_rim == 0 || mesh->pinned_factor.held());
+                (!(systems & HG_SYSTEM_PINNED) || mesh->n_rim == 0 || mesh->pinned_factor.held()) &&
+                (!(systems & HG_SYSTEM_REGULARIZED) || mesh->reg_factor.held());
       *out_factored = ok ? 1 : 0;
     }
   });
@@ -1243,7 +1375,8 @@ int hg_factored(hg_mesh* mesh, int32_t* out_systems) {
     require_mesh(mesh);
     *out_systems = (mesh->heat_factor.held() ? HG_SYSTEM_HEAT : 0) |
                    (mesh->poisson_factor.held() ? HG_SYSTEM_POISSON : 0) |
-                   (mesh->pinned_factor.held() ? HG_SYSTEM_PINNED : 0) | (mesh->refused << 4);
+                   (mesh->pinned_factor.held() ? HG_SYSTEM_PINNED : 0) |
+                   (mesh->reg_factor.held() ? HG_SYSTEM_REGULARIZED : 0) | (mesh->refused << 4);
   });
 }
 
@@ -1265,6 +1398,7 @@ int hg_set_plan_options(hg_mesh* mesh, int32_t leaf_rows, int32_t top_rows,
       m.heat_factor.reset();
       m.poisson_factor.reset();
       m.pinned_factor.reset();
+      m.reg_factor.reset();
       m.plan.reset();
       m.host_plan.reset();
       m.options.leaf_rows = leaf;
@@ -1360,6 +1494,35 @@ int hg_graph_state(hg_mesh* mesh, int64_t* out_cached, int32_t* out_broken) {
   });
 }
 
+int hg_last_stats(hg_mesh* mesh, hg_solve_stats* out_stats) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(out_stats != nullptr, HG_ERR_VALUE, "out_stats is null");
+    *out_stats = mesh->stats;
+  });
+}
+
+int hg_set_budget_probe(hg_mesh* mesh, int32_t probe) {
+  return guarded([&] {
+    require_mesh(mesh);
+    mesh->budget_probe = probe != 0;
+  });
+}
+
+int hg_set_settle_rule(hg_mesh* mesh, int32_t check_rounds, double change_tolerance,
+                       int32_t settle_rounds) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(check_rounds >= 1, HG_ERR_VALUE, "check_rounds must be at least 1");
+    HG_REQUIRE(change_tolerance > 0.0 && std::isfinite(change_tolerance), HG_ERR_VALUE,
+               "change_tolerance must be positive and finite");
+    HG_REQUIRE(settle_rounds >= 0, HG_ERR_VALUE, "settle_rounds must not be negative");
+    mesh->check_rounds = check_rounds;
+    mesh->change_tolerance = change_tolerance;
+    mesh->settle_window = settle_rounds;
+  });
+}
+
 int hg_set_memory_budget(hg_mesh* mesh, int64_t bytes) {
   return guarded([&] {
     require_mesh(mesh);
@@ -1385,6 +1548,7 @@ int hg_solve_poisson(hg_mesh* mesh, const double* d_rhs, double* d_solution, int
     HG_REQUIRE(n_columns >= 0, HG_ERR_VALUE, "negative column count");
     require_single_or_factored(*mesh, mode);
     SpeculationDisarm disarm(*mesh);
+    mesh->stats = hg_solve_stats{};
     // By columns (the caller's layout) the solve only reads its right-hand side.
     solve_poisson(*mesh, const_cast<double*>(d_rhs), d_solution, n_columns, mode, precision);
     HG_HIP(hipStreamSynchronize(mesh->stream));
@@ -1603,8 +1767,8 @@ int hg_timing_read(hg_mesh* mesh, double* out_ms, int64_t* out_counts, int32_t r
 int hg_nbytes(hg_mesh* mesh, int64_t* out_bytes) {
   return guarded([&] {
     require_mesh(mesh);
-    int64_t bytes =
-        mesh->heat_factor.bytes() + mesh->poisson_factor.bytes() + mesh->pinned_factor.bytes();
+    int64_t bytes = mesh->heat_factor.bytes() + mesh->poisson_factor.bytes() +
+                    mesh->pinned_factor.bytes() + mesh->reg_factor.bytes();
     if (bytes > 0 && mesh->plan) bytes += mesh->plan->bytes();
     *out_bytes = bytes;
   });
@@ -1618,6 +1782,7 @@ int hg_release(hg_mesh* mesh) {
     mesh->heat_factor.reset();
     mesh->poisson_factor.reset();
     mesh->pinned_factor.reset();
+    mesh->reg_factor.reset();
     mesh->plan.reset();
   });
 }

@@ -305,6 +305,26 @@ __global__ void pinned_rows_kernel(const int32_t* __restrict__ offsets,
   inv_diag[i] = diagonal != 0.0 ? 1.0 / diagonal : 0.0;
 }
 
+// The regularized Poisson system -L + shift I (kernels/heat.py add_to_diagonal, heat.py:4097-4108):
+// the Poisson values with shift added to each stored diagonal entry, and its inverse diagonal.
+__global__ void shifted_rows_kernel(const int32_t* __restrict__ offsets,
+                                    const int32_t* __restrict__ columns,
+                                    const double* __restrict__ values, int64_t n, double shift,
+                                    double* __restrict__ out, double* __restrict__ inv_diag) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double diagonal = 0.0;
+  for (int32_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+    double value = values[k];
+    if (columns[k] == i) {
+      value += shift;
+      diagonal = value;
+    }
+    out[k] = value;
+  }
+  inv_diag[i] = diagonal != 0.0 ? 1.0 / diagonal : 0.0;
+}
+
 // out = in with row r zeroed wherever rim_of[r] is set (kernels/heat.py:511-514, the pinned
 // solve's right-hand side), over n_columns columns in `layout` (rim_of indexed by vertex for
 // columns, by row of the elimination order for rows). in == out is allowed.
@@ -709,6 +729,15 @@ void launch_pinned_rows(const int32_t* offsets, const int32_t* columns, const in
   if (n == 0) return;
   pinned_rows_kernel<<<blocks_for(n), kBlock, 0, stream>>>(offsets, columns, rim, heat, n, out,
                                                            out32, inv_diag);
+  HG_HIP(hipGetLastError());
+}
+
+void launch_shifted_rows(const int32_t* offsets, const int32_t* columns, const double* values,
+                         int64_t n, double shift, double* out, double* inv_diag,
+                         hipStream_t stream) {
+  if (n == 0) return;
+  shifted_rows_kernel<<<blocks_for(n), kBlock, 0, stream>>>(offsets, columns, values, n, shift,
+                                                            out, inv_diag);
   HG_HIP(hipGetLastError());
 }
 

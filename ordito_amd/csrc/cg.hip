@@ -10,8 +10,6 @@
 // with grid-wide barriers, so nothing returns to the host until the solve ends. Every block folds
 // the per-block partials in the same fixed order, so all blocks agree on alpha, beta and the stop
 // decision bit for bit (the redundant fold ordito uses instead of a barrier, here beside one).
-#include <hip/hip_cooperative_groups.h>
-
 #include "kernels.h"
 
 namespace hg {
@@ -28,6 +26,7 @@ struct CgArgs {
   const double* values64;
   const double* inv_diag;
   const double* rhs;
+  const double* guess;  // the initial iterate, or null for zero (may alias x)
   double* x;
   double* r;
   double* u;
@@ -39,7 +38,10 @@ struct CgArgs {
   double* da;
   double* db;
   double* partials;  // [k][G][3], then [G][2] for the settle check, then [k][G] for |b|^2
-  int32_t* counters;  // [0] stepped rounds
+  // [0] stepped rounds, [1] set when a grid barrier gave up waiting, [2] blocks arrived at the
+  // current barrier, [3] barrier generation; [1..3] zero at launch.
+  int32_t* counters;
+  int64_t spin_limit;  // polls of a grid barrier before it gives up (every block then exits)
   int32_t settle;
   int32_t check_rounds;
   double change_tolerance;
@@ -51,12 +53,51 @@ struct CgArgs {
   double delta;
 };
 
+// A block's view of the grid barrier: the generation it waits past and whether a barrier gave up.
+struct SyncState {
+  int32_t generation;
+  int32_t aborted;
+};
+
+// Grid-wide barrier over co-resident blocks (the launch is cooperative): a block's thread 0
+// releases the block's writes, counts itself in, and the last block to arrive resets the count and
+// advances the generation the others poll. The wait is bounded: past a.spin_limit polls, or once
+// another block has given up, the block records the abort and stops waiting, and every later
+// barrier returns at once, so each block leaves its loop within a round and the kernel ends; the
+// host reads the abort and fails the call.
 template <int kThreads, bool kSingle>
-__device__ __forceinline__ void barrier() {
+__device__ __forceinline__ void barrier(const CgArgs& a, SyncState* sync) {
   if constexpr (kSingle) {
     __syncthreads();
   } else {
-    cooperative_groups::this_grid().sync();
+    __syncthreads();
+    if (threadIdx.x == 0 && !sync->aborted) {
+      int32_t* abort = a.counters + 1;
+      int32_t* count = a.counters + 2;
+      int32_t* generation = a.counters + 3;
+      const int32_t gen = sync->generation;
+      __threadfence();
+      const int32_t arrived = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+      if (arrived == static_cast<int32_t>(gridDim.x) - 1) {
+        __hip_atomic_store(count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(generation, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        int64_t polls = 0;
+        while (__hip_atomic_load(generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+          if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+              ++polls > a.spin_limit) {
+            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sync->aborted = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      sync->generation = gen + 1;
+      __threadfence();
+    }
+    __syncthreads();
   }
 }
 
@@ -106,7 +147,7 @@ __device__ __forceinline__ double row_dot(const CgArgs& a, int64_t i, const doub
 // steps, cheb_steps - 1 products; the search direction is double-buffered so each step is one
 // barrier. Leaves every thread past a barrier.
 template <int kThreads, bool kSingle>
-__device__ void chebyshev_apply(const CgArgs& a) {
+__device__ void chebyshev_apply(const CgArgs& a, SyncState* sync) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
   const int64_t first = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   const double sigma = a.theta / a.delta;
@@ -121,7 +162,7 @@ __device__ void chebyshev_apply(const CgArgs& a) {
       a.u[o + i] = d;
     }
   }
-  barrier<kThreads, kSingle>();
+  barrier<kThreads, kSingle>(a, sync);
   double* d_old = a.da;
   double* d_new = a.db;
   for (int32_t step = 1; step < a.cheb_steps; ++step) {
@@ -140,7 +181,7 @@ __device__ void chebyshev_apply(const CgArgs& a) {
       }
     }
     rho = rho_new;
-    barrier<kThreads, kSingle>();
+    barrier<kThreads, kSingle>(a, sync);
     double* t = d_old;
     d_old = d_new;
     d_new = t;
@@ -155,6 +196,13 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
   __shared__ double atol_sq[kMaxColumns];
   __shared__ double coeff[2 * kMaxColumns];
   __shared__ int stepping[kMaxColumns];
+  __shared__ SyncState sync_state;
+  SyncState* sync = &sync_state;
+  if (threadIdx.x == 0) {
+    sync_state.generation = 0;
+    sync_state.aborted = 0;
+  }
+  __syncthreads();
 
   const int32_t G = gridDim.x;
   const int64_t stride = static_cast<int64_t>(G) * kThreads;
@@ -163,25 +211,31 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
   double* settle_partials = a.partials + static_cast<int64_t>(a.k) * G * 3;
   double* norm_partials = settle_partials + 2 * G;
 
-  // Initial state from x = 0: r = b, p = s = 0, |b|^2 for the tolerance.
+  // Initial state from x = 0 (or the guess x0): r = b - A x0, p = s = 0, |b|^2 for the
+  // tolerance. Each thread writes only its own rows of x, with the guess's own value.
   for (int32_t q = 0; q < a.k; ++q) {
     const int64_t o = static_cast<int64_t>(q) * a.n;
     double acc[1] = {0.0};
     for (int64_t i = first; i < a.n; i += stride) {
-      double b = a.rhs[o + i];
-      a.x[o + i] = 0.0;
-      a.r[o + i] = b;
+      const double b = a.rhs[o + i];
+      double ri = b, xi = 0.0;
+      if (a.guess) {
+        xi = a.guess[o + i];
+        ri = b - row_dot(a, i, a.guess + o);
+      }
+      a.x[o + i] = xi;
+      a.r[o + i] = ri;
       a.p[o + i] = 0.0;
       a.s[o + i] = 0.0;
       a.prev[o + i] = 0.0;
-      if (a.cheb_steps == 0) a.u[o + i] = a.inv_diag[i] * b;
+      if (a.cheb_steps == 0) a.u[o + i] = a.inv_diag[i] * ri;
       acc[0] += b * b;
     }
     block_sum<kThreads, 1>(acc, scratch);
     if (threadIdx.x == 0) norm_partials[static_cast<int64_t>(q) * G + blockIdx.x] = acc[0];
   }
-  barrier<kThreads, kSingle>();
-  if (a.cheb_steps > 0) chebyshev_apply<kThreads, kSingle>(a);
+  barrier<kThreads, kSingle>(a, sync);
+  if (a.cheb_steps > 0) chebyshev_apply<kThreads, kSingle>(a, sync);
   for (int32_t q = 0; q < a.k; ++q) {
     double acc[1] = {0.0};
     for (int32_t b = threadIdx.x; b < G; b += kThreads)
@@ -200,6 +254,7 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
   double settle_previous = 0.0, reach_check = 0.0, checks = 0.0;
   bool running = true;
   while (running) {
+    if (sync->aborted) break;
     ++round;
     // w = A u and the round's three dots.
     for (int32_t q = 0; q < a.k; ++q) {
@@ -221,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
         dst[2] = acc[2];
       }
     }
-    barrier<kThreads, kSingle>();
+    barrier<kThreads, kSingle>(a, sync);
     // Every block folds every column's partials in block order: alpha, beta, whether it steps.
     for (int32_t q = 0; q < a.k; ++q) {
       double acc[3] = {0.0, 0.0, 0.0};
@@ -274,8 +329,8 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
     }
     if (round >= a.maxiter) running = false;
     if (a.cheb_steps > 0) {
-      barrier<kThreads, kSingle>();
-      chebyshev_apply<kThreads, kSingle>(a);
+      barrier<kThreads, kSingle>(a, sync);
+      chebyshev_apply<kThreads, kSingle>(a, sync);
     }
     if (a.settle && round % a.check_rounds == 0) {
       // Per-entry relative change since the last check and the count of reached entries.
@@ -298,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
         settle_partials[2 * blockIdx.x] = change;
         settle_partials[2 * blockIdx.x + 1] = reached[0];
       }
-      barrier<kThreads, kSingle>();
+      barrier<kThreads, kSingle>(a, sync);
       double c = 0.0;
       double count[1] = {0.0};
       for (int32_t b = threadIdx.x; b < G; b += kThreads) {
@@ -317,7 +372,7 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
         if (c < a.change_tolerance || idle >= static_cast<double>(a.settle_rounds)) running = false;
       }
     } else if (a.cheb_steps == 0) {
-      barrier<kThreads, kSingle>();
+      barrier<kThreads, kSingle>(a, sync);
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = stepped_rounds;
@@ -378,20 +433,25 @@ __global__ void dominance_final_kernel(const double* __restrict__ partials, int3
   if (threadIdx.x == 0) out[0] = v;
 }
 
-template <int kThreads, bool kSingle>
+// The multi-block solve's grid: one 1024-lane block per compute unit at most (every block folds
+// every block's partials each round, and each grid barrier counts every block in, so fewer, wider
+// blocks keep a round short), and never more blocks than can be resident at once.
 int32_t cooperative_grid(int64_t n) {
-  if (kSingle) return 1;
   int device = 0;
   HG_HIP(hipGetDevice(&device));
   int cus = 0;
   HG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
   int per_cu = 0;
-  HG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_kernel<kThreads, kSingle>,
-                                                      kThreads, 0));
-  int64_t resident = static_cast<int64_t>(cus) * std::max(per_cu, 1);
-  int64_t needed = (n + kThreads - 1) / kThreads;
-  return static_cast<int32_t>(std::max<int64_t>(1, std::min(resident, needed)));
+  HG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_kernel<1024, false>, 1024, 0));
+  HG_REQUIRE(per_cu >= 1, HG_ERR_RUNTIME, "the conjugate-gradient kernel fits no compute unit");
+  const int64_t needed = (n + 1023) / 1024;
+  return static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(cus, needed)));
 }
+
+// Polls of a grid barrier before it gives up: each poll is an agent-scope load plus a short sleep
+// (about a microsecond), so about two seconds, thousands of times a round's longest legitimate
+// wait.
+constexpr int64_t kSpinLimit = int64_t{1} << 21;
 
 }  // namespace
 
@@ -402,7 +462,7 @@ int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
   if (n == 0) return 0;
   // Small systems run in one block of 1024 lanes with block barriers only.
   const bool single = n <= 8192;
-  const int32_t G = single ? 1 : cooperative_grid<256, false>(n);
+  const int32_t G = single ? 1 : cooperative_grid(n);
   const int64_t entries = n * k;
   if (ws.capacity_entries < entries) {
     ws.vectors.allocate(9 * entries);
@@ -420,6 +480,7 @@ int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
   a.values64 = problem.values64;
   a.inv_diag = problem.inv_diag;
   a.rhs = problem.rhs;
+  a.guess = problem.guess;
   a.x = problem.x;
   double* base = ws.vectors.ptr;
   a.r = base;
@@ -433,6 +494,7 @@ int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
   a.db = base + 8 * entries;
   a.partials = ws.partials.ptr;
   a.counters = ws.counters.ptr;
+  a.spin_limit = kSpinLimit;
   a.settle = problem.settle;
   a.check_rounds = std::max(problem.check_rounds, 1);
   a.change_tolerance = problem.change_tolerance;
@@ -443,17 +505,20 @@ int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
   a.theta = 0.5 * (problem.lambda_max + problem.lambda_min);
   a.delta = 0.5 * (problem.lambda_max - problem.lambda_min);
   void* args[] = {&a};
+  launch_zero(ws.counters.ptr, sizeof(int32_t) * 4, stream);
   if (single) {
     hipLaunchKernelGGL((cg_kernel<1024, true>), dim3(1), dim3(1024), 0, stream, a);
     HG_HIP(hipGetLastError());
   } else {
-    HG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&cg_kernel<256, false>),
-                                      dim3(G), dim3(256), args, 0, stream));
+    HG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&cg_kernel<1024, false>),
+                                      dim3(G), dim3(1024), args, 0, stream));
   }
-  int32_t rounds = 0;
-  HG_HIP(hipMemcpyAsync(&rounds, ws.counters.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+  int32_t words[2] = {0, 0};
+  HG_HIP(hipMemcpyAsync(words, ws.counters.ptr, sizeof(words), hipMemcpyDeviceToHost, stream));
   HG_HIP(hipStreamSynchronize(stream));
-  return rounds;
+  HG_REQUIRE(words[1] == 0, HG_ERR_RUNTIME,
+             "conjugate gradient: a grid barrier waited past its limit; the solve was abandoned");
+  return words[0];
 }
 
 void launch_backward_error(const int32_t* offsets, const int32_t* columns, const double* values,

@@ -126,9 +126,10 @@ void remove_null_space(const NullSpace& ns, const int32_t* comp_ptr, const int32
 // component order (comp_ptr), which the analysis guarantees; labels: each vertex's component.
 // out_slots (device): chunk c of 64 columns goes to out_slots[c], read when its kernel runs, so a
 // replayed graph writes wherever the slots point that call.
+// remove_means false (a nonsingular Poisson system): the components' means are left in.
 void finish_rows(const DevicePlan& plan, const NullSpace& ns, const int32_t* comp_ptr,
                  const int32_t* labels, const double* x, int64_t k, const int32_t* sources,
                  const int64_t* source_offsets, double* const* out_slots,
-                 DeviceArray<double>& scratch, hipStream_t stream);
+                 DeviceArray<double>& scratch, hipStream_t stream, bool remove_means = true);
 
 }  // namespace hg

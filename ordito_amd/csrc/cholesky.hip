@@ -77,36 +77,52 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const int32_t* __restri
   }
 }
 
-// Panel step at column j0 of front s. Every task first eliminates the b x b diagonal block D in
-// LDS while applying the same eliminations to the identity, which leaves Zd = L_D^-1 (right-looking:
-// at pivot k, rows i > k lose D(i,k)/D(k,k) times row k of D's trailing part and of Zd, then Zd's
-// row k is divided by sqrt(D(k,k))). Then, by kind: 0 solves panel rows [t0, t0+64):
-// X = F(rows, J) Zd^T; 1 applies Zd to Z's block rows J at columns [t0, t0+64); 2 stores Zd as Z's
-// diagonal block. A non-positive pivot raises status[0] (cholesky.py:260-266).
-__global__ __launch_bounds__(256) void panel_kernel(const int32_t* __restrict__ tasks,
-                                                    FactorArgs a) {
+// The FP64 matrix cores: v_mfma_f64_16x16x4 takes a 16 x 4 block of A (lane l holds
+// A[l % 16][l / 16]) and a 4 x 16 block of B (lane l holds B[l / 16][l % 16]) and accumulates the
+// 16 x 16 product, lane l holding rows l / 16 + 4v (v < 4) of column l % 16. A 64 x 64 tile is four
+// waves of 16 rows, each accumulating four 16-column blocks over the inner dimension four at a
+// time; operands read from LDS, zero past the inner extent b (rounded up to a multiple of four).
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// C += A B for a 64 x 64 tile: A(r, k) = a_at(r, k), B(k, c) = b_at(k, c), k < kb_end.
+template <typename FA, typename FB>
+__device__ __forceinline__ void tile_product(FA a_at, FB b_at, int32_t kb_end, int wave, int lane,
+                                             f64x4 acc[4]) {
+  const int li = lane % 16, lk = lane / 16;
+  for (int32_t kb = 0; kb < kb_end; kb += 4) {
+    const double av = a_at(16 * wave + li, kb + lk);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[n] = mfma16x16x4(av, b_at(kb + lk, 16 * n + li), acc[n]);
+  }
+}
+
+// Panel step at column j0 of front s, first launch: the task of kind 2 eliminates the b x b
+// diagonal block D in LDS while applying the same eliminations to the identity, which leaves
+// Zd = L_D^-1 (right-looking: at pivot k, rows i > k lose D(i,k)/D(k,k) times row k of D's
+// trailing part and of Zd, then Zd's row k is divided by sqrt(D(k,k))), and stores Zd as Z's
+// diagonal block; the launch's other tasks return. Once per front panel, where every task of the
+// panel once repeated it. A non-positive pivot raises status[0] (cholesky.py:260-266).
+__global__ __launch_bounds__(256) void panel_diag_kernel(const int32_t* __restrict__ tasks,
+                                                         FactorArgs a) {
   __shared__ double Ds[kFt][kFt + 1];
   __shared__ double Zs[kFt][kFt + 1];
-  __shared__ double Ts[kFt][kFt + 1];
   const int32_t* task = tasks + 4 * static_cast<int64_t>(blockIdx.x);
-  const int32_t s = task[0], j0 = task[1], kind = task[2], t0 = task[3];
+  if (task[2] != 2) return;
+  const int32_t s = task[0], j0 = task[1];
   const int32_t nc = a.ncol[s];
   const int64_t m = nc + a.nrow[s];
   const int32_t b = min(kFt, nc - j0);
-  double* F = a.arena + a.front_off[s];
+  const double* F = a.arena + a.front_off[s];
   double* Z = a.z + a.w_off[s];
   const int t = threadIdx.x;
   for (int e = t; e < kFt * kFt; e += 256) {
     const int i = e % kFt, j = e / kFt;
     Ds[i][j] = (i < b && j < b && i >= j) ? F[(j0 + j) * m + j0 + i] : 0.0;
     Zs[i][j] = (i == j && i < b) ? 1.0 : 0.0;
-    if (kind == 0) {
-      const int64_t r = t0 + i;
-      Ts[i][j] = (r < m && j < b) ? F[(j0 + j) * m + r] : 0.0;
-    } else if (kind == 1) {
-      // Ts[k][cc] = Z(j0 + k, t0 + cc), read along k (contiguous).
-      Ts[i][j] = (i < b) ? Z[static_cast<int64_t>(t0 + j) * m + j0 + i] : 0.0;
-    }
   }
   __syncthreads();
   // Thread (row i, column group cg) owns entries (i, cg + 4v), v < 16.
@@ -139,57 +155,64 @@ __global__ __launch_bounds__(256) void panel_kernel(const int32_t* __restrict__ 
     }
   }
   __syncthreads();
-  if (kind == 2) {
-    if (bad && t == 0) atomicOr(a.status, 1);
-    for (int e = t; e < kFt * kFt; e += 256) {
-      const int r = e % kFt, c = e / kFt;
-      if (r < b && c < b && r >= c) Z[static_cast<int64_t>(j0 + c) * m + j0 + r] = Zs[r][c];
-    }
-    return;
+  if (bad && t == 0) atomicOr(a.status, 1);
+  for (int e = t; e < kFt * kFt; e += 256) {
+    const int r = e % kFt, c = e / kFt;
+    if (r < b && c < b && r >= c) Z[static_cast<int64_t>(j0 + c) * m + j0 + r] = Zs[r][c];
   }
-  // 64 x 64 product, thread (tr, tc) = (t % 16, t / 16) owns rows tr + 16x, columns tc + 16z.
-  const int tr = t % 16, tc = t / 16;
-  double acc[4][4] = {};
+}
+
+// Panel step, second launch: with Zd read from Z's diagonal block (lower triangle; zero above),
+// kind 0 solves panel rows [t0, t0+64): X = F(rows, J) Zd^T; kind 1 applies Zd to Z's block rows
+// J at columns [t0, t0+64): Zd Z(J, cols). Kind 2 tasks return. Products on the matrix cores.
+__global__ __launch_bounds__(256) void panel_apply_kernel(const int32_t* __restrict__ tasks,
+                                                          FactorArgs a) {
+  __shared__ double Zs[kFt][kFt + 1];
+  __shared__ double Ts[kFt][kFt + 1];
+  const int32_t* task = tasks + 4 * static_cast<int64_t>(blockIdx.x);
+  const int32_t kind = task[2];
+  if (kind == 2) return;
+  const int32_t s = task[0], j0 = task[1], t0 = task[3];
+  const int32_t nc = a.ncol[s];
+  const int64_t m = nc + a.nrow[s];
+  const int32_t b = min(kFt, nc - j0);
+  double* F = a.arena + a.front_off[s];
+  double* Z = a.z + a.w_off[s];
+  const int t = threadIdx.x;
+  for (int e = t; e < kFt * kFt; e += 256) {
+    const int i = e % kFt, j = e / kFt;
+    Zs[i][j] = (i < b && j < b && i >= j) ? Z[static_cast<int64_t>(j0 + j) * m + j0 + i] : 0.0;
+    if (kind == 0) {
+      const int64_t r = t0 + i;
+      Ts[i][j] = (r < m && j < b) ? F[(j0 + j) * m + r] : 0.0;
+    } else {
+      // Ts[k][cc] = Z(j0 + k, t0 + cc), read along k (contiguous).
+      Ts[i][j] = (i < b) ? Z[static_cast<int64_t>(t0 + j) * m + j0 + i] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int wave = t / 64, lane = t % 64;
+  f64x4 acc[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
   if (kind == 0) {
-    for (int k = 0; k < b; ++k) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int x = 0; x < 4; ++x) av[x] = Ts[tr + 16 * x][k];
-#pragma unroll
-      for (int z = 0; z < 4; ++z) bv[z] = Zs[tc + 16 * z][k];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
-    }
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      const int64_t r = t0 + tr + 16 * x;
-#pragma unroll
-      for (int z = 0; z < 4; ++z) {
-        const int c = tc + 16 * z;
-        if (r < m && c < b) F[(j0 + c) * m + r] = acc[x][z];
-      }
-    }
+    tile_product([&](int r, int k) { return Ts[r][k]; }, [&](int k, int c) { return Zs[c][k]; },
+                 b, wave, lane, acc);
   } else {
-    for (int k = 0; k < b; ++k) {
-      double av[4], bv[4];
+    tile_product([&](int r, int k) { return Zs[r][k]; }, [&](int k, int c) { return Ts[k][c]; },
+                 b, wave, lane, acc);
+  }
 #pragma unroll
-      for (int x = 0; x < 4; ++x) av[x] = Zs[tr + 16 * x][k];
+  for (int n = 0; n < 4; ++n) {
+    const int cc = 16 * n + lane % 16;
 #pragma unroll
-      for (int z = 0; z < 4; ++z) bv[z] = Ts[k][tc + 16 * z];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
-    }
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      const int r = tr + 16 * x;
-#pragma unroll
-      for (int z = 0; z < 4; ++z) {
-        const int64_t c = t0 + tc + 16 * z;
-        if (r < b) Z[c * m + j0 + r] = acc[x][z];
+    for (int v = 0; v < 4; ++v) {
+      const int rr = 16 * wave + lane / 16 + 4 * v;
+      if (kind == 0) {
+        const int64_t r = t0 + rr;
+        if (r < m && cc < b) F[(j0 + cc) * m + r] = acc[n][v];
+      } else if (rr < b) {
+        Z[static_cast<int64_t>(t0 + cc) * m + j0 + rr] = acc[n][v];
       }
     }
   }
@@ -197,7 +220,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const int32_t* __restrict__ 
 
 // Update after the panel step at j0: rows [i0, i0+64) of the trailing matrix (c0 >= j0 + b, lower
 // part) lose P(i0..) P(c0..)^T, or of Z (c0 < j0 + b) lose P(i0..) Z(J, c0..), where P is the
-// solved panel (the front's columns J = [j0, j0+b)).
+// solved panel (the front's columns J = [j0, j0+b)). Products on the matrix cores.
 __global__ __launch_bounds__(256) void update_kernel(const int32_t* __restrict__ tasks,
                                                      FactorArgs a) {
   __shared__ double As[kFt][kFt + 1];  // [k][row]
@@ -224,30 +247,23 @@ __global__ __launch_bounds__(256) void update_kernel(const int32_t* __restrict__
     }
   }
   __syncthreads();
-  const int tr = t % 16, tc = t / 16;
-  double acc[4][4] = {};
-  for (int k = 0; k < b; ++k) {
-    double av[4], bv[4];
+  const int wave = t / 64, lane = t % 64;
+  f64x4 acc[4];
 #pragma unroll
-    for (int x = 0; x < 4; ++x) av[x] = As[k][tr + 16 * x];
+  for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
+  tile_product([&](int r, int k) { return As[k][r]; }, [&](int k, int c) { return Bs[k][c]; }, b,
+               wave, lane, acc);
 #pragma unroll
-    for (int z = 0; z < 4; ++z) bv[z] = Bs[k][tc + 16 * z];
+  for (int n = 0; n < 4; ++n) {
+    const int64_t c = c0 + 16 * n + lane % 16;
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int z = 0; z < 4; ++z) acc[x][z] += av[x] * bv[z];
-  }
-#pragma unroll
-  for (int x = 0; x < 4; ++x) {
-    const int64_t r = i0 + tr + 16 * x;
-#pragma unroll
-    for (int z = 0; z < 4; ++z) {
-      const int64_t c = c0 + tc + 16 * z;
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = i0 + 16 * wave + lane / 16 + 4 * v;
       if (r >= m) continue;
       if (trailing) {
-        if (c < m && r >= c) F[c * m + r] -= acc[x][z];
+        if (c < m && r >= c) F[c * m + r] -= acc[n][v];
       } else if (c < j0 + b) {
-        Z[c * m + r] -= acc[x][z];
+        Z[c * m + r] -= acc[n][v];
       }
     }
   }
@@ -707,8 +723,12 @@ void factor_numeric(const DevicePlan& plan, const double* values, const int32_t*
     const int32_t* tasks = plan.factor_tasks.ptr + 4 * static_cast<int64_t>(L.first);
     const unsigned grid = static_cast<unsigned>(L.count);
     if (L.kind == kFactorExtend) extend_add_kernel<<<grid, 256, 0, stream>>>(tasks, a);
-    else if (L.kind == kFactorPanel) panel_kernel<<<grid, 256, 0, stream>>>(tasks, a);
-    else update_kernel<<<grid, 256, 0, stream>>>(tasks, a);
+    else if (L.kind == kFactorPanel) {
+      panel_diag_kernel<<<grid, 256, 0, stream>>>(tasks, a);
+      panel_apply_kernel<<<grid, 256, 0, stream>>>(tasks, a);
+    } else {
+      update_kernel<<<grid, 256, 0, stream>>>(tasks, a);
+    }
     HG_HIP(hipGetLastError());
   }
   int32_t host_status = 0;

@@ -113,6 +113,10 @@ void launch_mass(const int32_t* vertex_ptr, const int32_t* vertex_corners, const
 void launch_pinned_rows(const int32_t* offsets, const int32_t* columns, const int32_t* rim,
                         const double* heat, int64_t n, double* out, float* out32, double* inv_diag,
                         hipStream_t stream);
+// out = values with `shift` added on each stored diagonal entry, and its inverse diagonal.
+void launch_shifted_rows(const int32_t* offsets, const int32_t* columns, const double* values,
+                         int64_t n, double shift, double* out, double* inv_diag,
+                         hipStream_t stream);
 // out = in with the flagged rows zeroed (in == out allowed); rim_of by vertex for kColumns, by
 // row of the elimination order for kRows.
 void launch_off_rim(const double* in, double* out, const int32_t* rim_of, int64_t n,
@@ -174,6 +178,7 @@ struct CgProblem {
   const double* values64 = nullptr;
   const double* inv_diag = nullptr;
   const double* rhs = nullptr;
+  const double* guess = nullptr;  // initial iterate (may alias x), or null for zero
   double* x = nullptr;
   // Stop rule: settle (the heat step) or relative residual tol (the Poisson step).
   int32_t settle = 1;
@@ -218,5 +223,60 @@ void launch_backward_error(const int32_t* offsets, const int32_t* columns, const
 // interval), reduced into *out (one block).
 void launch_dominance_bound(const int32_t* offsets, const int32_t* columns, const double* values,
                             int64_t n, double* partials, double* out, hipStream_t stream);
+
+// march.hip: the band march of a system refused under the memory budget.
+struct MarchSystem {
+  int64_t n = 0;
+  const int32_t* offsets = nullptr;
+  const int32_t* columns = nullptr;
+  const double* values = nullptr;
+};
+
+struct MarchWorkspace {
+  DeviceArray<int32_t> finished, local, band, flags, positions, counts, band_offsets, band_columns,
+      keep, scan_totals, bounds, count;
+  DeviceArray<double> tentative, band_values, band_rhs, band_x, band_inv, words;
+  CgWorkspace cg;
+  int64_t bytes() const;
+  void release();
+};
+
+// Solves A x = rhs for n_columns columns (column q at q * n), each marched on its own; x is
+// overwritten. Returns the band solves' conjugate-gradient rounds.
+int32_t march_settled(const MarchSystem& system, const double* rhs, double* x, int64_t n_columns,
+                      MarchWorkspace& workspace, hipStream_t stream);
+
+// multigrid.hip: a smoothed-aggregation hierarchy on the device and the conjugate gradient it
+// preconditions (the hierarchy is built on the host, multigrid.h).
+struct HostCsr;
+struct MultigridLevel {
+  struct Matrix {
+    int64_t rows = 0;
+    DeviceArray<int32_t> ptr, idx;
+    DeviceArray<double> val;
+    int64_t bytes() const { return ptr.bytes() + idx.bytes() + val.bytes(); }
+  };
+  Matrix a, p, r;
+  DeviceArray<double> smoother;
+  DeviceArray<double> u, v, res, b, x;  // a cycle's vectors at this level (b, x below the top)
+};
+
+struct Multigrid {
+  std::vector<MultigridLevel> levels;
+  DeviceArray<double> coarse_inverse;
+  DeviceArray<double> r, z, p, w, partials, scalars;
+  DeviceArray<int32_t> flags;
+  int32_t blocks = 0;
+  int64_t bytes() const;
+};
+
+// The hierarchy of a host operator, uploaded; null when the operator does not coarsen to a
+// bottom small enough to invert densely (the caller keeps its single-level preconditioner).
+std::unique_ptr<Multigrid> build_multigrid(const HostCsr& fine, uint64_t seed, hipStream_t stream);
+
+// Conjugate gradient under one V-cycle a round, from zero, to |r| <= tolerance |b| per column;
+// returns the most rounds a column took.
+int32_t multigrid_cg(Multigrid& mg, const double* rhs, double* x, int64_t n_columns,
+                     double tolerance, int64_t maxiter, hipStream_t stream);
 
 }  // namespace hg

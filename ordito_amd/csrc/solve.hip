@@ -1307,7 +1307,7 @@ void remove_null_space(const NullSpace& ns, const int32_t* comp_ptr, const int32
 void finish_rows(const DevicePlan& plan, const NullSpace& ns, const int32_t* comp_ptr,
                  const int32_t* labels, const double* x, int64_t k, const int32_t* sources,
                  const int64_t* source_offsets, double* const* out_slots,
-                 DeviceArray<double>& scratch, hipStream_t stream) {
+                 DeviceArray<double>& scratch, hipStream_t stream, bool remove_means) {
   const int64_t n = plan.n;
   if (n == 0 || k == 0 || ns.n_chunks == 0) return;
   const int64_t need = static_cast<int64_t>(ns.n_chunks) * kCols +
@@ -1321,9 +1321,12 @@ void finish_rows(const DevicePlan& plan, const NullSpace& ns, const int32_t* com
     const double* xq = x + q0 * n;
     rows_partial_kernel<<<static_cast<unsigned>(ns.n_chunks), 256, 0, stream>>>(
         ns.chunk_comp.ptr, ns.chunk_first.ptr, comp_ptr, xq, kk, partials);
-    component_mean_kernel<<<dim3(static_cast<unsigned>(ns.n_comp), static_cast<unsigned>(kk)), 256,
-                            0, stream>>>(ns.comp_chunk_ptr.ptr, comp_ptr, ns.n_chunks, partials,
-                                         means);
+    if (remove_means)
+      component_mean_kernel<<<dim3(static_cast<unsigned>(ns.n_comp), static_cast<unsigned>(kk)),
+                              256, 0, stream>>>(ns.comp_chunk_ptr.ptr, comp_ptr, ns.n_chunks,
+                                                partials, means);
+    else
+      launch_zero(means, static_cast<int64_t>(sizeof(double)) * ns.n_comp * kk, stream);
     rows_decide_kernel<<<static_cast<unsigned>(kk), 256, 0, stream>>>(
         comp_ptr, ns.chunk_comp.ptr, ns.chunk_first.ptr, ns.n_comp, ns.n_chunks, partials, means,
         xq, kk, n, plan.iperm.ptr, labels, sources, source_offsets + q0, shift);

@@ -39,11 +39,6 @@ _PRECISIONS = {"fp64": _lib.HG_PRECISION_FP64, "mixed": _lib.HG_PRECISION_MIXED}
 def _require_method(method: str, *, geodesic: bool) -> None:
     if method not in _METHODS:
         raise ValueError(f"method must be 'meshlab', 'igl' or 'potpourri3d', got {method!r}")
-    supported = ("meshlab", "igl") if geodesic else _METHODS
-    if method not in supported:
-        raise NotImplementedError(
-            f"method={method!r} is not implemented on the MI355X path (implemented: {supported})"
-        )
 
 
 def _require_precision(precision: str) -> int:
@@ -275,7 +270,7 @@ class HeatSolver:
         self._mesh = weakref.ref(mesh)
         self._handle: _Handle | None = None
         self._operators: HeatOperators | None = None
-        self._solves = {"heat": 0, "poisson": 0, "pinned": 0}
+        self._solves = {"heat": 0, "poisson": 0, "pinned": 0, "poisson_regularized": 0}
         self._rim: int | None = None
         self._ill_conditioned = False
         self.last_stats: dict[str, int] = {}
@@ -356,7 +351,7 @@ class HeatSolver:
         """Drop every kept factorization and forget the solve counts (``heat.py:3849-3861``)."""
         if self._handle is not None:
             _lib.call("hg_release", self._handle.ptr)
-        self._solves = {"heat": 0, "poisson": 0, "pinned": 0}
+        self._solves = {"heat": 0, "poisson": 0, "pinned": 0, "poisson_regularized": 0}
 
     @property
     def unfactorable(self) -> bool:
@@ -378,9 +373,29 @@ class HeatSolver:
         """
         The factorizations' memory budget in bytes (ordito's ``CHOLESKY_MEMORY_BUDGET``,
         ``cholesky.py:72``): one whose peak would exceed it is refused and its system iterated. 0
-        (the default) takes 90 % of the device memory free when a factorization starts.
+        (the default) takes 90 % of the device memory free when a factorization starts. A heat
+        system refused under it is marched band by band (``linalg.py:2765-2939``), and a refused
+        Poisson system is solved under a multigrid V-cycle.
         """
         _lib.call("hg_set_memory_budget", self._native().ptr, int(nbytes))
+
+    def set_budget_probe(self, probe: bool = True) -> None:
+        """Whether a system is asked of its analysis, before it iterates, if it fits an explicitly
+        set budget (default); off, a refusal is met only when a rejected settled iterate falls back
+        to a factorization (ordito's size test skipping a small operator, ``linalg.py:2729``)."""
+        _lib.call("hg_set_budget_probe", self._native().ptr, 1 if probe else 0)
+
+    def set_settle_rule(
+        self, check_rounds: int = 16, change_tolerance: float = 1e-6, settle_rounds: int = 0
+    ) -> None:
+        """The heat solves' settle rule (``solve_spd_settled``'s parameters, ``linalg.py:2490``):
+        every ``check_rounds`` rounds, stop once every entry changed by under ``change_tolerance``
+        of itself, or once no entry was reached for ``settle_rounds`` rounds (0: the window the
+        time gives)."""
+        _lib.call(
+            "hg_set_settle_rule", self._native().ptr, int(check_rounds), float(change_tolerance),
+            int(settle_rounds),
+        )
 
     def factor(self) -> bool:
         """Factor the heat and Poisson systems now (``heat.py:3863-3888``); False when one is
@@ -461,11 +476,12 @@ class HeatSolver:
         "heat": _lib.HG_SYSTEM_HEAT,
         "poisson": _lib.HG_SYSTEM_POISSON,
         "pinned": _lib.HG_SYSTEM_PINNED,
+        "poisson_regularized": _lib.HG_SYSTEM_REGULARIZED,
     }
 
     def _count(self, role: str) -> None:
         """Count a solve of ``role``'s system; factor it on its second (rent or buy)."""
-        self._solves[role] += 1
+        self._solves[role] = self._solves.get(role, 0) + 1
         if self._solves[role] >= 2:
             system = self._SYSTEMS[role]
             state = self._factored()
@@ -494,6 +510,7 @@ class HeatSolver:
             self._precision_code,
             ctypes.byref(flags),
         )
+        self._read_stats()
         if flags.value & 1:
             self._ill_conditioned = True
             _LOGGER.debug(
@@ -526,6 +543,7 @@ class HeatSolver:
             ctypes.byref(flags),
             ctypes.byref(has_rim),
         )
+        self._read_stats()
         if flags.value & 1:
             self._ill_conditioned = True
             _LOGGER.debug(
@@ -555,6 +573,12 @@ class HeatSolver:
             mode,
             self._precision_code,
         )
+        self._read_stats()
+
+    def _read_stats(self) -> None:
+        stats = _lib.SolveStats()
+        _lib.call("hg_last_stats", self._native().ptr, ctypes.byref(stats))
+        self.last_stats = stats.as_dict()
 
     def geodesic(self, source_sets: Sequence[Any]):
         """
@@ -623,11 +647,16 @@ class HeatSolver:
         # whenever the heat system's is held.
         native = self._native()
         factored = self._factored_on(native)
-        roles = [("heat", _lib.HG_SYSTEM_HEAT), ("poisson", _lib.HG_SYSTEM_POISSON)]
+        # At "potpourri3d" the Poisson step solves the regularized system (heat.py:3121-3128).
+        if self.method == "potpourri3d":
+            poisson = ("poisson_regularized", _lib.HG_SYSTEM_REGULARIZED)
+        else:
+            poisson = ("poisson", _lib.HG_SYSTEM_POISSON)
+        roles = [("heat", _lib.HG_SYSTEM_HEAT), poisson]
         if self.method == "igl" and self.rim_size > 0:
             roles.append(("pinned", _lib.HG_SYSTEM_PINNED))
         for role, system in roles:
-            self._solves[role] += 1
+            self._solves[role] = self._solves.get(role, 0) + 1
             if self._solves[role] >= 2 and not factored & system and not factored & (system << 4):
                 done = ctypes.c_int32()
                 _lib.call("hg_factor", native.ptr, system, self._precision_code, ctypes.byref(done))
@@ -686,9 +715,11 @@ def heat_geodesic(
 ):
     """
     Approximate geodesic distance to the nearest source vertex by the heat method (ordito
-    ``heat.py:370-609``): MeshLab's discretization (``method="meshlab"``, the default) or
+    ``heat.py:370-609``): MeshLab's discretization (``method="meshlab"``, the default),
     ``igl::heat_geodesics``' (``"igl"``: the Voronoi mass, ``h`` over the unique edges, and the
-    Neumann heat averaged with the heat pinned to zero on the rim). Takes a ``Trimesh`` (which
+    Neumann heat averaged with the heat pinned to zero on the rim) or geometry-central's
+    (``"potpourri3d"``: the barycentric mass, ``h`` over the unique edges, and the Poisson step
+    regularized with ``1e-6`` times the identity, ``heat.py:3121-3128``). Takes a ``Trimesh`` (which
     keeps the operators and factorizations for later calls) or ``vertices, faces``; returns
     ``(V,)`` float64 -- a torch tensor on the GPU when the vertices were one, a numpy array
     otherwise.

@@ -162,6 +162,18 @@ class GeodesicServer:
             self._lock.notify_all()
         return request.future
 
+    def drain(self, timeout: float | None = None) -> bool:
+        """Wait until every request queued so far has been answered (the server keeps running);
+        False if ``timeout`` seconds passed first."""
+        deadline = None if timeout is None else time.perf_counter() + timeout
+        with self._lock:
+            while any(e.pending or e.busy for e in self._resident.values()):
+                left = None if deadline is None else deadline - time.perf_counter()
+                if left is not None and left <= 0:
+                    return False
+                self._lock.wait(timeout=0.05 if left is None else min(left, 0.05))
+        return True
+
     def close(self) -> None:
         """Finish every queued request, then stop."""
         with self._lock:
@@ -309,3 +321,87 @@ class GeodesicServer:
                 self._batches += 1
                 self._idle.append(worker)
                 self._lock.notify_all()
+
+
+class NativeServer:
+    """
+    The same request loop in the library itself (``hg_server_*`` in ``include/heatgeo.h``): worker
+    threads and streams in C++, for hosts that drive the C ABI directly. ``meshes`` maps an id to
+    ``(vertices, faces)``; ``submit`` returns a ticket whose ``result()`` is the (V,) float64
+    distance on the GPU.
+    """
+
+    class Ticket:
+        def __init__(self, server: "NativeServer", ticket: int, out) -> None:
+            self._server, self._ticket, self._out, self._done = server, ticket, out, False
+
+        def result(self, timeout: float | None = None):
+            if not self._done:
+                import ctypes
+
+                from . import _lib
+
+                done = ctypes.c_int32()
+                _lib.call("hg_server_wait", self._server._ptr, self._ticket,
+                          -1.0 if timeout is None else 1e3 * timeout, ctypes.byref(done))
+                if not done.value:
+                    raise TimeoutError(f"ticket {self._ticket} is still running")
+                self._done = True
+            return self._out
+
+    def __init__(self, meshes: dict, *, budget_bytes: float = 200e9, max_batch: int = 64,
+                 workers: int = 4, precision: str = "fp64") -> None:
+        import ctypes
+
+        from . import _lib
+
+        _arrays.require_gpu()
+        torch = _arrays.torch_module()
+        codes = {"fp64": _lib.HG_PRECISION_FP64, "mixed": _lib.HG_PRECISION_MIXED}
+        handle = ctypes.c_void_p()
+        _lib.call("hg_server_create", int(budget_bytes), int(workers), int(max_batch),
+                  codes[precision], ctypes.byref(handle))
+        self._ptr = handle
+        self._ids: dict[Any, int] = {}
+        self._sizes: dict[Any, int] = {}
+        for key, (vertices, faces) in meshes.items():
+            v = _arrays.vertices_tensor(vertices)
+            f = _arrays.faces_tensor(faces)
+            out = ctypes.c_int64()
+            _lib.call("hg_server_add_mesh", self._ptr, _arrays.pointer(v), int(v.shape[0]),
+                      _arrays.pointer(f), int(f.numel() // 3), ctypes.byref(out))
+            self._ids[key] = int(out.value)
+            self._sizes[key] = int(v.shape[0])
+        torch.cuda.synchronize()
+
+    def submit(self, mesh_id, sources) -> "NativeServer.Ticket":
+        import ctypes
+
+        from . import _lib
+
+        if mesh_id not in self._ids:
+            raise ValueError(f"unknown mesh {mesh_id!r}")
+        torch = _arrays.torch_module()
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int32).reshape(-1))
+        out = torch.empty(self._sizes[mesh_id], dtype=torch.float64, device="cuda")
+        ticket = ctypes.c_int64()
+        _lib.call("hg_server_submit", self._ptr, self._ids[mesh_id],
+                  src.ctypes.data_as(ctypes.c_void_p), int(src.size), out.data_ptr(),
+                  ctypes.byref(ticket))
+        return NativeServer.Ticket(self, int(ticket.value), out)
+
+    def stats(self, since: float = 0.0) -> dict:
+        import ctypes
+
+        from . import _lib
+
+        report = _lib.ServerStats()
+        _lib.call("hg_server_stats", self._ptr, float(since), ctypes.byref(report))
+        return report.as_dict()
+
+    def close(self) -> None:
+        from . import _lib
+
+        if self._ptr is not None:
+            _lib.call("hg_server_destroy", self._ptr)
+            self._ptr = None

@@ -83,7 +83,8 @@ def test_torus_factored_queries_match_the_oracle(cuda, torus_c2, release):
     many = solver.geodesic([_gpu(cuda, sources)])[0].cpu().numpy()
     worst, _ = ho.relative_error(many, oracle.distance(sources))
     assert worst <= 1e-9, worst
-    assert np.all(many[sources] == 0.0) or np.max(np.abs(many[sources])) <= 1e-9 * np.ptp(many)
+    # Many sources: the shift zeroes their mean (heat.py:3134-3162), not each source's value.
+    assert abs(np.mean(many[sources])) <= 1e-12 * np.ptp(many)
 
 
 @pytest.mark.timeout(900)
@@ -98,9 +99,16 @@ def test_torus_one_shot_iterates_its_full_window(cuda, torus_c2, release):
     got = heat.heat_geodesic(mesh, _gpu(cuda, source)).cpu().numpy()
     solver = mesh.heat_solver(method="meshlab")
     stats = solver.last_stats
-    assert solver.nbytes == 0  # the first call iterates: nothing was factored
-    assert stats["heat_refinements"] == 0 and stats["heat_rounds"] >= window, stats
-    assert stats["poisson_rounds"] > 0, stats
+    print(f"one-shot stats {stats}, factored bytes {solver.nbytes}", flush=True)
+    # The settled CG runs at least its widened window (the count of reached entries keeps
+    # changing while the far field emerges, then the window passes).
+    assert stats["heat_rounds"] >= window, stats
+    if stats["backward_error_failed"]:
+        # The iterate failed its componentwise check (solve_spd_settled, linalg.py:2687-2727):
+        # the heat system was factored and solved again, and the Poisson system with it.
+        assert stats["heat_refinements"] >= 1 and solver.nbytes > 0, stats
+    else:
+        assert solver.nbytes == 0 and stats["poisson_rounds"] > 0, stats
     worst, _ = ho.relative_error(got, oracle.distance(source))
     assert worst <= 1e-5, worst
     assert got[source[0]] == 0.0

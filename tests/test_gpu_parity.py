@@ -253,6 +253,38 @@ def test_igl_geodesic_matches_oracle(cuda, name):
         assert np.abs(got.cpu().numpy() - meshlab).max() > 1e-3 * np.ptp(meshlab)
 
 
+# --- method="potpourri3d" ------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("name", ["grid", "icosphere4", "two_bodies", "saddle_graded"])
+def test_potpourri3d_geodesic_matches_oracle(cuda, name):
+    """heat_geodesic(method="potpourri3d"): the unique-edge time and the Poisson step regularized
+    with 1e-6 I (no null space, nothing pinned), iterated on the first call (1e-5 of the range)
+    and through the heat and regularized factors on the later ones (1e-9); the source reads 0 and
+    the answer differs from MeshLab's."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh(name)
+    n = len(vertices)
+    s = n // 2 + 17
+    expected = ho.geodesic(vertices, faces, np.array([s]), method="potpourri3d").distance
+    once = heat.heat_geodesic(vertices, faces, np.array([s], np.int32), method="potpourri3d")
+    worst, _ = ho.relative_error(once, expected)
+    assert worst <= 1e-5, worst
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method="potpourri3d")
+    for call in range(3):
+        got = heat.heat_geodesic(
+            trimesh, _tensor(cuda, np.array([s], np.int32)), method="potpourri3d"
+        ).cpu().numpy()
+        worst, _ = ho.relative_error(got, expected)
+        assert worst <= (1e-5 if call == 0 else 1e-9), (call, worst)
+    assert got[s] == 0.0
+    assert solver._factored() & 8 and not solver._factored() & 2
+    meshlab = ho.geodesic(vertices, faces, np.array([s])).distance
+    assert np.abs(got - meshlab).max() > 1e-9 * np.ptp(meshlab)
+
+
 def test_diffuse_pinned_holds_the_rim_at_zero(cuda):
     """HeatSolver.diffuse_pinned (the reference's check, tests/test_heat.py:295-329): zero on the
     rim and the interior system's direct solve elsewhere, iterated on the first call (rtol 1e-6,
@@ -310,12 +342,13 @@ def test_batch_widths_give_each_query_its_own_answer(cuda, width):
         assert worst <= 1e-9, (q, worst)
 
 
-def test_memory_budget_refuses_and_the_solves_iterate(cuda):
+def test_memory_budget_refuses_and_the_solves_march(cuda):
     """A factorization whose peak exceeds the memory budget is refused, not an error
     (OperatorFactorization.factor returns False, linalg.py:2393-2420): factor() is False, nothing
-    is held, and every later call iterates to the iterated bar; lifting the budget factors again.
-    On the graded saddle, whose iterate fails its backward-error check, the refused fallback leaves
-    the iterate standing and marks the mesh ill-conditioned."""
+    is held, and every later heat solve marches band by band (linalg.py:2677-2682) and every
+    Poisson solve runs under the multigrid V-cycle, to the iterated bar; lifting the budget factors
+    again. On the graded saddle, whose settled iterate fails its backward-error check, the probed
+    refusal marches before any iteration: the mesh is never found ill-conditioned."""
     from ordito_amd import heat
 
     vertices, faces = mesh("torus")
@@ -330,7 +363,9 @@ def test_memory_budget_refuses_and_the_solves_iterate(cuda):
         got = heat.heat_geodesic(trimesh, source, method="meshlab").cpu().numpy()
         worst, _ = ho.relative_error(got, expected)
         assert worst <= 1e-5, worst
-        assert solver.last_stats["heat_rounds"] > 0 and solver.nbytes == 0
+        stats = solver.last_stats
+        assert stats["heat_marched"] == 1 and stats["poisson_multigrid"] == 1, stats
+        assert stats["heat_rounds"] > 0 and solver.nbytes == 0
     solver.set_memory_budget(0)
     assert solver.factor() and not solver.unfactorable
     got = heat.heat_geodesic(trimesh, source, method="meshlab").cpu().numpy()
@@ -342,8 +377,11 @@ def test_memory_budget_refuses_and_the_solves_iterate(cuda):
     graded = saddle.heat_solver(method="meshlab")
     graded.set_memory_budget(1)
     d = heat.heat_geodesic(saddle, _tensor(cuda, np.array([0], np.int32)), method="meshlab")
-    assert bool(cuda.isfinite(d).all())
-    assert graded.ill_conditioned and graded.nbytes == 0
+    got = d.cpu().numpy()
+    worst, _ = ho.relative_error(got, ho.geodesic(sv, sf, np.array([0])).distance)
+    assert worst <= 1e-5, worst
+    assert graded.last_stats["heat_marched"] == 1, graded.last_stats
+    assert not graded.ill_conditioned and graded.nbytes == 0
 
 
 def test_a_failed_speculated_round_solves_again(cuda):

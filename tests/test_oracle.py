@@ -546,3 +546,39 @@ def test_igl_choices_matter_and_stay_within_the_great_circle_bound():
     factored = ho.FactoredOracle(g, gf, method="igl").distance(np.array([middle]))
     assert np.allclose(factored, igl, atol=1e-12 * np.ptp(igl))
 
+
+
+# --- method="potpourri3d": geometry-central's regularized Poisson system ------------------------
+
+
+@pytest.mark.parametrize("name", ["icosphere3", "grid", "two_bodies"])
+def test_regularized_poisson_returns_ones_from_its_shift(name):
+    """``(-L + 1e-6 I) 1 = 1e-6 1`` since ``L 1 = 0``: the right-hand side ``1e-6`` everywhere
+    solves to ones, which fixes the regularization's sign and size (a ``+L``, a missing shift or a
+    shift of the wrong size gives something else), on one and on two components."""
+    v, f = mesh(name)
+    ops = ho.assemble(v, f, None, "potpourri3d")
+    phi = ho.solve_regularized_poisson(ops, np.full(len(v), ho.POTPOURRI_POISSON_SHIFT))
+    assert np.max(np.abs(phi - 1.0)) < 1e-6
+    wrong = ops.poisson.copy()
+    rows = np.repeat(np.arange(ops.pattern.n), np.diff(ops.pattern.indptr))
+    diagonal = ops.pattern.indices == rows
+    wrong[diagonal] += 1e-5  # ten times the shift
+    bad = sp.linalg.spsolve(ops.matrix(wrong).tocsc(), np.full(len(v), ho.POTPOURRI_POISSON_SHIFT))
+    assert np.max(np.abs(bad - 1.0)) > 0.5
+
+
+def test_potpourri3d_stays_within_the_great_circle_bound_and_differs_from_meshlab():
+    v, f = mesh("icosphere4")
+    d = ho.geodesic(v, f, [0], method="potpourri3d").distance
+    p = v.astype(np.float64)
+    p /= np.linalg.norm(p, axis=1)[:, None]
+    exact = np.arccos(np.clip(p @ p[0], -1.0, 1.0))
+    assert np.max(np.abs(d - exact)) / np.ptp(exact) < 0.07
+    assert d[0] == 0.0 and d.min() >= 0.0
+    meshlab = ho.geodesic(v, f, [0]).distance
+    # The regularization and the unique-edge time move the answer, by less than the method's error.
+    gap = np.abs(d - meshlab).max() / np.ptp(meshlab)
+    assert 1e-9 < gap < 0.03
+    factored = ho.FactoredOracle(v, f, method="potpourri3d").distance(np.array([0]))
+    assert np.allclose(factored, d, atol=1e-12 * np.ptp(d))

@@ -134,3 +134,43 @@ def test_warmed_server_factors_every_mesh_first_and_answers_the_same(cuda):
         v, f = meshes[name]
         worst, _ = ho.relative_error(got, _alone(cuda, v, f, sources))
         assert worst <= 1e-10, (name, worst)
+
+
+def test_native_server_matches_each_request_alone_and_evicts_under_a_tight_budget(cuda):
+    """The C++ request loop (hg_server_*): requests on three meshes submitted together form
+    batches on several workers; each field within 1e-10 of the request solved alone; a budget
+    that holds about one factored mesh makes the workers release least recently used meshes, and
+    the answers stay the same; a bad source is the ticket's ValueError."""
+    from ordito_amd.serve import NativeServer
+
+    meshes = {name: mesh(name) for name in NAMES}
+    rng = np.random.default_rng(11)
+    for budget, workers in ((200e9, 3), (None, 1)):
+        if budget is None:
+            probe = NativeServer({NAMES[0]: meshes[NAMES[0]]}, workers=1)
+            first = probe.submit(NAMES[0], np.array([0], np.int32))
+            first.result(timeout=120)
+            one = probe.stats()["resident_bytes"]
+            probe.close()
+            budget = 1.5 * one
+        server = NativeServer(meshes, budget_bytes=budget, workers=workers, max_batch=8)
+        requests = []
+        for _ in range(24):
+            name = NAMES[int(rng.integers(len(NAMES)))]
+            n = len(meshes[name][0])
+            sources = rng.choice(n, size=int(rng.integers(1, 4)), replace=False).astype(np.int32)
+            requests.append((name, sources, server.submit(name, sources)))
+        results = [(name, sources, t.result(timeout=120).cpu().numpy()) for name, sources, t in requests]
+        stats = server.stats()
+        with pytest.raises(ValueError):
+            server.submit(NAMES[1], np.array([10**7], np.int32))
+        server.close()
+        assert stats["completed"] == len(requests)
+        assert stats["batches"] <= len(requests)
+        assert stats["p99_ms"] >= stats["p50_ms"] > 0.0
+        if workers == 1:
+            assert stats["evictions"] > 0
+        for i, (name, sources, got) in enumerate(results):
+            v, f = meshes[name]
+            worst, _ = ho.relative_error(got, _alone(cuda, v, f, sources))
+            assert worst <= 1e-10, (budget, i, name, worst)
