@@ -256,6 +256,19 @@ int hg_shift_to_sources(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
                         const int64_t* source_offsets, int64_t n_columns);
 
 /*
+ * hg_shift_to_sources without allocating: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same kernels and result as hg_shift_to_sources, with the queries' offsets, the
+ * decisions and the column partials in a fixed device array of the library (8 MiB + 128 KiB per
+ * device, held until the process ends) instead of per-mesh buffers. Limits: n_columns <= 4096 and
+ * n_columns * ceil(V / 256) <= 2^20 (about 64 columns at a million vertices); past them
+ * HG_ERR_VALUE. Every call on one device shares that array: calls that overlap on a device (two
+ * streams, two threads) race. Synchronizes the mesh's stream before returning.
+ */
+int hg_shift_to_sources_noalloc(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
+                                const int64_t* source_offsets, int64_t n_columns);
+
+/*
  * The whole path for n_queries source sets (heat_geodesic, heat.py:370-609): seed, diffuse (at
  * HG_METHOD_IGL averaged with the rim-pinned diffusion on a mesh with a rim), divergence, Poisson,
  * shift. Query q's sources are d_sources[off[q]..off[q+1]) with

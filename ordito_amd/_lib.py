@@ -58,6 +58,7 @@ EXPORTED = (
     "hg_unit_gradient_divergence",
     "hg_seed_sources",
     "hg_shift_to_sources",
+    "hg_shift_to_sources_noalloc",
     "hg_geodesic",
     "hg_mesh_create_packed",
     "hg_geodesic_packed",
@@ -176,6 +177,7 @@ _SIGNATURES = {
     "hg_unit_gradient_divergence": ([_P, _P, _P, _I64], ctypes.c_int),
     "hg_seed_sources": ([_P, _P, _PI64, _I64, _P], ctypes.c_int),
     "hg_shift_to_sources": ([_P, _P, _P, _PI64, _I64], ctypes.c_int),
+    "hg_shift_to_sources_noalloc": ([_P, _P, _P, _PI64, _I64], ctypes.c_int),
     "hg_geodesic": ([_P, _P, _PI64, _I64, _P, _I32, _I32, ctypes.POINTER(SolveStats)], ctypes.c_int),
     "hg_mesh_create_packed": ([_P, _PI64, _P, _PI64, _I64, _P, ctypes.POINTER(_P)], ctypes.c_int),
     "hg_geodesic_packed": (

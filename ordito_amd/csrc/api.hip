@@ -5,6 +5,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 
 #include "cholesky.h"
 #include "multigrid.h"
@@ -1584,6 +1585,88 @@ int hg_shift_to_sources(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
                "packed meshes shift each query in hg_geodesic_packed");
     upload_sources(*mesh, d_sources, source_offsets, n_columns);
     shift(*mesh, d_phi, n_columns);
+    HG_HIP(hipStreamSynchronize(mesh->stream));
+  });
+}
+
+// ------------------------------------------------------------------------------------------------
+// The shift without allocation: its column partials, decisions and the queries' offsets live in
+// one file-scope device array (each device holds its own copy, found by hipGetSymbolAddress once
+// per device). Layout: offsets (k+1 int64) at 0, the decisions (2k doubles) at kNoAllocShiftAt,
+// the column partials (blocks x k doubles) at kNoAllocPartialsAt. Every word a call reads it
+// writes first, so nothing needs clearing; calls that overlap on one device share the array and
+// race.
+// ------------------------------------------------------------------------------------------------
+
+namespace {
+
+constexpr int64_t kNoAllocColumns = 4096;
+constexpr int64_t kNoAllocPartials = int64_t{1} << 20;
+constexpr size_t kNoAllocShiftAt = 64 * 1024;
+constexpr size_t kNoAllocPartialsAt = 128 * 1024;
+constexpr size_t kNoAllocBytes = kNoAllocPartialsAt + sizeof(double) * kNoAllocPartials;
+static_assert(sizeof(int64_t) * (kNoAllocColumns + 1) <= kNoAllocShiftAt, "offsets overrun");
+static_assert(kNoAllocShiftAt + sizeof(double) * 2 * kNoAllocColumns <= kNoAllocPartialsAt,
+              "decisions overrun");
+
+constexpr int kNoAllocDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+char* g_scratch[kNoAllocDevices] = {};
+
+// The current device's copy of `scratch`: its address on that device, looked up once.
+char* device_scratch() {
+  int device = 0;
+  HG_HIP(hipGetDevice(&device));
+  HG_REQUIRE(device >= 0 && device < kNoAllocDevices, HG_ERR_RUNTIME,
+             "device ordinal past the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    HG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    HG_REQUIRE(ptr != nullptr, HG_ERR_RUNTIME, "no device address for the shift scratch");
+    g_scratch[device] = static_cast<char*>(ptr);
+  }
+  return g_scratch[device];
+}
+
+}  // namespace
+
+int hg_shift_to_sources_noalloc(hg_mesh* mesh, double* d_phi, const int32_t* d_sources,
+                                const int64_t* source_offsets, int64_t n_columns) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(mesh->segments() == 1, HG_ERR_VALUE,
+               "packed meshes shift each query in hg_geodesic_packed");
+    HG_REQUIRE(n_columns >= 0 && n_columns <= kNoAllocColumns, HG_ERR_VALUE,
+               "hg_shift_to_sources_noalloc takes at most " + std::to_string(kNoAllocColumns) +
+                   " columns");
+    if (n_columns == 0 || mesh->n == 0) return;
+    HG_REQUIRE(source_offsets != nullptr, HG_ERR_VALUE, "source_offsets is null");
+    for (int64_t q = 0; q < n_columns; ++q)
+      HG_REQUIRE(source_offsets[q] >= 0 && source_offsets[q] <= source_offsets[q + 1],
+                 HG_ERR_VALUE, "source_offsets must be non-negative and non-decreasing");
+    const int64_t blocks = shift_partial_blocks(mesh->n);
+    HG_REQUIRE(blocks * n_columns <= kNoAllocPartials, HG_ERR_VALUE,
+               "hg_shift_to_sources_noalloc holds " + std::to_string(kNoAllocPartials) +
+                   " column partials (one per 256 vertices per column): " +
+                   std::to_string(blocks * n_columns) + " asked");
+    char* base = device_scratch();
+    int64_t* d_offsets = reinterpret_cast<int64_t*>(base);
+    double* d_shift = reinterpret_cast<double*>(base + kNoAllocShiftAt);
+    double* d_partials = reinterpret_cast<double*>(base + kNoAllocPartialsAt);
+    // The caller's offsets index d_sources as they stand; the kernels read sources[e] for e in
+    // [off[q], off[q+1]), the same entries the allocating path copies out relative to off[0].
+    HG_HIP(hipMemcpyAsync(d_offsets, source_offsets, sizeof(int64_t) * (n_columns + 1),
+                          hipMemcpyHostToDevice, mesh->stream));
+    launch_shift(d_phi, d_sources, d_offsets, mesh->n, n_columns, nullptr, nullptr, d_partials,
+                 d_shift, mesh->stream);
     HG_HIP(hipStreamSynchronize(mesh->stream));
   });
 }
