@@ -566,7 +566,6 @@ def extra_workloads(args, oracle_c2, c2_vertices, c2_faces, c2_sources) -> dict:
     log("== 64 served meshes")
     sub = copy.copy(args)
     sub.meshes, sub.min_vertices, sub.max_vertices = 64, 1e4, 1e6
-    sub.duration = min(args.duration, 20.0)
     probes = [float(r) for r in args.serve_probes.split(",") if r.strip()]
     extras["served_64"] = serve_result(sub, probe_rates=probes)
     torch.cuda.empty_cache()

@@ -34,6 +34,7 @@ MESHES = {
     "bumpy": lambda: __import__("tests.meshes", fromlist=["x"]).bumpy_hull(),
     "saddle_graded": lambda: __import__("tests.meshes", fromlist=["x"]).graded_saddle(),
     "two_bodies": lambda: _two_bodies(),
+    "hub_disk": lambda: __import__("tests.meshes", fromlist=["x"]).hub_disk(),
 }
 
 

@@ -1,5 +1,5 @@
 """
-Two ill-shaped test surfaces, written from their geometric description.
+Ill-shaped test surfaces, written from their geometric description.
 
 Provenance: ordito's test suite names two meshes that stress a heat solver -- a saddle grid graded
 towards x = 0 (``saddle_graded``, ordito ``tests/conftest.py:471-502``) and a bumpy star-shaped
@@ -86,3 +86,37 @@ def bumpy_hull(points: int = 500, squeeze: float = 0.97, seed: int = 5) -> tuple
     start = rng.integers(0, 3, len(tri))
     tri = np.stack([tri[np.arange(len(tri)), (start + c) % 3] for c in range(3)], axis=1)
     return surface[order].astype(np.float32), tri.astype(np.int32).reshape(-1)
+
+
+def hub_disk(spokes: int = 400, rings: int = 20, seed: int = 7) -> tuple[np.ndarray, np.ndarray]:
+    """
+    A polar grid on the paraboloid ``z = 0.3 r^2`` over the unit disk: a centre vertex joined to
+    every vertex of the first of ``rings`` rings of ``spokes`` vertices (a hub of degree
+    ``spokes``, far past what a row of the divergence's face groups holds), the rings joined as
+    quads split along alternating diagonals, angles jittered by a tenth of a spoke. Open, with a
+    rim (the last ring).
+    """
+    rng = np.random.default_rng(seed)
+    step = 2.0 * np.pi / spokes
+    theta = np.arange(spokes) * step + rng.uniform(-0.1, 0.1, (rings, spokes)) * step
+    radius = (np.arange(1, rings + 1) / rings)[:, None]
+    x, y = radius * np.cos(theta), radius * np.sin(theta)
+    ring_xyz = np.stack([x, y, 0.3 * (x * x + y * y)], axis=-1).reshape(-1, 3)
+    vertices = np.concatenate([np.zeros((1, 3)), ring_xyz])
+
+    def at(i, j):  # vertex of ring i (0-based), spoke j
+        return 1 + i * spokes + (j % spokes)
+
+    j = np.arange(spokes)
+    faces = [np.stack([np.zeros(spokes, np.int64), at(0, j), at(0, j + 1)], axis=1)]
+    for i in range(rings - 1):
+        a, b, c, d = at(i, j), at(i, j + 1), at(i + 1, j + 1), at(i + 1, j)
+        even = (i + j) % 2 == 0
+        faces.append(np.where(even[:, None], np.stack([a, c, d], 1), np.stack([a, b, d], 1)))
+        faces.append(np.where(even[:, None], np.stack([a, b, c], 1), np.stack([b, c, d], 1)))
+    tri = np.concatenate(faces)
+    # The surface is a graph over the plane: every face wound with its normal upward.
+    p0, p1, p2 = vertices[tri[:, 0]], vertices[tri[:, 1]], vertices[tri[:, 2]]
+    down = np.cross(p1 - p0, p2 - p0)[:, 2] < 0.0
+    tri[down, 1], tri[down, 2] = tri[down, 2].copy(), tri[down, 1].copy()
+    return vertices.astype(np.float32), tri.astype(np.int32).reshape(-1)

@@ -65,6 +65,51 @@ def test_factored_fp64_matches_oracle(cuda, name):
     assert worst <= 1e-9, worst
 
 
+@pytest.mark.parametrize(
+    "name, method",
+    [("icosphere4", "meshlab"), ("bumpy", "meshlab"), ("two_bodies", "meshlab"),
+     ("saddle_graded", "meshlab"), ("hub_disk", "meshlab"), ("hub_disk", "igl")],
+)
+def test_wide_batches_match_the_oracle(cuda, name, method):
+    """40 queries in one factored call (the wide form of every by-rows kernel: a wave a row in the
+    divergence, over a hub row of 400 corners on hub_disk): single sources spread over the mesh,
+    two multi-source queries and an empty one, each against the factored oracle to 1e-9 of its
+    range."""
+    from ordito_amd import creation, heat
+
+    vertices, faces = mesh(name)
+    n = len(vertices)
+    picks = creation.random_sources(n, 37, seed=11)
+    picks[0] = 0
+    sets = [np.array([p], np.int32) for p in picks]
+    sets += [np.array([1, n // 3, n - 1], np.int32), np.array([n // 2, n // 2 + 1], np.int32)]
+    sets.append(np.zeros(0, np.int32))
+    print(f"wide batch: {name} {method}, {len(sets)} queries", flush=True)
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method=method)
+    assert solver.factor()
+    got = solver.geodesic([_tensor(cuda, q) for q in sets]).cpu().numpy()
+    oracle = ho.FactoredOracle(vertices, faces, method=method)
+    for q, sources in enumerate(sets):
+        if sources.size == 0:
+            assert not got[q].any()
+            continue
+        worst, _ = ho.relative_error(got[q], oracle.distance(sources))
+        assert worst <= 1e-9, (q, worst)
+
+
+def test_hub_disk_iterated_matches_oracle(cuda):
+    """The one-shot path on a mesh with a vertex of degree 400: its long row in the settled CG,
+    the Chebyshev Poisson solve and the divergence."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh("hub_disk")
+    for sources in (np.array([0], np.int32), np.array([4321], np.int32)):
+        got = heat.heat_geodesic(vertices, faces, sources)
+        worst, _ = ho.relative_error(got, ho.geodesic(vertices, faces, sources).distance)
+        assert worst <= 1e-5, (sources, worst)
+
+
 @pytest.mark.parametrize("name", ["icosphere4", "torus", "grid", "bumpy", "two_bodies"])
 def test_iterated_matches_oracle(cuda, name):
     from ordito_amd import heat
