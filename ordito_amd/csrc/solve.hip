@@ -156,17 +156,46 @@ __device__ __forceinline__ void gather_add(const SweepArgs& a, const int64_t row
 // stage's global loads are issued before the current stage's products, so they overlap.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 constexpr int kInner = 16;
+// Staged rows are kTile + kPad doubles apart: the four inner rows an MFMA operand read spans start
+// 34 banks apart (mod 64), and the backward's transposed stores of sixteen inner rows at one
+// column land in distinct bank pairs.
+constexpr int kPad = 17;
 
 __device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Four 4-deep steps over one stage: A from As[k][16 * wave + l % 16] (the wave's 16 output rows),
-// B from Bs[k][16 n + l % 16] for each 16-column block n holding right-hand sides.
-template <int kDepth = kInner>
-__device__ __forceinline__ void stage_products(const double (*As)[kTile + 1],
-                                               const double (*Bs)[kCols + 1], int wave, int lane,
-                                               int blocks, f64x4 acc[4]) {
+// The 4-deep steps over one stage: A from As[k][16 * wave + l % 16] (the wave's 16 output rows),
+// B from Bs[k][16 n + l % 16] for each of the four 16-column blocks n of right-hand sides (columns
+// past the chunk's are staged as zeros, so every block is multiplied). kBatch steps' operands are
+// read before their products, so the reads overlap and the products issue back to back.
+template <int kDepth = kInner, int kBatch = 2>
+__device__ __forceinline__ void stage_products(const double (*As)[kTile + kPad],
+                                               const double (*Bs)[kCols + kPad], int wave, int lane,
+                                               f64x4 acc[4]) {
+  const int li = lane % 16, lk = lane / 16;
+#pragma unroll
+  for (int kb = 0; kb < kDepth; kb += 4 * kBatch) {
+    double av[kBatch], bv[kBatch][4];
+#pragma unroll
+    for (int h = 0; h < kBatch; ++h) {
+      av[h] = As[kb + 4 * h + lk][16 * wave + li];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) bv[h][n] = Bs[kb + 4 * h + lk][16 * n + li];
+    }
+#pragma unroll
+    for (int h = 0; h < kBatch; ++h)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[n] = mfma16x16x4(av[h], bv[h][n], acc[n]);
+  }
+}
+
+// stage_products over the column blocks that hold right-hand sides only (blocks of them), a step
+// at a time: the backward's form, whose registers the unrolled form would spill.
+template <int kDepth>
+__device__ __forceinline__ void stage_products_blocks(const double (*As)[kTile + kPad],
+                                                      const double (*Bs)[kCols + kPad], int wave,
+                                                      int lane, int blocks, f64x4 acc[4]) {
   const int li = lane % 16, lk = lane / 16;
 #pragma unroll 1
   for (int kb = 0; kb < kDepth; kb += 4) {
@@ -188,8 +217,8 @@ __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
     const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_r0,
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
-  __shared__ double Ws[kInner][kTile + 1];  // [inner][row]
-  __shared__ double Fs[kInner][kCols + 1];  // [inner][right-hand side]
+  __shared__ double Ws[kInner][kTile + kPad];  // [inner][row]
+  __shared__ double Fs[kInner][kCols + kPad];  // [inner][right-hand side]
   const int32_t task = xcd_block();
   const int32_t s = task_s[task];
   const int32_t r0 = task_r0[task];
@@ -201,7 +230,6 @@ __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
   const TW* W = w + a.w_off[s];
   const int64_t vbase = a.vec_off[s];
   const int t = threadIdx.x, wave = t / 64, lane = t % 64;
-  const int blocks = (kk + 15) / 16;
   const bool rows_live = r0 + 16 * wave < m;  // wave-uniform
   // The epilogue's gather pairs (this lane's update rows lane / 16 + 4v), loaded before the stages.
   int2 epair[4];
@@ -272,7 +300,7 @@ __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
       load(j0 + kInner);
       if (j0 + 2 * kInner < k1) load_pairs(j0 + 2 * kInner);
     }
-    if (rows_live) stage_products(Ws, Fs, wave, lane, blocks, acc);
+    if (rows_live) stage_products(Ws, Fs, wave, lane, acc);
     __syncthreads();
   }
   if (!rows_live) return;
@@ -399,8 +427,8 @@ __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
     const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
     const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
   constexpr int kD = kBwdInner, kPer = kD * kTile / 256;  // entries a thread stages per operand
-  __shared__ double Ws[kD][kTile + 1];  // [inner][inverse-factor column]
-  __shared__ double Gs[kD][kCols + 1];  // [inner][right-hand side]
+  __shared__ double Ws[kD][kTile + kPad];  // [inner][inverse-factor column]
+  __shared__ double Gs[kD][kCols + kPad];  // [inner][right-hand side]
   const int32_t task = xcd_block();
   const int32_t s = task_s[task];
   const int32_t j0 = task_j0[task];
@@ -412,7 +440,6 @@ __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
   const TW* W = w + a.w_off[s];
   const int32_t* srows = a.rows + a.row_ptr[s];
   const int t = threadIdx.x, wave = t / 64, lane = t % 64;
-  const int blocks = (kk + 15) / 16;
   const bool cols_live = j0 + 16 * wave < nc;  // wave-uniform
   // Staging: entry e = t + 256u is inverse-factor column e / kD at inner e % kD (contiguous in
   // memory), and g's right-hand side e % 64 at inner e / 64.
@@ -470,7 +497,7 @@ __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
       load(i0 + kD);
       if (i0 + 2 * kD < k1) load_rows(i0 + 2 * kD);
     }
-    if (cols_live) stage_products<kD>(Ws, Gs, wave, lane, blocks, acc);
+    if (cols_live) stage_products_blocks<kD>(Ws, Gs, wave, lane, (kk + 15) / 16, acc);
     __syncthreads();
   }
   if (!cols_live) return;
