@@ -39,6 +39,11 @@ extern "C" {
                                    Neumann heat with the heat pinned to zero on the rim */
 #define HG_METHOD_POTPOURRI3D 2 /* barycentric mass, h over the unique edges; a geodesic's Poisson
                                    step solves -L + 1e-6 I (geometry-central's regularization) */
+#define HG_METHOD_ROBUST 0x100  /* or'ed into HG_METHOD_MESHLAB / HG_METHOD_IGL: use_robust
+                                   (heat.py:229-243, 2041-2052), the half-cotangents from side
+                                   lengths mollified by the smallest constant that leaves every
+                                   triangle-inequality margin 1e-5 of the mean side
+                                   (mollify_intrinsic, laplacian.py:779-874) */
 
 /* How a solve runs (HeatSolver policy, heat.py:4348-4423). */
 #define HG_MODE_ITERATE 0  /* settled Jacobi-CG heat solve; Jacobi-Chebyshev CG Poisson solve */
@@ -76,6 +81,7 @@ typedef struct hg_operator_view {
   int32_t settle_rounds;    /* the settle window of the heat solve (heat.py:5234-5243) */
   int32_t time_from_edges;  /* 1: t is the device-formed squared mean edge length (the method's
                                edges) */
+  double mollify_delta;     /* HG_METHOD_ROBUST: the constant added to every side (0: none) */
 } hg_operator_view;
 
 /* Statistics of the last solve on a mesh. */

@@ -35,6 +35,12 @@ Pinning (what checks each function; see ``tests/test_oracle.py``):
 - ``geodesic(method="igl")``: a source on the rim leaves only the (halved) Neumann heat, so the
   distance is the Voronoi Neumann one bit for bit; the great-circle bound on a sphere; ``"igl"``
   and ``"meshlab"`` differ where the choices matter.
+- ``mollify_intrinsic`` / ``intrinsic_half_cotangents`` (``use_robust``): the reference's
+  docstring known answers (the equilateral triangle's ``cot(60)/2`` thrice, the 3-4-5 triangle's
+  ``0, 0.375, 0.6667``; a closed box needs no constant, the collinear triangle of lengths 1, 2, 1
+  needs ``1e-5`` of its mean ``4/3``), every margin of the mollified table at least the constant
+  ``epsilon`` asked for, and on a clean mesh the law-of-cosines weights within float32 rounding of
+  the position-based ``face_terms`` ones (a different formula for the same numbers).
 - ``solve_regularized_poisson`` / ``geodesic(method="potpourri3d")``: ``(-L + 1e-6 I) 1 = 1e-6``
   (``L 1 = 0``), so the right-hand side ``1e-6`` gives ones exactly up to rounding and the
   regularization's sign and size are fixed; an unreferenced vertex stays zero; the great-circle
@@ -64,6 +70,8 @@ SETTLE_DIRECT_TOLERANCE = 1e-12
 CG_TOLERANCE = 1e-8
 # heat.py:190-193: geometry-central's identity multiple on the Poisson system ("potpourri3d").
 POTPOURRI_POISSON_SHIFT = 1e-6
+# constants.py:31: use_robust's triangle-inequality margin, relative to the mean edge length.
+TOLERANCE_MOLLIFY = 1e-5
 
 
 def _f32(x):
@@ -141,6 +149,75 @@ def face_terms(vertices: np.ndarray, faces: np.ndarray) -> FaceTerms:
     cot = np.stack([c0, c1, c2], axis=1)
     cot[~dpos] = np.float32(0.0)
     return FaceTerms(cot.astype(np.float32), normals, areas)
+
+
+def face_edge_lengths(vertices: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    """
+    ``(F, 3)`` float32 side lengths, column ``e`` opposite corner ``e``: ``|v1 - v2|``,
+    ``|v2 - v0|``, ``|v0 - v1|``, each the float32 square root of a left-to-right float32 sum of
+    squares (``kernels/triangles.py:204-221``, ``kernels/predicates.py:221-228``).
+    """
+    p = _f32(vertices)
+    tri = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    v0, v1, v2 = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
+    sides = []
+    for a, b in ((v1, v2), (v2, v0), (v0, v1)):
+        d = (a - b).astype(np.float32)
+        sides.append(np.sqrt(_dot3(d, d)).astype(np.float32))
+    return np.stack(sides, axis=1).astype(np.float32)
+
+
+def mollify_intrinsic(
+    vertices: np.ndarray, faces: np.ndarray, epsilon: float = TOLERANCE_MOLLIFY
+) -> tuple[np.ndarray, float]:
+    """
+    The smallest constant added to every edge length that leaves every triangle's
+    triangle-inequality margin ``a + b - c`` at least ``epsilon`` times the mean edge length, and
+    the mollified ``(F, 3)`` float32 table (``laplacian.py:779-874``,
+    ``kernels/laplacian.py:1585-1630``). Adding ``delta`` to all three sides grows each margin by
+    exactly ``delta``, so ``delta = max(epsilon * mean - min margin, 0)``, formed in float32; the
+    mean is the float64 mean of the float32 table. A clean mesh gets ``delta = 0`` and its
+    lengths unchanged.
+    """
+    lengths = face_edge_lengths(vertices, faces)
+    if lengths.shape[0] == 0:
+        return lengths, 0.0
+    a, b, c = lengths[:, 0], lengths[:, 1], lengths[:, 2]
+    margin = np.minimum(np.minimum((a + b) - c, (b + c) - a), (c + a) - b).min()
+    mean = float(lengths.astype(np.float64).sum()) / lengths.size
+    eps_scaled = np.float32(epsilon * mean)
+    delta = float(max(np.float32(eps_scaled - np.float32(margin)), np.float32(0.0)))
+    if delta <= 0.0:
+        return lengths, 0.0
+    return (lengths + np.float32(delta)).astype(np.float32), delta
+
+
+def intrinsic_half_cotangents(lengths: np.ndarray) -> np.ndarray:
+    """
+    ``(F, 3)`` float32 half-cotangents from side lengths (column ``e`` opposite corner ``e``), in
+    float32: ``(l_{e+1}^2 + l_{e+2}^2 - l_e^2) / (4 * 2A)`` with twice the area from Kahan's
+    ordering of Heron's formula, largest side first, ``0.5 * sqrt(max(arg, 0))``; a zero-area face
+    gives zeros (``kernels/laplacian.py:59-90``, ``kernels/predicates.py:194-210``,
+    ``laplacian.py:269-340``).
+    """
+    lengths = _f32(lengths)
+    srt = np.sort(lengths, axis=1)
+    c, b, a = srt[:, 0], srt[:, 1], srt[:, 2]
+    arg = (a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c))
+    with np.errstate(invalid="ignore"):
+        dbl = (np.float32(0.5) * np.sqrt(np.maximum(arg, np.float32(0.0)))).astype(np.float32)
+    dbl = np.where(np.isnan(dbl), np.float32(0.0), dbl)
+    den = np.float32(4.0) * dbl
+    ok = den > np.float32(0.0)
+    safe = np.where(ok, den, np.float32(1.0))
+    l2 = lengths * lengths
+    out = np.stack(
+        [((l2[:, 1] + l2[:, 2]) - l2[:, 0]) / safe, ((l2[:, 2] + l2[:, 0]) - l2[:, 1]) / safe,
+         ((l2[:, 0] + l2[:, 1]) - l2[:, 2]) / safe],
+        axis=1,
+    ).astype(np.float32)
+    out[~ok] = np.float32(0.0)
+    return out
 
 
 @dataclass
@@ -386,7 +463,11 @@ def resolve_time(
 
 
 def assemble(
-    vertices: np.ndarray, faces: np.ndarray, t: float | None = None, method: str = "meshlab"
+    vertices: np.ndarray,
+    faces: np.ndarray,
+    t: float | None = None,
+    method: str = "meshlab",
+    use_robust: bool = False,
 ) -> Operators:
     """
     ``L`` (igl sign: negative diagonal), ``M - tL`` and ``-L`` over one pattern, and ``m``
@@ -399,10 +480,15 @@ def assemble(
     ``kernels/linalg.py:1612-1621``) with ``factor = -t`` and, for ``t=None``, ``-h^2`` from the
     face-edge sums (``kernels/linalg.py:1593-1609``); Poisson ``-L_ij``.
     ``t`` here is the time as resolved (explicit, or the floor), ``None`` for ``h^2``.
+    ``use_robust``: the half-cotangents (the Laplacian's and the divergence's) from the mollified
+    side lengths, everything else from the positions (``heat.py:2041-2052``).
     """
     vertices = _f32(vertices)
     n = vertices.shape[0]
     terms = face_terms(vertices, faces)
+    if use_robust:
+        lengths, _ = mollify_intrinsic(vertices, faces)
+        terms = FaceTerms(intrinsic_half_cotangents(lengths), terms.normals, terms.areas)
     pattern = operator_pattern(n, faces)
     lumping, edges = METHODS[method]
     if lumping == "voronoi":
@@ -573,11 +659,12 @@ class FactoredOracle:
         t: float | None = None,
         ordering: str = "MMD_AT_PLUS_A",
         method: str = "meshlab",
+        use_robust: bool = False,
     ) -> None:
         self.vertices = _f32(vertices)
         self.faces = np.asarray(faces, dtype=np.int64).reshape(-1)
         t_resolved, _ = resolve_time(self.vertices, self.faces, t, method)
-        self.ops = assemble(self.vertices, self.faces, t_resolved, method)
+        self.ops = assemble(self.vertices, self.faces, t_resolved, method, use_robust)
         self.heat_lu = factorize(self.ops.matrix(self.ops.heat), ordering)
         self.rim = rim_vertices(self.ops.pattern)
         self.pinned_lu = None
@@ -734,6 +821,7 @@ def geodesic(
     sources: np.ndarray,
     t: float | None = None,
     method: str = "meshlab",
+    use_robust: bool = False,
 ) -> GeodesicResult:
     """
     ``heat_geodesic(vertices, faces, sources, t, method=method)`` solved directly in FP64
@@ -745,7 +833,8 @@ def geodesic(
     ``heat.py:3714-3736``; heat step ``(M - tL) u = u0`` (``heat.py:3890-3914``); divergence
     (``heat.py:3052-3131``); Poisson ``-L phi = -b`` (``heat.py:4062-4151``); shift and orient
     (``heat.py:3134-3162``). Both solves are direct, the reference's factored form (whose
-    refinement reaches 1e-12 componentwise / 1e-8 normwise).
+    refinement reaches 1e-12 componentwise / 1e-8 normwise). ``use_robust``: the half-cotangents
+    from mollified side lengths (``assemble``).
     """
     vertices = _f32(vertices)
     faces = np.asarray(faces, dtype=np.int64).reshape(-1)
@@ -756,7 +845,7 @@ def geodesic(
         return GeodesicResult(z, z, z, None)  # type: ignore[arg-type]
     u0 = seed_indicator(n, sources)
     t_resolved, _ = resolve_time(vertices, faces, t, method)
-    ops = assemble(vertices, faces, t_resolved, method)
+    ops = assemble(vertices, faces, t_resolved, method, use_robust)
     heat = solve_direct(ops.matrix(ops.heat), u0)
     if method == "igl":
         pinned = diffuse_pinned(ops, u0)

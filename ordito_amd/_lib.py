@@ -24,6 +24,7 @@ HG_ERR_BUDGET = 4
 HG_METHOD_MESHLAB = 0
 HG_METHOD_IGL = 1
 HG_METHOD_POTPOURRI3D = 2
+HG_METHOD_ROBUST = 0x100
 HG_MODE_ITERATE = 0
 HG_MODE_FACTORED = 1
 HG_PRECISION_FP64 = 0
@@ -106,6 +107,7 @@ class OperatorView(ctypes.Structure):
         ("t", ctypes.c_double),
         ("settle_rounds", ctypes.c_int32),
         ("time_from_edges", ctypes.c_int32),
+        ("mollify_delta", ctypes.c_double),
     ]
 
 

@@ -109,6 +109,9 @@ struct hg_mesh {
   int32_t refused = 0;
   // Whether a system is asked of its analysis before it iterates (hg_set_budget_probe).
   bool budget_probe = true;
+  // use_robust: the half-cotangents from side lengths mollified by mollify_delta.
+  bool robust = false;
+  double mollify_delta = 0.0;
   // The settle rule of the heat solves' iteration (hg_set_settle_rule); settle_window 0 keeps the
   // window the operators' time gives (settle_rounds).
   int32_t check_rounds = kHeatCheckRounds;
@@ -306,7 +309,8 @@ double host_diffusion_floor(const hg_mesh& m, int64_t s) {
   return (diag / kDiffusionDecay) * (diag / kDiffusionDecay);
 }
 
-void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default) {
+void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default,
+              bool robust = false) {
   const hipStream_t st = m.stream;
   const int64_t n = m.n, nnz = static_cast<int64_t>(m.pattern.columns.size());
   const int32_t face_edges = method == HG_METHOD_MESHLAB ? 1 : 0;
@@ -351,6 +355,10 @@ void assemble(hg_mesh& m, double t_request, int32_t method, bool solver_default)
   for (int64_t s = 0; s < n_seg; ++s) longest = std::max(longest, m.seg_v[s + 1] - m.seg_v[s]);
   m.edge_partials.allocate(2 * edge_sum_blocks(longest));
   launch_face_terms(m.xyz.ptr, m.faces.ptr, m.f, m.cot.ptr, m.normals.ptr, m.areas.ptr, st);
+  // use_robust: the half-cotangents (the Laplacian's and the divergence's) from mollified side
+  // lengths, the areas, normals, mass and time from the positions (heat.py:2041-2052).
+  m.robust = robust;
+  m.mollify_delta = robust ? launch_mollified_cot(m.xyz.ptr, m.faces.ptr, m.f, m.cot.ptr, st) : 0.0;
   launch_face_geometry(m.xyz.ptr, m.faces.ptr, m.normals.ptr, m.areas.ptr, m.cot.ptr, m.f,
                        m.face_geom.ptr, st);
   if (method == HG_METHOD_IGL) {
@@ -1315,11 +1323,18 @@ int hg_mesh_set_stream(hg_mesh* mesh, void* stream) {
 int hg_operators(hg_mesh* mesh, double t, int32_t method, int32_t solver_default_time) {
   return guarded([&] {
     require_mesh(mesh);
+    const bool robust = (method & HG_METHOD_ROBUST) != 0;
+    method &= ~HG_METHOD_ROBUST;
     HG_REQUIRE(method == HG_METHOD_MESHLAB || method == HG_METHOD_IGL ||
                    method == HG_METHOD_POTPOURRI3D,
                HG_ERR_VALUE, "method must be 'meshlab', 'igl' or 'potpourri3d'");
+    HG_REQUIRE(!robust || method != HG_METHOD_POTPOURRI3D, HG_ERR_VALUE,
+               "use_robust at 'potpourri3d' is geometry-central's intrinsic Delaunay method, which "
+               "this library does not build; use 'meshlab' or 'igl'");
+    HG_REQUIRE(!robust || mesh->segments() == 1, HG_ERR_VALUE,
+               "use_robust takes one mesh, not a packed set");
     HG_REQUIRE(std::isnan(t) || t > 0.0, HG_ERR_VALUE, "t must be positive");
-    assemble(*mesh, t, method, solver_default_time != 0);
+    assemble(*mesh, t, method, solver_default_time != 0, robust);
   });
 }
 
@@ -1343,6 +1358,7 @@ int hg_operator_views(hg_mesh* mesh, hg_operator_view* out) {
     out->t = mesh->t;
     out->settle_rounds = mesh->settle_rounds;
     out->time_from_edges = mesh->time_from_edges;
+    out->mollify_delta = mesh->mollify_delta;
   });
 }
 

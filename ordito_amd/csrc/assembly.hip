@@ -8,6 +8,10 @@
 // rounding per operation, exactly the arithmetic the reference's kernels spell.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "kernels.h"
 
 namespace hg {
@@ -74,6 +78,80 @@ __global__ void face_terms_kernel(const float* __restrict__ xyz, const int32_t* 
     c0 = dot(sub(v1, v0), sub(v2, v0)) / denominator;
     c1 = dot(sub(v2, v1), sub(v0, v1)) / denominator;
     c2 = dot(sub(v0, v2), sub(v1, v2)) / denominator;
+  }
+  cot[3 * f] = c0;
+  cot[3 * f + 1] = c1;
+  cot[3 * f + 2] = c2;
+}
+
+// use_robust's first pass (laplacian.py:779-874, kernels/laplacian.py:1585-1630): per face the
+// float32 side lengths, column e opposite corner e (|v1 - v2|, |v2 - v0|, |v0 - v1|), and per
+// block of faces the smallest triangle-inequality margin a + b - c over its faces and sides and
+// the float64 sum of its lengths, added in a fixed tree.
+__global__ __launch_bounds__(256) void face_lengths_kernel(
+    const float* __restrict__ xyz, const int32_t* __restrict__ faces, int64_t n_faces,
+    float* __restrict__ lengths, float* __restrict__ block_margin,
+    double* __restrict__ block_sum) {
+  __shared__ float low[256];
+  __shared__ double total[256];
+  const int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  float margin = INFINITY;
+  double sum = 0.0;
+  if (f < n_faces) {
+    const F3 v0 = load3(xyz, faces[3 * f]), v1 = load3(xyz, faces[3 * f + 1]),
+             v2 = load3(xyz, faces[3 * f + 2]);
+    const F3 e0 = sub(v1, v2), e1 = sub(v2, v0), e2 = sub(v0, v1);
+    const float a = sqrtf(dot(e0, e0)), b = sqrtf(dot(e1, e1)), c = sqrtf(dot(e2, e2));
+    lengths[3 * f] = a;
+    lengths[3 * f + 1] = b;
+    lengths[3 * f + 2] = c;
+    margin = fminf(fminf((a + b) - c, (b + c) - a), (c + a) - b);
+    sum = (static_cast<double>(a) + static_cast<double>(b)) + static_cast<double>(c);
+  }
+  low[threadIdx.x] = margin;
+  total[threadIdx.x] = sum;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (static_cast<int>(threadIdx.x) < stride) {
+      low[threadIdx.x] = fminf(low[threadIdx.x], low[threadIdx.x + stride]);
+      total[threadIdx.x] += total[threadIdx.x + stride];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    block_margin[blockIdx.x] = low[0];
+    block_sum[blockIdx.x] = total[0];
+  }
+}
+
+// use_robust's half-cotangents from the side lengths plus the mollifying constant delta, in
+// float32 (kernels/laplacian.py:59-90, kernels/predicates.py:194-210): twice the area by Kahan's
+// ordering of Heron's formula (largest side first, a negative product clamped, NaN to zero) and
+// (l_{e+1}^2 + l_{e+2}^2 - l_e^2) / (4 * 2A); a zero-area face weighs nothing.
+__global__ void intrinsic_cot_kernel(const float* __restrict__ lengths, float delta,
+                                     int64_t n_faces, float* __restrict__ cot) {
+  const int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  float l[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) l[e] = delta > 0.0f ? lengths[3 * f + e] + delta : lengths[3 * f + e];
+  // Ascending c <= b <= a: the first smallest side, the last largest, and the third.
+  int lo = 0, hi = 0;
+  for (int e = 1; e < 3; ++e) {
+    if (l[e] < l[lo]) lo = e;
+    if (l[e] >= l[hi]) hi = e;
+  }
+  const float c = l[lo], a = l[hi], b = l[3 - lo - hi];
+  const float arg = (a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c));
+  float dbl = 0.5f * sqrtf(fmaxf(arg, 0.0f));
+  if (isnan(dbl)) dbl = 0.0f;
+  const float den = 4.0f * dbl;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  if (den > 0.0f) {
+    const float s0 = l[0] * l[0], s1 = l[1] * l[1], s2 = l[2] * l[2];
+    c0 = ((s1 + s2) - s0) / den;
+    c1 = ((s2 + s0) - s1) / den;
+    c2 = ((s0 + s1) - s2) / den;
   }
   cot[3 * f] = c0;
   cot[3 * f + 1] = c1;
@@ -706,6 +784,38 @@ void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, 
   face_terms_kernel<<<blocks_for(n_faces), kBlock, 0, stream>>>(xyz, faces, n_faces, cot, normals,
                                                                  areas);
   HG_HIP(hipGetLastError());
+}
+
+double launch_mollified_cot(const float* xyz, const int32_t* faces, int64_t n_faces, float* cot,
+                            hipStream_t stream) {
+  if (n_faces == 0) return 0.0;
+  const unsigned blocks = blocks_for(n_faces);
+  DeviceArray<float> lengths(3 * n_faces), margins(blocks);
+  DeviceArray<double> sums(blocks);
+  face_lengths_kernel<<<blocks, kBlock, 0, stream>>>(xyz, faces, n_faces, lengths.ptr, margins.ptr,
+                                                     sums.ptr);
+  HG_HIP(hipGetLastError());
+  std::vector<float> h_margin(blocks);
+  std::vector<double> h_sum(blocks);
+  HG_HIP(hipMemcpyAsync(h_margin.data(), margins.ptr, sizeof(float) * blocks,
+                        hipMemcpyDeviceToHost, stream));
+  HG_HIP(hipMemcpyAsync(h_sum.data(), sums.ptr, sizeof(double) * blocks, hipMemcpyDeviceToHost,
+                        stream));
+  HG_HIP(hipStreamSynchronize(stream));
+  float margin = INFINITY;
+  double total = 0.0;
+  for (unsigned b = 0; b < blocks; ++b) {
+    margin = std::min(margin, h_margin[b]);
+    total += h_sum[b];
+  }
+  // delta = max(epsilon * mean - margin, 0) in float32 (laplacian.py:866-870).
+  const float eps_scaled = static_cast<float>(kMollifyTolerance * (total / (3.0 * n_faces)));
+  const float shortfall = eps_scaled - margin;
+  const float delta = shortfall > 0.0f ? shortfall : 0.0f;
+  intrinsic_cot_kernel<<<blocks, kBlock, 0, stream>>>(lengths.ptr, delta, n_faces, cot);
+  HG_HIP(hipGetLastError());
+  HG_HIP(hipStreamSynchronize(stream));
+  return delta;
 }
 
 void launch_voronoi_shares(const float* xyz, const int32_t* faces, int64_t n_faces, float* shares,

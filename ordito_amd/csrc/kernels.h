@@ -103,6 +103,13 @@ struct PhaseScope {
 // assembly.hip
 void launch_face_terms(const float* xyz, const int32_t* faces, int64_t n_faces, float* cot,
                        float* normals, float* areas, hipStream_t stream);
+// use_robust (laplacian.py:779-874, 269-340): cot (F,3) float32 rebuilt from the side lengths
+// plus the smallest constant that leaves every triangle-inequality margin at least
+// kMollifyTolerance times the mean side; returns that constant (0 on a clean mesh). Synchronizes
+// `stream` (the constant is decided on the host).
+constexpr double kMollifyTolerance = 1e-5;  // constants.py:31
+double launch_mollified_cot(const float* xyz, const int32_t* faces, int64_t n_faces, float* cot,
+                            hipStream_t stream);
 // Each corner's Voronoi share of its face's area (F,3) float32.
 void launch_voronoi_shares(const float* xyz, const int32_t* faces, int64_t n_faces, float* shares,
                            hipStream_t stream);

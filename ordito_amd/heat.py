@@ -41,6 +41,19 @@ def _require_method(method: str, *, geodesic: bool) -> None:
         raise ValueError(f"method must be 'meshlab', 'igl' or 'potpourri3d', got {method!r}")
 
 
+def _method_code(method: str, use_robust: bool) -> int:
+    """The library's method code, with ``use_robust`` (mollified side lengths for the
+    half-cotangents, ordito ``heat.py:229-243``) as its flag. At ``"potpourri3d"`` the reference's
+    ``use_robust`` runs geometry-central's fully intrinsic method on an intrinsic Delaunay
+    triangulation of the tufted cover (``heat.py:425-430``), which this library does not build."""
+    if use_robust and method == "potpourri3d":
+        raise NotImplementedError(
+            "use_robust at method='potpourri3d' (geometry-central's intrinsic Delaunay heat "
+            "method) is not implemented; method='meshlab' and 'igl' take use_robust"
+        )
+    return _METHOD_CODES[method] | (_lib.HG_METHOD_ROBUST if use_robust else 0)
+
+
 def _require_precision(precision: str) -> int:
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
@@ -206,7 +219,8 @@ def heat_operators(
     edges; igl and potpourri3d: the unique edges); the lumped mass is barycentric, or Voronoi at
     ``method="igl"``. A ``cot_entries`` table, when given, must have shape
     ``(n_faces, 3)``; the table is formed from the positions either way (what
-    ``cotmatrix_entries`` forms). ``pattern`` is accepted for signature compatibility.
+    ``cotmatrix_entries`` forms), or with ``use_robust`` from the mollified side lengths.
+    ``pattern`` is accepted for signature compatibility.
     """
     if cot_entries is not None and use_robust:
         raise ValueError(
@@ -214,8 +228,7 @@ def heat_operators(
             "half-cotangent table from mollified edge lengths."
         )
     _require_method(method, geodesic=False)
-    if use_robust:
-        raise NotImplementedError("use_robust (mollified edge lengths) is not implemented")
+    code = _method_code(method, use_robust)
     v = _arrays.vertices_tensor(vertices)
     f = _arrays.faces_tensor(faces)
     if cot_entries is not None:
@@ -232,7 +245,7 @@ def heat_operators(
             "hg_operators",
             handle.ptr,
             math.nan if t is None else float(t),
-            _METHOD_CODES[method],
+            code,
             0,
         )
         return _operators_from_views(_views(handle))
@@ -260,8 +273,7 @@ class HeatSolver:
         precision: str = "fp64",
     ) -> None:
         _require_method(method, geodesic=False)
-        if use_robust:
-            raise NotImplementedError("use_robust (mollified edge lengths) is not implemented")
+        self._method_code = _method_code(method, use_robust)
         self.t = None if t is None else float(t)
         self.use_robust = bool(use_robust)
         self.method = method
@@ -291,7 +303,7 @@ class HeatSolver:
                 "hg_operators",
                 handle.ptr,
                 math.nan if self.t is None else float(self.t),
-                _METHOD_CODES[self.method],
+                self._method_code,
                 1,
             )
             self._handle = handle
@@ -313,6 +325,11 @@ class HeatSolver:
     @property
     def settle_rounds(self) -> int:
         return int(_views(self._native()).settle_rounds)
+
+    @property
+    def mollify_delta(self) -> float:
+        """With ``use_robust``, the constant added to every side length (0 on a clean mesh)."""
+        return float(_views(self._native()).mollify_delta)
 
     @property
     def ill_conditioned(self) -> bool:
@@ -724,13 +741,19 @@ def heat_geodesic(
     ``(V,)`` float64 -- a torch tensor on the GPU when the vertices were one, a numpy array
     otherwise.
 
-    Raises ``ValueError`` for an unknown ``method`` or a source outside ``[0, n_vertices)``.
+    ``use_robust`` (``"meshlab"`` and ``"igl"``) builds the half-cotangents from side lengths
+    mollified by the smallest constant that leaves every triangle's triangle-inequality margin
+    ``1e-5`` of the mean side (``heat.py:229-243``, ``laplacian.py:779-874``): a mesh with
+    degenerate triangles then solves; a clean mesh's weights move only by float32 rounding.
+
+    Raises ``ValueError`` for an unknown ``method`` or a source outside ``[0, n_vertices)``, and
+    ``NotImplementedError`` for ``use_robust`` at ``"potpourri3d"`` (geometry-central's intrinsic
+    Delaunay method).
     """
     if method not in _METHODS:
         raise ValueError(f"method must be 'meshlab', 'igl' or 'potpourri3d', got {method!r}")
     _require_method(method, geodesic=True)
-    if use_robust:
-        raise NotImplementedError("use_robust (mollified edge lengths) is not implemented")
+    _method_code(method, use_robust)
     _require_precision(precision)
     bound, owned, sources, t, template = _resolve_arguments(mesh, faces, sources, t, vertices)
     torch = _arrays.torch_module()

@@ -35,6 +35,7 @@ MESHES = {
     "saddle_graded": lambda: __import__("tests.meshes", fromlist=["x"]).graded_saddle(),
     "two_bodies": lambda: _two_bodies(),
     "hub_disk": lambda: __import__("tests.meshes", fromlist=["x"]).hub_disk(),
+    "pinched_grid": lambda: __import__("tests.meshes", fromlist=["x"]).pinched_grid(),
 }
 
 

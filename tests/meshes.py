@@ -120,3 +120,22 @@ def hub_disk(spokes: int = 400, rings: int = 20, seed: int = 7) -> tuple[np.ndar
     down = np.cross(p1 - p0, p2 - p0)[:, 2] < 0.0
     tri[down, 1], tri[down, 2] = tri[down, 2].copy(), tri[down, 1].copy()
     return vertices.astype(np.float32), tri.astype(np.int32).reshape(-1)
+
+
+# The needle of pinched_grid: an interior face of the 30 x 24 grid.
+PINCHED_FACE = 2 * 29 * 23 // 2 + 7
+
+
+def pinched_grid() -> tuple[np.ndarray, np.ndarray]:
+    """
+    The jittered 30 x 24 planar grid with one interior triangle's third corner moved onto the
+    midpoint of its opposite side: a needle of (nearly) zero area, the degenerate face that
+    ``use_robust``'s mollified lengths exist for.
+    """
+    from ordito_amd import creation
+
+    v, f = creation.planar_grid(30, 24, 1.0, 0.2, 4)
+    v, tri = np.array(v, np.float32), np.asarray(f).reshape(-1, 3)
+    a, b, c = tri[PINCHED_FACE]
+    v[c] = (v[a] + v[b]) * np.float32(0.5)
+    return v, tri.reshape(-1).astype(np.int32)

@@ -503,3 +503,48 @@ def test_graph_replays_equal_the_eager_path(cuda, name):
     expected = ho.geodesic(vertices, faces, sets[0], method=method).distance
     worst, _ = ho.relative_error(eager[0], expected)
     assert worst <= 1e-9, worst
+
+
+@pytest.mark.parametrize(
+    "name, method",
+    [("icosphere4", "meshlab"), ("bumpy", "meshlab"), ("pinched_grid", "meshlab"),
+     ("pinched_grid", "igl")],
+)
+def test_use_robust_matches_the_oracle(cuda, name, method):
+    """use_robust (heat.py:229-243): the half-cotangent table from the mollified side lengths bit
+    for bit, the constant itself, and the factored and one-shot distances against the oracle built
+    on the same weights (1e-9 and 1e-5 of the range). The pinched grid's needle, 6e6 by the
+    positions, weighs about 1e2 mollified; a clean mesh needs no constant."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh(name)
+    lengths, delta = ho.mollify_intrinsic(vertices, faces)
+    assert (delta > 0.0) == (name == "pinched_grid")
+    print(f"use_robust: {name} {method}", flush=True)
+    trimesh = heat.Trimesh(_tensor(cuda, vertices), _tensor(cuda, faces))
+    solver = trimesh.heat_solver(method=method, use_robust=True)
+    cot = solver.operators[5].cpu().numpy()
+    assert solver.mollify_delta == delta
+    assert np.array_equal(cot, ho.intrinsic_half_cotangents(lengths))
+    assert solver.factor()
+    sets = [np.array([0], np.int32), np.array([len(vertices) // 2], np.int32),
+            np.array([3, len(vertices) - 5], np.int32)]
+    got = solver.geodesic([_tensor(cuda, q) for q in sets]).cpu().numpy()
+    oracle = ho.FactoredOracle(vertices, faces, method=method, use_robust=True)
+    for q, sources in enumerate(sets):
+        worst, _ = ho.relative_error(got[q], oracle.distance(sources))
+        assert worst <= 1e-9, (q, worst)
+    one_shot = heat.heat_geodesic(vertices, faces, sets[1], method=method, use_robust=True)
+    worst, _ = ho.relative_error(one_shot, oracle.distance(sets[1]))
+    assert worst <= 1e-5, worst
+
+
+def test_use_robust_at_potpourri3d_is_refused(cuda):
+    """The reference's use_robust at "potpourri3d" is geometry-central's intrinsic Delaunay
+    method (heat.py:425-430), not built here: refused before any work, not silently replaced."""
+    from ordito_amd import heat
+
+    vertices, faces = mesh("icosphere3")
+    with pytest.raises(NotImplementedError):
+        heat.heat_geodesic(vertices, faces, np.array([0], np.int32), method="potpourri3d",
+                           use_robust=True)

@@ -582,3 +582,97 @@ def test_potpourri3d_stays_within_the_great_circle_bound_and_differs_from_meshla
     assert 1e-9 < gap < 0.03
     factored = ho.FactoredOracle(v, f, method="potpourri3d").distance(np.array([0]))
     assert np.allclose(factored, d, atol=1e-12 * np.ptp(d))
+
+
+# ---- use_robust: mollified side lengths ----------------------------------------------------------
+
+
+def _box() -> tuple[np.ndarray, np.ndarray]:
+    """The unit cube's 8 corners and 12 outward-wound triangles."""
+    v = np.array([[x, y, z] for x in (0, 1) for y in (0, 1) for z in (0, 1)], np.float32)
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    tri = []
+    for a, b, c, d in quads:
+        tri += [(a, b, c), (a, c, d)]
+    return v, np.array(tri, np.int32).reshape(-1)
+
+
+def test_intrinsic_half_cotangents_reference_known_answers():
+    """laplacian.py:319-322: the equilateral triangle weighs each side cot(60)/2, the 3-4-5
+    triangle its hypotenuse nothing, then 0.375 and 2/3."""
+    got = ho.intrinsic_half_cotangents(np.array([[1, 1, 1], [5, 4, 3]], np.float32))
+    assert np.allclose(got[0], 1.0 / (2.0 * math.sqrt(3.0)), rtol=1e-6)
+    assert got[1, 0] == 0.0
+    assert np.allclose(got[1, 1:], [0.375, 2.0 / 3.0], rtol=1e-6)
+
+
+def test_mollify_reference_known_answers():
+    """laplacian.py:830-837: a cube needs no constant; the collinear triangle of sides 1, 2, 1
+    needs 1e-5 of its mean side 4/3, added to every side."""
+    v, f = _box()
+    lengths, delta = ho.mollify_intrinsic(v, f)
+    assert delta == 0.0 and np.array_equal(lengths, ho.face_edge_lengths(v, f))
+    v = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0]], np.float32)
+    lengths, delta = ho.mollify_intrinsic(v, np.array([0, 1, 2]))
+    assert math.isclose(delta, 1e-5 * 4.0 / 3.0, rel_tol=1e-6)
+    assert np.array_equal(lengths, np.array([[1, 2, 1]], np.float32) + np.float32(delta))
+
+
+def test_mollified_margins_reach_the_tolerance_and_clean_meshes_keep_their_weights():
+    """After mollification every triangle-inequality margin is at least epsilon times the mean
+    side (the constant's definition, up to float32 rounding of the sum); on a clean mesh the
+    law-of-cosines weights equal the position-based ones within float32 rounding -- two formulas
+    for the same numbers, so a wrong side, sign or Heron ordering fails one of them."""
+    from tests.meshes import PINCHED_FACE, pinched_grid
+
+    v, f = pinched_grid()
+    face = PINCHED_FACE
+    lengths, delta = ho.mollify_intrinsic(v, f)
+    base = ho.face_edge_lengths(v, f)
+    assert delta > 0.0
+    eps = 1e-5 * float(base.astype(np.float64).mean())
+    a, b, c = (lengths[:, k].astype(np.float64) for k in range(3))
+    assert min((a + b - c).min(), (b + c - a).min(), (c + a - b).min()) >= eps * (1 - 1e-3)
+    # The needle's weights are finite and large after mollification, zero-area guarded before.
+    cot = ho.intrinsic_half_cotangents(lengths)
+    assert np.isfinite(cot).all() and np.abs(cot[face]).max() > 100.0
+    for name in ("icosphere4", "bumpy", "saddle_graded"):
+        vv, ff = mesh(name)
+        lengths, delta = ho.mollify_intrinsic(vv, ff)
+        assert delta == 0.0, name
+        ext = ho.face_terms(vv, ff).cot
+        got = ho.intrinsic_half_cotangents(lengths)
+        scale = np.maximum(np.abs(ext), 1.0)
+        # Thin faces (the hull's, the saddle's needles) lose a few float32 digits to the law of
+        # cosines' cancellation; a wrong side or sign is off by O(1).
+        assert (np.abs(got - ext) / scale).max() < (1e-5 if name == "icosphere4" else 1e-3), name
+
+
+def test_heron_takes_the_largest_side_first():
+    """kernels/predicates.py:197-210: Kahan's ordering keeps a needle's float32 double area within
+    a few ulp of the float64 one; the smallest side first loses digits on the graded saddle's
+    needles. The pin compares against float64 Heron of the same float32 sides."""
+    vv, ff = mesh("saddle_graded")
+    lengths = ho.face_edge_lengths(vv, ff)
+    srt = np.sort(lengths.astype(np.float64), axis=1)
+    c, b, a = srt[:, 0], srt[:, 1], srt[:, 2]
+    exact = 0.5 * np.sqrt(np.maximum((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c)), 0))
+
+    def dbl_area(order_largest_first: bool) -> np.ndarray:
+        s32 = np.sort(lengths, axis=1)
+        if order_largest_first:
+            c32, b32, a32 = s32[:, 0], s32[:, 1], s32[:, 2]
+        else:
+            a32, b32, c32 = s32[:, 0], s32[:, 1], s32[:, 2]
+        arg = (a32 + (b32 + c32)) * (c32 - (a32 - b32)) * (c32 + (a32 - b32)) * (a32 + (b32 - c32))
+        return (np.float32(0.5) * np.sqrt(np.maximum(arg, np.float32(0)))).astype(np.float64)
+
+    right = np.abs(dbl_area(True) - exact) / exact
+    wrong = np.abs(dbl_area(False) - exact) / exact
+    assert right.max() < 1e-5
+    assert wrong.max() > 10 * right.max()
+    # And the oracle's weights are the right ordering's.
+    cot = ho.intrinsic_half_cotangents(lengths).astype(np.float64)
+    l2 = lengths.astype(np.float64) ** 2
+    want = (l2[:, 1] + l2[:, 2] - l2[:, 0]) / (4 * exact)
+    assert (np.abs(cot[:, 0] - want) / np.maximum(np.abs(want), 1.0)).max() < 1e-4
