@@ -85,12 +85,30 @@ def parse() -> argparse.Namespace:
     )
     parser.add_argument(
         "--workload",
-        choices=("torus", "sphere", "batch", "serve", "igl"),
+        choices=("torus", "sphere", "batch", "serve", "igl", "oneshot"),
         default="torus",
         help="torus: the headline configuration (default); sphere: the 2.6 M-vertex displaced "
         "sphere, mixed and FP64 against the oracle; batch: many meshes packed into one system; "
         "serve: the continuous-batching server under Poisson arrivals; igl: method='igl' on an "
-        "open grid (its rim-pinned heat solve), beside method='meshlab'",
+        "open grid (its rim-pinned heat solve), beside method='meshlab'; oneshot: only the one-shot "
+        "(iterated) calls on icosphere(4) and the torus",
+    )
+    parser.add_argument(
+        "--summarize-profiles",
+        metavar="DIR",
+        help="print the markdown summary of the rocprofv3 outputs under DIR (kt/, fetch/, write/, "
+        "mfma/, copy_fetch/, copy_write/) and exit",
+    )
+    parser.add_argument(
+        "--traffic-out",
+        help="with --summarize-profiles: write the sweep's measured HBM bytes per sweep there "
+        "(the record bench.py reads back as roofline.traffic)",
+    )
+    parser.add_argument(
+        "--solve-block-entries",
+        type=int,
+        default=0,
+        help="with --traffic-out: the profiled plan's stored solve-block entries",
     )
     many = parser.add_argument_group("--workload batch / serve")
     many.add_argument("--meshes", type=int, help="default 256 (batch), 64 (serve)")
@@ -523,19 +541,37 @@ def one_shot_result(oracle_c2, c2_vertices, c2_faces, c2_source: int) -> dict:
     src = torch.tensor([c2_source], dtype=torch.int32, device="cuda")
     lat = []
     stats = {}
+    cg_ms = []
+    plan = {}
     for _ in range(2):
         t0 = time.perf_counter()
         trimesh = heat.Trimesh(vg, fg)
+        solver = trimesh.heat_solver(method="meshlab")
+        solver.enable_timing(True)
         got = heat.heat_geodesic(trimesh, src)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
-        stats = dict(trimesh.heat_solver(method="meshlab").last_stats)
-        del trimesh
+        stats = dict(solver.last_stats)
+        cg_ms.append(solver.read_timing(reset=True)["cg"][0])
+        view = heat._views(solver._native())
+        plan = {"n": int(view.n_vertices), "nnz": int(view.nnz)}
+        del solver, trimesh
+    # The settled Jacobi-CG's algorithmic bytes a round (one column): the FP32 operator values and
+    # their columns once, the row offsets, the direction gathered once, and the Chronopoulos-Gear
+    # update's seven float64 vector reads (u, p, w, s, x, r, the inverse diagonal) and six writes
+    # (w, p, s, x, r, u).
+    rounds = max(stats.get("heat_rounds", 0), 1)
+    round_bytes = plan["nnz"] * 8 + plan["n"] * (4 + 14 * 8)
+    round_s = float(np.mean(cg_ms)) / 1e3 / rounds
     out["torus1024x512"] = {
         "n_vertices": int(len(c2_vertices)),
         "mean_ms": round(1e3 * float(np.mean(lat)), 3),
         "calls": len(lat),
         "rounds": stats,
+        "heat_cg_ms": round(float(np.mean(cg_ms)), 3),
+        "cg_round_us": round(1e6 * round_s, 2),
+        "cg_round_bytes": round_bytes,
+        "cg_round_gbs": round(round_bytes / round_s / 1e9, 1),
         "rel_err_max": ho.relative_error(got.cpu().numpy(), oracle_c2.distance(np.array([c2_source])))[0],
     }
     log(f"   one-shot torus: {out['torus1024x512']}")
@@ -575,6 +611,155 @@ def extra_workloads(args, oracle_c2, c2_vertices, c2_faces, c2_sources) -> dict:
     return extras
 
 
+# The served window of a profiled C2 run: every dispatch from the first to the last 64-column
+# sweep tile, whose sweep kernels (tiles and split-K reductions) make the triangular sweeps.
+_SWEEP_TILES = ("fwd_tile_kernel<double", "bwd_tile_kernel<double")
+_SWEEP_KERNELS = _SWEEP_TILES + ("fwd_reduce_kernel", "bwd_reduce_kernel")
+
+
+def _short_name(name: str) -> str:
+    import re
+
+    name = name.replace("hg::(anonymous namespace)::", "").replace("hg::", "").replace("void ", "")
+    return re.sub(r"\(.*", "", name)
+
+
+def _rows(directory: Path, suffix: str) -> list[dict]:
+    import csv
+
+    files = sorted(directory.rglob(f"*{suffix}"))
+    return list(csv.DictReader(files[0].open())) if files else []
+
+
+def _dispatch_counters(directory: Path) -> dict[int, dict]:
+    out: dict[int, dict] = {}
+    for r in _rows(directory, "counter_collection.csv"):
+        d = out.setdefault(int(r["Dispatch_Id"]), {"name": _short_name(r["Kernel_Name"])})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def _copy_scale(directory: Path, counter: str) -> float:
+    """Bytes a 1 GiB device copy moves per KiB its counter reads (reads: 1 GiB, writes: 1 GiB)."""
+    copies = [d for d in _dispatch_counters(directory).values() if "copy" in d["name"].lower()]
+    if not copies:
+        return float("nan")
+    kib = max(d.get(counter, 0.0) for d in copies)
+    return (1 << 30) / (kib * 1024.0) if kib > 0 else float("nan")
+
+
+def _window(names: list[str]) -> tuple[int, int]:
+    idx = [i for i, n in enumerate(names) if n.startswith(_SWEEP_TILES)]
+    return (idx[0], idx[-1] + 1) if idx else (0, 0)
+
+
+def summarize_profiles(root: Path, traffic_out: Path | None = None,
+                       solve_block_entries: int = 0) -> str:
+    """Markdown summary of the profiling runs listed in profiles/triangular_sweep_torus1024x512.md:
+    per-level sweep times, per-kernel times and HBM traffic (FETCH_SIZE / WRITE_SIZE scaled by a
+    1 GiB copy's readings), the factorization's time and matrix-core busy fraction, the one-shot
+    CG."""
+    import collections
+
+    lines: list[str] = []
+    trace = sorted(_rows(root / "kt", "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
+    names = [_short_name(r["Kernel_Name"]) for r in trace]
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace]
+    lo, hi = _window(names)
+    sweeps = sum(1 for n in names[lo:hi] if n.startswith("fwd_tile_kernel<double, false>"))
+    sweep_us = sum(d for n, d in zip(names[lo:hi], dur[lo:hi]) if n.startswith(_SWEEP_KERNELS))
+    lines.append(f"- sweeps in the served window: {sweeps}")
+    lines.append(f"- sweep kernels' time per sweep (rocprofv3): **{sweep_us / max(sweeps, 1) / 1e3:.4f} ms**")
+    # The last sweep level by level.
+    starts = [i for i in range(lo, hi) if names[i].startswith("fwd_tile_kernel<double, false>")]
+    if starts:
+        lines += ["", "One sweep, launch by launch (the last of the window):", "",
+                  "| launch | grid | µs |", "|---|---:|---:|"]
+        i = starts[-1]
+        while i < hi and names[i].startswith(_SWEEP_KERNELS):
+            grid = f"{trace[i].get('Grid_Size_X', trace[i].get('Grid_Size', '?'))}"
+            lines.append(f"| `{names[i]}` | {grid} | {dur[i]:.2f} |")
+            i += 1
+            if i < hi and names[i].startswith("fwd_tile_kernel<double, false>"):
+                break
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for n, d in zip(names[lo:hi], dur[lo:hi]):
+        agg[n][0] += 1
+        agg[n][1] += d
+    lines += ["", "Kernels in the served window:", "", "| kernel | calls | total ms | avg µs |",
+              "|---|---:|---:|---:|"]
+    for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"| `{n}` | {c} | {t / 1e3:.3f} | {t / c:.2f} |")
+    whole = collections.defaultdict(lambda: [0, 0.0])
+    for n, d in zip(names, dur):
+        whole[n][0] += 1
+        whole[n][1] += d
+    avg_us = {n: t / c for n, (c, t) in whole.items()}
+    # HBM traffic.
+    fs = _copy_scale(root / "copy_fetch", "FETCH_SIZE")
+    ws = _copy_scale(root / "copy_write", "WRITE_SIZE")
+    fetch = _dispatch_counters(root / "fetch")
+    write = _dispatch_counters(root / "write")
+    lines += ["", f"Counter scales from a 1 GiB device copy: FETCH_SIZE x {fs:.4f}, WRITE_SIZE x {ws:.4f}"
+              " (bytes per KiB read)."]
+    per_sweep: dict[str, float] = {}
+    for label, counters, key, scale in (("FETCH_SIZE", fetch, "FETCH_SIZE", fs),
+                                        ("WRITE_SIZE", write, "WRITE_SIZE", ws)):
+        ids = sorted(counters)
+        nm = [counters[d]["name"] for d in ids]
+        a, b = _window(nm)
+        n_sw = sum(1 for x in nm[a:b] if x.startswith("fwd_tile_kernel<double, false>"))
+        total = sum(counters[ids[j]].get(key, 0.0) for j in range(a, b) if nm[j].startswith(_SWEEP_KERNELS))
+        per_sweep[label] = total * 1024 * scale / max(n_sw, 1)
+        lines.append(f"- {label}: {per_sweep[label] / 1e9:.3f} GB per sweep "
+                     f"({n_sw} sweeps in that run's window)")
+    lines += ["", "Per-kernel HBM bytes per dispatch (counters x scale) and the rate over the kernel "
+              "trace's mean duration:", "", "| kernel | dispatches | read MB | written MB | µs | GB/s |",
+              "|---|---:|---:|---:|---:|---:|"]
+    per = collections.defaultdict(lambda: [0, 0.0, 0.0])
+    for d in fetch.values():
+        per[d["name"]][0] += 1
+        per[d["name"]][1] += d.get("FETCH_SIZE", 0.0) * 1024 * fs
+    for d in write.values():
+        per[d["name"]][2] += d.get("WRITE_SIZE", 0.0) * 1024 * ws
+    for n, (c, rd, wr) in sorted(per.items(), key=lambda kv: -(kv[1][1] + kv[1][2]))[:16]:
+        us = avg_us.get(n)
+        rate = (rd + wr) / c / (us * 1e-6) / 1e9 if us else float("nan")
+        lines.append(f"| `{n}` | {c} | {rd / c / 1e6:.2f} | {wr / c / 1e6:.2f} | "
+                     f"{us if us else float('nan'):.1f} | {rate:.0f} |")
+    if traffic_out is not None and per_sweep:
+        record = {
+            "kernel": "triangular sweep (forward + backward)",
+            "workload": "torus 1024x512 R=1 r=0.3 normal jitter 0.1h, 64 single-source queries per "
+            "step as one batch, factors resident",
+            "columns": 64,
+            "solve_block_entries": solve_block_entries,
+            "factor_bytes": 8,
+            "traffic_bytes_per_sweep": round(per_sweep["FETCH_SIZE"] + per_sweep["WRITE_SIZE"]),
+            "fetch_bytes": round(per_sweep["FETCH_SIZE"]),
+            "write_bytes": round(per_sweep["WRITE_SIZE"]),
+            "fetch_scale": round(fs, 4),
+            "write_scale": round(ws, 4),
+            "solve_hip_sha256_16": _solve_source_hash(),
+        }
+        traffic_out.write_text(json.dumps(record, indent=1) + "\n")
+    # Factorization: time and matrix-core busy.
+    mfma = _dispatch_counters(root / "mfma")
+    fact = ("panel_diag_kernel", "panel_apply_kernel", "update_kernel", "extend_add_kernel")
+    lines += ["", "Numeric factorization kernels (whole run: both systems' factorizations), time and "
+              "matrix-core busy (SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE x 1024 SIMDs / 8 XCDs):",
+              "", "| kernel | calls | total ms | MFMA busy | FP64 MFMA GFLOP |", "|---|---:|---:|---:|---:|"]
+    for n in fact:
+        c, t = whole.get(n, [0, 0.0])
+        busy = [d for d in mfma.values() if d["name"] == n]
+        cycles = sum(d.get("GRBM_GUI_ACTIVE", 0.0) for d in busy) / 8.0
+        mfma_busy = sum(d.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) for d in busy)
+        flops = sum(d.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) for d in busy) * 512
+        frac = mfma_busy / (cycles * 1024) if cycles else float("nan")
+        lines.append(f"| `{n}` | {c} | {t / 1e3:.3f} | {100 * frac:.1f}% | {flops / 1e9:.2f} |")
+    return "\n".join(lines)
+
+
 def log(message: str) -> None:
     print(message, file=sys.stderr, flush=True)
 
@@ -595,18 +780,22 @@ def calibrate_copy() -> int:
 TRAFFIC_FILE = ROOT / "profiles" / "triangular_sweep_torus1024x512_traffic.json"
 
 
-def measured_traffic(columns: int, solve_block_entries: int, factor_bytes: int) -> float | None:
-    """HBM bytes per sweep from the committed FETCH_SIZE / WRITE_SIZE passes, when they were taken
-    on this workload with the sweep kernels' current source; otherwise None."""
+def _solve_source_hash() -> str:
     import hashlib
 
+    return hashlib.sha256((ROOT / "ordito_amd" / "csrc" / "solve.hip").read_bytes()).hexdigest()[:16]
+
+
+def measured_traffic(columns: int, solve_block_entries: int, factor_bytes: int) -> float | None:
+    """HBM bytes per sweep from the committed FETCH_SIZE / WRITE_SIZE passes, when they were taken
+    on this workload (its column count, stored solve-block entries and factor width) with the sweep
+    kernels' current source; otherwise None."""
     try:
         record = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None
-    source = (ROOT / "ordito_amd" / "csrc" / "solve.hip").read_bytes()
     if (
-        record.get("solve_hip_sha256_16") != hashlib.sha256(source).hexdigest()[:16]
+        record.get("solve_hip_sha256_16") != _solve_source_hash()
         or record.get("columns") != columns
         or record.get("solve_block_entries") != solve_block_entries
         or record.get("factor_bytes") != factor_bytes
@@ -617,13 +806,15 @@ def measured_traffic(columns: int, solve_block_entries: int, factor_bytes: int) 
 
 def sweep_bytes(stats: dict[str, int], columns: int, factor_bytes: int, first_round: bool) -> float:
     """Algorithmic bytes of one forward + backward sweep over `columns` right-hand sides: the
-    inverse factors read twice, and per column these float64 vector touches -- forward: r read,
+    inverse factors' structural nonzeros (each front's lower-triangular leading block and the rows
+    below it; not the zero triangle the stored blocks carry) read twice, and per column these
+    float64 vector touches -- forward: r read,
     y write (n each), update vectors written and gathered by the parents (structure rows each);
     backward: y read, d write (n each), d gathered at the structure rows, and after the first
     round x read and x + d write (n each; the first round starts from x = 0 and its d is x)."""
     n = stats["rows"]
     structure_rows = stats["front_vector_entries"] - n
-    w = stats["solve_block_entries"]
+    w = stats["solve_block_nonzeros"]
     vector_n = 4.0 if first_round else 6.0
     return 2.0 * w * factor_bytes + columns * (vector_n * n + 3.0 * structure_rows) * 8.0
 
@@ -663,6 +854,10 @@ def ms(values, q) -> float:
 
 def main() -> int:
     args = parse()
+    if args.summarize_profiles:
+        out = Path(args.traffic_out) if args.traffic_out else None
+        print(summarize_profiles(Path(args.summarize_profiles), out, args.solve_block_entries))
+        return 0
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: the heat-geodesic path here is built and measured on a "
@@ -688,6 +883,21 @@ def main() -> int:
         return sphere_workload(args)
     if args.workload == "igl":
         return igl_workload(args)
+    if args.workload == "oneshot":
+        log("== one-shot (iterated) calls")
+        vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)
+        from oracle import heat_oracle as ho
+
+        source = int(creation.random_sources(vertices.shape[0], args.queries, seed=1)[0])
+        line = {
+            "metric": "one-shot heat_geodesic calls (operators built and both solves iterated "
+            "each call)",
+            "n_gpus": 1,
+            "data": "synthetic",
+            "workloads": one_shot_result(ho.FactoredOracle(vertices, faces), vertices, faces, source),
+        }
+        print(json.dumps(line))
+        return 0
     log(f"== generating torus {args.major}x{args.minor}")
     vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)
     n = vertices.shape[0]
@@ -777,6 +987,7 @@ def main() -> int:
             "supernodes": stats["supernodes"],
             "levels": stats["levels"],
             "solve_block_entries": stats["solve_block_entries"],
+            "solve_block_nonzeros": stats["solve_block_nonzeros"],
         },
         "p50_ms": ms(latencies, 50),
         "p99_ms": ms(latencies, 99),

@@ -390,6 +390,15 @@ int hg_device_bytes(hg_mesh* mesh, int64_t* out_bytes);
 int hg_plan_stats(hg_mesh* mesh, int64_t* out_stats8);
 
 /*
+ * The inverse factors' entry counts (analysing if needed): out3[0] the structural nonzeros (each
+ * front's lower-triangular leading block and its rows below), out3[1] the entries one sweep
+ * direction reads (whole 64-row tiles of the leading block), out3[2] the stored entries (every
+ * front's full ncol x m block). No counterpart in the reference: the algorithmic bytes of the
+ * sweep's roofline are the nonzeros'.
+ */
+int hg_inverse_factor_entries(hg_mesh* mesh, int64_t* out3);
+
+/*
  * Host-only access to the Cholesky analysis (sparse_cholesky's ordering and symbolic phase,
  * cholesky.py:538-1273), for inspection and tests; needs no GPU. The pattern is a symmetric CSR
  * with sorted rows, xyz float32 (n,3) host coordinates for the geometric nested dissection.

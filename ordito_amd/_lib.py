@@ -50,6 +50,7 @@ EXPORTED = (
     "hg_set_memory_budget",
     "hg_set_budget_probe",
     "hg_set_settle_rule",
+    "hg_inverse_factor_entries",
     "hg_set_refinement",
     "hg_set_plan_options",
     "hg_diffuse",
@@ -173,6 +174,7 @@ _SIGNATURES = {
     "hg_set_memory_budget": ([_P, _I64], ctypes.c_int),
     "hg_set_budget_probe": ([_P, _I32], ctypes.c_int),
     "hg_set_settle_rule": ([_P, _I32, ctypes.c_double, _I32], ctypes.c_int),
+    "hg_inverse_factor_entries": ([_P, _PI64], ctypes.c_int),
     "hg_set_graphs": ([_P, _I32], ctypes.c_int),
     "hg_graph_state": ([_P, _PI64, _PI32], ctypes.c_int),
     "hg_solve_poisson": ([_P, _P, _P, _I64, _I32, _I32], ctypes.c_int),

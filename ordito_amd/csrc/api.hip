@@ -1911,6 +1911,29 @@ int hg_plan_stats(hg_mesh* mesh, int64_t* out) {
   });
 }
 
+int hg_inverse_factor_entries(hg_mesh* mesh, int64_t* out3) {
+  return guarded([&] {
+    require_mesh(mesh);
+    HG_REQUIRE(out3 != nullptr, HG_ERR_VALUE, "out3 is null");
+    ensure_operators(*mesh);
+    ensure_plan(*mesh);
+    const Plan& p = *mesh->host_plan;
+    int64_t nonzero = 0, read = 0, stored = 0;
+    for (int32_t s = 0; s < p.n_super; ++s) {
+      const int64_t nc = p.ncol[s], m = nc + p.nrow[s];
+      stored += nc * m;
+      nonzero += nc * (nc + 1) / 2 + (m - nc) * nc;
+      // A forward tile of rows [t, t + 64) reads inner [0, min(nc, t + 64)) inside the leading
+      // block and all nc below it (the backward reads the transposed set).
+      for (int64_t t = 0; t < m; t += 64)
+        read += std::min<int64_t>(64, m - t) * (t < nc ? std::min<int64_t>(nc, t + 64) : nc);
+    }
+    out3[0] = nonzero;
+    out3[1] = read;
+    out3[2] = stored;
+  });
+}
+
 struct hg_plan {
   Plan plan;
 };

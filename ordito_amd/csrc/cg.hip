@@ -38,8 +38,9 @@ struct CgArgs {
   double* da;
   double* db;
   double* partials;  // [k][G][3], then [G][2] for the settle check, then [k][G] for |b|^2
-  // [0] stepped rounds, [1] set when a grid barrier gave up waiting, [2] blocks arrived at the
-  // current barrier, [3] barrier generation; [1..3] zero at launch.
+  // [0] stepped rounds, [1] set when a grid barrier gave up waiting, [2] barrier groups arrived
+  // at the current barrier, [3] barrier generation, [4, 12) each group's blocks arrived; [1..12)
+  // zero at launch.
   int32_t* counters;
   int64_t spin_limit;  // polls of a grid barrier before it gives up (every block then exits)
   int32_t settle;
@@ -59,12 +60,17 @@ struct SyncState {
   int32_t aborted;
 };
 
-// Grid-wide barrier over co-resident blocks (the launch is cooperative): a block's thread 0
-// releases the block's writes, counts itself in, and the last block to arrive resets the count and
-// advances the generation the others poll. The wait is bounded: past a.spin_limit polls, or once
+// Grid-wide barrier over co-resident blocks (the launch is cooperative), counted in two stages so
+// no counter takes more than about a sixteenth of the grid's arrivals: a block's thread 0 releases
+// the block's writes and counts itself into its group's counter (blocks b with the same b mod 8,
+// the XCD the hardware deals b to); the last of a group counts the group into the grid's counter,
+// and the last group resets the counts and advances the generation the others poll. The polls are
+// relaxed loads, one acquire after the wait. The wait is bounded: past a.spin_limit polls, or once
 // another block has given up, the block records the abort and stops waiting, and every later
 // barrier returns at once, so each block leaves its loop within a round and the kernel ends; the
 // host reads the abort and fails the call.
+constexpr int kBarrierGroups = 8;
+
 template <int kThreads, bool kSingle>
 __device__ __forceinline__ void barrier(const CgArgs& a, SyncState* sync) {
   if constexpr (kSingle) {
@@ -73,18 +79,29 @@ __device__ __forceinline__ void barrier(const CgArgs& a, SyncState* sync) {
     __syncthreads();
     if (threadIdx.x == 0 && !sync->aborted) {
       int32_t* abort = a.counters + 1;
-      int32_t* count = a.counters + 2;
+      int32_t* grid_count = a.counters + 2;
       int32_t* generation = a.counters + 3;
+      int32_t* group_count = a.counters + 4;
+      const int32_t blocks = static_cast<int32_t>(gridDim.x);
+      const int32_t group = static_cast<int32_t>(blockIdx.x) % kBarrierGroups;
+      const int32_t groups = min(blocks, kBarrierGroups);
+      const int32_t members = blocks / kBarrierGroups + (group < blocks % kBarrierGroups ? 1 : 0);
       const int32_t gen = sync->generation;
-      __threadfence();
-      const int32_t arrived = __hip_atomic_fetch_add(count, 1, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-      if (arrived == static_cast<int32_t>(gridDim.x) - 1) {
-        __hip_atomic_store(count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int32_t in_group = __hip_atomic_fetch_add(group_count + group, 1, __ATOMIC_ACQ_REL,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+      bool last = false;
+      if (in_group == members - 1) {
+        __hip_atomic_store(group_count + group, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int32_t in_grid = __hip_atomic_fetch_add(grid_count, 1, __ATOMIC_ACQ_REL,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+        last = in_grid == groups - 1;
+      }
+      if (last) {
+        __hip_atomic_store(grid_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(generation, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       } else {
         int64_t polls = 0;
-        while (__hip_atomic_load(generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        while (__hip_atomic_load(generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
           if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
               ++polls > a.spin_limit) {
             __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -93,39 +110,48 @@ __device__ __forceinline__ void barrier(const CgArgs& a, SyncState* sync) {
           }
           __builtin_amdgcn_s_sleep(2);
         }
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
       }
       sync->generation = gen + 1;
-      __threadfence();
     }
     __syncthreads();
   }
 }
 
-// Fixed-tree block sum of up to three values; every thread gets the result.
+// Fixed-order block sum of up to three values, every thread getting the same bits: a butterfly
+// within each wave (each step adds two lanes' values, which commute, so every lane holds the
+// same sum), then the waves' sums in wave order. Two block barriers.
 template <int kThreads, int kWidth>
 __device__ __forceinline__ void block_sum(double (&v)[kWidth], double* scratch) {
-  for (int c = 0; c < kWidth; ++c) scratch[c * kThreads + threadIdx.x] = v[c];
+  constexpr int kWaves = kThreads / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+#pragma unroll
+  for (int c = 0; c < kWidth; ++c)
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) v[c] += __shfl_xor(v[c], offset, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < kWidth; ++c) scratch[c * kWaves + wave] = v[c];
   __syncthreads();
-  for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
-    if (threadIdx.x < stride)
-      for (int c = 0; c < kWidth; ++c)
-        scratch[c * kThreads + threadIdx.x] += scratch[c * kThreads + threadIdx.x + stride];
-    __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kWidth; ++c) {
+    double total = 0.0;
+    for (int w = 0; w < kWaves; ++w) total += scratch[c * kWaves + w];
+    v[c] = total;
   }
-  for (int c = 0; c < kWidth; ++c) v[c] = scratch[c * kThreads];
   __syncthreads();
 }
 
 template <int kThreads>
 __device__ __forceinline__ double block_max(double v, double* scratch) {
-  scratch[threadIdx.x] = v;
+  constexpr int kWaves = kThreads / 64;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) v = fmax(v, __shfl_xor(v, offset, 64));
+  if (lane == 0) scratch[wave] = v;
   __syncthreads();
-  for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
-    if (threadIdx.x < stride)
-      scratch[threadIdx.x] = fmax(scratch[threadIdx.x], scratch[threadIdx.x + stride]);
-    __syncthreads();
-  }
   double out = scratch[0];
+  for (int w = 1; w < kWaves; ++w) out = fmax(out, scratch[w]);
   __syncthreads();
   return out;
 }
@@ -378,6 +404,201 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) a.counters[0] = stepped_rounds;
 }
 
+// One column of at most kRows * 1024 rows in one workgroup, the iteration of cg_kernel with its
+// state where one compute unit reaches it fastest: each thread's rows (tid + 1024 j, j < kRows)
+// keep x, r, u, p, s, prev and the Chebyshev residual in registers, and the vectors the products
+// read from every row (u, and the Chebyshev directions) live in LDS, so a product is one round
+// trip for the operator's entries (issued a row at a time, four or eight together) and LDS
+// gathers.
+// Same recurrences, stop rules, sums in fixed order (a thread's rows in order, then block_sum).
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallRows = 3;  // rows a thread: systems of at most 3,072 rows
+
+template <int kRows>
+__device__ __forceinline__ double small_row_product(const CgArgs& a, int32_t begin, int32_t end,
+                                                    const double* __restrict__ vec) {
+  constexpr int kBatch = kRows > 2 ? 4 : 8;
+  double sum = 0.0;
+  for (int32_t e0 = begin; e0 < end; e0 += kBatch) {
+    int32_t col[kBatch];
+    double val[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int32_t e = min(e0 + u, end - 1);
+      col[u] = a.columns[e];
+      val[u] = a.values32 ? static_cast<double>(a.values32[e]) : a.values64[e];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (e0 + u < end) sum += val[u] * vec[col[u]];
+  }
+  return sum;
+}
+
+template <int kRows, bool kCheb>
+__global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(CgArgs a) {
+  constexpr int kWaves = kSmallThreads / 64;
+  constexpr int kN = kRows * kSmallThreads;
+  __shared__ double u_lds[kN];
+  __shared__ double d_lds[2][kN];
+  __shared__ double scratch[3 * kWaves];
+  const int32_t n = static_cast<int32_t>(a.n);
+  const int t = threadIdx.x;
+  bool live[kRows];
+  int32_t begin[kRows], end[kRows];
+  // The Chebyshev residual and direction (the Poisson step) or the settle check's last iterate
+  // (the heat step): an instance holds one or the other.
+  constexpr int kCheck = kCheb ? 1 : kRows, kPoly = kCheb ? kRows : 1;
+  double x[kRows], r[kRows], u[kRows], p[kRows], sv[kRows], dinv[kRows];
+  double prev[kCheck], res[kPoly], dcur[kPoly];
+  double bb[1] = {0.0};
+#pragma unroll
+  for (int j = 0; j < kRows; ++j) {
+    const int32_t i = t + kSmallThreads * j;
+    live[j] = i < n;
+    begin[j] = live[j] ? a.offsets[i] : 0;
+    end[j] = live[j] ? a.offsets[i + 1] : 0;
+    dinv[j] = live[j] ? a.inv_diag[i] : 0.0;
+    x[j] = live[j] && a.guess ? a.guess[i] : 0.0;
+    u_lds[i] = x[j];
+  }
+  __syncthreads();
+  // r = b - A x0 (x0 the guess, else zero).
+#pragma unroll
+  for (int j = 0; j < kRows; ++j) {
+    const int32_t i = t + kSmallThreads * j;
+    const double b = live[j] ? a.rhs[i] : 0.0;
+    r[j] = live[j] && a.guess ? b - small_row_product<kRows>(a, begin[j], end[j], u_lds) : b;
+    if (live[j]) a.x[i] = x[j];
+    p[j] = 0.0;
+    sv[j] = 0.0;
+    if constexpr (!kCheb) prev[j] = 0.0;
+    u[j] = dinv[j] * r[j];
+    bb[0] += b * b;
+  }
+  block_sum<kSmallThreads, 1>(bb, scratch);
+  const double atol_sq = a.tol_sq * bb[0];
+  double gamma_old = INFINITY, alpha_old = 1.0;
+  const double sigma = a.cheb_steps > 0 ? a.theta / a.delta : 0.0;
+  // u = p(D^-1 A) D^-1 r for the Poisson step's rounds (chebyshev_apply's recurrence).
+  auto chebyshev = [&]() {
+    if constexpr (kCheb) {
+      double rho = 1.0 / sigma;
+#pragma unroll
+      for (int j = 0; j < kRows; ++j) {
+        const int32_t i = t + kSmallThreads * j;
+        const double c = dinv[j] * r[j];
+        const double d = c / a.theta;
+        res[j] = c;
+        dcur[j] = d;
+        u[j] = d;
+        if (live[j]) d_lds[0][i] = d;
+      }
+      __syncthreads();
+      int cur = 0;
+      for (int32_t step = 1; step < a.cheb_steps; ++step) {
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        const double keep = rho_new * rho;
+        const double push = 2.0 * rho_new / a.delta;
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) {
+          const int32_t i = t + kSmallThreads * j;
+          if (!live[j]) continue;
+          const double ad = small_row_product<kRows>(a, begin[j], end[j], d_lds[cur]);
+          res[j] = res[j] - dinv[j] * ad;
+          const double d = keep * dcur[j] + push * res[j];
+          dcur[j] = d;
+          u[j] += d;
+          d_lds[cur ^ 1][i] = d;
+        }
+        rho = rho_new;
+        cur ^= 1;
+        __syncthreads();
+      }
+    }
+  };
+  if (kCheb) chebyshev();
+#pragma unroll
+  for (int j = 0; j < kRows; ++j)
+    if (live[j]) u_lds[t + kSmallThreads * j] = u[j];
+  __syncthreads();
+
+  int32_t stepped_rounds = 0;
+  int64_t round = 0;
+  double settle_previous = 0.0, reach_check = 0.0, checks = 0.0;
+  bool running = true;
+  while (running) {
+    ++round;
+    double w[kRows];
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) {
+      w[j] = live[j] ? small_row_product<kRows>(a, begin[j], end[j], u_lds) : 0.0;
+      acc[0] += r[j] * u[j];
+      acc[1] += w[j] * u[j];
+      acc[2] += r[j] * r[j];
+    }
+    block_sum<kSmallThreads, 3>(acc, scratch);
+    double alpha = 0.0, beta = 0.0;
+    bool step = false;
+    if (acc[2] > atol_sq && round <= a.maxiter && acc[0] != 0.0) {
+      beta = acc[0] / gamma_old;
+      const double denominator = acc[1] - beta * acc[0] / alpha_old;
+      if (denominator != 0.0) {
+        alpha = acc[0] / denominator;
+        step = true;
+      }
+    }
+    if (!step) break;
+    gamma_old = acc[0];
+    alpha_old = alpha;
+    ++stepped_rounds;
+#pragma unroll
+    for (int j = 0; j < kRows; ++j) {
+      p[j] = u[j] + beta * p[j];
+      sv[j] = w[j] + beta * sv[j];
+      x[j] += alpha * p[j];
+      r[j] = r[j] - alpha * sv[j];
+      if (!kCheb) u[j] = dinv[j] * r[j];
+    }
+    if (round >= a.maxiter) running = false;
+    if (kCheb) chebyshev();
+    if (!kCheb && a.settle && round % a.check_rounds == 0) {
+      double change = 0.0;
+      double reached[1] = {0.0};
+#pragma unroll
+      for (int j = 0; j < kCheck; ++j) {
+        if (live[j] && x[j] != 0.0) {
+          change = fmax(change, fabs(x[j] - prev[j]) / fabs(x[j]));
+          reached[0] += 1.0;
+        }
+        prev[j] = x[j];
+      }
+      const double c = block_max<kSmallThreads>(change, scratch);
+      block_sum<kSmallThreads, 1>(reached, scratch);
+      checks += 1.0;
+      const double total = reached[0] + 1.0;
+      if (total != settle_previous) {
+        settle_previous = total;
+        reach_check = checks;
+      } else {
+        const double idle = (checks - reach_check) * static_cast<double>(a.check_rounds);
+        if (c < a.change_tolerance || idle >= static_cast<double>(a.settle_rounds)) running = false;
+      }
+    }
+    // The next round's product reads u from LDS: every thread is past this round's (block_sum's
+    // barriers), so the rows are rewritten, then published.
+#pragma unroll
+    for (int j = 0; j < kRows; ++j)
+      if (live[j]) u_lds[t + kSmallThreads * j] = u[j];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < kRows; ++j)
+    if (live[j]) a.x[t + kSmallThreads * j] = x[j];
+  if (t == 0) a.counters[0] = stepped_rounds;
+}
+
 // Componentwise backward error flag (kernels/linalg.py:1128-1230): "not within" so NaN flags.
 __global__ void backward_error_kernel(const int32_t* __restrict__ offsets,
                                       const int32_t* __restrict__ columns,
@@ -470,7 +691,7 @@ int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
   }
   const int64_t partial_size = k * G * 3 + 2 * G + k * G;
   if (ws.partials.size < partial_size) ws.partials.allocate(partial_size);
-  if (ws.counters.size < 4) ws.counters.allocate(4);
+  if (ws.counters.size < 4 + kBarrierGroups) ws.counters.allocate(4 + kBarrierGroups);
   CgArgs a{};
   a.n = n;
   a.k = static_cast<int32_t>(k);
@@ -505,8 +726,16 @@ int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
   a.theta = 0.5 * (problem.lambda_max + problem.lambda_min);
   a.delta = 0.5 * (problem.lambda_max - problem.lambda_min);
   void* args[] = {&a};
-  launch_zero(ws.counters.ptr, sizeof(int32_t) * 4, stream);
-  if (single) {
+  launch_zero(ws.counters.ptr, sizeof(int32_t) * (4 + kBarrierGroups), stream);
+  if (k == 1 && n <= static_cast<int64_t>(kSmallRows) * kSmallThreads) {
+    const int64_t rows = (n + kSmallThreads - 1) / kSmallThreads;
+    const bool cheb = a.cheb_steps > 0;
+    auto kernel = rows == 1 ? (cheb ? cg_small_kernel<1, true> : cg_small_kernel<1, false>)
+                  : rows == 2 ? (cheb ? cg_small_kernel<2, true> : cg_small_kernel<2, false>)
+                              : (cheb ? cg_small_kernel<3, true> : cg_small_kernel<3, false>);
+    hipLaunchKernelGGL(kernel, dim3(1), dim3(kSmallThreads), 0, stream, a);
+    HG_HIP(hipGetLastError());
+  } else if (single) {
     hipLaunchKernelGGL((cg_kernel<1024, true>), dim3(1), dim3(1024), 0, stream, a);
     HG_HIP(hipGetLastError());
   } else {

@@ -454,7 +454,8 @@ class HeatSolver:
         return {name: (float(ms[i]), int(counts[i])) for i, name in enumerate(_lib.PHASES)}
 
     def plan_stats(self) -> dict[str, int]:
-        """Statistics of the Cholesky analysis: supernodes, levels, solve-block entries, flops."""
+        """Statistics of the Cholesky analysis: supernodes, levels, solve-block entries (stored,
+        structural nonzeros, and those one sweep direction reads), flops."""
         out = (ctypes.c_int64 * 8)()
         _lib.call("hg_plan_stats", self._native().ptr, out)
         keys = (
@@ -467,7 +468,12 @@ class HeatSolver:
             "front_vector_entries",
             "rows",
         )
-        return dict(zip(keys, (int(v) for v in out)))
+        stats = dict(zip(keys, (int(v) for v in out)))
+        entries = (ctypes.c_int64 * 3)()
+        _lib.call("hg_inverse_factor_entries", self._native().ptr, entries)
+        stats["solve_block_nonzeros"] = int(entries[0])
+        stats["solve_block_read"] = int(entries[1])
+        return stats
 
     # -- solves ----------------------------------------------------------------------------------
 
