@@ -722,7 +722,11 @@ def summarize_profiles(root: Path, traffic_out: Path | None = None,
         per[d["name"]][1] += d.get("FETCH_SIZE", 0.0) * 1024 * fs
     for d in write.values():
         per[d["name"]][2] += d.get("WRITE_SIZE", 0.0) * 1024 * ws
-    for n, (c, rd, wr) in sorted(per.items(), key=lambda kv: -(kv[1][1] + kv[1][2]))[:16]:
+    ranked = sorted(per.items(), key=lambda kv: -(kv[1][1] + kv[1][2]))
+    assembly = ("face_terms_kernel", "edge_sums_kernel", "mass_kernel", "operator_rows_kernel",
+                "face_geometry_kernel")
+    shown = ranked[:24] + [kv for kv in ranked[24:] if kv[0] in assembly]
+    for n, (c, rd, wr) in shown:
         us = avg_us.get(n)
         rate = (rd + wr) / c / (us * 1e-6) / 1e9 if us else float("nan")
         lines.append(f"| `{n}` | {c} | {rd / c / 1e6:.2f} | {wr / c / 1e6:.2f} | "
