@@ -131,6 +131,26 @@ int hg_operators(hg_mesh* mesh, double t, int32_t method, int32_t solver_default
 int hg_operator_views(hg_mesh* mesh, hg_operator_view* out_view);
 
 /*
+ * The cotangent operator's pattern built on the device from device faces, with no mesh bound:
+ * mesh_operator_pattern(faces, n_vertices) (laplacian.py:1071-1141, operator="cotmatrix"; the
+ * sorted-key build of _mesh_operator_pattern, laplacian.py:1212-1389, and
+ * kernels/laplacian.py:707-760). Every buffer is caller-owned device memory and may be NULL to
+ * skip it; capacities as in the reference, of which out_sizes reports what was written:
+ *   d_offsets (V+1) row bounds; d_columns (cap 6F+V) sorted columns per row;
+ *   d_corner_ptr (cap 6F+V+1) each entry's run into d_corner_slots (the reference's run_start);
+ *   d_corner_slots (cap 6F) the corner slots 3f+e opposite each entry's edge, ascending per run
+ *     (the reference's payload order);
+ *   d_vertex_ptr (V+1), d_vertex_corners (3F) each vertex's corners ascending (_corner_order,
+ *     heat.py:3170-3190).
+ * out_sizes[0] = entries (offsets[V]), out_sizes[1] = corner slots. A face index outside [0, V)
+ * is a ValueError. Returns after the copies are complete.
+ */
+int hg_operator_pattern(const int32_t* d_faces, int64_t n_faces, int64_t n_vertices, void* stream,
+                        int32_t* d_offsets, int32_t* d_columns, int32_t* d_corner_ptr,
+                        int32_t* d_corner_slots, int32_t* d_vertex_ptr, int32_t* d_vertex_corners,
+                        int64_t* out_sizes);
+
+/*
  * Factor the systems named by `systems` (HeatSolver.factor, heat.py:3863-3888; the Poisson
  * factorization shares the heat analysis, cholesky.py:313-362). Sets *out_factored to 1 when
  * every named system holds a factor. `precision` HG_PRECISION_MIXED also keeps FP32 solve blocks.

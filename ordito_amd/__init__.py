@@ -7,7 +7,7 @@ ordito_amd: ordito's heat-method geodesic path, rebuilt for the AMD Instinct MI3
 The compute runs in ``libheatgeo.so`` (hand-written HIP for CDNA4) through a ctypes C ABI.
 """
 
-from . import creation, heat, mesh  # noqa: F401
+from . import creation, heat, laplacian, mesh  # noqa: F401
 from .heat import HeatSolver, heat_geodesic, heat_operators  # noqa: F401
 from .mesh import Trimesh  # noqa: F401
 

@@ -43,6 +43,7 @@ EXPORTED = (
     "hg_mesh_set_stream",
     "hg_operators",
     "hg_operator_views",
+    "hg_operator_pattern",
     "hg_factor",
     "hg_factored",
     "hg_set_graphs",
@@ -164,6 +165,7 @@ _SIGNATURES = {
     "hg_mesh_set_stream": ([_P, _P], ctypes.c_int),
     "hg_operators": ([_P, ctypes.c_double, _I32, _I32], ctypes.c_int),
     "hg_operator_views": ([_P, ctypes.POINTER(OperatorView)], ctypes.c_int),
+    "hg_operator_pattern": ([_P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _PI64], ctypes.c_int),
     "hg_factor": ([_P, _I32, _I32, _PI32], ctypes.c_int),
     "hg_factored": ([_P, _PI32], ctypes.c_int),
     "hg_set_refinement": ([_P, _I32, ctypes.c_double, ctypes.c_double], ctypes.c_int),

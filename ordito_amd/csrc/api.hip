@@ -1183,7 +1183,12 @@ bool solve_columns(hg_mesh& m, const int64_t* offsets, int64_t k, double* phi, i
 // The host and device state of a mesh from its host positions and faces (global indices).
 void bind(hg_mesh& m) {
   const int64_t n_vertices = m.n;
-  m.pattern = build_pattern(n_vertices, m.h_faces.data(), m.f);
+  hipStream_t st = m.stream;
+  m.faces.upload(m.h_faces, st);
+  build_pattern_device(n_vertices, m.faces.ptr, m.f, st,
+                       {m.offsets, m.columns, m.corner_ptr, m.corner_slots, m.vertex_ptr,
+                        m.vertex_corners},
+                       &m.pattern);
   m.labels = component_labels(m.pattern, &m.n_comp);
   std::vector<int32_t> comp_ptr(m.n_comp + 1, 0), members(n_vertices);
   for (int64_t v = 0; v < n_vertices; ++v) comp_ptr[m.labels[v] + 1]++;
@@ -1192,15 +1197,7 @@ void bind(hg_mesh& m) {
     std::vector<int32_t> at(comp_ptr.begin(), comp_ptr.end() - 1);
     for (int64_t v = 0; v < n_vertices; ++v) members[at[m.labels[v]]++] = static_cast<int32_t>(v);
   }
-  hipStream_t st = m.stream;
   m.xyz.upload(m.h_xyz, st);
-  m.faces.upload(m.h_faces, st);
-  m.offsets.upload(m.pattern.offsets, st);
-  m.columns.upload(m.pattern.columns, st);
-  m.corner_ptr.upload(m.pattern.corner_ptr, st);
-  m.corner_slots.upload(m.pattern.corner_slots, st);
-  m.vertex_ptr.upload(m.pattern.vertex_ptr, st);
-  m.vertex_corners.upload(m.pattern.vertex_corners, st);
   // The rim: rows with an off-diagonal entry of a single corner, i.e. an edge on one face
   // (kernels/laplacian.py:1286-1321).
   m.rim.assign(static_cast<size_t>(n_vertices), 0);
@@ -1335,6 +1332,45 @@ int hg_operators(hg_mesh* mesh, double t, int32_t method, int32_t solver_default
                "use_robust takes one mesh, not a packed set");
     HG_REQUIRE(std::isnan(t) || t > 0.0, HG_ERR_VALUE, "t must be positive");
     assemble(*mesh, t, method, solver_default_time != 0, robust);
+  });
+}
+
+int hg_operator_pattern(const int32_t* d_faces, int64_t n_faces, int64_t n_vertices, void* stream,
+                        int32_t* d_offsets, int32_t* d_columns, int32_t* d_corner_ptr,
+                        int32_t* d_corner_slots, int32_t* d_vertex_ptr, int32_t* d_vertex_corners,
+                        int64_t* out_sizes) {
+  return guarded([&] {
+    HG_REQUIRE(n_vertices >= 0 && n_faces >= 0, HG_ERR_VALUE, "negative size");
+    HG_REQUIRE(n_vertices < (1ll << 31) && 3 * n_faces < (1ll << 31), HG_ERR_VALUE,
+               "mesh too large for int32 indices");
+    HG_REQUIRE(out_sizes != nullptr && (n_faces == 0 || d_faces != nullptr), HG_ERR_VALUE,
+               "bad arguments");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DeviceArray<int32_t> offsets, columns, corner_ptr, corner_slots, vertex_ptr, vertex_corners;
+    build_pattern_device(n_vertices, d_faces, n_faces, st,
+                         {offsets, columns, corner_ptr, corner_slots, vertex_ptr, vertex_corners},
+                         nullptr);
+    int32_t totals[2] = {0, 0};  // entries, corner slots
+    HG_HIP(hipMemcpyAsync(&totals[0], offsets.ptr + n_vertices, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, st));
+    HG_HIP(hipStreamSynchronize(st));
+    HG_HIP(hipMemcpyAsync(&totals[1], corner_ptr.ptr + totals[0], sizeof(int32_t),
+                          hipMemcpyDeviceToHost, st));
+    HG_HIP(hipStreamSynchronize(st));
+    const int64_t nnz = totals[0], n_slots = totals[1];
+    auto copy = [&](int32_t* dst, const DeviceArray<int32_t>& src, int64_t count) {
+      if (dst != nullptr && count > 0)
+        HG_HIP(hipMemcpyAsync(dst, src.ptr, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, st));
+    };
+    copy(d_offsets, offsets, n_vertices + 1);
+    copy(d_columns, columns, nnz);
+    copy(d_corner_ptr, corner_ptr, nnz + 1);
+    copy(d_corner_slots, corner_slots, n_slots);
+    copy(d_vertex_ptr, vertex_ptr, n_vertices + 1);
+    copy(d_vertex_corners, vertex_corners, 3 * n_faces);
+    HG_HIP(hipStreamSynchronize(st));
+    out_sizes[0] = nnz;
+    out_sizes[1] = n_slots;
   });
 }
 

@@ -120,7 +120,20 @@ struct HostPattern {
   std::vector<int32_t> diagonal;      // n: slot of the diagonal entry, -1 for an empty row
 };
 
-HostPattern build_pattern(int64_t n_vertices, const int32_t* faces, int64_t n_faces);
+// The pattern's device arrays (the HostPattern fields but the diagonal).
+struct PatternArrays {
+  DeviceArray<int32_t>& offsets;
+  DeviceArray<int32_t>& columns;
+  DeviceArray<int32_t>& corner_ptr;
+  DeviceArray<int32_t>& corner_slots;
+  DeviceArray<int32_t>& vertex_ptr;
+  DeviceArray<int32_t>& vertex_corners;
+};
+
+// Builds the pattern on the device from the device faces (range-checked there) and, when host is
+// not null, downloads it there for the host-side work.
+void build_pattern_device(int64_t n_vertices, const int32_t* d_faces, int64_t n_faces,
+                          hipStream_t stream, PatternArrays out, HostPattern* host);
 
 // Connected components of the pattern's graph.
 std::vector<int32_t> component_labels(const HostPattern& pattern, int32_t* n_components);
