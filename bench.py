@@ -9,7 +9,11 @@ stay resident in HBM; a *step* is one batch of the 64 single-source queries thro
 path: seed, heat solve, unit-gradient divergence, Poisson solve, shift -- all on the GPU.
 
 - ``value``: served queries/s over the timed steps: the 64 queries of a step as one batch, one
-  64-column refined solve per system through the FP64 inverse factors.
+  64-column refined solve per system through the FP64 inverse factors, with two batches in flight
+  (``--lanes``): two handles of the mesh (each with its own factors, the serving layer's lanes) on
+  two host threads and HIP streams, the K steps split between them, so one batch's latency-bound
+  sweep levels overlap the other's bandwidth-bound ones. ``single_lane`` is the same with one
+  batch at a time (the form the roofline and phase times are measured in).
 - ``served_mixed``: the same batch through FP32 inverse factors with FP64 refinement (the
   mixed-precision form), timed beside it, with its refinement rounds and its error against the
   served fields.
@@ -59,6 +63,8 @@ def parse() -> argparse.Namespace:
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--queries", type=int, default=64)
     parser.add_argument("--plain-queries", type=int, default=8)
+    parser.add_argument("--lanes", type=int, default=2,
+                        help="batches in flight for the served value (handles of the mesh)")
     parser.add_argument("--cpu-sample", type=int, default=4)
     parser.add_argument(
         "--cpu-sample-all", type=int, default=32, help="queries of the all-cores CPU baseline"
@@ -852,6 +858,59 @@ def timed_steps(solver, source_sets, steps: int, warmup: int):
     return elapsed, latencies, timing, out, dict(solver.last_stats)
 
 
+def in_flight_steps(solvers, source_sets, steps: int, warmup: int):
+    """The served batch with len(solvers) batches in flight: each solver (an independent handle
+    of the mesh, its own factors) runs on its own host thread and stream, the `steps` steps split
+    between them. Returns (seconds, per-step latencies, each solver's last output)."""
+    import threading
+
+    import torch
+
+    flat = torch.cat(source_sets).contiguous()
+    offsets = [0]
+    for s in source_sets:
+        offsets.append(offsets[-1] + s.numel())
+    lanes = len(solvers)
+    streams = [torch.cuda.Stream() for _ in solvers]
+    for solver, stream in zip(solvers, streams):
+        with torch.cuda.stream(stream):
+            for _ in range(warmup):
+                solver.geodesic_flat(flat, offsets)
+    torch.cuda.synchronize()
+    counts = [steps // lanes + (1 if i < steps % lanes else 0) for i in range(lanes)]
+    latencies = [[] for _ in solvers]
+    outs = [None] * lanes
+    errors = []
+    start = threading.Barrier(lanes + 1, timeout=600)
+
+    def run(i: int) -> None:
+        try:
+            with torch.cuda.stream(streams[i]):
+                start.wait()
+                for _ in range(counts[i]):
+                    s0 = time.perf_counter()
+                    outs[i] = solvers[i].geodesic_flat(flat, offsets)
+                    streams[i].synchronize()
+                    latencies[i].append(time.perf_counter() - s0)
+        except BaseException as error:  # reported by the caller
+            errors.append(error)
+            start.abort()
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(lanes)]
+    for thread in threads:
+        thread.start()
+    torch.cuda.synchronize()
+    start.wait()
+    t0 = time.perf_counter()
+    for thread in threads:
+        thread.join()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    return elapsed, [x for lane in latencies for x in lane], outs
+
+
 def ms(values, q) -> float:
     return round(1e3 * float(np.percentile(values, q)), 3)
 
@@ -924,11 +983,30 @@ def main() -> int:
     # Factored: every later solve goes through the held factors (the rent-or-buy count is past).
     solver._solves = {"heat": 2, "poisson": 2}
 
-    log(f"== served: {args.steps} steps of {k} queries, FP64 inverse factors")
-    elapsed, latencies, timing, out, served_stats = timed_steps(solver, source_sets, args.steps, args.warmup)
-    value = k * args.steps / elapsed
-    log(f"   served {value:.1f} queries/s, step {1e3 * elapsed / args.steps:.2f} ms, stats {served_stats}")
+    log(f"== served, one batch at a time: {args.steps} steps of {k} queries, FP64 inverse factors")
+    elapsed1, latencies1, timing, out, served_stats = timed_steps(solver, source_sets, args.steps, args.warmup)
+    value1 = k * args.steps / elapsed1
+    log(f"   {value1:.1f} queries/s, step {1e3 * elapsed1 / args.steps:.2f} ms, stats {served_stats}")
     served_host = out.cpu().numpy()
+    lanes = max(1, args.lanes)
+    solvers = [solver]
+    lane_meshes = []
+    t0 = time.perf_counter()
+    for _ in range(lanes - 1):
+        lane_mesh = heat.Trimesh(v_gpu, f_gpu)
+        lane = lane_mesh.heat_solver(method="meshlab", precision="fp64")
+        assert lane.factor()
+        lane._solves = {"heat": 2, "poisson": 2}
+        lane_meshes.append(lane_mesh)
+        solvers.append(lane)
+    torch.cuda.synchronize()
+    setup_lanes = time.perf_counter() - t0
+    log(f"== served, {lanes} batches in flight: {args.steps} steps of {k} queries")
+    elapsed, latencies, lane_outs = in_flight_steps(solvers, source_sets, args.steps, args.warmup)
+    value = k * args.steps / elapsed
+    lanes_equal = all(o is None or np.array_equal(o.cpu().numpy(), served_host) for o in lane_outs)
+    log(f"   served {value:.1f} queries/s, {1e3 * elapsed / args.steps:.2f} ms a step, lanes equal {lanes_equal}")
+    del solvers, lane_outs
 
     log(f"== plain FP64, one query per call, {args.plain_queries} queries")
     solver.geodesic_flat(source_sets[0], [0, 1])
@@ -983,7 +1061,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"torus {args.major}x{args.minor} R=1 r=0.3 normal jitter 0.1h, "
-            f"{k} single-source queries per step as one batch, factors resident",
+            f"{k} single-source queries per step as one batch, factors resident, "
+            f"{lanes} batches in flight",
+            "in_flight": lanes,
             "n_vertices": int(n),
             "n_faces": int(faces.size // 3),
             "queries_per_step": k,
@@ -995,7 +1075,15 @@ def main() -> int:
         },
         "p50_ms": ms(latencies, 50),
         "p99_ms": ms(latencies, 99),
-        "latency_note": "a query's latency is its step's: all 64 complete together",
+        "latency_note": "a query's latency is its step's: all 64 complete together; with "
+        "batches in flight a step's latency is measured on its own lane",
+        "single_lane": {
+            "value": round(value1, 2),
+            "ms_per_step": round(1e3 * elapsed1 / args.steps, 3),
+            "p50_ms": ms(latencies1, 50),
+            "p99_ms": ms(latencies1, 99),
+        },
+        "lanes_bit_equal": lanes_equal,
         "plain_fp64": {
             "value": round(plain_value, 2),
             "unit": "queries/s",
@@ -1005,7 +1093,7 @@ def main() -> int:
             "refinements": plain_stats,
         },
         "served_vs_plain": round(value / plain_value, 2) if plain_value > 0 else None,
-        "setup_s": {"factor_fp64": round(setup_fp64, 3)},
+        "setup_s": {"factor_fp64": round(setup_fp64, 3), "further_lanes": round(setup_lanes, 3)},
         "phase_ms_per_step": {key: round(v[0] / args.steps, 3) for key, v in timing.items() if v[1]},
         "refinements": served_stats,
         "roofline": roofline,
@@ -1077,7 +1165,7 @@ def main() -> int:
             "queries_checked": len(sample),
         }
     if not args.no_extra:
-        del solver, mesh
+        del solver, mesh, lane_meshes
         torch.cuda.empty_cache()
         result["workloads"] = extra_workloads(args, oracle, vertices, faces, sources)
     print(json.dumps(result))

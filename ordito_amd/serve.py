@@ -20,9 +20,16 @@ needs around that, with no counterpart in the reference:
 - **Latency.** Every request records its submit and completion times; ``stats()`` reports
   throughput and p50/p99.
 
-One mesh is never solved by two workers at once (an ``hg_mesh`` serves one stream at a time,
-``heat.py:298-308``). A batch's results are the ones each request gets alone, within the
-refinement's agreement (both are refined solves through the same factors).
+- **Lanes.** An ``hg_mesh`` serves one stream at a time (``heat.py:298-308``), so a mesh is
+  solved through *lanes*: independent handles of it, each with its own operators, factors and
+  workspaces. A mesh opens a second lane (up to ``replicas``) when a full batch of its requests
+  waits while every lane it has is busy and the budget holds another copy; its batches then run on
+  two workers at once, and the latency-bound levels of one batch's triangular sweeps overlap the
+  bandwidth-bound levels of the other's. The factorization is deterministic, so every lane returns
+  the same bits for the same sources.
+
+A batch's results are the ones each request gets alone, within the refinement's agreement (both
+are refined solves through the same factors).
 """
 
 from __future__ import annotations
@@ -64,17 +71,32 @@ class Completion:
 
 
 @dataclass
+class _Lane:
+    """One handle of a mesh: its own operators, factors and workspaces."""
+
+    trimesh: Trimesh | None = None  # made by the lane's first batch's worker, on its stream
+    solver: HeatSolver | None = None
+    factored: bool = False
+    nbytes: int = 0  # every device byte the lane holds, measured after each batch
+    busy: bool = False
+
+
+@dataclass
 class _Resident:
     n_vertices: int
     n_faces: int
-    trimesh: Trimesh | None = None  # made by the first batch's worker, on its stream
-    solver: HeatSolver | None = None
-    factored: bool = False
-    nbytes: int = 0  # every device byte the mesh's state holds, measured after each batch
-    estimate: int = 0
+    estimate: int = 0  # a lane's footprint before its first batch measures it
     last_used: float = 0.0
-    busy: bool = False
+    lanes: list = field(default_factory=list)
     pending: collections.deque = field(default_factory=collections.deque)
+
+    @property
+    def busy(self) -> bool:
+        return any(lane.busy for lane in self.lanes)
+
+    @property
+    def factored(self) -> bool:
+        return any(lane.factored for lane in self.lanes)
 
 
 def footprint_estimate(n_vertices: int, n_faces: int, precision: str = "fp64",
@@ -100,10 +122,13 @@ class GeodesicServer:
     """
     ``meshes``: mapping from a mesh id to ``(vertices, faces)``. ``budget_bytes``: the HBM the
     resident meshes may hold. ``workers``: batches solved at once (each on its own stream).
+    ``replicas``: the lanes (independent handles) one mesh may hold, so that up to that many of its
+    batches run at once.
     """
 
     def __init__(self, meshes: dict, *, budget_bytes: float = 200e9, max_batch: int = 64,
-                 workers: int = 4, precision: str = "fp64", warm_columns: int = 16) -> None:
+                 workers: int = 4, precision: str = "fp64", warm_columns: int = 16,
+                 replicas: int = 2) -> None:
         _arrays.require_gpu()
         self._torch = _arrays.torch_module()
         self._meshes = dict(meshes)
@@ -111,6 +136,9 @@ class GeodesicServer:
         self.max_batch = int(max_batch)
         self.warm_columns = int(warm_columns)
         self.precision = precision
+        if int(replicas) < 1:
+            raise ValueError(f"replicas must be at least 1, got {replicas}")
+        self.replicas = int(replicas)
         self._resident: dict[Any, _Resident] = {}
         self._lock = threading.Condition()
         self._closing = False
@@ -200,6 +228,7 @@ class GeodesicServer:
             "batches": self._batches,
             "evictions": self._evictions,
             "resident": sum(1 for e in self._resident.values() if e.factored),
+            "lanes": sum(1 for e in self._resident.values() for lane in e.lanes if lane.factored),
             "resident_bytes": self._held_bytes(),
         }
 
@@ -215,45 +244,69 @@ class GeodesicServer:
         self._resident[mesh_id] = entry
         return entry
 
-    def _held_bytes(self, exclude=None) -> int:
-        """Device bytes of every mesh with state on the device but ``exclude`` (a busy mesh's as of
-        its last batch; a mesh being built for the first time counts its estimate)."""
-        return sum(max(e.nbytes, e.estimate if e.busy and not e.factored else 0)
-                   for k, e in self._resident.items()
-                   if k != exclude and (e.solver is not None or e.busy))
+    def _held_bytes(self, exclude: _Lane | None = None) -> int:
+        """Device bytes of every lane with state on the device but ``exclude`` (a busy lane's as of
+        its last batch; a lane being built for the first time counts its mesh's estimate)."""
+        return sum(max(lane.nbytes, e.estimate if lane.busy and not lane.factored else 0)
+                   for e in self._resident.values() for lane in e.lanes
+                   if lane is not exclude and (lane.solver is not None or lane.busy))
 
-    def _admit(self, mesh_id, entry: _Resident) -> None:
-        """Evict least recently used idle meshes, freeing all their device state, until ``entry``
-        fits the budget."""
-        need = max(entry.estimate, entry.nbytes)
-        while self._held_bytes(exclude=mesh_id) + need > self.budget:
-            idle = [(e.last_used, k) for k, e in self._resident.items()
-                    if e.solver is not None and not e.busy and k != mesh_id]
+    def _admit(self, entry: _Resident, lane: _Lane) -> bool:
+        """Evict least recently used idle lanes, freeing all their device state, until ``lane``
+        fits the budget. A mesh's first lane is admitted whatever is left (it must serve); a further
+        lane only if it fits: False leaves the budget as it was and the lane unbuilt."""
+        need = max(entry.estimate, lane.nbytes)
+        first = len(entry.lanes) == 1
+        if not first and self._held_bytes(exclude=lane) + need > self.budget:
+            # A second copy of a mesh is worth no other mesh's eviction.
+            return False
+        while self._held_bytes(exclude=lane) + need > self.budget:
+            idle = [(e.last_used, n, k) for k, e in self._resident.items()
+                    for n, other in enumerate(e.lanes)
+                    if other.solver is not None and not other.busy and other is not lane]
             if not idle:
                 break
-            _, victim = min(idle)
+            _, n, victim = min(idle)
             e = self._resident[victim]
-            e.solver.release()
-            e.solver = None
-            e.trimesh = None  # its hg_mesh (operators, pattern, workspaces) goes with the solver
-            e.factored = False
-            e.nbytes = 0
+            other = e.lanes.pop(n)
+            other.solver.release()
+            # Its hg_mesh (operators, pattern, workspaces) goes with the solver.
+            other.solver = None
+            other.trimesh = None
             self._evictions += 1
+        return True
+
+    def _lane_for(self, entry: _Resident) -> _Lane | None:
+        """An idle lane of ``entry``, else a new one when a full batch waits behind busy lanes (or
+        the mesh has none) and fewer than ``replicas`` exist, else None."""
+        idle = [lane for lane in entry.lanes if not lane.busy]
+        if idle:
+            return max(idle, key=lambda lane: lane.factored)
+        if not entry.lanes or (len(entry.lanes) < self.replicas
+                               and len(entry.pending) >= self.max_batch):
+            lane = _Lane()
+            entry.lanes.append(lane)
+            return lane
+        return None
 
     def _next_batch(self):
-        """The mesh whose oldest waiting request is oldest and not busy, with up to max_batch."""
-        best = None
-        for mesh_id, entry in self._resident.items():
-            if entry.busy or not entry.pending:
+        """The mesh whose oldest waiting request is oldest and which has a lane to solve it on,
+        with up to max_batch of its requests."""
+        order = sorted((e.pending[0].submitted, k) for k, e in self._resident.items() if e.pending)
+        for _, mesh_id in order:
+            entry = self._resident[mesh_id]
+            lane = self._lane_for(entry)
+            if lane is None:
                 continue
-            head = entry.pending[0].submitted
-            if best is None or head < best[0]:
-                best = (head, mesh_id)
-        if best is None:
-            return None
-        entry = self._resident[best[1]]
-        batch = [entry.pending.popleft() for _ in range(min(self.max_batch, len(entry.pending)))]
-        return best[1], entry, batch
+            lane.busy = True
+            if not lane.factored and not self._admit(entry, lane) and lane.solver is None:
+                # A further lane that does not fit is not built; the batch waits for a lane.
+                lane.busy = False
+                entry.lanes.remove(lane)
+                continue
+            batch = [entry.pending.popleft() for _ in range(min(self.max_batch, len(entry.pending)))]
+            return mesh_id, entry, lane, batch
+        return None
 
     def _schedule(self) -> None:
         while True:
@@ -266,38 +319,36 @@ class GeodesicServer:
                             and len(self._idle) == len(self._streams):
                         return
                     self._lock.wait(timeout=0.05)
-                mesh_id, entry, batch = pick
-                entry.busy = True
+                mesh_id, entry, lane, batch = pick
                 worker = self._idle.pop()
-                if not entry.factored:
-                    self._admit(mesh_id, entry)
-            self._pool.submit(self._run, worker, mesh_id, entry, batch)
+            self._pool.submit(self._run, worker, mesh_id, entry, lane, batch)
 
-    def _run(self, worker: int, mesh_id, entry: _Resident, batch: list[_Request]) -> None:
+    def _run(self, worker: int, mesh_id, entry: _Resident, lane: _Lane,
+             batch: list[_Request]) -> None:
         torch = self._torch
         try:
             start = time.perf_counter()
             for r in batch:
                 r.started = start
             with torch.cuda.stream(self._streams[worker]):
-                if entry.solver is None:
+                if lane.solver is None:
                     vertices, faces = self._meshes[mesh_id]
-                    entry.trimesh = Trimesh(vertices, faces)
-                    entry.solver = entry.trimesh.heat_solver(method="meshlab", precision=self.precision)
+                    lane.trimesh = Trimesh(vertices, faces)
+                    lane.solver = lane.trimesh.heat_solver(method="meshlab", precision=self.precision)
                     # Graph replay pays per new (mesh, batch width) capture, and launches from
                     # several threads contend: measured p50 4.5 ms eager against 10.6 ms replayed.
-                    entry.solver.set_graphs(False)
-                if not entry.factored:
-                    # A refusal under the library's own budget leaves the mesh iterating.
-                    entry.factored = entry.solver.factor()
+                    lane.solver.set_graphs(False)
+                if not lane.factored:
+                    # A refusal under the library's own budget leaves the lane iterating.
+                    lane.factored = lane.solver.factor()
                 # The batch's sources in one flat upload, its sets delimited by offsets.
                 offsets = [0]
                 for r in batch:
                     offsets.append(offsets[-1] + int(r.sources.size))
                 flat = torch.from_numpy(np.concatenate([r.sources for r in batch])).to("cuda")
-                fields = entry.solver.geodesic_flat(flat, offsets)
+                fields = lane.solver.geodesic_flat(flat, offsets)
                 self._streams[worker].synchronize()
-                entry.nbytes = entry.solver.device_bytes
+                lane.nbytes = lane.solver.device_bytes
             # The results' memory comes from this worker stream's pool: each consumer's stream is
             # recorded on it, so a result freed while its reads are still queued there is not
             # reused by this worker's next batch before they finish.
@@ -314,7 +365,7 @@ class GeodesicServer:
                     r.future.set_exception(error)
         finally:
             with self._lock:
-                entry.busy = False
+                lane.busy = False
                 entry.last_used = time.perf_counter()
                 self._completed.extend(Completion(r.mesh_id, r.submitted, r.started, r.done)
                                        for r in batch if r.done)

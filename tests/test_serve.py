@@ -87,16 +87,54 @@ def test_admission_counts_every_device_byte_a_mesh_holds(cuda):
     for name in meshes:
         server.submit(name, np.array([0, 5], dtype=np.int32)).result(timeout=120)
     entries = [server._resident[name] for name in meshes]
+    assert all(len(e.lanes) == 1 for e in entries)  # one request at a time: one lane a mesh
+    lanes = [e.lanes[0] for e in entries]
     held = server.stats()["resident_bytes"]
-    assert held == sum(e.solver.device_bytes for e in entries)
-    assert held > sum(e.solver.nbytes for e in entries) > 0
-    server._admit("torus", server._resident["torus"])  # fits: nothing goes
+    assert held == sum(lane.solver.device_bytes for lane in lanes)
+    assert held > sum(lane.solver.nbytes for lane in lanes) > 0
+    torus = server._resident["torus"]
+    assert server._admit(torus, torus.lanes[0])  # fits: nothing goes
     assert server.stats()["evictions"] == 0
-    server.budget = 1  # nothing fits: every other idle mesh goes, with all its device state
-    server._admit("torus", server._resident["torus"])
-    assert server._resident["icosphere3"].solver is None and server.stats()["evictions"] == 1
-    assert server._held_bytes() == server._resident["torus"].nbytes
+    server.budget = 1  # nothing fits: every other idle lane goes, with all its device state
+    assert server._admit(torus, torus.lanes[0])
+    assert server._resident["icosphere3"].lanes == [] and server.stats()["evictions"] == 1
+    assert server._held_bytes() == torus.lanes[0].nbytes
     server.close()
+
+
+def test_a_hot_mesh_opens_a_second_lane_with_the_same_answers(cuda):
+    """A full batch waiting while a mesh's only lane is busy opens a second lane (an independent
+    handle with its own factors) on another worker, never more than ``replicas``; every field
+    matches the request solved alone, and a further lane that does not fit the budget is not
+    built."""
+    from ordito_amd.serve import GeodesicServer
+
+    meshes = {"torus": mesh("torus")}
+    n = len(meshes["torus"][0])
+    rng = np.random.default_rng(3)
+    sources = [rng.choice(n, size=1).astype(np.int32) for _ in range(48)]
+    server = GeodesicServer(meshes, workers=3, max_batch=8, replicas=2)
+    futures = [server.submit("torus", s) for s in sources]
+    results = [f.result(timeout=120).cpu().numpy() for f in futures]
+    stats = server.stats()
+    lanes = len(server._resident["torus"].lanes)
+    server.close()
+    assert stats["lanes"] == lanes == 2
+    v, f = meshes["torus"]
+    for i in (0, 17, 47):
+        worst, _ = ho.relative_error(results[i], _alone(cuda, v, f, sources[i]))
+        assert worst <= 1e-10, (i, worst)
+    # One lane's worth of budget: the second lane is refused and the first serves everything.
+    server = GeodesicServer(meshes, workers=3, max_batch=8, replicas=2)
+    server.submit("torus", sources[0]).result(timeout=120)
+    server.budget = server._held_bytes() + 1
+    futures = [server.submit("torus", s) for s in sources]
+    tight = [f.result(timeout=120).cpu().numpy() for f in futures]
+    stats = server.stats()
+    server.close()
+    assert stats["lanes"] == 1 and stats["evictions"] == 0
+    assert all(np.array_equal(a, b) for a, b in zip(tight, results)) or all(
+        ho.relative_error(a, b)[0] <= 1e-10 for a, b in zip(tight, results))
 
 
 def test_unknown_mesh_and_closed_server(cuda):
