@@ -91,13 +91,14 @@ def parse() -> argparse.Namespace:
     )
     parser.add_argument(
         "--workload",
-        choices=("torus", "sphere", "batch", "serve", "igl", "oneshot"),
+        choices=("torus", "sphere", "batch", "serve", "igl", "oneshot", "pattern"),
         default="torus",
         help="torus: the headline configuration (default); sphere: the 2.6 M-vertex displaced "
         "sphere, mixed and FP64 against the oracle; batch: many meshes packed into one system; "
         "serve: the continuous-batching server under Poisson arrivals; igl: method='igl' on an "
         "open grid (its rim-pinned heat solve), beside method='meshlab'; oneshot: only the one-shot "
-        "(iterated) calls on icosphere(4) and the torus",
+        "(iterated) calls on icosphere(4) and the torus; pattern: the operator pattern built on "
+        "the device (mesh_operator_pattern) for the torus, --steps builds",
     )
     parser.add_argument(
         "--summarize-profiles",
@@ -505,6 +506,50 @@ def serve_result(args, probe_rates=()) -> dict:
 
 def serve_workload(args) -> int:
     print(json.dumps(serve_result(args)))
+    return 0
+
+
+def pattern_workload(args) -> int:
+    """The operator pattern of the torus built on the device from device-resident faces
+    (radix sorts of the 6F + V edge keys and of the 3F corners), `steps` builds timed after
+    `warmup`; checked once against the oracle's sorted keys."""
+    import torch
+
+    from oracle import heat_oracle as ho
+    from ordito_amd import creation
+    from ordito_amd.laplacian import mesh_operator_pattern
+
+    vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)
+    n = vertices.shape[0]
+    f_gpu = torch.from_numpy(faces.reshape(-1)).cuda()
+    for _ in range(max(args.warmup, 1)):
+        pattern = mesh_operator_pattern(f_gpu, n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pattern = mesh_operator_pattern(f_gpu, n)
+    torch.cuda.synchronize()
+    seconds = (time.perf_counter() - t0) / args.steps
+    want = ho.operator_pattern(n, faces.reshape(-1, 3))
+    exact = bool(
+        np.array_equal(pattern.offsets.cpu().numpy(), want.indptr)
+        and np.array_equal(pattern.columns.cpu().numpy(), want.indices)
+        and np.array_equal(pattern.corner_ptr.cpu().numpy(), want.corner_ptr)
+        and np.array_equal(pattern.corner_slots.cpu().numpy(), want.corner_slots)
+    )
+    keys = 6 * (faces.size // 3) + n
+    print(json.dumps({
+        "metric": "operator pattern built on the device (mesh_operator_pattern)",
+        "value": round(1e3 * seconds, 4),
+        "unit": "ms per build",
+        "higher_is_better": False,
+        "steps": args.steps,
+        "config": {"workload": f"torus {args.major}x{args.minor}", "n_vertices": int(n),
+                   "n_faces": int(faces.size // 3), "keys": int(keys),
+                   "entries": int(pattern.columns.numel())},
+        "keys_per_s": round(keys / seconds, 1),
+        "matches_oracle": exact,
+    }))
     return 0
 
 
@@ -946,6 +991,8 @@ def main() -> int:
         return sphere_workload(args)
     if args.workload == "igl":
         return igl_workload(args)
+    if args.workload == "pattern":
+        return pattern_workload(args)
     if args.workload == "oneshot":
         log("== one-shot (iterated) calls")
         vertices, faces = creation.torus(args.major, args.minor, 1.0, 0.3, jitter=0.1, seed=0)

@@ -4,8 +4,10 @@
 // by column; for each entry the run of corner slots 3f+e whose opposite edge it is, in ascending
 // slot order (the order its half-cotangents are summed in, kernels/laplacian.py:970-989); and each
 // vertex's corners in ascending order. One stable radix sort of the keys i*n+j carrying their
-// slots gives the entries and runs; one of the corners by vertex gives the vertex lists. The host
-// keeps a copy for the work that runs there (components, the rim, the analysis).
+// slots gives the entries and runs; one of the corners by vertex gives the vertex lists; row and
+// vertex bounds are bisections of the sorted keys (no atomic counts). The temporaries live in two
+// allocations. The host keeps a copy for the work that runs there (components, the rim, the
+// analysis).
 #include <hipcub/hipcub.hpp>
 
 #include "kernels.h"
@@ -64,7 +66,7 @@ __global__ void pattern_heads_kernel(const uint64_t* __restrict__ keys,
 }
 
 // With entry = the inclusive count of heads and position = the exclusive count of slots: each
-// entry's column, the start of its run and its row's count; every slot into the runs.
+// entry's column and the start of its run; every slot into the runs.
 __global__ void pattern_entries_kernel(const uint64_t* __restrict__ keys,
                                        const int32_t* __restrict__ slots,
                                        const int32_t* __restrict__ head,
@@ -72,27 +74,52 @@ __global__ void pattern_entries_kernel(const uint64_t* __restrict__ keys,
                                        const int32_t* __restrict__ position, int64_t count,
                                        int64_t n, int32_t* __restrict__ columns,
                                        int32_t* __restrict__ corner_ptr,
-                                       int32_t* __restrict__ row_count,
                                        int32_t* __restrict__ corner_slots) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= count) return;
   if (head[k]) {
     const int32_t e = entry[k] - 1;
-    const uint64_t row = keys[k] / static_cast<uint64_t>(n);
-    columns[e] = static_cast<int32_t>(keys[k] - row * n);
+    columns[e] = static_cast<int32_t>(keys[k] % static_cast<uint64_t>(n));
     corner_ptr[e] = position[k];
-    atomicAdd(&row_count[row], 1);
   }
   if (slots[k] >= 0) corner_slots[position[k]] = slots[k];
 }
 
-__global__ void vertex_counts_kernel(const int32_t* __restrict__ faces, int64_t count,
-                                     int32_t* __restrict__ corner_ids,
-                                     int32_t* __restrict__ vertex_count) {
+// Row bounds from the sorted keys: row v starts at the entries before the first key of row v or
+// later (its lower bound among the keys, by bisection), so rows with no entries get bounds too
+// and no count is accumulated.
+__global__ void row_offsets_kernel(const uint64_t* __restrict__ keys,
+                                   const int32_t* __restrict__ entry, int64_t count, int64_t n,
+                                   int32_t* __restrict__ offsets) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  const uint64_t target = static_cast<uint64_t>(v) * static_cast<uint64_t>(n);
+  int64_t lo = 0, hi = count;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] < target) lo = mid + 1;
+    else hi = mid;
+  }
+  offsets[v] = lo == 0 ? 0 : entry[lo - 1];
+}
+
+// Each vertex's first corner in the corners sorted by vertex (its lower bound, by bisection).
+__global__ void vertex_offsets_kernel(const uint32_t* __restrict__ vertex_of, int64_t count,
+                                      int64_t n, int32_t* __restrict__ vertex_ptr) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  int64_t lo = 0, hi = count;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (static_cast<int64_t>(vertex_of[mid]) < v) lo = mid + 1;
+    else hi = mid;
+  }
+  vertex_ptr[v] = static_cast<int32_t>(lo);
+}
+
+__global__ void iota_kernel(int32_t* __restrict__ out, int64_t count) {
   const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= count) return;
-  corner_ids[q] = static_cast<int32_t>(q);
-  atomicAdd(&vertex_count[faces[q]], 1);
+  if (q < count) out[q] = static_cast<int32_t>(q);
 }
 
 __global__ void set_word_kernel(int32_t* p, int32_t value) { p[0] = value; }
@@ -108,41 +135,40 @@ int bits_for(uint64_t bound) {
   return bits;
 }
 
-void grow(DeviceArray<uint8_t>& temp, size_t bytes) {
-  if (temp.size < static_cast<int64_t>(bytes)) temp.allocate(static_cast<int64_t>(bytes));
+// hipCUB calls on caller-provided temporary storage: each first asked its size (temp null), then
+// run with it.
+size_t exclusive_scan(const int32_t* in, int32_t* out, int64_t items, void* temp, size_t bytes,
+                      hipStream_t stream) {
+  HG_HIP(hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, static_cast<int>(items), stream));
+  return bytes;
 }
 
-// out[0..n] = the exclusive scan of in[0..n], so out[n] is the total of in[0..n) when in[n] = 0.
-void exclusive_scan(const int32_t* in, int32_t* out, int64_t n, DeviceArray<uint8_t>& temp,
-                    hipStream_t stream) {
-  size_t bytes = 0;
-  const int items = static_cast<int>(n + 1);
-  HG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, items, stream));
-  grow(temp, bytes);
-  HG_HIP(hipcub::DeviceScan::ExclusiveSum(temp.ptr, bytes, in, out, items, stream));
-}
-
-void inclusive_scan(const int32_t* in, int32_t* out, int64_t n, DeviceArray<uint8_t>& temp,
-                    hipStream_t stream) {
-  size_t bytes = 0;
-  const int items = static_cast<int>(n);
-  HG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out, items, stream));
-  grow(temp, bytes);
-  HG_HIP(hipcub::DeviceScan::InclusiveSum(temp.ptr, bytes, in, out, items, stream));
+size_t inclusive_scan(const int32_t* in, int32_t* out, int64_t items, void* temp, size_t bytes,
+                      hipStream_t stream) {
+  HG_HIP(hipcub::DeviceScan::InclusiveSum(temp, bytes, in, out, static_cast<int>(items), stream));
+  return bytes;
 }
 
 template <typename K, typename V>
-void sort_pairs(const K* keys_in, K* keys_out, const V* values_in, V* values_out, int64_t count,
-                int end_bit, DeviceArray<uint8_t>& temp, hipStream_t stream) {
-  if (count == 0) return;
-  size_t bytes = 0;
-  const int items = static_cast<int>(count);
-  HG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in, keys_out, values_in,
-                                            values_out, items, 0, end_bit, stream));
-  grow(temp, bytes);
-  HG_HIP(hipcub::DeviceRadixSort::SortPairs(temp.ptr, bytes, keys_in, keys_out, values_in,
-                                            values_out, items, 0, end_bit, stream));
+size_t sort_pairs(const K* keys_in, K* keys_out, const V* values_in, V* values_out, int64_t items,
+                  int end_bit, void* temp, size_t bytes, hipStream_t stream) {
+  if (items == 0) return 0;
+  HG_HIP(hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, values_in, values_out,
+                                            static_cast<int>(items), 0, end_bit, stream));
+  return bytes;
 }
+
+// Carves 256-byte aligned pieces out of one allocation.
+struct Carver {
+  uint8_t* base;
+  size_t at = 0;
+  template <typename T>
+  T* take(int64_t count) {
+    T* p = base ? reinterpret_cast<T*>(base + at) : nullptr;  // a null base only sizes
+    at += (sizeof(T) * static_cast<size_t>(std::max<int64_t>(count, 1)) + 255) & ~size_t{255};
+    return p;
+  }
+};
 
 template <typename T>
 std::vector<T> download(const T* p, int64_t n, hipStream_t stream) {
@@ -158,82 +184,124 @@ void build_pattern_device(int64_t n, const int32_t* faces, int64_t n_faces, hipS
   const int64_t corners = 3 * n_faces;
   HG_REQUIRE(6 * n_faces + n < (int64_t{1} << 31), HG_ERR_VALUE,
              "mesh too large for int32 pattern offsets");
-  DeviceArray<uint8_t> temp;
-  // Range check and the referenced vertices, counted by a scan (each array has room for a total).
-  DeviceArray<int32_t> referenced(n + 1), ref_pos(n + 1), invalid(1);
-  launch_zero(referenced.ptr, sizeof(int32_t) * (n + 1), stream);
-  launch_zero(invalid.ptr, sizeof(int32_t), stream);
+  // Range check and the referenced vertices, counted by a scan (each array has room for a total),
+  // in one allocation.
+  size_t scan_a = exclusive_scan(nullptr, nullptr, n + 1, nullptr, 0, stream);
+  Carver size_a{nullptr};
+  size_a.take<int32_t>(n + 1);
+  size_a.take<int32_t>(n + 1);
+  size_a.take<int32_t>(1);
+  size_a.take<uint8_t>(static_cast<int64_t>(scan_a));
+  DeviceArray<uint8_t> arena_a(static_cast<int64_t>(size_a.at));
+  Carver ca{arena_a.ptr};
+  int32_t* referenced = ca.take<int32_t>(n + 1);
+  int32_t* ref_pos = ca.take<int32_t>(n + 1);
+  int32_t* invalid = ca.take<int32_t>(1);
+  void* temp_a = ca.take<uint8_t>(static_cast<int64_t>(scan_a));
+  launch_zero(referenced, sizeof(int32_t) * (n + 1), stream);
+  launch_zero(invalid, sizeof(int32_t), stream);
   if (corners > 0)
-    check_faces_kernel<<<grid_of(corners), kBlock, 0, stream>>>(faces, corners, n, referenced.ptr,
-                                                                 invalid.ptr);
+    check_faces_kernel<<<grid_of(corners), kBlock, 0, stream>>>(faces, corners, n, referenced,
+                                                                 invalid);
   HG_HIP(hipGetLastError());
-  exclusive_scan(referenced.ptr, ref_pos.ptr, n, temp, stream);
+  exclusive_scan(referenced, ref_pos, n + 1, temp_a, scan_a, stream);
   int32_t words[2] = {0, 0};
-  HG_HIP(hipMemcpyAsync(&words[0], invalid.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-  HG_HIP(hipMemcpyAsync(&words[1], ref_pos.ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost,
-                        stream));
+  HG_HIP(hipMemcpyAsync(&words[0], invalid, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+  HG_HIP(hipMemcpyAsync(&words[1], ref_pos + n, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
   HG_HIP(hipStreamSynchronize(stream));
   HG_REQUIRE(words[0] == 0, HG_ERR_VALUE,
              "every face index must be a vertex index in [0, n_vertices)");
   const int64_t count = 2 * corners + words[1];
-  const int64_t room = std::max<int64_t>(count, 1);
+  const int64_t n_slots = 2 * corners;  // every face key carries its slot, no diagonal does
+
+  // Everything else in a second allocation: the keys and slots (sorted and not), the heads and
+  // slot flags and their scans, the corners sorted by vertex, and the sorts' and scans' storage.
+  const int key_bits = bits_for(n > 0 ? static_cast<uint64_t>(n) * n - 1 : 0);
+  const int vertex_bits = bits_for(n > 0 ? static_cast<uint64_t>(n - 1) : 0);
+  size_t temp_bytes = 0;
+  temp_bytes = std::max(temp_bytes, sort_pairs<uint64_t, int32_t>(nullptr, nullptr, nullptr,
+                                                                  nullptr, count, key_bits,
+                                                                  nullptr, 0, stream));
+  temp_bytes = std::max(temp_bytes, sort_pairs<uint32_t, int32_t>(nullptr, nullptr, nullptr,
+                                                                  nullptr, corners, vertex_bits,
+                                                                  nullptr, 0, stream));
+  temp_bytes = std::max(temp_bytes, inclusive_scan(nullptr, nullptr, count + 1, nullptr, 0, stream));
+  temp_bytes = std::max(temp_bytes, exclusive_scan(nullptr, nullptr, count + 1, nullptr, 0, stream));
+  auto carve_b = [&](Carver& c) {
+    struct {
+      uint64_t *keys, *sorted_keys;
+      int32_t *slots, *sorted_slots, *head, *carries, *entry, *position, *corner_ids;
+      uint32_t* vertex_of;
+      void* temp;
+    } b;
+    b.keys = c.take<uint64_t>(count);
+    b.sorted_keys = c.take<uint64_t>(count);
+    b.slots = c.take<int32_t>(count);
+    b.sorted_slots = c.take<int32_t>(count);
+    b.head = c.take<int32_t>(count + 1);
+    b.carries = c.take<int32_t>(count + 1);
+    b.entry = c.take<int32_t>(count + 1);
+    b.position = c.take<int32_t>(count + 1);
+    b.corner_ids = c.take<int32_t>(corners);
+    b.vertex_of = c.take<uint32_t>(corners);
+    b.temp = c.take<uint8_t>(static_cast<int64_t>(temp_bytes));
+    return b;
+  };
+  Carver size_b{nullptr};
+  carve_b(size_b);
+  DeviceArray<uint8_t> arena_b(static_cast<int64_t>(size_b.at));
+  Carver cb{arena_b.ptr};
+  const auto b = carve_b(cb);
 
   // The entries: one stable sort of the keys carrying their slots.
-  DeviceArray<uint64_t> keys(room), sorted_keys(room);
-  DeviceArray<int32_t> slots(room), sorted_slots(room);
-  pattern_keys_kernel<<<grid_of(corners + n), kBlock, 0, stream>>>(faces, n_faces, referenced.ptr,
-                                                                    ref_pos.ptr, n, keys.ptr,
-                                                                    slots.ptr);
+  pattern_keys_kernel<<<grid_of(corners + n), kBlock, 0, stream>>>(faces, n_faces, referenced,
+                                                                    ref_pos, n, b.keys, b.slots);
   HG_HIP(hipGetLastError());
-  const uint64_t largest = n > 0 ? static_cast<uint64_t>(n) * n - 1 : 0;
-  sort_pairs(keys.ptr, sorted_keys.ptr, slots.ptr, sorted_slots.ptr, count, bits_for(largest),
-             temp, stream);
-  DeviceArray<int32_t> head(count + 1), carries(count + 1), entry(count + 1), position(count + 1);
-  launch_zero(head.ptr, sizeof(int32_t) * (count + 1), stream);
-  launch_zero(carries.ptr, sizeof(int32_t) * (count + 1), stream);
+  sort_pairs(b.keys, b.sorted_keys, b.slots, b.sorted_slots, count, key_bits, b.temp, temp_bytes,
+             stream);
+  launch_zero(b.head + count, sizeof(int32_t), stream);
+  launch_zero(b.carries + count, sizeof(int32_t), stream);
   if (count > 0)
-    pattern_heads_kernel<<<grid_of(count), kBlock, 0, stream>>>(sorted_keys.ptr, sorted_slots.ptr,
-                                                                 count, head.ptr, carries.ptr);
+    pattern_heads_kernel<<<grid_of(count), kBlock, 0, stream>>>(b.sorted_keys, b.sorted_slots,
+                                                                 count, b.head, b.carries);
   HG_HIP(hipGetLastError());
-  inclusive_scan(head.ptr, entry.ptr, count + 1, temp, stream);
-  exclusive_scan(carries.ptr, position.ptr, count, temp, stream);
-  int32_t sizes[2] = {0, 0};  // entries, slots
-  HG_HIP(hipMemcpyAsync(&sizes[0], entry.ptr + count, sizeof(int32_t), hipMemcpyDeviceToHost,
-                        stream));
-  HG_HIP(hipMemcpyAsync(&sizes[1], position.ptr + count, sizeof(int32_t), hipMemcpyDeviceToHost,
-                        stream));
+  inclusive_scan(b.head, b.entry, count + 1, b.temp, temp_bytes, stream);
+  exclusive_scan(b.carries, b.position, count + 1, b.temp, temp_bytes, stream);
+  int32_t entries = 0;
+  HG_HIP(hipMemcpyAsync(&entries, b.entry + count, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
   HG_HIP(hipStreamSynchronize(stream));
-  const int64_t nnz = sizes[0], n_slots = sizes[1];
-  DeviceArray<int32_t> row_count(n + 1);
-  launch_zero(row_count.ptr, sizeof(int32_t) * (n + 1), stream);
+  const int64_t nnz = entries;
   out.columns.allocate(std::max<int64_t>(nnz, 1));
   out.corner_ptr.allocate(nnz + 1);
   out.corner_slots.allocate(std::max<int64_t>(n_slots, 1));
+  out.offsets.allocate(n + 1);
   if (count > 0)
     pattern_entries_kernel<<<grid_of(count), kBlock, 0, stream>>>(
-        sorted_keys.ptr, sorted_slots.ptr, head.ptr, entry.ptr, position.ptr, count, n,
-        out.columns.ptr, out.corner_ptr.ptr, row_count.ptr, out.corner_slots.ptr);
+        b.sorted_keys, b.sorted_slots, b.head, b.entry, b.position, count, n, out.columns.ptr,
+        out.corner_ptr.ptr, out.corner_slots.ptr);
   set_word_kernel<<<1, 1, 0, stream>>>(out.corner_ptr.ptr + nnz, static_cast<int32_t>(n_slots));
+  row_offsets_kernel<<<grid_of(n + 1), kBlock, 0, stream>>>(b.sorted_keys, b.entry, count, n,
+                                                             out.offsets.ptr);
   HG_HIP(hipGetLastError());
-  out.offsets.allocate(n + 1);
-  exclusive_scan(row_count.ptr, out.offsets.ptr, n, temp, stream);
 
   // Each vertex's corners: a stable sort of the corner ids by vertex.
-  DeviceArray<int32_t> corner_ids(std::max<int64_t>(corners, 1)), vertex_count(n + 1);
-  DeviceArray<uint32_t> vertex_of(std::max<int64_t>(corners, 1));
-  launch_zero(vertex_count.ptr, sizeof(int32_t) * (n + 1), stream);
   out.vertex_corners.allocate(std::max<int64_t>(corners, 1));
-  if (corners > 0)
-    vertex_counts_kernel<<<grid_of(corners), kBlock, 0, stream>>>(faces, corners, corner_ids.ptr,
-                                                                   vertex_count.ptr);
-  HG_HIP(hipGetLastError());
-  sort_pairs(reinterpret_cast<const uint32_t*>(faces), vertex_of.ptr, corner_ids.ptr, out.vertex_corners.ptr, corners,
-             bits_for(n > 0 ? static_cast<uint64_t>(n - 1) : 0), temp, stream);
   out.vertex_ptr.allocate(n + 1);
-  exclusive_scan(vertex_count.ptr, out.vertex_ptr.ptr, n, temp, stream);
+  if (corners > 0) iota_kernel<<<grid_of(corners), kBlock, 0, stream>>>(b.corner_ids, corners);
+  HG_HIP(hipGetLastError());
+  sort_pairs(reinterpret_cast<const uint32_t*>(faces), b.vertex_of,
+             static_cast<const int32_t*>(b.corner_ids), out.vertex_corners.ptr, corners,
+             vertex_bits, b.temp, temp_bytes, stream);
+  vertex_offsets_kernel<<<grid_of(n + 1), kBlock, 0, stream>>>(b.vertex_of, corners, n,
+                                                                out.vertex_ptr.ptr);
+  HG_HIP(hipGetLastError());
+  if (host == nullptr) {
+    // The temporaries are freed on return: their work must be done first.
+    HG_HIP(hipStreamSynchronize(stream));
+    return;
+  }
 
   // The host copy, and each row's diagonal entry.
-  if (host == nullptr) return;
   HostPattern& h = *host;
   h = HostPattern{};
   h.n = n;
