@@ -414,25 +414,43 @@ __global__ __launch_bounds__(kThreads) void cg_kernel(CgArgs a) {
 constexpr int kSmallThreads = 1024;
 constexpr int kSmallRows = 3;  // rows a thread: systems of at most 3,072 rows
 
-template <int kRows>
-__device__ __forceinline__ double small_row_product(const CgArgs& a, int32_t begin, int32_t end,
-                                                    const double* __restrict__ vec) {
-  constexpr int kBatch = kRows > 2 ? 4 : 8;
-  double sum = 0.0;
-  for (int32_t e0 = begin; e0 < end; e0 += kBatch) {
-    int32_t col[kBatch];
-    double val[kBatch];
+// The products of a thread's rows, kBatch entries of every row a step: each step's columns and
+// values of all the rows are loaded together (one round trip for the rows, where a row at a time
+// waited once per row), then gathered from LDS and summed, each row in entry order. kWide: the
+// operator's float64 values only (the Chebyshev instance's: the Poisson systems have no narrowed
+// copy), else the narrowed values when there are.
+template <int kRows, bool kWide>
+__device__ __forceinline__ void small_row_products(const CgArgs& a, const int32_t (&begin)[kRows],
+                                                   const int32_t (&end)[kRows],
+                                                   const double* __restrict__ vec,
+                                                   double (&out)[kRows]) {
+  constexpr int kBatch = kRows == 1 ? 8 : (kRows == 2 ? 4 : (kWide ? 2 : 3));  // no spills
+  int32_t longest = 0;
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int32_t e = min(e0 + u, end - 1);
-      col[u] = a.columns[e];
-      val[u] = a.values32 ? static_cast<double>(a.values32[e]) : a.values64[e];
-    }
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u)
-      if (e0 + u < end) sum += val[u] * vec[col[u]];
+  for (int j = 0; j < kRows; ++j) {
+    out[j] = 0.0;
+    longest = max(longest, end[j] - begin[j]);
   }
-  return sum;
+  for (int32_t base = 0; base < longest; base += kBatch) {
+    int32_t col[kRows][kBatch];
+    double val[kRows][kBatch];
+#pragma unroll
+    for (int j = 0; j < kRows; ++j)
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const bool has = begin[j] + base + u < end[j];
+        const int32_t e = has ? begin[j] + base + u : 0;
+        col[j][u] = has ? a.columns[e] : 0;
+        val[j][u] = !has ? 0.0
+                    : kWide || !a.values32 ? a.values64[e]
+                                           : static_cast<double>(a.values32[e]);
+      }
+#pragma unroll
+    for (int j = 0; j < kRows; ++j)
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u)
+        if (begin[j] + base + u < end[j]) out[j] += val[j][u] * vec[col[j][u]];
+  }
 }
 
 template <int kRows, bool kCheb>
@@ -464,11 +482,13 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(CgArgs a) {
   }
   __syncthreads();
   // r = b - A x0 (x0 the guess, else zero).
+  double ax[kRows] = {};
+  if (a.guess) small_row_products<kRows, kCheb>(a, begin, end, u_lds, ax);
 #pragma unroll
   for (int j = 0; j < kRows; ++j) {
     const int32_t i = t + kSmallThreads * j;
     const double b = live[j] ? a.rhs[i] : 0.0;
-    r[j] = live[j] && a.guess ? b - small_row_product<kRows>(a, begin[j], end[j], u_lds) : b;
+    r[j] = live[j] && a.guess ? b - ax[j] : b;
     if (live[j]) a.x[i] = x[j];
     p[j] = 0.0;
     sv[j] = 0.0;
@@ -500,12 +520,13 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(CgArgs a) {
         const double rho_new = 1.0 / (2.0 * sigma - rho);
         const double keep = rho_new * rho;
         const double push = 2.0 * rho_new / a.delta;
+        double ad[kRows];
+        small_row_products<kRows, kCheb>(a, begin, end, d_lds[cur], ad);
 #pragma unroll
         for (int j = 0; j < kRows; ++j) {
           const int32_t i = t + kSmallThreads * j;
           if (!live[j]) continue;
-          const double ad = small_row_product<kRows>(a, begin[j], end[j], d_lds[cur]);
-          res[j] = res[j] - dinv[j] * ad;
+          res[j] = res[j] - dinv[j] * ad[j];
           const double d = keep * dcur[j] + push * res[j];
           dcur[j] = d;
           u[j] += d;
@@ -531,9 +552,9 @@ __global__ __launch_bounds__(kSmallThreads) void cg_small_kernel(CgArgs a) {
     ++round;
     double w[kRows];
     double acc[3] = {0.0, 0.0, 0.0};
+    small_row_products<kRows, kCheb>(a, begin, end, u_lds, w);
 #pragma unroll
     for (int j = 0; j < kRows; ++j) {
-      w[j] = live[j] ? small_row_product<kRows>(a, begin[j], end[j], u_lds) : 0.0;
       acc[0] += r[j] * u[j];
       acc[1] += w[j] * u[j];
       acc[2] += r[j] * r[j];
@@ -730,6 +751,9 @@ int32_t run_cg(const CgProblem& problem, CgWorkspace& ws, hipStream_t stream) {
   if (k == 1 && n <= static_cast<int64_t>(kSmallRows) * kSmallThreads) {
     const int64_t rows = (n + kSmallThreads - 1) / kSmallThreads;
     const bool cheb = a.cheb_steps > 0;
+    // The Chebyshev instance reads the float64 values only.
+    HG_REQUIRE(!cheb || a.values32 == nullptr, HG_ERR_RUNTIME,
+               "the Chebyshev-preconditioned solve takes float64 operator values");
     auto kernel = rows == 1 ? (cheb ? cg_small_kernel<1, true> : cg_small_kernel<1, false>)
                   : rows == 2 ? (cheb ? cg_small_kernel<2, true> : cg_small_kernel<2, false>)
                               : (cheb ? cg_small_kernel<3, true> : cg_small_kernel<3, false>);
