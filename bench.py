@@ -917,6 +917,11 @@ def in_flight_steps(solvers, source_sets, steps: int, warmup: int):
         offsets.append(offsets[-1] + s.numel())
     lanes = len(solvers)
     streams = [torch.cuda.Stream() for _ in solvers]
+    # Eager launches on each lane's own stream, as the server's workers run: a replayed graph runs
+    # on a stream the library keeps per host thread, and which hardware queue that lands on is not
+    # the caller's to choose (two lanes on one queue run one after the other).
+    for solver in solvers:
+        solver.set_graphs(False)
     for solver, stream in zip(solvers, streams):
         with torch.cuda.stream(stream):
             for _ in range(warmup):
@@ -951,6 +956,8 @@ def in_flight_steps(solvers, source_sets, steps: int, warmup: int):
         thread.join()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    for solver in solvers:
+        solver.set_graphs(True)
     if errors:
         raise errors[0]
     return elapsed, [x for lane in latencies for x in lane], outs
