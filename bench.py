@@ -462,7 +462,8 @@ def serve_result(args, probe_rates=()) -> dict:
         p = server.stats(since=p_start)
         probes.append({"offered_per_s": round(offered / 10.0, 1),
                        "completed_per_s": round(p["queries_per_s"], 1),
-                       "p50_ms": round(p["p50_ms"], 3), "p99_ms": round(p["p99_ms"], 3)})
+                       "p50_ms": round(p["p50_ms"], 3), "p99_ms": round(p["p99_ms"], 3),
+                       "lanes": p["lanes"], "resident_gb": round(p["resident_bytes"] / 1e9, 3)})
         log(f"   {probes[-1]}")
     server.close()  # every queued request finishes first
     # Accuracy: sampled answers on the smallest meshes against the FP64 oracle.
@@ -495,6 +496,7 @@ def serve_result(args, probe_rates=()) -> dict:
         "batches": stats["batches"],
         "evictions": stats["evictions"],
         "resident_meshes": stats["resident"],
+        "lanes": stats["lanes"],
         "resident_gb": round(stats["resident_bytes"] / 1e9, 3),
         "setup_s": round(setup, 2),
         "sampled_rel_err_max": max(errors) if errors else None,
