@@ -9,9 +9,13 @@
 // tree is one dense front of the multifrontal factorization (Liu 1992). The analysis runs once per
 // mesh on the host, where ordito runs it on the device; it is not per query.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <functional>
+#include <future>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "hg_internal.h"
 
@@ -23,12 +27,28 @@ struct Node {
   std::vector<int32_t> kids;
 };
 
+// The two halves of a split share no vertex and no edge (the separator takes every crossing edge's
+// endpoint on one side), and neither do two components: their dissections run on their own threads,
+// marking disjoint entries of `side` and reading no entry the other marks. Only the node list is
+// shared, behind a lock; a node's id is its position there, and the tree (each node's children, in
+// order) does not depend on the order the threads append in.
 struct Dissector {
   const HostPattern& pattern;
   const float* xyz;
   int32_t leaf_rows;
   std::vector<Node> nodes;
+  std::mutex nodes_lock;
   std::vector<int8_t> side;  // 0: outside the current set, 1: part A, 2: part B
+  // Splits of at least this many vertices hand one half to a thread of its own, down to this depth
+  // (at most 2^depth threads a component).
+  static constexpr size_t kThreadSet = 16384;
+  static constexpr int kThreadDepth = 4;
+
+  int32_t add(Node node) {
+    std::lock_guard<std::mutex> hold(nodes_lock);
+    nodes.push_back(std::move(node));
+    return static_cast<int32_t>(nodes.size() - 1);
+  }
 
   Dissector(const HostPattern& p, const float* c, int32_t leaf)
       : pattern(p), xyz(c), leaf_rows(leaf), side(static_cast<size_t>(p.n), 0) {}
@@ -65,14 +85,14 @@ struct Dissector {
     }
   }
 
-  // Returns the roots of the dissection forest of `set` (consumed).
-  std::vector<int32_t> dissect(std::vector<int32_t> set) {
+  // Returns the roots of the dissection forest of `set` (consumed). `depth` counts the splits
+  // above that may still hand a half to a thread of its own (0: none).
+  std::vector<int32_t> dissect(std::vector<int32_t> set, int depth = 0) {
     std::vector<int32_t> roots;
     if (set.empty()) return roots;
     if (static_cast<int32_t>(set.size()) <= leaf_rows) {
       std::sort(set.begin(), set.end());
-      nodes.push_back(Node{std::move(set), {}});
-      roots.push_back(static_cast<int32_t>(nodes.size() - 1));
+      roots.push_back(add(Node{std::move(set), {}}));
       return roots;
     }
     double axis[3], center[3];
@@ -132,14 +152,22 @@ struct Dissector {
       for (size_t k = 0; k < sep.size(); ++k) sep[k] = order[k].second;
     }
     std::vector<int32_t> sep_copy = sep;
-    std::vector<int32_t> kids_a = dissect(std::move(part_a));
-    std::vector<int32_t> kids_b = dissect(std::move(part_b));
+    std::vector<int32_t> kids_a, kids_b;
+    if (depth > 0 && part_a.size() >= kThreadSet && part_b.size() >= kThreadSet) {
+      auto half = std::async(std::launch::async, [this, &part_a, depth] {
+        return dissect(std::move(part_a), depth - 1);
+      });
+      kids_b = dissect(std::move(part_b), depth - 1);
+      kids_a = half.get();
+    } else {
+      kids_a = dissect(std::move(part_a), depth > 0 ? depth - 1 : 0);
+      kids_b = dissect(std::move(part_b), depth > 0 ? depth - 1 : 0);
+    }
     std::vector<int32_t> kids;
     kids.insert(kids.end(), kids_a.begin(), kids_a.end());
     kids.insert(kids.end(), kids_b.begin(), kids_b.end());
     if (sep_copy.empty()) return kids;
-    nodes.push_back(Node{std::move(sep_copy), std::move(kids)});
-    roots.push_back(static_cast<int32_t>(nodes.size() - 1));
+    roots.push_back(add(Node{std::move(sep_copy), std::move(kids)}));
     return roots;
   }
 };
@@ -159,10 +187,26 @@ Plan analyse(const HostPattern& pattern, const float* coordinates, const Analysi
     const std::vector<int32_t> label = component_labels(pattern, &n_comp);
     std::vector<std::vector<int32_t>> members(static_cast<size_t>(n_comp));
     for (int64_t v = 0; v < n; ++v) members[label[v]].push_back(static_cast<int32_t>(v));
-    for (std::vector<int32_t>& set : members) {
-      std::vector<int32_t> r = dissector.dissect(std::move(set));
-      roots.insert(roots.end(), r.begin(), r.end());
+    // One component: its splits spread over threads. Several: a pool of threads takes them in
+    // turn, each dissected on one thread. Either way every component's roots are kept in
+    // component order.
+    std::vector<std::vector<int32_t>> comp_roots(static_cast<size_t>(n_comp));
+    if (n_comp == 1) {
+      comp_roots[0] = dissector.dissect(std::move(members[0]), Dissector::kThreadDepth);
+    } else if (n_comp > 1) {
+      const int32_t workers = std::max<int32_t>(
+          1, std::min<int32_t>(n_comp, static_cast<int32_t>(std::thread::hardware_concurrency())));
+      std::atomic<int32_t> next_comp{0};
+      auto work = [&] {
+        for (int32_t c = next_comp++; c < n_comp; c = next_comp++)
+          comp_roots[c] = dissector.dissect(std::move(members[c]));
+      };
+      std::vector<std::future<void>> pool;
+      for (int32_t w = 1; w < workers; ++w) pool.push_back(std::async(std::launch::async, work));
+      work();
+      for (std::future<void>& f : pool) f.get();
     }
+    for (const std::vector<int32_t>& r : comp_roots) roots.insert(roots.end(), r.begin(), r.end());
   }
   std::vector<Node>& nodes = dissector.nodes;
 
