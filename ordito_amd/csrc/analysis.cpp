@@ -172,6 +172,23 @@ struct Dissector {
   }
 };
 
+// Runs body(begin, end, worker) over [0, count) in contiguous chunks, one a worker thread (the
+// caller's thread takes the first); waits for all, rethrowing the first failure.
+template <typename F>
+void parallel_chunks(int64_t count, int64_t grain, F&& body) {
+  const int64_t hw = std::max<int64_t>(1, std::thread::hardware_concurrency());
+  const int64_t workers = std::max<int64_t>(1, std::min<int64_t>(hw, (count + grain - 1) / grain));
+  const int64_t chunk = (count + workers - 1) / std::max<int64_t>(workers, 1);
+  std::vector<std::future<void>> pool;
+  for (int64_t w = 1; w < workers; ++w) {
+    const int64_t b = w * chunk, e = std::min(count, b + chunk);
+    if (b >= e) break;
+    pool.push_back(std::async(std::launch::async, [&body, b, e, w] { body(b, e, w); }));
+  }
+  body(0, std::min(count, chunk), 0);
+  for (std::future<void>& f : pool) f.get();
+}
+
 }  // namespace
 
 Plan analyse(const HostPattern& pattern, const float* coordinates, const AnalysisOptions& options) {
@@ -308,39 +325,50 @@ Plan analyse(const HostPattern& pattern, const float* coordinates, const Analysi
     for (int32_t s = 0; s < S; ++s) plan.level_nodes[at[plan.level[s]]++] = s;
   }
 
-  // Symbolic structure: rows(s) = (adjacency of cols(s) past s) U (rows(children) past s).
+  // Symbolic structure: rows(s) = (adjacency of cols(s) past s) U (rows(children) past s). The
+  // supernodes of a level are independent (their children sit on lower levels): each level's are
+  // formed on threads, each thread with its own stamps.
   plan.row_ptr.assign(S + 1, 0);
   plan.nrow.assign(S, 0);
-  std::vector<int32_t> stamp(static_cast<size_t>(n), -1);
   std::vector<std::vector<int32_t>> rows(S);
-  for (int32_t s = 0; s < S; ++s) {
-    const int32_t first = plan.col_begin[s];
-    const int32_t last = first + plan.ncol[s] - 1;
-    std::vector<int32_t>& list = rows[s];
-    for (int32_t c = first; c <= last; ++c) {
-      int32_t v = plan.perm[c];
-      for (int32_t e = pattern.offsets[v]; e < pattern.offsets[v + 1]; ++e) {
-        int32_t u = plan.iperm[pattern.columns[e]];
-        if (u > last && stamp[u] != s) {
-          stamp[u] = s;
-          list.push_back(u);
+  const int64_t hw_threads = std::max<int64_t>(1, std::thread::hardware_concurrency());
+  std::vector<std::vector<int32_t>> stamps(static_cast<size_t>(hw_threads));
+  for (int32_t lv = 0; lv < plan.n_levels; ++lv) {
+    const int32_t l0 = plan.level_ptr[lv], l1 = plan.level_ptr[lv + 1];
+    parallel_chunks(l1 - l0, 16, [&](int64_t b, int64_t e, int64_t worker) {
+      std::vector<int32_t>& stamp = stamps[worker];
+      if (stamp.empty()) stamp.assign(static_cast<size_t>(n), -1);
+      for (int64_t q = b; q < e; ++q) {
+        const int32_t s = plan.level_nodes[l0 + q];
+        const int32_t first = plan.col_begin[s];
+        const int32_t last = first + plan.ncol[s] - 1;
+        std::vector<int32_t>& list = rows[s];
+        for (int32_t c = first; c <= last; ++c) {
+          int32_t v = plan.perm[c];
+          for (int32_t e2 = pattern.offsets[v]; e2 < pattern.offsets[v + 1]; ++e2) {
+            int32_t u = plan.iperm[pattern.columns[e2]];
+            if (u > last && stamp[u] != s) {
+              stamp[u] = s;
+              list.push_back(u);
+            }
+          }
         }
-      }
-    }
-    for (int32_t k = plan.child_ptr[s]; k < plan.child_ptr[s + 1]; ++k) {
-      int32_t child = plan.children[k];
-      for (int32_t r : rows[child]) {
-        HG_REQUIRE(r >= first, HG_ERR_RUNTIME, "supernode tree is not an assembly tree");
-        if (r > last && stamp[r] != s) {
-          stamp[r] = s;
-          list.push_back(r);
+        for (int32_t k = plan.child_ptr[s]; k < plan.child_ptr[s + 1]; ++k) {
+          int32_t child = plan.children[k];
+          for (int32_t r : rows[child]) {
+            HG_REQUIRE(r >= first, HG_ERR_RUNTIME, "supernode tree is not an assembly tree");
+            if (r > last && stamp[r] != s) {
+              stamp[r] = s;
+              list.push_back(r);
+            }
+          }
         }
+        std::sort(list.begin(), list.end());
+        plan.nrow[s] = static_cast<int32_t>(list.size());
       }
-    }
-    std::sort(list.begin(), list.end());
-    plan.nrow[s] = static_cast<int32_t>(list.size());
-    plan.row_ptr[s + 1] = plan.row_ptr[s] + plan.nrow[s];
+    });
   }
+  for (int32_t s = 0; s < S; ++s) plan.row_ptr[s + 1] = plan.row_ptr[s] + plan.nrow[s];
   plan.rows.resize(plan.row_ptr[S]);
   for (int32_t s = 0; s < S; ++s)
     std::copy(rows[s].begin(), rows[s].end(), plan.rows.begin() + plan.row_ptr[s]);
@@ -395,30 +423,52 @@ Plan analyse(const HostPattern& pattern, const float* coordinates, const Analysi
   plan.flops = flops;
   plan.nnz_w = w;
 
-  // Lower-triangle operator entries in front coordinates.
-  for (int64_t i = 0; i < n; ++i) {
-    const int32_t ni = plan.iperm[i];
-    for (int32_t e = pattern.offsets[i]; e < pattern.offsets[i + 1]; ++e) {
-      const int32_t nj = plan.iperm[pattern.columns[e]];
-      if (ni < nj) continue;
-      const int32_t s = super_of_col[nj];
-      const int64_t m = plan.ncol[s] + plan.nrow[s];
-      int64_t rpos;
-      const int32_t first = plan.col_begin[s], last = first + plan.ncol[s] - 1;
-      if (ni <= last) {
-        rpos = ni - first;
-      } else {
-        const int32_t* srow = plan.rows.data() + plan.row_ptr[s];
-        const int32_t* it = std::lower_bound(srow, srow + plan.nrow[s], ni);
-        HG_REQUIRE(it != srow + plan.nrow[s] && *it == ni, HG_ERR_RUNTIME,
-                   "operator entry outside its supernode's structure");
-        rpos = plan.ncol[s] + (it - srow);
+  // Lower-triangle operator entries in front coordinates, in operator order: each thread forms a
+  // contiguous range of rows' into its own lists, appended in range order.
+  {
+    std::vector<std::vector<int32_t>> src(static_cast<size_t>(hw_threads)),
+        erow(static_cast<size_t>(hw_threads)), ecol(static_cast<size_t>(hw_threads));
+    std::vector<std::vector<int64_t>> dst(static_cast<size_t>(hw_threads));
+    parallel_chunks(n, 4096, [&](int64_t b, int64_t e_end, int64_t worker) {
+      // Local lists, moved out at the end: the workers' list headers sit side by side, and
+      // appending to them in place would share their cache lines.
+      std::vector<int32_t> my_src, my_row, my_col;
+      std::vector<int64_t> my_dst;
+      for (int64_t i = b; i < e_end; ++i) {
+        const int32_t ni = plan.iperm[i];
+        for (int32_t e = pattern.offsets[i]; e < pattern.offsets[i + 1]; ++e) {
+          const int32_t nj = plan.iperm[pattern.columns[e]];
+          if (ni < nj) continue;
+          const int32_t s = super_of_col[nj];
+          const int64_t m = plan.ncol[s] + plan.nrow[s];
+          int64_t rpos;
+          const int32_t first = plan.col_begin[s], last = first + plan.ncol[s] - 1;
+          if (ni <= last) {
+            rpos = ni - first;
+          } else {
+            const int32_t* srow = plan.rows.data() + plan.row_ptr[s];
+            const int32_t* it = std::lower_bound(srow, srow + plan.nrow[s], ni);
+            HG_REQUIRE(it != srow + plan.nrow[s] && *it == ni, HG_ERR_RUNTIME,
+                       "operator entry outside its supernode's structure");
+            rpos = plan.ncol[s] + (it - srow);
+          }
+          const int64_t cpos = nj - first;
+          my_src.push_back(e);
+          my_dst.push_back(plan.front_off[s] + cpos * m + rpos);
+          my_row.push_back(ni);
+          my_col.push_back(nj);
+        }
       }
-      const int64_t cpos = nj - first;
-      plan.entry_src.push_back(e);
-      plan.entry_dst.push_back(plan.front_off[s] + cpos * m + rpos);
-      plan.entry_row.push_back(ni);
-      plan.entry_col.push_back(nj);
+      src[worker] = std::move(my_src);
+      dst[worker] = std::move(my_dst);
+      erow[worker] = std::move(my_row);
+      ecol[worker] = std::move(my_col);
+    });
+    for (int64_t w = 0; w < hw_threads; ++w) {
+      plan.entry_src.insert(plan.entry_src.end(), src[w].begin(), src[w].end());
+      plan.entry_dst.insert(plan.entry_dst.end(), dst[w].begin(), dst[w].end());
+      plan.entry_row.insert(plan.entry_row.end(), erow[w].begin(), erow[w].end());
+      plan.entry_col.insert(plan.entry_col.end(), ecol[w].begin(), ecol[w].end());
     }
   }
   return plan;
