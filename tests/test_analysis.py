@@ -416,3 +416,30 @@ def _plan_for_validated(name):
     pattern = ho.operator_pattern(len(vertices), faces)
     for leaf in (8, 48):
         analysis.analyse(pattern.indptr, pattern.indices, vertices, leaf_rows=leaf, validate=True)
+
+
+def test_threaded_analysis_is_deterministic():
+    """The dissection, symbolic rows and entry maps run on threads (a split's halves, separate
+    components, a level's supernodes, ranges of rows): the plan does not depend on the order the
+    threads finish in. A 98k-vertex torus (splits large enough to go to threads) and forty packed
+    tori (one component a thread), each analysed three times, give the same arrays every time."""
+    from ordito_amd import analysis, creation
+
+    cases = [creation.torus(384, 256, 1.0, 0.3, 0.1, 1)]
+    vs, fs, off = [], [], 0
+    for i in range(40):
+        v, f = creation.torus(24 + i, 12 + i // 2, 1.0, 0.3, 0.1, i)
+        vs.append(v + np.array([3.0 * i, 0.0, 0.0], dtype=np.float32))
+        fs.append(np.asarray(f).reshape(-1, 3) + off)
+        off += len(v)
+    cases.append((np.concatenate(vs), np.concatenate(fs)))
+    fields = ("perm", "ncol", "nrow", "parent", "level", "rows", "rel", "entry_src", "entry_dst")
+    for vertices, faces in cases:
+        pattern = ho.operator_pattern(len(vertices), np.asarray(faces).reshape(-1, 3))
+        plans = [
+            analysis.analyse(pattern.indptr, pattern.indices, vertices, leaf_rows=64)
+            for _ in range(3)
+        ]
+        for name in fields:
+            first = getattr(plans[0], name)
+            assert all(np.array_equal(first, getattr(p, name)) for p in plans[1:]), name
