@@ -924,10 +924,6 @@ def in_flight_steps(solvers, source_sets, steps: int, warmup: int):
     # the caller's to choose (two lanes on one queue run one after the other).
     for solver in solvers:
         solver.set_graphs(False)
-    for solver, stream in zip(solvers, streams):
-        with torch.cuda.stream(stream):
-            for _ in range(warmup):
-                solver.geodesic_flat(flat, offsets)
     torch.cuda.synchronize()
     counts = [steps // lanes + (1 if i < steps % lanes else 0) for i in range(lanes)]
     latencies = [[] for _ in solvers]
@@ -938,6 +934,12 @@ def in_flight_steps(solvers, source_sets, steps: int, warmup: int):
     def run(i: int) -> None:
         try:
             with torch.cuda.stream(streams[i]):
+                # Each lane warms up on its own thread and stream, as a server's worker would, the
+                # lanes together (at least five steps: the first concurrent steps of a process have
+                # run one lane after the other now and then).
+                for _ in range(max(warmup, 5)):
+                    solvers[i].geodesic_flat(flat, offsets)
+                streams[i].synchronize()
                 start.wait()
                 for _ in range(counts[i]):
                     s0 = time.perf_counter()
@@ -1170,17 +1172,41 @@ def main() -> int:
             float(np.max(np.abs(mixed_host[q] - served_host[q])) / np.ptp(served_host[q]))
             for q in range(k)
         ]
+        # The same batches with the FP64 line's batches in flight (lanes of the mixed factors).
+        mixed_lanes = [mixed]
+        mixed_meshes = []
+        for _ in range(lanes - 1):
+            lane_mesh = heat.Trimesh(v_gpu, f_gpu)
+            lane = lane_mesh.heat_solver(method="meshlab", precision="mixed")
+            assert lane.factor()
+            lane._solves = {"heat": 2, "poisson": 2}
+            mixed_meshes.append(lane_mesh)
+            mixed_lanes.append(lane)
+        mf_elapsed, mf_lat, mf_outs = in_flight_steps(mixed_lanes, source_sets, args.steps, 1)
+        mixed_equal = all(o is None or np.array_equal(o.cpu().numpy(), mixed_host) for o in mf_outs)
+        log(f"   mixed: {k * args.steps / mf_elapsed:.1f} queries/s with {lanes} in flight, "
+            f"{k * args.steps / m_elapsed:.1f} one at a time")
         result["served_mixed"] = {
-            "value": round(k * args.steps / m_elapsed, 2),
+            "value": round(k * args.steps / mf_elapsed, 2),
             "unit": "queries/s",
-            "ms_per_step": round(1e3 * m_elapsed / args.steps, 3),
-            "p50_ms": ms(m_lat, 50),
-            "p99_ms": ms(m_lat, 99),
+            "in_flight": lanes,
+            "ms_per_step": round(1e3 * mf_elapsed / args.steps, 3),
+            "p50_ms": ms(mf_lat, 50),
+            "p99_ms": ms(mf_lat, 99),
+            "single_lane": {
+                "value": round(k * args.steps / m_elapsed, 2),
+                "ms_per_step": round(1e3 * m_elapsed / args.steps, 3),
+                "p50_ms": ms(m_lat, 50),
+                "p99_ms": ms(m_lat, 99),
+            },
+            "lanes_bit_equal": mixed_equal,
+            "vs_plain_fp64": round(k * args.steps / mf_elapsed / plain_value, 2)
+            if plain_value > 0 else None,
             "refinements": m_stats,
             "max_rel_diff_from_served": max(from_served),
             "phase_ms_per_step": {key: round(v[0] / args.steps, 3) for key, v in m_timing.items() if v[1]},
         }
-        del mixed, mesh_mixed
+        del mixed, mesh_mixed, mixed_lanes, mixed_meshes, mf_outs
 
     oracle = None
     oracle_setup = 0.0
