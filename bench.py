@@ -127,6 +127,8 @@ def parse() -> argparse.Namespace:
     many.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     many.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget")
     many.add_argument("--workers", type=int, default=4, help="serve: batches solved at once")
+    many.add_argument("--pack-vertices", type=int, default=0,
+                      help="serve: meshes of at most this many vertices served in packed groups")
     args = parser.parse_args()
     defaults = {"batch": (256, 1e3, 2e5), "serve": (64, 1e4, 1e6)}.get(args.workload)
     if defaults:
@@ -408,7 +410,8 @@ def serve_result(args, probe_rates=()) -> dict:
     log(f"== generating {args.meshes} meshes")
     meshes = creation.mesh_collection(args.meshes, args.min_vertices, args.max_vertices, rng)
     sizes = {i: len(v) for i, (v, _) in meshes.items()}
-    server = GeodesicServer(meshes, budget_bytes=args.budget_gb * 1e9, workers=args.workers)
+    server = GeodesicServer(meshes, budget_bytes=args.budget_gb * 1e9, workers=args.workers,
+                            pack_vertices=args.pack_vertices)
     log(f"== warming {args.meshes} meshes ({sum(sizes.values())} vertices)")
     t0 = time.perf_counter()
     server.warm()
@@ -485,7 +488,9 @@ def serve_result(args, probe_rates=()) -> dict:
             "workload": f"{args.meshes} meshes, V log-uniform in [{args.min_vertices:.0f}, "
             f"{args.max_vertices:.0f}] (icosphere / torus / planar grid), Poisson arrivals at "
             f"{args.rate:.0f}/s for {args.duration:.0f}s, 1-32 sources per query, "
-            f"{args.budget_gb:.0f} GB budget, {args.workers} workers",
+            f"{args.budget_gb:.0f} GB budget, {args.workers} workers"
+            + (f", meshes of at most {args.pack_vertices} vertices in packed groups"
+               if args.pack_vertices else ""),
             "total_vertices": int(sum(sizes.values())),
         },
         "offered_per_s": round(submitted / args.duration, 2),
@@ -496,6 +501,7 @@ def serve_result(args, probe_rates=()) -> dict:
         "batches": stats["batches"],
         "evictions": stats["evictions"],
         "resident_meshes": stats["resident"],
+        "packs": stats.get("packs", 0),
         "lanes": stats["lanes"],
         "resident_gb": round(stats["resident_bytes"] / 1e9, 3),
         "setup_s": round(setup, 2),
@@ -507,7 +513,8 @@ def serve_result(args, probe_rates=()) -> dict:
 
 
 def serve_workload(args) -> int:
-    print(json.dumps(serve_result(args)))
+    probes = [float(r) for r in args.serve_probes.split(",") if r.strip()]
+    print(json.dumps(serve_result(args, probe_rates=probes)))
     return 0
 
 

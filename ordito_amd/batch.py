@@ -111,6 +111,13 @@ class MeshBatch:
         _lib.call("hg_nbytes", self._ptr, ctypes.byref(out))
         return int(out.value)
 
+    @property
+    def device_bytes(self) -> int:
+        """Every device byte the batch's state holds: operators, plan, factors and workspaces."""
+        out = ctypes.c_int64()
+        _lib.call("hg_device_bytes", self._ptr, ctypes.byref(out))
+        return int(out.value)
+
     def release(self) -> None:
         _lib.call("hg_release", self._ptr)
 

@@ -28,6 +28,14 @@ needs around that, with no counterpart in the reference:
   bandwidth-bound levels of the other's. The factorization is deterministic, so every lane returns
   the same bits for the same sources.
 
+- **Batching across meshes.** With ``pack_vertices``, meshes of at most that many vertices are
+  packed at start-up into groups (``MeshBatch``: one block-diagonal system of up to
+  ``pack_group_vertices`` vertices, ``hg_mesh_create_packed``), and a group is served like one
+  mesh: its requests share one queue, and a batch takes them whatever their mesh, solved in one
+  batched call (``hg_geodesic_packed``), each query a column of its own mesh's rows. A packed
+  batch sweeps the whole group's factors whichever meshes it asks about, so packing pays where the
+  group's meshes are all asked at once (high load, many small meshes) and not at a trickle.
+
 A batch's results are the ones each request gets alone, within the refinement's agreement (both
 are refined solves through the same factors).
 """
@@ -44,8 +52,16 @@ from typing import Any
 import numpy as np
 
 from . import _arrays
+from .batch import MeshBatch
 from .heat import HeatSolver
 from .mesh import Trimesh
+
+
+@dataclass(frozen=True)
+class _Pack:
+    """The key of a group of packed meshes."""
+
+    index: int
 
 
 @dataclass
@@ -57,6 +73,7 @@ class _Request:
     consumer: Any = None  # the submitting thread's current stream, which will read the result
     started: float = 0.0
     done: float = 0.0
+    member: int = 0  # the mesh's position in its packed group (0 for a mesh served alone)
 
 
 @dataclass(frozen=True)
@@ -75,7 +92,7 @@ class _Lane:
     """One handle of a mesh: its own operators, factors and workspaces."""
 
     trimesh: Trimesh | None = None  # made by the lane's first batch's worker, on its stream
-    solver: HeatSolver | None = None
+    solver: HeatSolver | MeshBatch | None = None  # a packed group's lane holds a MeshBatch
     factored: bool = False
     nbytes: int = 0  # every device byte the lane holds, measured after each batch
     busy: bool = False
@@ -123,12 +140,14 @@ class GeodesicServer:
     ``meshes``: mapping from a mesh id to ``(vertices, faces)``. ``budget_bytes``: the HBM the
     resident meshes may hold. ``workers``: batches solved at once (each on its own stream).
     ``replicas``: the lanes (independent handles) one mesh may hold, so that up to that many of its
-    batches run at once.
+    batches run at once. ``pack_vertices``: meshes of at most this many vertices are served in
+    packed groups of up to ``pack_group_vertices`` vertices (0: none).
     """
 
     def __init__(self, meshes: dict, *, budget_bytes: float = 200e9, max_batch: int = 64,
                  workers: int = 4, precision: str = "fp64", warm_columns: int = 16,
-                 replicas: int = 2) -> None:
+                 replicas: int = 2, pack_vertices: int = 0,
+                 pack_group_vertices: int = 2_000_000) -> None:
         _arrays.require_gpu()
         self._torch = _arrays.torch_module()
         self._meshes = dict(meshes)
@@ -139,6 +158,31 @@ class GeodesicServer:
         if int(replicas) < 1:
             raise ValueError(f"replicas must be at least 1, got {replicas}")
         self.replicas = int(replicas)
+        # Serving units: a mesh alone, or a packed group (greedily in mesh order).
+        self._units: dict[Any, list] = {}
+        self._unit_of: dict[Any, tuple[Any, int]] = {}
+        group: list = []
+        group_vertices = 0
+
+        def close_group() -> None:
+            key = _Pack(sum(isinstance(k, _Pack) for k in self._units))
+            self._units[key] = list(group)
+            for member, mesh_id in enumerate(group):
+                self._unit_of[mesh_id] = (key, member)
+
+        for mesh_id, (vertices, _) in self._meshes.items():
+            n = int(np.prod(vertices.shape)) // 3
+            if int(pack_vertices) > 0 and n <= int(pack_vertices):
+                if group and (group_vertices + n > int(pack_group_vertices) or len(group) >= 256):
+                    close_group()
+                    group, group_vertices = [], 0
+                group.append(mesh_id)
+                group_vertices += n
+            else:
+                self._units[mesh_id] = [mesh_id]
+                self._unit_of[mesh_id] = (mesh_id, 0)
+        if group:
+            close_group()
         self._resident: dict[Any, _Resident] = {}
         self._lock = threading.Condition()
         self._closing = False
@@ -161,12 +205,14 @@ class GeodesicServer:
         """
         columns = self.warm_columns if columns is None else int(columns)
         futures = []
-        for mesh_id in self._meshes:
+        for unit, members in self._units.items():
             with self._lock:
-                entry = self._resident.get(mesh_id) or self._new_entry(mesh_id)
-                for _ in range(int(columns)):
-                    request = _Request(mesh_id, np.zeros(1, dtype=np.int32),
-                                       concurrent.futures.Future(), time.perf_counter())
+                entry = self._resident.get(unit) or self._new_entry(unit)
+                for q in range(int(columns)):
+                    member = q % len(members)
+                    request = _Request(members[member], np.zeros(1, dtype=np.int32),
+                                       concurrent.futures.Future(), time.perf_counter(),
+                                       member=member)
                     entry.pending.append(request)
                     futures.append(request.future)
                 self._lock.notify_all()
@@ -177,15 +223,16 @@ class GeodesicServer:
         """Queue one query; the future resolves to the (V,) float64 distance on the GPU."""
         if mesh_id not in self._meshes:
             raise ValueError(f"unknown mesh {mesh_id!r}")
+        unit, member = self._unit_of[mesh_id]
         request = _Request(mesh_id, np.asarray(sources, dtype=np.int32).reshape(-1),
                            concurrent.futures.Future(), time.perf_counter(),
-                           consumer=self._torch.cuda.current_stream())
+                           consumer=self._torch.cuda.current_stream(), member=member)
         with self._lock:
             if self._closing:
                 raise RuntimeError("the server is closed")
-            entry = self._resident.get(mesh_id)
+            entry = self._resident.get(unit)
             if entry is None:
-                entry = self._new_entry(mesh_id)
+                entry = self._new_entry(unit)
             entry.pending.append(request)
             self._lock.notify_all()
         return request.future
@@ -227,21 +274,25 @@ class GeodesicServer:
             "mean_ms": 1e3 * float(latency.mean()),
             "batches": self._batches,
             "evictions": self._evictions,
-            "resident": sum(1 for e in self._resident.values() if e.factored),
+            "resident": sum(len(self._units[k]) for k, e in self._resident.items() if e.factored),
+            "packs": sum(1 for k, e in self._resident.items() if isinstance(k, _Pack) and e.factored),
             "lanes": sum(1 for e in self._resident.values() for lane in e.lanes if lane.factored),
             "resident_bytes": self._held_bytes(),
         }
 
     # -- scheduling ------------------------------------------------------------------------------
 
-    def _new_entry(self, mesh_id) -> _Resident:
-        vertices, faces = self._meshes[mesh_id]
-        n_vertices = int(np.prod(vertices.shape)) // 3
-        n_faces = int(np.prod(faces.shape)) // 3
-        entry = _Resident(n_vertices, n_faces,
-                          estimate=footprint_estimate(n_vertices, n_faces, self.precision,
-                                                      self.max_batch))
-        self._resident[mesh_id] = entry
+    def _new_entry(self, unit) -> _Resident:
+        n_vertices = n_faces = estimate = 0
+        for mesh_id in self._units[unit]:
+            vertices, faces = self._meshes[mesh_id]
+            v = int(np.prod(vertices.shape)) // 3
+            f = int(np.prod(faces.shape)) // 3
+            n_vertices += v
+            n_faces += f
+            estimate += footprint_estimate(v, f, self.precision, self.max_batch)
+        entry = _Resident(n_vertices, n_faces, estimate=estimate)
+        self._resident[unit] = entry
         return entry
 
     def _held_bytes(self, exclude: _Lane | None = None) -> int:
@@ -290,11 +341,12 @@ class GeodesicServer:
         return None
 
     def _next_batch(self):
-        """The mesh whose oldest waiting request is oldest and which has a lane to solve it on,
-        with up to max_batch of its requests."""
-        order = sorted((e.pending[0].submitted, k) for k, e in self._resident.items() if e.pending)
-        for _, mesh_id in order:
-            entry = self._resident[mesh_id]
+        """The serving unit (a mesh, or a packed group) whose oldest waiting request is oldest and
+        which has a lane to solve it on, with up to max_batch of its requests."""
+        order = sorted(((e.pending[0].submitted, k) for k, e in self._resident.items() if e.pending),
+                       key=lambda head: head[0])
+        for _, unit in order:
+            entry = self._resident[unit]
             lane = self._lane_for(entry)
             if lane is None:
                 continue
@@ -305,7 +357,7 @@ class GeodesicServer:
                 entry.lanes.remove(lane)
                 continue
             batch = [entry.pending.popleft() for _ in range(min(self.max_batch, len(entry.pending)))]
-            return mesh_id, entry, lane, batch
+            return unit, entry, lane, batch
         return None
 
     def _schedule(self) -> None:
@@ -319,42 +371,54 @@ class GeodesicServer:
                             and len(self._idle) == len(self._streams):
                         return
                     self._lock.wait(timeout=0.05)
-                mesh_id, entry, lane, batch = pick
+                unit, entry, lane, batch = pick
                 worker = self._idle.pop()
-            self._pool.submit(self._run, worker, mesh_id, entry, lane, batch)
+            self._pool.submit(self._run, worker, unit, entry, lane, batch)
 
-    def _run(self, worker: int, mesh_id, entry: _Resident, lane: _Lane,
+    def _run(self, worker: int, unit, entry: _Resident, lane: _Lane,
              batch: list[_Request]) -> None:
         torch = self._torch
         try:
             start = time.perf_counter()
             for r in batch:
                 r.started = start
+            packed = isinstance(unit, _Pack)
             with torch.cuda.stream(self._streams[worker]):
-                if lane.solver is None:
-                    vertices, faces = self._meshes[mesh_id]
+                if lane.solver is None and packed:
+                    lane.solver = MeshBatch([self._meshes[m] for m in self._units[unit]],
+                                            precision=self.precision)
+                elif lane.solver is None:
+                    vertices, faces = self._meshes[unit]
                     lane.trimesh = Trimesh(vertices, faces)
                     lane.solver = lane.trimesh.heat_solver(method="meshlab", precision=self.precision)
                     # Graph replay pays per new (mesh, batch width) capture, and launches from
                     # several threads contend: measured p50 4.5 ms eager against 10.6 ms replayed.
                     lane.solver.set_graphs(False)
                 if not lane.factored:
-                    # A refusal under the library's own budget leaves the lane iterating.
+                    # A refusal under the library's own budget leaves the lane iterating (a packed
+                    # group solves through its factors only, and raises instead).
                     lane.factored = lane.solver.factor()
-                # The batch's sources in one flat upload, its sets delimited by offsets.
-                offsets = [0]
-                for r in batch:
-                    offsets.append(offsets[-1] + int(r.sources.size))
-                flat = torch.from_numpy(np.concatenate([r.sources for r in batch])).to("cuda")
-                fields = lane.solver.geodesic_flat(flat, offsets)
+                if packed:
+                    # One batched call over the group: each query a column of its mesh's rows.
+                    fields = lane.solver.geodesic([(r.member, r.sources) for r in batch])
+                    storage = fields[0] if fields else None
+                else:
+                    # The batch's sources in one flat upload, its sets delimited by offsets.
+                    offsets = [0]
+                    for r in batch:
+                        offsets.append(offsets[-1] + int(r.sources.size))
+                    flat = torch.from_numpy(np.concatenate([r.sources for r in batch])).to("cuda")
+                    fields = lane.solver.geodesic_flat(flat, offsets)
+                    storage = fields
                 self._streams[worker].synchronize()
                 lane.nbytes = lane.solver.device_bytes
             # The results' memory comes from this worker stream's pool: each consumer's stream is
-            # recorded on it, so a result freed while its reads are still queued there is not
-            # reused by this worker's next batch before they finish.
+            # recorded on it (a packed batch's fields are views of one allocation), so a result
+            # freed while its reads are still queued there is not reused by this worker's next
+            # batch before they finish.
             for consumer in {r.consumer for r in batch if r.consumer is not None}:
-                if consumer != self._streams[worker]:
-                    fields.record_stream(consumer)
+                if consumer != self._streams[worker] and storage is not None:
+                    storage.record_stream(consumer)
             done = time.perf_counter()
             for q, r in enumerate(batch):
                 r.done = done

@@ -212,3 +212,36 @@ def test_native_server_matches_each_request_alone_and_evicts_under_a_tight_budge
             v, f = meshes[name]
             worst, _ = ho.relative_error(got, _alone(cuda, v, f, sources))
             assert worst <= 1e-10, (budget, i, name, worst)
+
+
+def test_packed_group_batches_requests_across_meshes(cuda):
+    """With pack_vertices every test mesh is packed into one group: requests for different meshes
+    share a queue and a batched call (fewer batches than requests, one pack resident), and every
+    field matches the same request solved alone; a mesh above the threshold is served alone beside
+    the group."""
+    from ordito_amd.serve import GeodesicServer
+
+    names = ["icosphere3", "torus", "grid"]
+    meshes = {name: mesh(name) for name in names}
+    meshes["big"] = mesh("icosphere4")
+    limit = max(len(meshes[name][0]) for name in names)
+    assert len(meshes["big"][0]) > limit
+    server = GeodesicServer(meshes, workers=2, max_batch=16, pack_vertices=limit)
+    rng = np.random.default_rng(5)
+    requests = []
+    for i in range(40):
+        name = (names + ["big"])[i % 4]
+        n = len(meshes[name][0])
+        sources = rng.choice(n, size=int(rng.integers(1, 4)), replace=False).astype(np.int32)
+        requests.append((name, sources, server.submit(name, sources)))
+    results = [(name, s, f.result(timeout=120).cpu().numpy()) for name, s, f in requests]
+    stats = server.stats()
+    server.close()
+    assert stats["completed"] == len(requests) and stats["batches"] < len(requests)
+    assert stats["packs"] == 1 and stats["resident"] == 4
+    for i, (name, sources, got) in enumerate(results):
+        v, f = meshes[name]
+        assert got.shape == (len(v),)
+        if i < 12:
+            worst, _ = ho.relative_error(got, _alone(cuda, v, f, sources))
+            assert worst <= 1e-10, (i, name, worst)
