@@ -32,6 +32,13 @@ struct DevicePlan {
     std::vector<int32_t> reduce_ptr;  // reduction groups per level (empty level: no split)
     std::vector<int32_t> slots;       // partial slots each level uses
     DeviceArray<int32_t> s, tile, k0, k1, slot;  // per product task; slot -1 writes directly
+    // The same per task with its front's figures, three int4 a task read in one round trip:
+    // {s, tile, k0, k1}, {slot, ncol, col_begin, ncol + nrow}, {w_off (two words), vec_off
+    // (forward) or row_ptr (backward) as two words}.
+    DeviceArray<int4> desc;
+    // The reduction groups likewise: {s, tile, first slot, slot count}, {0, ncol, col_begin, m},
+    // {0, 0, vec_off (forward) as two words}.
+    DeviceArray<int4> rdesc;
     DeviceArray<int32_t> rs, rtile, rfirst, rcount;  // per reduction group
   };
   SweepTasks fwd, bwd;

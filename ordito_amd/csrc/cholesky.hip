@@ -565,7 +565,29 @@ void validate_factor_tasks(const Plan& plan, const HostFactorTasks& h) {
 
 namespace {
 
-void upload_sweep(const HostSweep& h, DevicePlan::SweepTasks& out, hipStream_t stream) {
+void upload_sweep(const Plan& plan, bool forward, const HostSweep& h,
+                  DevicePlan::SweepTasks& out, hipStream_t stream) {
+  std::vector<int4> desc(3 * h.s.size());
+  for (size_t t = 0; t < h.s.size(); ++t) {
+    const int32_t s = h.s[t];
+    const int64_t w_off = plan.w_off[s];
+    const int64_t third = forward ? plan.vec_off[s] : static_cast<int64_t>(plan.row_ptr[s]);
+    desc[3 * t] = make_int4(s, h.tile[t], h.k0[t], h.k1[t]);
+    desc[3 * t + 1] = make_int4(h.slot[t], plan.ncol[s], plan.col_begin[s], plan.ncol[s] + plan.nrow[s]);
+    desc[3 * t + 2] = make_int4(static_cast<int32_t>(w_off & 0xffffffff), static_cast<int32_t>(w_off >> 32),
+                                static_cast<int32_t>(third & 0xffffffff), static_cast<int32_t>(third >> 32));
+  }
+  out.desc.upload(desc, stream);
+  std::vector<int4> rdesc(3 * h.rs.size());
+  for (size_t g = 0; g < h.rs.size(); ++g) {
+    const int32_t s = h.rs[g];
+    const int64_t third = forward ? plan.vec_off[s] : 0;
+    rdesc[3 * g] = make_int4(s, h.rtile[g], h.rfirst[g], h.rcount[g]);
+    rdesc[3 * g + 1] = make_int4(0, plan.ncol[s], plan.col_begin[s], plan.ncol[s] + plan.nrow[s]);
+    rdesc[3 * g + 2] = make_int4(0, 0, static_cast<int32_t>(third & 0xffffffff),
+                                 static_cast<int32_t>(third >> 32));
+  }
+  out.rdesc.upload(rdesc, stream);
   out.level_ptr = h.level_ptr;
   out.reduce_ptr = h.reduce_ptr;
   out.slots = h.slots;
@@ -586,8 +608,8 @@ void DevicePlan::upload_sweeps(const Plan& plan, int32_t target_tasks, hipStream
   const HostSweep hf = sweep_tasks(plan, true, target_tasks);
   const HostSweep hb = sweep_tasks(plan, false, target_tasks);
   max_slots = std::max(hf.max_slots, hb.max_slots);
-  upload_sweep(hf, fwd, stream);
-  upload_sweep(hb, bwd, stream);
+  upload_sweep(plan, true, hf, fwd, stream);
+  upload_sweep(plan, false, hb, bwd, stream);
   split_target = target_tasks;
 }
 
@@ -677,6 +699,7 @@ int64_t DevicePlan::bytes() const {
          level_nodes.bytes() + front_off.bytes() + w_off.bytes() + vec_off.bytes() +
          entry_src.bytes() + entry_dst.bytes() + entry_row.bytes() + entry_col.bytes() +
          fwd.s.bytes() * 5 + fwd.rs.bytes() * 4 + bwd.s.bytes() * 5 + bwd.rs.bytes() * 4 +
+         fwd.desc.bytes() + fwd.rdesc.bytes() + bwd.desc.bytes() + bwd.rdesc.bytes() +
          gather_ptr.bytes() + gather_src.bytes() + gather_pair.bytes() +
          p_offsets.bytes() + p_columns.bytes() + p_src.bytes() + factor_tasks.bytes();
 }

@@ -212,23 +212,44 @@ __device__ __forceinline__ void stage_products_blocks(const double (*As)[kTile +
 // kGathers: the level's fronts have children whose update vectors f gathers (a.gathers); the
 // leaves' instance carries none of that code, and fits 80 registers: six waves a SIMD instead of
 // four.
+// A task's descriptor: {s, tile, k0, k1}, {slot, ncol, col_begin, m}, {a 64-bit offset of the
+// front's solve block, a 64-bit second offset}, read together.
+struct Task {
+  int32_t s, tile, k0, k1, slot, nc, c0, m;
+  int64_t w_off, other;
+};
+
+__device__ __forceinline__ Task load_task(const int4* __restrict__ desc, int32_t task) {
+  const int4* d = desc + 3 * static_cast<int64_t>(task);
+  const int4 d0 = d[0], d1 = d[1], d2 = d[2];
+  Task t;
+  t.s = d0.x;
+  t.tile = d0.y;
+  t.k0 = d0.z;
+  t.k1 = d0.w;
+  t.slot = d1.x;
+  t.nc = d1.y;
+  t.c0 = d1.z;
+  t.m = d1.w;
+  t.w_off = static_cast<int64_t>(static_cast<uint32_t>(d2.x)) | (static_cast<int64_t>(d2.y) << 32);
+  t.other = static_cast<int64_t>(static_cast<uint32_t>(d2.z)) | (static_cast<int64_t>(d2.w) << 32);
+  return t;
+}
+
 template <typename TW, bool kGathers>
 __global__ __launch_bounds__(256, kGathers ? 4 : 6) void fwd_tile_kernel(
-    const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_r0,
-    const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
-    const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
+    const int4* __restrict__ desc, const TW* __restrict__ w, SweepArgs a) {
   __shared__ double Ws[kInner][kTile + kPad];  // [inner][row]
   __shared__ double Fs[kInner][kCols + kPad];  // [inner][right-hand side]
-  const int32_t task = xcd_block();
-  const int32_t s = task_s[task];
-  const int32_t r0 = task_r0[task];
-  const int32_t k0 = task_k0[task], k1 = task_k1[task];
-  const int32_t slot = task_slot[task];
-  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
-  const int32_t m = nc + a.nrow[s];
+  const Task tk = load_task(desc, xcd_block());
+  const int32_t r0 = tk.tile;
+  const int32_t k0 = tk.k0, k1 = tk.k1;
+  const int32_t slot = tk.slot;
+  const int32_t nc = tk.nc, c0 = tk.c0;
+  const int32_t m = tk.m;
   const int32_t kk = a.kk;
-  const TW* W = w + a.w_off[s];
-  const int64_t vbase = a.vec_off[s];
+  const TW* W = w + tk.w_off;
+  const int64_t vbase = tk.other;
   const int t = threadIdx.x, wave = t / 64, lane = t % 64;
   const bool rows_live = r0 + 16 * wave < m;  // wave-uniform
   // The epilogue's gather pairs (this lane's update rows lane / 16 + 4v), loaded before the stages.
@@ -370,15 +391,14 @@ __device__ __forceinline__ void add_partials(const double* __restrict__ partials
 
 // Adds a split forward tile's partials in slot order and applies the sum; blockIdx.y picks 16 of
 // the tile's 64 rows.
-__global__ __launch_bounds__(256) void fwd_reduce_kernel(
-    const int32_t* __restrict__ grp_s, const int32_t* __restrict__ grp_r0,
-    const int32_t* __restrict__ grp_first, const int32_t* __restrict__ grp_count, SweepArgs a) {
-  const int32_t task = xcd_block();
-  const int32_t s = grp_s[task], r0 = grp_r0[task];
-  const int32_t first = grp_first[task], count = grp_count[task];
-  const int32_t nc = a.ncol[s], m = nc + a.nrow[s], c0 = a.col_begin[s];
+__global__ __launch_bounds__(256) void fwd_reduce_kernel(const int4* __restrict__ rdesc,
+                                                         SweepArgs a) {
+  const Task g = load_task(rdesc, xcd_block());
+  const int32_t r0 = g.tile;
+  const int32_t first = g.k0, count = g.k1;
+  const int32_t nc = g.nc, m = g.m, c0 = g.c0;
   const int32_t kk = a.kk;
-  const int64_t vbase = a.vec_off[s];
+  const int64_t vbase = g.other;
   constexpr int kRows = kTile / kReduceSplit;
   const int32_t lo = blockIdx.y * kRows;
   const int32_t rows = min(kRows, m - r0 - lo);
@@ -423,22 +443,19 @@ constexpr int kBwdInner = 16;
 
 template <typename TW, bool kIterate>
 __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
-    const int32_t* __restrict__ task_s, const int32_t* __restrict__ task_j0,
-    const int32_t* __restrict__ task_k0, const int32_t* __restrict__ task_k1,
-    const int32_t* __restrict__ task_slot, const TW* __restrict__ w, SweepArgs a) {
+    const int4* __restrict__ desc, const TW* __restrict__ w, SweepArgs a) {
   constexpr int kD = kBwdInner, kPer = kD * kTile / 256;  // entries a thread stages per operand
   __shared__ double Ws[kD][kTile + kPad];  // [inner][inverse-factor column]
   __shared__ double Gs[kD][kCols + kPad];  // [inner][right-hand side]
-  const int32_t task = xcd_block();
-  const int32_t s = task_s[task];
-  const int32_t j0 = task_j0[task];
-  const int32_t k0 = task_k0[task], k1 = task_k1[task];
-  const int32_t slot = task_slot[task];
-  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
-  const int32_t m = nc + a.nrow[s];
+  const Task tk = load_task(desc, xcd_block());
+  const int32_t j0 = tk.tile;
+  const int32_t k0 = tk.k0, k1 = tk.k1;
+  const int32_t slot = tk.slot;
+  const int32_t nc = tk.nc, c0 = tk.c0;
+  const int32_t m = tk.m;
   const int32_t kk = a.kk;
-  const TW* W = w + a.w_off[s];
-  const int32_t* srows = a.rows + a.row_ptr[s];
+  const TW* W = w + tk.w_off;
+  const int32_t* srows = a.rows + tk.other;
   const int t = threadIdx.x, wave = t / 64, lane = t % 64;
   const bool cols_live = j0 + 16 * wave < nc;  // wave-uniform
   // Staging: entry e = t + 256u is inverse-factor column e / kD at inner e % kD (contiguous in
@@ -535,13 +552,12 @@ __global__ __launch_bounds__(256, 6) void bwd_tile_kernel(
   }
 }
 
-__global__ __launch_bounds__(256) void bwd_reduce_kernel(
-    const int32_t* __restrict__ grp_s, const int32_t* __restrict__ grp_j0,
-    const int32_t* __restrict__ grp_first, const int32_t* __restrict__ grp_count, SweepArgs a) {
-  const int32_t task = xcd_block();
-  const int32_t s = grp_s[task], j0 = grp_j0[task];
-  const int32_t first = grp_first[task], count = grp_count[task];
-  const int32_t nc = a.ncol[s], c0 = a.col_begin[s];
+__global__ __launch_bounds__(256) void bwd_reduce_kernel(const int4* __restrict__ rdesc,
+                                                         SweepArgs a) {
+  const Task g = load_task(rdesc, xcd_block());
+  const int32_t j0 = g.tile;
+  const int32_t first = g.k0, count = g.k1;
+  const int32_t nc = g.nc, c0 = g.c0;
   const int32_t kk = a.kk;
   constexpr int kRows = kTile / kReduceSplit;
   const int32_t lo = blockIdx.y * kRows;
@@ -1105,14 +1121,13 @@ void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream)
                                                       F.slot.ptr + t0, w, a);
     } else {
       auto tile = a.gathers ? fwd_tile_kernel<TW, true> : fwd_tile_kernel<TW, false>;
-      tile<<<tasks, 256, 0, stream>>>(F.s.ptr + t0, F.tile.ptr + t0, F.k0.ptr + t0, F.k1.ptr + t0,
-                                      F.slot.ptr + t0, w, a);
+      tile<<<tasks, 256, 0, stream>>>(F.desc.ptr + 3 * static_cast<int64_t>(t0), w, a);
     }
     const int32_t r0 = F.reduce_ptr[level];
     const int32_t groups = F.reduce_ptr[level + 1] - r0;
     if (groups > 0) {
       fwd_reduce_kernel<<<dim3(groups, kReduceSplit), 256, 0, stream>>>(
-          F.rs.ptr + r0, F.rtile.ptr + r0, F.rfirst.ptr + r0, F.rcount.ptr + r0, a);
+          F.rdesc.ptr + 3 * static_cast<int64_t>(r0), a);
     }
   }
   a.gathers = 0;
@@ -1126,14 +1141,13 @@ void sweep(const DevicePlan& plan, const TW* w, SweepArgs a, hipStream_t stream)
                                                       B.slot.ptr + t0, w, a);
     } else {
       auto tile = a.x ? bwd_tile_kernel<TW, true> : bwd_tile_kernel<TW, false>;
-      tile<<<tasks, 256, 0, stream>>>(B.s.ptr + t0, B.tile.ptr + t0, B.k0.ptr + t0, B.k1.ptr + t0,
-                                      B.slot.ptr + t0, w, a);
+      tile<<<tasks, 256, 0, stream>>>(B.desc.ptr + 3 * static_cast<int64_t>(t0), w, a);
     }
     const int32_t r0 = B.reduce_ptr[level];
     const int32_t groups = B.reduce_ptr[level + 1] - r0;
     if (groups > 0) {
       bwd_reduce_kernel<<<dim3(groups, kReduceSplit), 256, 0, stream>>>(
-          B.rs.ptr + r0, B.rtile.ptr + r0, B.rfirst.ptr + r0, B.rcount.ptr + r0, a);
+          B.rdesc.ptr + 3 * static_cast<int64_t>(r0), a);
     }
   }
   HG_HIP(hipGetLastError());
