@@ -131,11 +131,13 @@ int hg_operators(hg_mesh* mesh, double t, int32_t method, int32_t solver_default
 int hg_operator_views(hg_mesh* mesh, hg_operator_view* out_view);
 
 /*
- * The cotangent operator's pattern built on the device from device faces, with no mesh bound:
+ * The cotangent operator's pattern built on the device from the faces (device or host memory,
+ * copied into the library's first), with no mesh bound:
  * mesh_operator_pattern(faces, n_vertices) (laplacian.py:1071-1141, operator="cotmatrix"; the
  * sorted-key build of _mesh_operator_pattern, laplacian.py:1212-1389, and
- * kernels/laplacian.py:707-760). Every buffer is caller-owned device memory and may be NULL to
- * skip it; capacities as in the reference, of which out_sizes reports what was written:
+ * kernels/laplacian.py:707-760). Every output buffer is caller-owned (device or host memory) and
+ * may be NULL to skip it; capacities as in the reference, of which out_sizes reports what was
+ * written:
  *   d_offsets (V+1) row bounds; d_columns (cap 6F+V) sorted columns per row;
  *   d_corner_ptr (cap 6F+V+1) each entry's run into d_corner_slots (the reference's run_start);
  *   d_corner_slots (cap 6F) the corner slots 3f+e opposite each entry's edge, ascending per run

@@ -1346,21 +1346,25 @@ int hg_operator_pattern(const int32_t* d_faces, int64_t n_faces, int64_t n_verti
     HG_REQUIRE(out_sizes != nullptr && (n_faces == 0 || d_faces != nullptr), HG_ERR_VALUE,
                "bad arguments");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    // The faces are copied into library memory first (from device or host memory alike), as a
+    // mesh's are, so no kernel reads the caller's pointer directly.
+    DeviceArray<int32_t> faces(std::max<int64_t>(3 * n_faces, 1));
+    if (n_faces > 0)
+      HG_HIP(hipMemcpyAsync(faces.ptr, d_faces, sizeof(int32_t) * 3 * n_faces, hipMemcpyDefault,
+                            st));
     DeviceArray<int32_t> offsets, columns, corner_ptr, corner_slots, vertex_ptr, vertex_corners;
-    build_pattern_device(n_vertices, d_faces, n_faces, st,
+    build_pattern_device(n_vertices, faces.ptr, n_faces, st,
                          {offsets, columns, corner_ptr, corner_slots, vertex_ptr, vertex_corners},
                          nullptr);
-    int32_t totals[2] = {0, 0};  // entries, corner slots
-    HG_HIP(hipMemcpyAsync(&totals[0], offsets.ptr + n_vertices, sizeof(int32_t),
+    int32_t entries = 0;
+    HG_HIP(hipMemcpyAsync(&entries, offsets.ptr + n_vertices, sizeof(int32_t),
                           hipMemcpyDeviceToHost, st));
     HG_HIP(hipStreamSynchronize(st));
-    HG_HIP(hipMemcpyAsync(&totals[1], corner_ptr.ptr + totals[0], sizeof(int32_t),
-                          hipMemcpyDeviceToHost, st));
-    HG_HIP(hipStreamSynchronize(st));
-    const int64_t nnz = totals[0], n_slots = totals[1];
+    // Every face key carries its corner slot: two a corner.
+    const int64_t nnz = entries, n_slots = 6 * n_faces;
     auto copy = [&](int32_t* dst, const DeviceArray<int32_t>& src, int64_t count) {
       if (dst != nullptr && count > 0)
-        HG_HIP(hipMemcpyAsync(dst, src.ptr, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, st));
+        HG_HIP(hipMemcpyAsync(dst, src.ptr, sizeof(int32_t) * count, hipMemcpyDefault, st));
     };
     copy(d_offsets, offsets, n_vertices + 1);
     copy(d_columns, columns, nnz);

@@ -90,3 +90,30 @@ def test_mesh_pattern_is_the_device_pattern(cuda):
     want = ho.operator_pattern(len(v), np.asarray(f).reshape(-1, 3))
     assert np.array_equal(ops[2].offsets.cpu().numpy(), want.indptr)
     assert np.array_equal(ops[2].columns.cpu().numpy(), want.indices)
+
+
+def test_pattern_from_host_buffers(cuda):
+    # The C ABI takes the faces and the outputs in host memory too (copied through the library).
+    import ctypes
+
+    from ordito_amd import _lib
+
+    v, f = mesh("grid")
+    faces = np.ascontiguousarray(np.asarray(f, dtype=np.int32).reshape(-1))
+    n, n_faces = len(v), faces.size // 3
+    cap = 6 * n_faces + n
+    offsets = np.zeros(n + 1, dtype=np.int32)
+    columns = np.zeros(cap, dtype=np.int32)
+    run_ptr = np.zeros(cap + 1, dtype=np.int32)
+    slots = np.zeros(6 * n_faces, dtype=np.int32)
+    sizes = (ctypes.c_int64 * 2)()
+    ptr = lambda a: a.ctypes.data  # noqa: E731
+    _lib.call("hg_operator_pattern", ptr(faces), n_faces, n, None, ptr(offsets), ptr(columns),
+              ptr(run_ptr), ptr(slots), None, None, sizes)
+    want = ho.operator_pattern(n, faces.reshape(-1, 3))
+    nnz = int(sizes[0])
+    assert nnz == want.indices.size and int(sizes[1]) == want.corner_slots.size
+    assert np.array_equal(offsets, want.indptr)
+    assert np.array_equal(columns[:nnz], want.indices)
+    assert np.array_equal(run_ptr[: nnz + 1], want.corner_ptr)
+    assert np.array_equal(slots, want.corner_slots)
