@@ -317,7 +317,7 @@ class GeodesicServer:
                     if other.solver is not None and not other.busy and other is not lane]
             if not idle:
                 break
-            _, n, victim = min(idle)
+            _, n, victim = min(idle, key=lambda item: item[:2])  # never compares unit keys
             e = self._resident[victim]
             other = e.lanes.pop(n)
             other.solver.release()
