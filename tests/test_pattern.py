@@ -117,3 +117,10 @@ def test_pattern_from_host_buffers(cuda):
     assert np.array_equal(columns[:nnz], want.indices)
     assert np.array_equal(run_ptr[: nnz + 1], want.corner_ptr)
     assert np.array_equal(slots, want.corner_slots)
+
+
+def test_pattern_degenerate_faces(cuda):
+    # A face repeating a vertex: its self-edge lands on that vertex's diagonal entry with the
+    # corner slots it carries, and its two other edges both ways, as the sorted keys give them.
+    faces = np.array([[0, 1, 2], [2, 2, 3], [3, 1, 1], [0, 4, 0]], dtype=np.int32)
+    _check(5, faces)
