@@ -135,22 +135,36 @@ __global__ __launch_bounds__(256) void panel_diag_kernel(const int32_t* __restri
       d = 1.0;
     }
     if (i > k && i < b) {
+      // Every operand of this pivot's sixteen updates is read before any is written back, with
+      // no branch per entry: none is written during the pivot (column k of D and row k of Zd stay
+      // as they are until it ends). Entry j of the thread's row is Zd's (j <= k) or D's; past
+      // the diagonal (j > i) it is D's unused upper triangle, updated without effect.
       const double lik = Ds[i][k] / d;
+      double* at[16];
+      double own[16], other[16];
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int j = cg + 4 * v;
-        if (j > k && j <= i) Ds[i][j] -= lik * Ds[j][k];
-        else if (j <= k) Zs[i][j] -= lik * Zs[k][j];
+        const bool z = j <= k;
+        at[v] = z ? &Zs[i][j] : &Ds[i][j];
+        own[v] = *at[v];
+        other[v] = z ? Zs[k][j] : Ds[j][k];
       }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) *at[v] = own[v] - lik * other[v];
     }
     __syncthreads();
     // Row k is read by no later pivot's updates, so its scaling needs no barrier of its own.
     if (i == k) {
+      // Loaded together and stored together, as the updates are (entries past k stay as they are).
       const double inv = 1.0 / sqrt(d);
+      double row[16];
+#pragma unroll
+      for (int v = 0; v < 16; ++v) row[v] = Zs[k][cg + 4 * v];
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int j = cg + 4 * v;
-        if (j <= k) Zs[k][j] *= inv;
+        Zs[k][j] = j <= k ? row[v] * inv : row[v];
       }
     }
   }
