@@ -15,6 +15,8 @@
 // the product of the panels' elementary inverses [I 0 0; 0 D^-1 0; 0 -P D^-1 I] (D the panel's
 // diagonal block factor, P the panel below it), so each panel step updates Z with the same tile
 // product that updates the trailing matrix, and no separate triangular inversion is needed.
+#include <utility>
+
 #include "cholesky.h"
 
 namespace hg {
@@ -101,78 +103,116 @@ __device__ __forceinline__ void tile_product(FA a_at, FB b_at, int32_t kb_end, i
 }
 
 // Panel step at column j0 of front s, first launch: the task of kind 2 eliminates the b x b
-// diagonal block D in LDS while applying the same eliminations to the identity, which leaves
+// diagonal block D while applying the same eliminations to the identity, which leaves
 // Zd = L_D^-1 (right-looking: at pivot k, rows i > k lose D(i,k)/D(k,k) times row k of D's
 // trailing part and of Zd, then Zd's row k is divided by sqrt(D(k,k))), and stores Zd as Z's
 // diagonal block; the launch's other tasks return. Once per front panel, where every task of the
 // panel once repeated it. A non-positive pivot raises status[0] (cholesky.py:260-266).
+//
+// Thread (row i, column group cg = its wave) keeps entries (i, cg + 4v), v < 16, in registers:
+// D's lower triangle (zero above) on entry. Zd's column k only becomes nonzero at pivot k, when
+// D's column k is spent, so one register serves both: column j holds D's entry before pivot j and
+// Zd's after it. A pivot's shared operands (Zd's row k before its scaling, 1 at the diagonal, D's
+// column k below it, and D(k,k)) are published by the threads that hold them into a two-slot LDS
+// buffer before the pivot's barrier; the next pivot's go to the other slot, so one barrier a pivot
+// orders both. The pivots are unrolled (one instance per k), so every register index is fixed at
+// compile time. Slot layout: entry j at (j % 4) * 16 + j / 4, so wave cg's sixteen operands are
+// contiguous.
+__device__ __forceinline__ int pivot_slot(int j) { return (j % 4) * 16 + j / 4; }
+
+struct PanelPivots {
+  double e[16];
+  double (*ops)[kFt];
+  double* piv;
+  int i, cg, b;
+  bool bad;
+  double dk;  // D(i, i) at pivot i (1 if it was not positive)
+
+  template <int K>
+  __device__ __forceinline__ void pivot() {
+    constexpr int p = K & 1, q = p ^ 1;
+    __syncthreads();
+    double d = piv[p];
+    if (!(d > 0.0)) {
+      bad = true;
+      d = 1.0;
+    }
+    if (i == K) dk = d;
+    if (i > K && i < b) {
+      const double lik = ops[p][pivot_slot(i)] / d;
+      double other[16];
+#pragma unroll
+      for (int v = 0; v < 16; ++v) other[v] = ops[p][cg * 16 + v];
+      // Zd's entry (i, K) is zero before this pivot (the identity's) while e holds D's there: the
+      // update starts from zero.
+      if (cg == K % 4) e[K / 4] = 0.0;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) e[v] = e[v] - lik * other[v];
+    }
+    // Pivot K + 1's operands, from this pivot's results: row K + 1 of Zd (its lane in every wave),
+    // D's column K + 1 below the diagonal and D(K + 1, K + 1) (wave (K + 1) % 4).
+    if (K + 1 < b) {
+      if (i == K + 1) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+          if (cg + 4 * v <= K) ops[q][cg * 16 + v] = e[v];
+      }
+      if (cg == (K + 1) % 4 && i >= K + 1) {
+        const double c = e[(K + 1) / 4];
+        ops[q][pivot_slot(i)] = i == K + 1 ? 1.0 : c;
+        if (i == K + 1) piv[q] = c;
+      }
+    }
+  }
+
+  // Pivots 0, 1, ... in order while K < b (&& stops at the first that is not).
+  template <int... K>
+  __device__ __forceinline__ void run(std::integer_sequence<int, K...>) {
+    (void)((K < b && (pivot<K>(), true)) && ...);
+  }
+};
+
 __global__ __launch_bounds__(256) void panel_diag_kernel(const int32_t* __restrict__ tasks,
                                                          FactorArgs a) {
-  __shared__ double Ds[kFt][kFt + 1];
-  __shared__ double Zs[kFt][kFt + 1];
+  __shared__ double ops[2][kFt];
+  __shared__ double piv[2];
   const int32_t* task = tasks + 4 * static_cast<int64_t>(blockIdx.x);
   if (task[2] != 2) return;
   const int32_t s = task[0], j0 = task[1];
   const int32_t nc = a.ncol[s];
   const int64_t m = nc + a.nrow[s];
-  const int32_t b = min(kFt, nc - j0);
   const double* F = a.arena + a.front_off[s];
   double* Z = a.z + a.w_off[s];
   const int t = threadIdx.x;
-  for (int e = t; e < kFt * kFt; e += 256) {
-    const int i = e % kFt, j = e / kFt;
-    Ds[i][j] = (i < b && j < b && i >= j) ? F[(j0 + j) * m + j0 + i] : 0.0;
-    Zs[i][j] = (i == j && i < b) ? 1.0 : 0.0;
+  PanelPivots pp;
+  pp.ops = ops;
+  pp.piv = piv;
+  pp.i = t % kFt;
+  pp.cg = t / kFt;
+  pp.b = min(kFt, nc - j0);
+  pp.bad = false;
+  pp.dk = 1.0;
+  const int i = pp.i, cg = pp.cg, b = pp.b;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int j = cg + 4 * v;
+    pp.e[v] = (i < b && j <= i) ? F[(j0 + j) * m + j0 + i] : 0.0;
   }
-  __syncthreads();
-  // Thread (row i, column group cg) owns entries (i, cg + 4v), v < 16.
-  const int i = t % kFt, cg = t / kFt;
-  bool bad = false;
-  for (int k = 0; k < b; ++k) {
-    double d = Ds[k][k];
-    if (!(d > 0.0)) {
-      bad = true;
-      d = 1.0;
-    }
-    if (i > k && i < b) {
-      // Every operand of this pivot's sixteen updates is read before any is written back, with
-      // no branch per entry: none is written during the pivot (column k of D and row k of Zd stay
-      // as they are until it ends). Entry j of the thread's row is Zd's (j <= k) or D's; past
-      // the diagonal (j > i) it is D's unused upper triangle, updated without effect.
-      const double lik = Ds[i][k] / d;
-      double* at[16];
-      double own[16], other[16];
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int j = cg + 4 * v;
-        const bool z = j <= k;
-        at[v] = z ? &Zs[i][j] : &Ds[i][j];
-        own[v] = *at[v];
-        other[v] = z ? Zs[k][j] : Ds[j][k];
-      }
-#pragma unroll
-      for (int v = 0; v < 16; ++v) *at[v] = own[v] - lik * other[v];
-    }
-    __syncthreads();
-    // Row k is read by no later pivot's updates, so its scaling needs no barrier of its own.
-    if (i == k) {
-      // Loaded together and stored together, as the updates are (entries past k stay as they are).
-      const double inv = 1.0 / sqrt(d);
-      double row[16];
-#pragma unroll
-      for (int v = 0; v < 16; ++v) row[v] = Zs[k][cg + 4 * v];
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int j = cg + 4 * v;
-        Zs[k][j] = j <= k ? row[v] * inv : row[v];
-      }
-    }
+  // Pivot 0's operands: D's column 0 (wave 0 holds it in e[0]), 1 at the diagonal, D(0,0).
+  if (cg == 0) {
+    ops[0][pivot_slot(i)] = i == 0 ? 1.0 : pp.e[0];
+    if (i == 0) piv[0] = pp.e[0];
   }
-  __syncthreads();
-  if (bad && t == 0) atomicOr(a.status, 1);
-  for (int e = t; e < kFt * kFt; e += 256) {
-    const int r = e % kFt, c = e / kFt;
-    if (r < b && c < b && r >= c) Z[static_cast<int64_t>(j0 + c) * m + j0 + r] = Zs[r][c];
+  pp.run(std::make_integer_sequence<int, kFt>{});
+  if (pp.bad && t == 0) atomicOr(a.status, 1);
+  // Zd's row i divided by sqrt(D(i,i)): no pivot after i reads it, so the scaling waits for the
+  // end (its diagonal entry is the identity's 1 until then).
+  const double inv = 1.0 / sqrt(pp.dk);
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int j = cg + 4 * v;
+    const double z = j == i ? 1.0 * inv : pp.e[v] * inv;
+    if (i < b && j <= i) Z[static_cast<int64_t>(j0 + j) * m + j0 + i] = z;
   }
 }
 
