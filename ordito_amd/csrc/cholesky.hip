@@ -72,11 +72,30 @@ __global__ __launch_bounds__(256) void extend_add_kernel(const int32_t* __restri
   const double* child = a.arena + a.front_off[c];
   double* parent = a.arena + a.front_off[s];
   const int32_t* map = a.rel + a.row_ptr[c];
-  for (int e = threadIdx.x; e < kFt * kFt; e += 256) {
+  // A thread's sixteen entries: the maps, then the child's and the parent's entries, all loaded
+  // before any sum is stored. Entries past the child's rows read its last row instead (a task
+  // only exists for rc >= 1), so every load is in bounds and none waits behind a branch.
+  int32_t mi[16], mk[16];
+  double add[16], cur[16];
+  bool live[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = threadIdx.x + 256 * u;
     const int32_t i = i0 + e % kFt, k = k0 + e / kFt;
-    if (i < rc && k < rc && i >= k)
-      parent[static_cast<int64_t>(map[k]) * ms + map[i]] += child[(ncc + k) * mc + ncc + i];
+    live[u] = i < rc && k < rc && i >= k;
+    const int32_t ic = min(i, rc - 1), kc = min(k, rc - 1);
+    mi[u] = map[ic];
+    mk[u] = map[kc];
+    add[u] = child[(ncc + kc) * mc + ncc + ic];
   }
+  int64_t at[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) at[u] = static_cast<int64_t>(mk[u]) * ms + mi[u];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) cur[u] = parent[at[u]];
+#pragma unroll
+  for (int u = 0; u < 16; ++u)
+    if (live[u]) parent[at[u]] = cur[u] + add[u];
 }
 
 // The FP64 matrix cores: v_mfma_f64_16x16x4 takes a 16 x 4 block of A (lane l holds
@@ -233,16 +252,27 @@ __global__ __launch_bounds__(256) void panel_apply_kernel(const int32_t* __restr
   double* F = a.arena + a.front_off[s];
   double* Z = a.z + a.w_off[s];
   const int t = threadIdx.x;
-  for (int e = t; e < kFt * kFt; e += 256) {
+  // Every load is issued before any LDS store (see update_kernel).
+  double zv[16], tv[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = t + 256 * u;
     const int i = e % kFt, j = e / kFt;
-    Zs[i][j] = (i < b && j < b && i >= j) ? Z[static_cast<int64_t>(j0 + j) * m + j0 + i] : 0.0;
+    zv[u] = (i < b && j < b && i >= j) ? Z[static_cast<int64_t>(j0 + j) * m + j0 + i] : 0.0;
     if (kind == 0) {
       const int64_t r = t0 + i;
-      Ts[i][j] = (r < m && j < b) ? F[(j0 + j) * m + r] : 0.0;
+      tv[u] = (r < m && j < b) ? F[(j0 + j) * m + r] : 0.0;
     } else {
       // Ts[k][cc] = Z(j0 + k, t0 + cc), read along k (contiguous).
-      Ts[i][j] = (i < b) ? Z[static_cast<int64_t>(t0 + j) * m + j0 + i] : 0.0;
+      tv[u] = (i < b) ? Z[static_cast<int64_t>(t0 + j) * m + j0 + i] : 0.0;
     }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = t + 256 * u;
+    const int i = e % kFt, j = e / kFt;
+    Zs[i][j] = zv[u];
+    Ts[i][j] = tv[u];
   }
   __syncthreads();
   const int wave = t / 64, lane = t % 64;
@@ -288,17 +318,28 @@ __global__ __launch_bounds__(256) void update_kernel(const int32_t* __restrict__
   double* F = a.arena + a.front_off[s];
   double* Z = a.z + a.w_off[s];
   const int t = threadIdx.x;
-  for (int e = t; e < kFt * kFt; e += 256) {
+  // A thread's sixteen entries of each operand: every load is issued before any LDS store, so
+  // they overlap instead of each waiting behind the store before it.
+  double av[16], bv[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = t + 256 * u;
     const int r = e % kFt, k = e / kFt;
-    As[k][r] = (i0 + r < m && k < b) ? F[(j0 + k) * m + i0 + r] : 0.0;
+    av[u] = (i0 + r < m && k < b) ? F[(j0 + k) * m + i0 + r] : 0.0;
     if (trailing) {
-      Bs[k][r] = (c0 + r < m && k < b) ? F[(j0 + k) * m + c0 + r] : 0.0;
+      bv[u] = (c0 + r < m && k < b) ? F[(j0 + k) * m + c0 + r] : 0.0;
     } else {
       // Bs[k][cc] = Z(j0 + k, c0 + cc): e % 64 runs along k here, contiguous in Z.
-      const int kk = e % kFt, cc = e / kFt;
-      Bs[kk][cc] = (kk < b && c0 + cc < j0 + b) ? Z[static_cast<int64_t>(c0 + cc) * m + j0 + kk]
-                                                 : 0.0;
+      bv[u] = (r < b && c0 + k < j0 + b) ? Z[static_cast<int64_t>(c0 + k) * m + j0 + r] : 0.0;
     }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = t + 256 * u;
+    const int r = e % kFt, k = e / kFt;
+    As[k][r] = av[u];
+    if (trailing) Bs[k][r] = bv[u];
+    else Bs[r][k] = bv[u];
   }
   __syncthreads();
   const int wave = t / 64, lane = t % 64;
@@ -307,18 +348,27 @@ __global__ __launch_bounds__(256) void update_kernel(const int32_t* __restrict__
   for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
   tile_product([&](int r, int k) { return As[k][r]; }, [&](int k, int c) { return Bs[k][c]; }, b,
                wave, lane, acc);
+  // The tile's entries are all loaded before any is stored back.
+  double* C = trailing ? F : Z;
+  double cv[4][4];
+  bool live[4][4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     const int64_t c = c0 + 16 * n + lane % 16;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int64_t r = i0 + 16 * wave + lane / 16 + 4 * v;
-      if (r >= m) continue;
-      if (trailing) {
-        if (c < m && r >= c) F[c * m + r] -= acc[n][v];
-      } else if (c < j0 + b) {
-        Z[c * m + r] -= acc[n][v];
-      }
+      live[n][v] = r < m && (trailing ? (c < m && r >= c) : c < j0 + b);
+      cv[n][v] = live[n][v] ? C[c * m + r] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int64_t c = c0 + 16 * n + lane % 16;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = i0 + 16 * wave + lane / 16 + 4 * v;
+      if (live[n][v]) C[c * m + r] = cv[n][v] - acc[n][v];
     }
   }
 }
