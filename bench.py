@@ -562,6 +562,64 @@ def pattern_workload(args) -> int:
     return 0
 
 
+def icosphere_rows() -> dict:
+    """
+    ordito's own ``heat_geodesic`` benchmark rows (``benchmarks/test_heat.py:136-230``, published in
+    ``docs/benchmarks/heat.md:52-57``): unit icospheres of 4, 6 and 7 subdivisions (sphere_small,
+    sphere_med, sphere_large), one source at vertex 0. ``full`` times
+    ``heat_geodesic(vertices, faces, sources)`` (operators built and both solves run inside the
+    call); ``amortized`` times ``heat_geodesic(mesh, sources)`` on a ``Trimesh`` warmed by two
+    calls, so both systems are factored and each call is two refined solves. Medians over 20 calls
+    (10 for the full icosphere(7)), each checked against the FP64 oracle on icosphere(4).
+    """
+    import torch
+
+    from ordito_amd import creation, heat
+    from oracle import heat_oracle as ho
+
+    published = {4: (3.47, 0.3243), 6: (6.60, 0.6897), 7: (21.63, 1.48)}
+    rows = {}
+    src = torch.tensor([0], dtype=torch.int32, device="cuda")
+    for level in (4, 6, 7):
+        v, f = creation.icosphere(level)
+        vg, fg = torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda()
+
+        def timed(call, count):
+            lat = []
+            for _ in range(count):
+                t0 = time.perf_counter()
+                got = call()
+                torch.cuda.synchronize()
+                lat.append(time.perf_counter() - t0)
+            return got, lat
+
+        for _ in range(2):
+            heat.heat_geodesic(vg, fg, src)
+        torch.cuda.synchronize()
+        got_full, full = timed(lambda: heat.heat_geodesic(vg, fg, src), 10 if level == 7 else 20)
+        mesh = heat.Trimesh(vg, fg)
+        for _ in range(2):
+            heat.heat_geodesic(mesh, src)
+        torch.cuda.synchronize()
+        got_amortized, amortized = timed(lambda: heat.heat_geodesic(mesh, src), 20)
+        row = {
+            "n_vertices": int(len(v)),
+            "full_p50_ms": ms(full, 50),
+            "amortized_p50_ms": ms(amortized, 50),
+            "ordito_rtx5090_ms": {"full": published[level][0], "amortized": published[level][1]},
+            "full_vs_amortized_max_abs": float(
+                (got_full - got_amortized).abs().max().item()),
+        }
+        if level == 4:
+            expected = ho.geodesic(v, f, np.array([0])).distance
+            row["rel_err_max"] = ho.relative_error(got_amortized.cpu().numpy(), expected)[0]
+        rows[f"icosphere{level}"] = row
+        log(f"   icosphere({level}): {row}")
+        del mesh
+        torch.cuda.empty_cache()
+    return rows
+
+
 def one_shot_result(oracle_c2, c2_vertices, c2_faces, c2_source: int) -> dict:
     """
     The reference's one-shot form (``heat_geodesic(vertices, faces, sources)``: every call builds
@@ -597,6 +655,7 @@ def one_shot_result(oracle_c2, c2_vertices, c2_faces, c2_source: int) -> dict:
         "rel_err_max": ho.relative_error(got.cpu().numpy(), expected)[0],
     }
     log(f"   one-shot icosphere(4): {out['icosphere4_jittered']}")
+    out["icospheres"] = icosphere_rows()
     vg, fg = torch.from_numpy(c2_vertices).cuda(), torch.from_numpy(c2_faces).cuda()
     src = torch.tensor([c2_source], dtype=torch.int32, device="cuda")
     lat = []

@@ -333,22 +333,9 @@ __global__ __launch_bounds__(256) void update_kernel(const int32_t* __restrict__
       bv[u] = (r < b && c0 + k < j0 + b) ? Z[static_cast<int64_t>(c0 + k) * m + j0 + r] : 0.0;
     }
   }
-#pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int e = t + 256 * u;
-    const int r = e % kFt, k = e / kFt;
-    As[k][r] = av[u];
-    if (trailing) Bs[k][r] = bv[u];
-    else Bs[r][k] = bv[u];
-  }
-  __syncthreads();
+  // The output tile's entries are loaded with the operands: no task of the launch writes them
+  // before this one does, and its products read none of them.
   const int wave = t / 64, lane = t % 64;
-  f64x4 acc[4];
-#pragma unroll
-  for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
-  tile_product([&](int r, int k) { return As[k][r]; }, [&](int k, int c) { return Bs[k][c]; }, b,
-               wave, lane, acc);
-  // The tile's entries are all loaded before any is stored back.
   double* C = trailing ? F : Z;
   double cv[4][4];
   bool live[4][4];
@@ -362,6 +349,20 @@ __global__ __launch_bounds__(256) void update_kernel(const int32_t* __restrict__
       cv[n][v] = live[n][v] ? C[c * m + r] : 0.0;
     }
   }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = t + 256 * u;
+    const int r = e % kFt, k = e / kFt;
+    As[k][r] = av[u];
+    if (trailing) Bs[k][r] = bv[u];
+    else Bs[r][k] = bv[u];
+  }
+  __syncthreads();
+  f64x4 acc[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
+  tile_product([&](int r, int k) { return As[k][r]; }, [&](int k, int c) { return Bs[k][c]; }, b,
+               wave, lane, acc);
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     const int64_t c = c0 + 16 * n + lane % 16;
