@@ -172,23 +172,6 @@ struct Dissector {
   }
 };
 
-// Runs body(begin, end, worker) over [0, count) in contiguous chunks, one a worker thread (the
-// caller's thread takes the first); waits for all, rethrowing the first failure.
-template <typename F>
-void parallel_chunks(int64_t count, int64_t grain, F&& body) {
-  const int64_t hw = std::max<int64_t>(1, std::thread::hardware_concurrency());
-  const int64_t workers = std::max<int64_t>(1, std::min<int64_t>(hw, (count + grain - 1) / grain));
-  const int64_t chunk = (count + workers - 1) / std::max<int64_t>(workers, 1);
-  std::vector<std::future<void>> pool;
-  for (int64_t w = 1; w < workers; ++w) {
-    const int64_t b = w * chunk, e = std::min(count, b + chunk);
-    if (b >= e) break;
-    pool.push_back(std::async(std::launch::async, [&body, b, e, w] { body(b, e, w); }));
-  }
-  body(0, std::min(count, chunk), 0);
-  for (std::future<void>& f : pool) f.get();
-}
-
 }  // namespace
 
 Plan analyse(const HostPattern& pattern, const float* coordinates, const AnalysisOptions& options) {

@@ -411,20 +411,28 @@ void ensure_plan(hg_mesh& m) {
   device->upload(plan, m.pattern, m.split_target, m.stream);
   {
     std::vector<int32_t> rows(m.h_faces.size());
-    for (size_t e = 0; e < rows.size(); ++e) rows[e] = plan.iperm[m.h_faces[e]];
+    parallel_chunks(static_cast<int64_t>(rows.size()), 1 << 16, [&](int64_t b, int64_t e, int64_t) {
+      for (int64_t i = b; i < e; ++i) rows[i] = plan.iperm[m.h_faces[i]];
+    });
     m.faces_rows.upload(rows, m.stream);
-    // Row r's corners are vertex perm[r]'s, in its ascending corner order.
+    // Row r's corners are vertex perm[r]'s, in its ascending corner order: the row offsets in
+    // one pass, then each chunk of rows fills its own records.
     std::vector<int32_t> ptr(static_cast<size_t>(m.n) + 1, 0);
-    std::vector<int4> corners(m.pattern.vertex_corners.size());
     for (int64_t r = 0; r < m.n; ++r) {
       const int32_t v = plan.perm[r];
-      int32_t at = ptr[r];
-      for (int32_t k = m.pattern.vertex_ptr[v]; k < m.pattern.vertex_ptr[v + 1]; ++k) {
-        const int32_t c = m.pattern.vertex_corners[k], f = c / 3;
-        corners[at++] = make_int4(c, rows[3 * f], rows[3 * f + 1], rows[3 * f + 2]);
-      }
-      ptr[r + 1] = at;
+      ptr[r + 1] = ptr[r] + (m.pattern.vertex_ptr[v + 1] - m.pattern.vertex_ptr[v]);
     }
+    std::vector<int4> corners(m.pattern.vertex_corners.size());
+    parallel_chunks(m.n, 1 << 14, [&](int64_t b, int64_t e, int64_t) {
+      for (int64_t r = b; r < e; ++r) {
+        const int32_t v = plan.perm[r];
+        int32_t at = ptr[r];
+        for (int32_t k = m.pattern.vertex_ptr[v]; k < m.pattern.vertex_ptr[v + 1]; ++k) {
+          const int32_t c = m.pattern.vertex_corners[k], f = c / 3;
+          corners[at++] = make_int4(c, rows[3 * f], rows[3 * f + 1], rows[3 * f + 2]);
+        }
+      }
+    });
     m.row_corners.upload(corners, m.stream);
     m.row_corner_ptr.upload(ptr, m.stream);
   }

@@ -736,22 +736,28 @@ void DevicePlan::upload(const Plan& plan, const HostPattern& pattern, int32_t ta
   // renumbered and sorted; src names each entry's slot in the original values.
   {
     const int64_t nn = plan.n;
-    std::vector<int32_t> offs(nn + 1, 0), cols, src;
-    cols.reserve(pattern.columns.size());
-    src.reserve(pattern.columns.size());
-    std::vector<std::pair<int32_t, int32_t>> row;
+    std::vector<int32_t> offs(nn + 1, 0);
     for (int64_t i = 0; i < nn; ++i) {
       const int32_t old = plan.perm[i];
-      row.clear();
-      for (int32_t e = pattern.offsets[old]; e < pattern.offsets[old + 1]; ++e)
-        row.push_back({plan.iperm[pattern.columns[e]], e});
-      std::sort(row.begin(), row.end());
-      for (const auto& entry : row) {
-        cols.push_back(entry.first);
-        src.push_back(entry.second);
-      }
-      offs[i + 1] = static_cast<int32_t>(cols.size());
+      offs[i + 1] = offs[i] + (pattern.offsets[old + 1] - pattern.offsets[old]);
     }
+    std::vector<int32_t> cols(static_cast<size_t>(offs[nn])), src(static_cast<size_t>(offs[nn]));
+    parallel_chunks(nn, 1 << 14, [&](int64_t b, int64_t e_end, int64_t) {
+      std::vector<std::pair<int32_t, int32_t>> row;
+      for (int64_t i = b; i < e_end; ++i) {
+        const int32_t old = plan.perm[i];
+        row.clear();
+        for (int32_t e = pattern.offsets[old]; e < pattern.offsets[old + 1]; ++e)
+          row.push_back({plan.iperm[pattern.columns[e]], e});
+        std::sort(row.begin(), row.end());
+        int32_t at = offs[i];
+        for (const auto& entry : row) {
+          cols[at] = entry.first;
+          src[at] = entry.second;
+          ++at;
+        }
+      }
+    });
     nnz = static_cast<int64_t>(cols.size());
     p_offsets.upload(offs, stream);
     p_columns.upload(cols, stream);
